@@ -1,0 +1,144 @@
+"""ctypes binding of libheddle_hip.so (include/heddle_hip.h).
+
+The library is the product: this module only declares its C ABI.  There is
+no Python fallback for any entry point -- if the shared library is missing
+the import fails loudly.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libheddle_hip.so")
+
+_c_u32p = ctypes.POINTER(ctypes.c_uint32)
+_c_i32p = ctypes.POINTER(ctypes.c_int32)
+_c_fp = ctypes.POINTER(ctypes.c_float)
+_c_dp = ctypes.POINTER(ctypes.c_double)
+_c_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+HMD_OK = 0
+STATUS_NAMES = {1: "HMD_ERR_GPU", 2: "HMD_ERR_INVALID", 3: "HMD_ERR_NBL_OVERFLOW",
+                4: "HMD_ERR_BOX_TOO_SMALL", 5: "HMD_ERR_UNSUPPORTED", 6: "HMD_ERR_NO_DEVICE"}
+
+# Every symbol include/heddle_hip.h declares (checked by the CPU tests).
+EXPORTED = [
+    "hmd_last_error", "hmd_version", "hmd_ctx_create", "hmd_ctx_destroy",
+    "hmd_ctx_synchronize", "hmd_ctx_device_info", "hmd_kernel_count", "hmd_kernel_name",
+    "hmd_particles_create", "hmd_particles_destroy", "hmd_particles_download",
+    "hmd_particles_upload", "hmd_particles_download_energies", "hmd_particles_download_lo",
+    "hmd_particles_kinetic_energy", "hmd_particles_energy_sums",
+    "hmd_forcefield_create", "hmd_forcefield_destroy", "hmd_forcefield_compute",
+    "hmd_forcefield_slots", "hmd_spme_scalars", "hmd_spme_grids", "hmd_spme_forces",
+    "hmd_nbl_request_rebuild", "hmd_nbl_pair_snapshot", "hmd_nbl_stats",
+    "hmd_integrator_create", "hmd_integrator_destroy", "hmd_integrator_step",
+    "hmd_integrator_capture", "hmd_integrator_replay", "hmd_integrator_steps",
+    "hmd_integrator_profile", "hmd_integrator_minimize",
+    "hmd_generate_velocities", "hmd_chacha_block",
+]
+
+
+class ForceFieldDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_types", ctypes.c_uint32),
+        ("lj_on", ctypes.c_int32),
+        ("lj_sigma", _c_dp), ("lj_epsilon", _c_dp), ("lj_cutoff", _c_dp), ("lj_switch", _c_dp),
+        ("coulomb_on", ctypes.c_int32),
+        ("alpha", ctypes.c_double), ("r_cut_real", ctypes.c_double), ("k_coulomb", ctypes.c_double),
+        ("spme_on", ctypes.c_int32),
+        ("grid", ctypes.c_uint32 * 3), ("spline_order", ctypes.c_uint32),
+        ("brick", ctypes.c_uint32 * 3),
+        ("r_skin", ctypes.c_double),
+        ("excl_offsets", _c_u32p), ("excl_partners", _c_u32p),
+        ("excl_lj_scale", _c_fp), ("excl_coul_scale", _c_fp),
+    ]
+
+
+class IntegratorDesc(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_char_p),
+        ("dt", ctypes.c_double),
+        ("compensated", ctypes.c_int32),
+        ("temperature", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("n_groups", ctypes.c_uint32),
+        ("group_atoms", _c_u32p),
+        ("d_oh", ctypes.c_double), ("d_hh", ctypes.c_double),
+        ("m_o", ctypes.c_double), ("m_h", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libheddle_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m heddlemd_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    vp = ctypes.c_void_p
+    pvp = ctypes.POINTER(ctypes.c_void_p)
+    st = ctypes.c_int32
+    sig = {
+        "hmd_last_error": (ctypes.c_char_p, []),
+        "hmd_version": (ctypes.c_char_p, []),
+        "hmd_ctx_create": (st, [ctypes.c_int, pvp]),
+        "hmd_ctx_destroy": (None, [vp]),
+        "hmd_ctx_synchronize": (st, [vp]),
+        "hmd_ctx_device_info": (st, [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_int]),
+        "hmd_kernel_count": (ctypes.c_int, []),
+        "hmd_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),
+        "hmd_particles_create": (st, [vp, ctypes.c_uint32, _c_fp, _c_fp, _c_fp, _c_u32p, _c_fp, pvp]),
+        "hmd_particles_destroy": (None, [vp]),
+        "hmd_particles_download": (st, [vp, _c_fp, _c_fp, _c_fp, _c_i32p]),
+        "hmd_particles_upload": (st, [vp, _c_fp, _c_fp]),
+        "hmd_particles_download_energies": (st, [vp, _c_fp, _c_fp]),
+        "hmd_particles_download_lo": (st, [vp, _c_dp, _c_dp]),
+        "hmd_particles_kinetic_energy": (st, [vp, _c_dp]),
+        "hmd_particles_energy_sums": (st, [vp, _c_dp, _c_dp]),
+        "hmd_forcefield_create": (st, [vp, vp, ctypes.POINTER(ForceFieldDesc), pvp]),
+        "hmd_forcefield_destroy": (None, [vp]),
+        "hmd_forcefield_compute": (st, [vp, ctypes.c_int32]),
+        "hmd_forcefield_slots": (st, [vp, ctypes.c_char_p, ctypes.c_int32]),
+        "hmd_spme_scalars": (st, [vp, _c_dp]),
+        "hmd_spme_grids": (st, [vp, _c_fp, _c_fp]),
+        "hmd_spme_forces": (st, [vp, _c_fp]),
+        "hmd_nbl_request_rebuild": (st, [vp]),
+        "hmd_nbl_pair_snapshot": (st, [vp, _c_u32p, ctypes.c_uint64, _c_u64p]),
+        "hmd_nbl_stats": (st, [vp, _c_u32p]),
+        "hmd_integrator_create": (st, [vp, vp, vp, ctypes.POINTER(IntegratorDesc), pvp]),
+        "hmd_integrator_destroy": (None, [vp]),
+        "hmd_integrator_step": (st, [vp, ctypes.c_uint32]),
+        "hmd_integrator_capture": (st, [vp]),
+        "hmd_integrator_replay": (st, [vp, ctypes.c_uint32]),
+        "hmd_integrator_steps": (ctypes.c_uint64, [vp]),
+        "hmd_integrator_profile": (st, [vp, ctypes.c_uint32, _c_dp, ctypes.c_int32,
+                                        _c_i32p, ctypes.c_char_p, ctypes.c_int32]),
+        "hmd_integrator_minimize": (st, [vp, ctypes.c_uint32, ctypes.c_double]),
+        "hmd_generate_velocities": (st, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
+                                         ctypes.c_uint64, _c_dp, _c_fp]),
+        "hmd_chacha_block": (None, [_c_u32p, ctypes.c_uint32, _c_u32p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+class HmdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def check(code):
+    if code != HMD_OK:
+        raise HmdError(code, lib().hmd_last_error().decode())

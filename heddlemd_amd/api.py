@@ -1,0 +1,290 @@
+"""Thin Python handles over the C ABI, named after the reference's host types
+(GpuContext, ParticleBuffers, ForceField, NeighborListState, Integrator).
+All computation happens in libheddle_hip.so."""
+import ctypes
+
+import numpy as np
+
+from ._lib import ForceFieldDesc, IntegratorDesc, check, lib
+
+_fp = ctypes.POINTER(ctypes.c_float)
+_dp = ctypes.POINTER(ctypes.c_double)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+
+def _ptr(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class GpuContext:
+    """init_device (src/gpu/device.rs:67): one device, one stream, the
+    embedded gfx950 code object loaded."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib().hmd_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def synchronize(self):
+        check(lib().hmd_ctx_synchronize(self.h))
+
+    def device_info(self):
+        cus = ctypes.c_int()
+        name = ctypes.create_string_buffer(64)
+        check(lib().hmd_ctx_device_info(self.h, ctypes.byref(cus), name, 64))
+        return cus.value, name.value.decode()
+
+    def close(self):
+        if self.h:
+            lib().hmd_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def kernel_names():
+    L = lib()
+    return [L.hmd_kernel_name(i).decode() for i in range(L.hmd_kernel_count())]
+
+
+class ParticleBuffers:
+    """ParticleBuffers (src/gpu/buffers.rs:11-178)."""
+
+    def __init__(self, ctx, posq, mass, lattice, vel=None, types=None):
+        self.ctx = ctx
+        posq = np.ascontiguousarray(posq, np.float32).reshape(-1, 4)
+        self.n = posq.shape[0]
+        mass = np.ascontiguousarray(mass, np.float32)
+        lat = np.ascontiguousarray(lattice, np.float32)
+        v = None if vel is None else np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+        t = None if types is None else np.ascontiguousarray(types, np.uint32)
+        h = ctypes.c_void_p()
+        check(lib().hmd_particles_create(ctx.h, self.n, _ptr(posq, _fp), _ptr(v, _fp),
+                                         _ptr(mass, _fp), _ptr(t, _u32p), _ptr(lat, _fp),
+                                         ctypes.byref(h)))
+        self.h = h
+        self.lattice = lat
+
+    def download(self):
+        n = self.n
+        posq = np.empty((n, 4), np.float32)
+        vel = np.empty((n, 3), np.float32)
+        f = np.empty((n, 3), np.float32)
+        im = np.empty((n, 3), np.int32)
+        check(lib().hmd_particles_download(self.h, _ptr(posq, _fp), _ptr(vel, _fp),
+                                           _ptr(f, _fp), _ptr(im, _i32p)))
+        return {"posq": posq, "vel": vel, "force": f, "images": im}
+
+    def forces(self):
+        f = np.empty((self.n, 3), np.float32)
+        check(lib().hmd_particles_download(self.h, None, None, _ptr(f, _fp), None))
+        return f
+
+    def upload(self, posq=None, vel=None):
+        p = None if posq is None else np.ascontiguousarray(posq, np.float32)
+        v = None if vel is None else np.ascontiguousarray(vel, np.float32)
+        check(lib().hmd_particles_upload(self.h, _ptr(p, _fp), _ptr(v, _fp)))
+
+    def energies(self):
+        e = np.empty(self.n, np.float32)
+        w = np.empty(self.n, np.float32)
+        check(lib().hmd_particles_download_energies(self.h, _ptr(e, _fp), _ptr(w, _fp)))
+        return e, w
+
+    def lo_parts(self):
+        a = np.empty((self.n, 3))
+        b = np.empty((self.n, 3))
+        check(lib().hmd_particles_download_lo(self.h, _ptr(a, _dp), _ptr(b, _dp)))
+        return a, b
+
+    def kinetic_energy(self):
+        ke = ctypes.c_double()
+        check(lib().hmd_particles_kinetic_energy(self.h, ctypes.byref(ke)))
+        return ke.value
+
+    def energy_sums(self):
+        pe = ctypes.c_double()
+        w = ctypes.c_double()
+        check(lib().hmd_particles_energy_sums(self.h, ctypes.byref(pe), ctypes.byref(w)))
+        return pe.value, w.value
+
+    def close(self):
+        if self.h:
+            lib().hmd_particles_destroy(self.h)
+            self.h = None
+
+
+class ForceField:
+    """ForceField (src/forces/mod.rs) built from the builtin potential registry;
+    `system` is a dict from heddlemd_amd.systems."""
+
+    def __init__(self, ctx, particles, ff):
+        self.ctx = ctx
+        self.particles = particles
+        d = ForceFieldDesc()
+        keep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dt)
+            keep.append(a)
+            return a
+
+        nt = int(ff.get("n_types", 1))
+        d.n_types = nt
+        if ff.get("lj") is not None:
+            lj = ff["lj"]
+            d.lj_on = 1
+            d.lj_sigma = _ptr(arr(lj["sigma"], np.float64), _dp)
+            d.lj_epsilon = _ptr(arr(lj["epsilon"], np.float64), _dp)
+            d.lj_cutoff = _ptr(arr(lj["cutoff"], np.float64), _dp)
+            d.lj_switch = _ptr(arr(lj["switch"], np.float64), _dp)
+        if ff.get("coulomb") is not None:
+            co = ff["coulomb"]
+            d.coulomb_on = 1
+            d.alpha = co["alpha"]
+            d.r_cut_real = co["r_cut"]
+            d.k_coulomb = co.get("k_coulomb", 1.0)
+        if ff.get("spme") is not None:
+            sp = ff["spme"]
+            d.spme_on = 1
+            for k in range(3):
+                d.grid[k] = int(sp["grid"][k])
+                d.brick[k] = int(sp.get("brick", (0, 0, 0))[k])
+            d.spline_order = int(sp.get("order", 4))
+        d.r_skin = ff["r_skin"]
+        if ff.get("exclusions") is not None:
+            off, par = ff["exclusions"]
+            d.excl_offsets = _ptr(arr(off, np.uint32), _u32p)
+            d.excl_partners = _ptr(arr(par, np.uint32), _u32p)
+        self._keep = keep
+        h = ctypes.c_void_p()
+        check(lib().hmd_forcefield_create(ctx.h, particles.h, ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+
+    def compute(self, level=0):
+        check(lib().hmd_forcefield_compute(self.h, level))
+
+    def slots(self):
+        b = ctypes.create_string_buffer(256)
+        check(lib().hmd_forcefield_slots(self.h, b, 256))
+        return [s for s in b.value.decode().split("\n") if s]
+
+    def spme_scalars(self):
+        out = np.zeros(3)
+        check(lib().hmd_spme_scalars(self.h, _ptr(out, _dp)))
+        return {"w_recip": out[0], "u_recip": out[1], "u_self": out[2]}
+
+    def spme_grids(self, shape):
+        m = int(np.prod(shape))
+        rho = np.empty(m, np.float32)
+        v = np.empty(m, np.float32)
+        check(lib().hmd_spme_grids(self.h, _ptr(rho, _fp), _ptr(v, _fp)))
+        return rho.reshape(shape), v.reshape(shape)
+
+    def spme_forces(self):
+        f = np.empty((self.particles.n, 3), np.float32)
+        check(lib().hmd_spme_forces(self.h, _ptr(f, _fp)))
+        return f
+
+    def request_rebuild(self):
+        check(lib().hmd_nbl_request_rebuild(self.h))
+
+    def pair_snapshot(self):
+        """Canonical (min, max) pairs within r_c + r_skin, sorted."""
+        cap = 1 << 20
+        while True:
+            out = np.empty((cap, 2), np.uint32)
+            k = ctypes.c_uint64()
+            check(lib().hmd_nbl_pair_snapshot(self.h, _ptr(out, _u32p), cap, ctypes.byref(k)))
+            if k.value <= cap:
+                pairs = out[:k.value]
+                order = np.lexsort((pairs[:, 1], pairs[:, 0]))
+                return pairs[order]
+            cap = int(k.value)
+
+    def stats(self):
+        out = np.zeros(8, np.uint32)
+        check(lib().hmd_nbl_stats(self.h, _ptr(out, _u32p)))
+        keys = ["n_work", "generation", "status", "high_water", "cap_entries",
+                "n_clusters", "fine_cells", "pad"]
+        return dict(zip(keys, (int(x) for x in out)))
+
+    def close(self):
+        if self.h:
+            lib().hmd_forcefield_destroy(self.h)
+            self.h = None
+
+
+class Integrator:
+    """Integrator + constraint plan (src/integrator/mod.rs)."""
+
+    def __init__(self, ctx, particles, forcefield, kind="velocity-verlet", dt=1.0,
+                 compensated=False, temperature=0.0, gamma=0.0, seed=0, settle=None):
+        d = IntegratorDesc()
+        self._kind = kind.encode()
+        d.kind = self._kind
+        d.dt = dt
+        d.compensated = 1 if compensated else 0
+        d.temperature = temperature
+        d.gamma = gamma
+        d.seed = seed
+        self._groups = None
+        if settle is not None:
+            g = np.ascontiguousarray(settle["groups"], np.uint32)
+            self._groups = g
+            d.n_groups = g.shape[0]
+            d.group_atoms = _ptr(g, _u32p)
+            d.d_oh, d.d_hh, d.m_o, d.m_h = settle["d_oh"], settle["d_hh"], settle["m_o"], settle["m_h"]
+        h = ctypes.c_void_p()
+        check(lib().hmd_integrator_create(ctx.h, particles.h, forcefield.h, ctypes.byref(d),
+                                          ctypes.byref(h)))
+        self.h = h
+
+    def step(self, n=1):
+        check(lib().hmd_integrator_step(self.h, n))
+
+    def capture(self):
+        check(lib().hmd_integrator_capture(self.h))
+
+    def replay(self, n):
+        check(lib().hmd_integrator_replay(self.h, n))
+
+    def steps(self):
+        return int(lib().hmd_integrator_steps(self.h))
+
+    def minimize(self, n_iter, max_step):
+        check(lib().hmd_integrator_minimize(self.h, n_iter, max_step))
+
+    def profile(self, n_steps):
+        us = np.zeros(64)
+        k = ctypes.c_int32()
+        names = ctypes.create_string_buffer(4096)
+        check(lib().hmd_integrator_profile(self.h, n_steps, _ptr(us, _dp), 64,
+                                           ctypes.byref(k), names, 4096))
+        labels = [s for s in names.value.decode().split("\n") if s]
+        return dict(zip(labels, us[:k.value].tolist()))
+
+    def close(self):
+        if self.h:
+            lib().hmd_integrator_destroy(self.h)
+            self.h = None
+
+
+def generate_velocities(masses, temperature, seed, n_constraints=0):
+    m = np.ascontiguousarray(masses, np.float64)
+    out = np.empty((m.size, 3), np.float32)
+    check(lib().hmd_generate_velocities(m.size, n_constraints, temperature, seed,
+                                        _ptr(m, _dp), _ptr(out, _fp)))
+    return out
+
+
+def chacha_block(state, rounds):
+    s = np.ascontiguousarray(state, np.uint32)
+    out = np.empty(16, np.uint32)
+    lib().hmd_chacha_block(_ptr(s, _u32p), rounds, _ptr(out, _u32p))
+    return out

@@ -1,0 +1,713 @@
+// ForceField host: the potential registry, the neighbour list, the fused
+// pair slot and the SPME reciprocal slot.
+//
+// Mirrors src/forces/mod.rs (ForceField, PotentialRegistry with builtins in
+// evaluation order), src/forces/neighbor_list.rs (NeighborListState) and
+// src/forces/spme.rs (SpmeReciprocalGrid / SpmeReciprocalState).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <functional>
+
+#include "forcefield.h"
+
+namespace hmd {
+
+// ------------------------------------------------------------ registry ---
+// PotentialRegistry::with_builtins (src/forces/mod.rs:363-398): builders in
+// evaluation order; each claims its part of the descriptor.
+struct PotentialBuilder {
+  const char* label;
+  std::function<bool(const hmd_forcefield_desc&)> claims;
+};
+
+static const std::vector<PotentialBuilder>& builtin_registry() {
+  static const std::vector<PotentialBuilder> reg = {
+      {"lennard-jones", [](const hmd_forcefield_desc& d) { return d.lj_on != 0; }},
+      {"spme-real", [](const hmd_forcefield_desc& d) { return d.coulomb_on != 0; }},
+      {"spme-reciprocal", [](const hmd_forcefield_desc& d) { return d.spme_on != 0; }},
+  };
+  return reg;
+}
+
+// Radix plan for the LDS FFT: 4s first, then 2, 3, 5.
+static std::vector<uint32_t> radix_plan(uint32_t n) {
+  std::vector<uint32_t> r;
+  uint32_t m = n;
+  while (m % 4 == 0) { r.push_back(4); m /= 4; }
+  while (m % 2 == 0) { r.push_back(2); m /= 2; }
+  while (m % 3 == 0) { r.push_back(3); m /= 3; }
+  while (m % 5 == 0) { r.push_back(5); m /= 5; }
+  if (m != 1 || r.size() > 8)
+    throw Error(HMD_ERR_INVALID, "SPME grid size " + std::to_string(n) +
+                                     " must factor into 2, 3 and 5");
+  return r;
+}
+
+static void twiddles(DevBuf<float>& buf, uint32_t n, hipStream_t s) {
+  std::vector<float> t(2 * (size_t)n);
+  for (uint32_t m = 0; m < n; ++m) {
+    double a = -2.0 * M_PI * (double)m / (double)n;
+    t[2 * m] = (float)std::cos(a);
+    t[2 * m + 1] = (float)std::sin(a);
+  }
+  buf.alloc(t.size(), false);
+  buf.upload(t.data(), t.size(), s);
+}
+
+// |b(k)|^2 per axis with near-zero neighbour averaging
+// (src/forces/spme.rs:524-585), host f64 -> f32.
+static double cardinal_bspline(int p, double x) {
+  std::vector<double> v(p);
+  for (int i = 0; i < p; ++i) {
+    double xi = x - i;
+    v[i] = (xi >= 0.0 && xi < 1.0) ? 1.0 : 0.0;
+  }
+  for (int order = 2; order <= p; ++order) {
+    double inv = 1.0 / (order - 1.0);
+    for (int i = 0; i < p - order + 1; ++i) {
+      double xi = x - i;
+      v[i] = xi * inv * v[i] + (order - xi) * inv * v[i + 1];
+    }
+  }
+  return v[0];
+}
+
+static std::vector<float> b_factors(uint32_t n, uint32_t p) {
+  std::vector<double> mod(n);
+  for (uint32_t k = 0; k < n; ++k) {
+    double th = 2.0 * M_PI * k / n, re = 0.0, im = 0.0;
+    for (uint32_t j = 0; j + 1 < p; ++j) {
+      double m = cardinal_bspline((int)p, j + 1.0);
+      re += m * std::cos(th * j);
+      im += m * std::sin(th * j);
+    }
+    mod[k] = re * re + im * im;
+  }
+  for (uint32_t k = 0; k < n; ++k)
+    if (mod[k] < 1.0e-7) mod[k] = 0.5 * (mod[(k + n - 1) % n] + mod[(k + 1) % n]);
+  std::vector<float> out(n);
+  for (uint32_t k = 0; k < n; ++k) out[k] = mod[k] > 0.0 ? (float)(1.0 / mod[k]) : 0.0f;
+  return out;
+}
+
+static uint32_t pick_brick(uint32_t n, uint32_t p) {
+  const uint32_t prefs[] = {16, 8, 12, 10, 6};
+  const uint32_t want = n >= 128 ? 16 : 8;
+  if (n % want == 0 && want >= p) return want;
+  for (uint32_t b : prefs)
+    if (b >= p && n % b == 0) return b;
+  for (uint32_t b = p; b <= n; ++b)
+    if (n % b == 0) return b;
+  return n;
+}
+
+static uint32_t lines_per_block(uint32_t n) {
+  uint32_t L = 40960u / (16u * (n + 1));
+  return std::max(1u, std::min(64u, L));
+}
+
+// ---------------------------------------------------------- construction --
+ForceField::ForceField(Context& ctx, Particles& part, const hmd_forcefield_desc& d)
+    : c(ctx), p(part) {
+  hipStream_t s = c.stream;
+  n = p.n;
+  for (const auto& b : builtin_registry())
+    if (b.claims(d)) slots.push_back(b.label);
+  if (slots.empty()) throw Error(HMD_ERR_INVALID, "force field has no active slot");
+  lj_on = d.lj_on != 0;
+  coul_on = d.coulomb_on != 0;
+  spme_on = d.spme_on != 0;
+  if (spme_on && !coul_on)
+    throw Error(HMD_ERR_INVALID, "spme-reciprocal requires the spme-real slot");
+
+  // ---- pair slot parameters (src/forces/lj.rs, src/forces/spme.rs)
+  n_types = std::max(1u, d.n_types);
+  if (n_types > 16) throw Error(HMD_ERR_UNSUPPORTED, "at most 16 particle types");
+  if (p.max_type >= n_types) throw Error(HMD_ERR_INVALID, "particle type index >= n_types");
+  double cut = 0.0;
+  std::vector<float> c6c12(2 * n_types * n_types, 0.0f);
+  double lj_cut = -1.0, lj_sw = -1.0;
+  if (lj_on) {
+    for (uint32_t k = 0; k < n_types * n_types; ++k) {
+      double sg = d.lj_sigma[k], ep = d.lj_epsilon[k];
+      double s6 = std::pow(sg, 6.0);
+      c6c12[2 * k] = (float)(4.0 * ep * s6);
+      c6c12[2 * k + 1] = (float)(4.0 * ep * s6 * s6);
+      if (lj_cut < 0.0) { lj_cut = d.lj_cutoff[k]; lj_sw = d.lj_switch[k]; }
+      if (d.lj_cutoff[k] != lj_cut || d.lj_switch[k] != lj_sw)
+        throw Error(HMD_ERR_UNSUPPORTED,
+                    "per-type-pair LJ cutoffs / switches differ (uniform only)");
+    }
+    cut = lj_cut;
+  }
+  if (coul_on) {
+    if (lj_on && d.r_cut_real != lj_cut)
+      throw Error(HMD_ERR_UNSUPPORTED, "LJ and real-space cutoffs must match");
+    cut = d.r_cut_real;
+    alpha = (float)d.alpha;
+    k_coulomb = (float)d.k_coulomb;
+  }
+  r_cut = (float)cut;
+  rc2 = (float)(cut * cut);
+  if (lj_on && lj_sw < lj_cut) {
+    lj_rs2 = (float)(lj_sw * lj_sw);
+    lj_inv_delta = (float)(1.0 / (lj_cut * lj_cut - lj_sw * lj_sw));
+  } else {
+    lj_rs2 = rc2;
+    lj_inv_delta = 0.0f;
+  }
+  lj_table.alloc(c6c12.size(), false);
+  lj_table.upload(c6c12.data(), c6c12.size(), s);
+
+  // ---- exclusions
+  std::vector<uint32_t> eoff(n + 1, 0u), epar;
+  if (d.excl_offsets) {
+    eoff.assign(d.excl_offsets, d.excl_offsets + n + 1);
+    for (uint32_t i = 0; i < n; ++i)
+      if (eoff[i + 1] < eoff[i]) throw Error(HMD_ERR_INVALID, "exclusion offsets not monotonic");
+    epar.assign(d.excl_partners, d.excl_partners + eoff[n]);
+    for (uint32_t m = 0; m < eoff[n]; ++m) {
+      if ((d.excl_lj_scale && d.excl_lj_scale[m] != 0.0f) ||
+          (d.excl_coul_scale && d.excl_coul_scale[m] != 0.0f))
+        throw Error(HMD_ERR_UNSUPPORTED, "exclusion scales other than 0");
+      if (epar[m] >= n) throw Error(HMD_ERR_INVALID, "exclusion partner out of range");
+    }
+  }
+  if (epar.empty()) epar.push_back(0u);
+  excl_off.alloc(eoff.size(), false);
+  excl_off.upload(eoff.data(), eoff.size(), s);
+  excl_par.alloc(epar.size(), false);
+  excl_par.upload(epar.data(), epar.size(), s);
+
+  init_neighbor_list(d);
+  if (spme_on) init_spme(d);
+  check(hipStreamSynchronize(s), "sync");
+  probe_capacity();
+}
+
+// compute_cell_layout analogue (src/forces/neighbor_list.rs:1563-1583):
+// the reference needs >= 3 cells of edge r_search per axis; the fine Morton
+// grid here only orders atoms, but the same box requirement keeps the
+// minimum-image pair set well defined.
+void ForceField::init_neighbor_list(const hmd_forcefield_desc& d) {
+  r_skin = (float)d.r_skin;
+  r_search = r_cut + r_skin;
+  r_search_sq = r_search * r_search;
+  const HmdLattice& L = p.lattice;
+  // perpendicular widths of the lower-triangular cell
+  double wa = L.lx, wb = L.ly, wc = L.lz;  // exact for orthorhombic
+  if (L.xy != 0.0f || L.xz != 0.0f || L.yz != 0.0f) {
+    double ax = L.lx, bx = L.xy, by = L.ly, cx = L.xz, cy = L.yz, cz = L.lz;
+    double vol = ax * by * cz;
+    // |b x c|, |c x a|, |a x b| for a=(ax,0,0), b=(bx,by,0), c=(cx,cy,cz)
+    double bxc = std::sqrt(std::pow(by * cz, 2) + std::pow(bx * cz, 2) +
+                           std::pow(bx * cy - by * cx, 2));
+    double cxa = std::sqrt(std::pow(cz * ax, 2) + std::pow(cy * ax, 2));
+    double axb = ax * by;
+    wa = vol / bxc; wb = vol / cxa; wc = vol / axb;
+  }
+  const double widths[3] = {wa, wb, wc};
+  for (int k = 0; k < 3; ++k)
+    if (std::floor(widths[k] / r_search) < 3.0) {
+      char msg[160];
+      std::snprintf(msg, sizeof msg,
+                    "box too small for cells: width %.4f < 3 * r_search = %.4f",
+                    widths[k], 3.0 * r_search);
+      throw Error(HMD_ERR_BOX_TOO_SMALL, msg);
+    }
+  orthorhombic = (L.xy == 0.0f && L.xz == 0.0f && L.yz == 0.0f);
+  ortho_fast = orthorhombic && 0.5 * L.lx > 1.001 * r_search &&
+               0.5 * L.ly > 1.001 * r_search && 0.5 * L.lz > 1.001 * r_search;
+
+  // Fine Morton grid: ~0.47 Bohr^-3 (water) -> ~1.5 atoms per cell.
+  const double vol = (double)L.lx * L.ly * L.lz;
+  const double cell = std::cbrt(1.6 * vol / n);
+  uint32_t gmax = 1;
+  for (int k = 0; k < 3; ++k) {
+    double w = k == 0 ? L.lx : (k == 1 ? L.ly : L.lz);
+    g[k] = (uint32_t)std::max(1.0, std::min(1024.0, std::floor(w / cell)));
+    gmax = std::max(gmax, g[k]);
+  }
+  uint32_t bits = 0;
+  while ((1u << bits) < gmax) ++bits;
+  n_keys = 1u << (3 * bits);
+  if (n_keys > (1u << 24)) throw Error(HMD_ERR_UNSUPPORTED, "fine grid too large");
+
+  n_clusters = ceil_div(n, HMD_CLUSTER);
+  n_pad = n_clusters * HMD_CLUSTER;
+  status.alloc(4);
+  atom_key.alloc(n);
+  cell_count.alloc(n_keys);
+  cell_start.alloc(n_keys + 1);
+  sorted_id.alloc(n_pad);
+  slot_of.alloc(n);
+  type_sorted.alloc(n_pad);
+  posq_sorted.alloc(4 * (size_t)n_pad);
+  ref_pos.alloc(4 * (size_t)n);
+  bbox.alloc(8 * (size_t)n_clusters);
+  entry_count.alloc(n_clusters);
+  entry_offset.alloc(n_clusters + 1);
+  n_work.alloc(1);
+  acc.alloc(5 * (size_t)n_pad);
+  cell_scan.init(n_keys);
+  entry_scan.init(n_clusters);
+  alloc_entries(8);
+}
+
+void ForceField::alloc_entries(uint32_t cap) {
+  cap_entries = cap;
+  const size_t slots_total = (size_t)n_clusters * cap * HMD_ENTRY;
+  entry_j.alloc(slots_total, false);
+  entry_excl.alloc(slots_total, false);
+  work.alloc((size_t)n_clusters * cap);
+}
+
+void ForceField::init_spme(const hmd_forcefield_desc& d) {
+  order = d.spline_order;
+  if (order != 4 && order != 5 && order != 6 && order != 8)
+    throw Error(HMD_ERR_UNSUPPORTED, "spline order must be 4, 5, 6 or 8");
+  for (int k = 0; k < 3; ++k) {
+    grid[k] = d.grid[k];
+    if (grid[k] < 2 * order)  // SpmeError::InvalidGrid (spme.rs:236-248)
+      throw Error(HMD_ERR_INVALID, "SPME grid must be at least 2 * spline_order per axis");
+    brick[k] = d.brick[k] ? d.brick[k] : pick_brick(grid[k], order);
+    if (grid[k] % brick[k] != 0 || brick[k] < order)
+      throw Error(HMD_ERR_INVALID, "SPME brick must divide the grid and be >= order");
+    nb[k] = grid[k] / brick[k];
+  }
+  if (grid[2] % 2 != 0) throw Error(HMD_ERR_UNSUPPORTED, "SPME grid[2] must be even");
+  if ((size_t)brick[0] * brick[1] * brick[2] * 8 > 65536 ||
+      (size_t)(brick[0] + order - 1) * (brick[1] + order - 1) * (brick[2] + order - 1) * 4 > 65536)
+    throw Error(HMD_ERR_INVALID, "SPME brick exceeds 64 KiB of LDS");
+  hipStream_t s = c.stream;
+  const uint32_t n0 = grid[0], n1 = grid[1], n2 = grid[2];
+  nch = n2 / 2 + 1;
+  M = (size_t)n0 * n1 * n2;
+  n_bins = nb[0] * nb[1] * nb[2];
+  atom_bin.alloc(n);
+  bin_count.alloc(n_bins);
+  bin_start.alloc(n_bins + 1);
+  bin_atoms.alloc(n);
+  bin_scan.init(n_bins);
+  rho.alloc(M);
+  V.alloc(M);
+  mesh_c.alloc(2 * (size_t)n0 * n1 * nch);
+  slow_f.alloc(3 * (size_t)n);
+  slow_e.alloc(n);
+  slow_w.alloc(n);
+  spme_out.alloc(2);
+  auto up = [&](DevBuf<float>& b, const std::vector<float>& v) {
+    b.alloc(v.size(), false);
+    b.upload(v.data(), v.size(), s);
+  };
+  up(bfac[0], b_factors(n0, order));
+  up(bfac[1], b_factors(n1, order));
+  up(bfac[2], b_factors(n2, order));
+  twiddles(tw_a, n0, s);
+  twiddles(tw_b, n1, s);
+  twiddles(tw_c, n2 / 2, s);
+  twiddles(tw_r2c, n2, s);
+  rad_a = radix_plan(n0);
+  rad_b = radix_plan(n1);
+  rad_c = radix_plan(n2 / 2);
+  // per-atom self energy k_C alpha/sqrt(pi) q^2 (spme.rs:808-818)
+  std::vector<float> posq(4 * (size_t)n);
+  p.posq.download(posq.data(), posq.size(), s);
+  std::vector<float> us(n);
+  u_self_total = 0.0;
+  const double pref = (double)d.k_coulomb * d.alpha / std::sqrt(M_PI);
+  for (uint32_t i = 0; i < n; ++i) {
+    double q = posq[4 * (size_t)i + 3];
+    double u = pref * q * q;
+    us[i] = (float)u;
+    u_self_total += u;
+  }
+  up(u_self, us);
+  const uint32_t La = lines_per_block(n0);
+  partial_blocks = ceil_div((size_t)n1 * nch, La);
+  partials.alloc(2 * (size_t)partial_blocks);
+}
+
+// Probe rebuild: size the per-cluster entry capacity from the measured high
+// water (the reference's probe sizing + x1.5 growth,
+// src/forces/neighbor_list.rs:1151-1221, 421-445).
+void ForceField::probe_capacity() {
+  request_rebuild();
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    rebuild_chain();
+    uint32_t st[4];
+    status.download(st, 4, c.stream);
+    uint32_t need = st[2];
+    if (!(st[0] & HMD_STATUS_ENTRY_OVERFLOW) && need <= cap_entries) {
+      if (cap_entries >= need + need / 2 + 1) break;
+    }
+    alloc_entries(std::max(4u, need + need / 2 + 2));
+    uint32_t clear[4] = {HMD_STATUS_DISPLACED, st[1], 0u, 0u};
+    status.upload(clear, 4, c.stream);
+  }
+  check(hipStreamSynchronize(c.stream), "sync");
+}
+
+void ForceField::request_rebuild() {
+  uint32_t st[4];
+  status.download(st, 4, c.stream);
+  st[0] |= HMD_STATUS_DISPLACED;
+  status.upload(st, 4, c.stream);
+  check(hipStreamSynchronize(c.stream), "sync");
+}
+
+// ----------------------------------------------------------------- step ---
+void ForceField::rebuild_chain() {
+  const uint32_t* gate = status.ptr;
+  const HmdLattice L = p.lattice;
+  NblKeyParams kp{p.posq.ptr, n, L, g[0], g[1], g[2], atom_key.ptr, cell_count.ptr,
+                  cell_start.ptr, sorted_id.ptr, gate};
+  launch(c, K_hmd_nbl_keys, ceil_div(n, 256), 256, 0, kp);
+  cell_scan.run(c, cell_count.ptr, cell_start.ptr, gate, HMD_STATUS_DISPLACED);
+  launch(c, K_hmd_nbl_scatter, ceil_div(n, 256), 256, 0, kp);
+  NblSortParams sp{cell_start.ptr, n_keys, sorted_id.ptr, gate};
+  launch(c, K_hmd_nbl_sort_cells, ceil_div(n_keys, 256), 256, 0, sp);
+  NblGatherParams gp{p.posq.ptr, p.type.ptr, n, n_pad, sorted_id.ptr, slot_of.ptr,
+                     posq_sorted.ptr, type_sorted.ptr, ref_pos.ptr, gate};
+  launch(c, K_hmd_nbl_gather, ceil_div(n_pad, 256), 256, 0, gp);
+  NblBboxParams bp{posq_sorted.ptr, n, n_clusters, bbox.ptr, gate};
+  launch(c, K_hmd_nbl_bbox, ceil_div(n_clusters, 4), 256, 0, bp);
+  NblBuildParams bd{};
+  bd.posq_sorted = posq_sorted.ptr;
+  bd.sorted_id = sorted_id.ptr;
+  bd.bbox = bbox.ptr;
+  bd.L = L;
+  bd.n = n;
+  bd.n_clusters = n_clusters;
+  bd.r_search_sq = r_search_sq;
+  bd.r_search = r_search;
+  bd.orthorhombic_fast = ortho_fast ? 1u : 0u;
+  bd.cap_entries = cap_entries;
+  bd.entry_j = entry_j.ptr;
+  bd.entry_excl = entry_excl.ptr;
+  bd.entry_count = entry_count.ptr;
+  bd.excl_offsets = excl_off.ptr;
+  bd.excl_partners = excl_par.ptr;
+  bd.status = status.ptr;
+  launch(c, K_hmd_nbl_build, ceil_div(n_clusters, 4), 256, 0, bd);
+  entry_scan.run(c, entry_count.ptr, entry_offset.ptr, gate, HMD_STATUS_DISPLACED);
+  NblWorkParams wp{entry_count.ptr, entry_offset.ptr, n_clusters, cap_entries,
+                   work.ptr, n_work.ptr, status.ptr};
+  launch(c, K_hmd_nbl_worklist, 1, 1024, 0, wp);
+}
+
+void ForceField::neighbor_pre_step() {
+  NblDispParams dp{p.posq.ptr, ref_pos.ptr, n, p.lattice,
+                   0.25f * r_skin * r_skin, status.ptr};
+  launch(c, K_hmd_nbl_displacement, ceil_div(n, 256), 256, 0, dp);
+  c.mark("nbl_displacement");
+  rebuild_chain();
+  c.mark("nbl_rebuild");
+}
+
+void ForceField::pair_forces(bool fev) {
+  PairParams pp{};
+  pp.posq_sorted = posq_sorted.ptr;
+  pp.type_sorted = type_sorted.ptr;
+  pp.work = work.ptr;
+  pp.n_work = n_work.ptr;
+  pp.entry_j = entry_j.ptr;
+  pp.entry_excl = entry_excl.ptr;
+  pp.bbox = bbox.ptr;
+  pp.L = p.lattice;
+  pp.n = n;
+  pp.cap_entries = cap_entries;
+  pp.n_types = n_types;
+  pp.lj_c6c12 = lj_table.ptr;
+  pp.lj_on = lj_on;
+  pp.coul_on = coul_on;
+  pp.rc2 = rc2;
+  pp.lj_rs2 = lj_rs2;
+  pp.lj_inv_delta = lj_inv_delta;
+  pp.alpha = alpha;
+  pp.k_coulomb = k_coulomb;
+  pp.shift_margin = r_cut + 0.5f * r_skin;
+  pp.acc = acc.ptr;
+  pp.acc_stride = n_pad;
+  const uint32_t blocks = std::max(1u, (uint32_t)c.num_cus * pair_blocks_per_cu);
+  launch(c, fev ? K_hmd_pair_force_fev : K_hmd_pair_force_f, blocks, 256, 0, pp);
+  c.mark("pair_force");
+}
+
+void ForceField::spme_forces(bool fev) {
+  const HmdLattice L = p.lattice;
+  const uint32_t n0 = grid[0], n1 = grid[1], n2 = grid[2];
+  SpmeBinParams bp{p.posq.ptr, n, L, n0, n1, n2, brick[0], brick[1], brick[2],
+                   nb[0], nb[1], nb[2], atom_bin.ptr, bin_count.ptr, bin_start.ptr,
+                   bin_atoms.ptr};
+  launch(c, K_hmd_spme_bin, ceil_div(n, 256), 256, 0, bp);
+  bin_scan.run(c, bin_count.ptr, bin_start.ptr, nullptr, 0);
+  launch(c, K_hmd_spme_bin_scatter, ceil_div(n, 256), 256, 0, bp);
+  c.mark("spme_bin");
+
+  SpmeSpreadParams sp{p.posq.ptr, n, L, n0, n1, n2, brick[0], brick[1], brick[2],
+                      nb[0], nb[1], nb[2], bin_start.ptr, bin_atoms.ptr, rho.ptr};
+  const size_t sh_spread = (size_t)brick[0] * brick[1] * brick[2] * 8;
+  KernelId ks = order == 4 ? K_hmd_spme_spread_p4
+                : order == 5 ? K_hmd_spme_spread_p5
+                : order == 6 ? K_hmd_spme_spread_p6 : K_hmd_spme_spread_p8;
+  launch(c, ks, n_bins, 256, sh_spread, sp);
+  c.mark("spme_spread");
+
+  FftParams f{};
+  f.L = L;
+  f.alpha = alpha;
+  f.k_coulomb = k_coulomb;
+  f.n0 = n0; f.n1 = n1; f.n2 = n2; f.nch = nch;
+  f.bfac0 = bfac[0].ptr; f.bfac1 = bfac[1].ptr; f.bfac2 = bfac[2].ptr;
+  auto set_radix = [&](const std::vector<uint32_t>& r) {
+    f.n_stages = (uint32_t)r.size();
+    for (size_t k = 0; k < r.size(); ++k) f.radix[k] = r[k];
+  };
+  // 1. c axis, R2C
+  {
+    FftParams q = f;
+    q.data = rho.ptr;
+    q.out = mesh_c.ptr;
+    q.n = n2 / 2;
+    q.n_lines = n0 * n1;
+    q.lines_per_block = lines_per_block(q.n);
+    q.twiddle = tw_c.ptr;
+    q.twiddle_r2c = tw_r2c.ptr;
+    q.sign = -1;
+    FftParams& r = q;
+    r.n_stages = (uint32_t)rad_c.size();
+    for (size_t k = 0; k < rad_c.size(); ++k) r.radix[k] = rad_c[k];
+    launch(c, K_hmd_fft_r2c_c, ceil_div(q.n_lines, q.lines_per_block), 256,
+           2 * (size_t)q.lines_per_block * (q.n + 1) * 8, q);
+  }
+  // 2-4. b forward, a forward * G * a inverse, b inverse
+  FftParams fb = f;
+  fb.data = mesh_c.ptr;
+  fb.n = n1;
+  fb.n_lines = n0 * nch;
+  fb.lines_per_block = lines_per_block(n1);
+  fb.elem_stride = nch;
+  fb.line_minor = nch;
+  fb.major_stride = (uint64_t)n1 * nch;
+  fb.minor_stride = 1;
+  fb.twiddle = tw_b.ptr;
+  fb.sign = -1;
+  fb.n_stages = (uint32_t)rad_b.size();
+  for (size_t k = 0; k < rad_b.size(); ++k) fb.radix[k] = rad_b[k];
+  const size_t sh_b = 2 * (size_t)fb.lines_per_block * (n1 + 1) * 8;
+  launch(c, K_hmd_fft_axis, ceil_div(fb.n_lines, fb.lines_per_block), 256, sh_b, fb);
+  FftParams fa = f;
+  fa.data = mesh_c.ptr;
+  fa.n = n0;
+  fa.n_lines = n1 * nch;
+  fa.lines_per_block = lines_per_block(n0);
+  fa.elem_stride = n1 * nch;
+  fa.line_minor = nch;
+  fa.major_stride = nch;
+  fa.minor_stride = 1;
+  fa.twiddle = tw_a.ptr;
+  fa.sign = -1;
+  fa.apply_influence = 1;
+  fa.partials = partials.ptr;
+  set_radix(rad_a);
+  fa.n_stages = f.n_stages;
+  for (int k = 0; k < 8; ++k) fa.radix[k] = f.radix[k];
+  launch(c, K_hmd_fft_axis, ceil_div(fa.n_lines, fa.lines_per_block), 256,
+         2 * (size_t)fa.lines_per_block * (n0 + 1) * 8, fa);
+  fb.sign = +1;
+  launch(c, K_hmd_fft_axis, ceil_div(fb.n_lines, fb.lines_per_block), 256, sh_b, fb);
+  // 5. c axis, C2R
+  {
+    FftParams q = f;
+    q.data = mesh_c.ptr;
+    q.out = V.ptr;
+    q.n = n2 / 2;
+    q.n_lines = n0 * n1;
+    q.lines_per_block = lines_per_block(q.n);
+    q.twiddle = tw_c.ptr;
+    q.twiddle_r2c = tw_r2c.ptr;
+    q.sign = +1;
+    q.n_stages = (uint32_t)rad_c.size();
+    for (size_t k = 0; k < rad_c.size(); ++k) q.radix[k] = rad_c[k];
+    launch(c, K_hmd_fft_c2r_c, ceil_div(q.n_lines, q.lines_per_block), 256,
+           2 * (size_t)q.lines_per_block * (q.n + 1) * 8, q);
+  }
+  c.mark("spme_fft_solve");
+  SpmeReduceParams rp{partials.ptr, partial_blocks, spme_out.ptr};
+  launch(c, K_hmd_spme_reduce, 1, 256, 0, rp);
+
+  SpmeGatherParams gp{p.posq.ptr, n, L, n0, n1, n2, brick[0], brick[1], brick[2],
+                      nb[0], nb[1], nb[2], bin_start.ptr, bin_atoms.ptr, V.ptr,
+                      u_self.ptr, spme_out.ptr, slow_f.ptr, slow_e.ptr, slow_w.ptr,
+                      fev ? 1u : 0u};
+  const size_t sh_gather = (size_t)(brick[0] + order - 1) * (brick[1] + order - 1) *
+                           (brick[2] + order - 1) * 4;
+  KernelId kg = order == 4 ? K_hmd_spme_gather_p4
+                : order == 5 ? K_hmd_spme_gather_p5
+                : order == 6 ? K_hmd_spme_gather_p6 : K_hmd_spme_gather_p8;
+  launch(c, kg, n_bins, 256, sh_gather, gp);
+  c.mark("spme_gather");
+}
+
+void ForceField::compute(int level) {
+  const bool fev = level != 0;
+  neighbor_pre_step();
+  PairFinalizeParams fp{};
+  pair_forces(fev);
+  if (spme_on) spme_forces(fev);
+  fp.acc = acc.ptr;
+  fp.acc_stride = n_pad;
+  fp.slot_of = slot_of.ptr;
+  fp.slow_f = spme_on ? slow_f.ptr : nullptr;
+  fp.slow_e = spme_on ? slow_e.ptr : nullptr;
+  fp.slow_w = spme_on ? slow_w.ptr : nullptr;
+  fp.f = p.force.ptr;
+  fp.e = p.energy.ptr;
+  fp.w = p.virial.ptr;
+  fp.n = n;
+  fp.fev = fev ? 1u : 0u;
+  fp.acc_clear = acc.ptr;
+  launch(c, K_hmd_pair_finalize, ceil_div(n, 256), 256, 0, fp);
+  c.mark("finalize");
+}
+
+// Host-side status check at a batch boundary: the analogue of the 4-byte
+// neighbour status read in pre_step (neighbor_list.rs:1308-1320).
+void ForceField::check_status() {
+  uint32_t st[4];
+  status.download(st, 4, c.stream);
+  if (st[0] & HMD_STATUS_ENTRY_OVERFLOW) {
+    uint32_t need = st[2];
+    alloc_entries(std::max(4u, need + need / 2 + 2));
+    uint32_t clear[4] = {HMD_STATUS_DISPLACED, st[1], 0u, 0u};
+    status.upload(clear, 4, c.stream);
+    check(hipStreamSynchronize(c.stream), "sync");
+    graph_invalid = true;
+    throw Error(HMD_ERR_NBL_OVERFLOW,
+                "packed neighbour list overflowed; capacity grown to " +
+                    std::to_string(cap_entries) + " entries per cluster");
+  }
+}
+
+}  // namespace hmd
+
+using namespace hmd;
+
+extern "C" {
+
+hmd_status hmd_forcefield_create(hmd_ctx* h, hmd_particles* hp,
+                                 const hmd_forcefield_desc* d, hmd_forcefield** out) {
+  HMD_GUARD({
+    if (!d) throw Error(HMD_ERR_INVALID, "null descriptor");
+    auto* f = new hmd_forcefield();
+    try {
+      f->ff = new ForceField(h->c, hp->p, *d);
+    } catch (...) {
+      delete f;
+      throw;
+    }
+    *out = f;
+  });
+}
+
+void hmd_forcefield_destroy(hmd_forcefield* f) {
+  if (!f) return;
+  delete f->ff;
+  delete f;
+}
+
+hmd_status hmd_forcefield_compute(hmd_forcefield* f, int32_t level) {
+  HMD_GUARD({
+    f->ff->compute(level);
+    check(hipStreamSynchronize(f->ff->c.stream), "sync");
+    f->ff->check_status();
+  });
+}
+
+hmd_status hmd_forcefield_slots(hmd_forcefield* f, char* buf, int32_t len) {
+  HMD_GUARD({
+    std::string s;
+    for (auto& l : f->ff->slots) s += l + "\n";
+    if (buf && len > 0) std::snprintf(buf, (size_t)len, "%s", s.c_str());
+  });
+}
+
+hmd_status hmd_spme_scalars(hmd_forcefield* f, double out[3]) {
+  HMD_GUARD({
+    ForceField& F = *f->ff;
+    if (!F.spme_on) throw Error(HMD_ERR_INVALID, "no spme-reciprocal slot");
+    double v[2];
+    F.spme_out.download(v, 2, F.c.stream);
+    out[0] = v[0];
+    out[1] = v[1];
+    out[2] = F.u_self_total;
+  });
+}
+
+hmd_status hmd_spme_grids(hmd_forcefield* f, float* rho, float* pot) {
+  HMD_GUARD({
+    ForceField& F = *f->ff;
+    if (!F.spme_on) throw Error(HMD_ERR_INVALID, "no spme-reciprocal slot");
+    if (rho) F.rho.download(rho, F.M, F.c.stream);
+    if (pot) F.V.download(pot, F.M, F.c.stream);
+  });
+}
+
+hmd_status hmd_spme_forces(hmd_forcefield* f, float* force) {
+  HMD_GUARD({
+    ForceField& F = *f->ff;
+    if (!F.spme_on) throw Error(HMD_ERR_INVALID, "no spme-reciprocal slot");
+    std::vector<float> soa(3 * (size_t)F.n);
+    F.slow_f.download(soa.data(), soa.size(), F.c.stream);
+    for (uint32_t i = 0; i < F.n; ++i)
+      for (int d = 0; d < 3; ++d) force[3 * (size_t)i + d] = soa[(size_t)d * F.n + i];
+  });
+}
+
+hmd_status hmd_nbl_request_rebuild(hmd_forcefield* f) {
+  HMD_GUARD(f->ff->request_rebuild());
+}
+
+hmd_status hmd_nbl_pair_snapshot(hmd_forcefield* f, uint32_t* pairs, uint64_t cap,
+                                 uint64_t* n_pairs) {
+  HMD_GUARD({
+    ForceField& F = *f->ff;
+    Context& c = F.c;
+    DevBuf<uint32_t> dpairs;
+    dpairs.alloc(std::max<uint64_t>(2, 2 * cap));
+    DevBuf<unsigned long long> cnt;
+    cnt.alloc(1);
+    uint32_t nw = 0;
+    F.n_work.download(&nw, 1, c.stream);
+    NblPairsParams pp{F.posq_sorted.ptr, F.sorted_id.ptr, F.work.ptr, F.n_work.ptr,
+                      F.entry_j.ptr, F.p.lattice, F.n, F.cap_entries, F.r_search_sq,
+                      dpairs.ptr, cnt.ptr, cap, nullptr};
+    launch(c, K_hmd_nbl_pairs, ceil_div(nw, 4), 256, 0, pp);
+    unsigned long long total = 0;
+    cnt.download(&total, 1, c.stream);
+    *n_pairs = total;
+    uint64_t k = std::min<uint64_t>(total, cap);
+    if (k) dpairs.download(pairs, 2 * k, c.stream);
+  });
+}
+
+hmd_status hmd_nbl_stats(hmd_forcefield* f, uint32_t out[8]) {
+  HMD_GUARD({
+    ForceField& F = *f->ff;
+    uint32_t st[4], nw = 0;
+    F.status.download(st, 4, F.c.stream);
+    F.n_work.download(&nw, 1, F.c.stream);
+    out[0] = nw;
+    out[1] = st[1];
+    out[2] = st[0];
+    out[3] = st[2];
+    out[4] = F.cap_entries;
+    out[5] = F.n_clusters;
+    out[6] = F.g[0];
+    out[7] = 0;
+  });
+}
+
+}  // extern "C"

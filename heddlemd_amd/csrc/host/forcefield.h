@@ -1,0 +1,69 @@
+// ForceField host object: potential slots, neighbour list, SPME mesh.
+#pragma once
+#include "hmd_internal.h"
+
+namespace hmd {
+
+struct ForceField {
+  Context& c;
+  Particles& p;
+  std::vector<std::string> slots;   // labels in evaluation order
+  uint32_t n = 0;
+
+  // ---- fused pair slot
+  bool lj_on = false, coul_on = false, spme_on = false;
+  uint32_t n_types = 1;
+  float r_cut = 0, rc2 = 0, lj_rs2 = 0, lj_inv_delta = 0;
+  float alpha = 0, k_coulomb = 1;
+  DevBuf<float> lj_table;
+  DevBuf<uint32_t> excl_off, excl_par;
+  uint32_t pair_blocks_per_cu = 8;
+
+  // ---- neighbour list
+  float r_skin = 0, r_search = 0, r_search_sq = 0;
+  bool orthorhombic = true, ortho_fast = false;
+  uint32_t g[3] = {1, 1, 1};
+  uint32_t n_keys = 1, n_clusters = 0, n_pad = 0, cap_entries = 0;
+  DevBuf<uint32_t> status;          // [0] flags [1] generation [2] high water
+  DevBuf<uint32_t> atom_key, cell_count, cell_start, sorted_id, slot_of, type_sorted;
+  DevBuf<float> posq_sorted, ref_pos, bbox;
+  DevBuf<uint32_t> entry_j, entry_count, entry_offset, work, n_work;
+  DevBuf<uint64_t> entry_excl;
+  DevBuf<long long> acc;            // [5][n_pad] fixed-point accumulators
+  Scanner cell_scan, entry_scan;
+  bool graph_invalid = false;
+
+  // ---- SPME reciprocal slot
+  uint32_t order = 4, grid[3] = {0, 0, 0}, brick[3] = {0, 0, 0}, nb[3] = {0, 0, 0};
+  uint32_t nch = 0, n_bins = 0, partial_blocks = 0;
+  size_t M = 0;
+  DevBuf<uint32_t> atom_bin, bin_count, bin_start, bin_atoms;
+  Scanner bin_scan;
+  DevBuf<float> rho, V, mesh_c, bfac[3], tw_a, tw_b, tw_c, tw_r2c, u_self;
+  DevBuf<float> slow_f, slow_e, slow_w;
+  DevBuf<double> partials, spme_out;
+  std::vector<uint32_t> rad_a, rad_b, rad_c;
+  double u_self_total = 0.0;
+
+  ForceField(Context& ctx, Particles& part, const hmd_forcefield_desc& d);
+
+  void init_neighbor_list(const hmd_forcefield_desc& d);
+  void init_spme(const hmd_forcefield_desc& d);
+  void alloc_entries(uint32_t cap);
+  void probe_capacity();
+  void request_rebuild();
+  void rebuild_chain();
+  void neighbor_pre_step();
+  void pair_forces(bool fev);
+  void spme_forces(bool fev);
+  // One force evaluation, enqueued only (no host synchronisation): safe to
+  // capture into a graph.
+  void compute(int level);
+  void check_status();
+};
+
+}  // namespace hmd
+
+struct hmd_forcefield {
+  hmd::ForceField* ff = nullptr;
+};

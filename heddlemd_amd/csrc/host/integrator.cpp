@@ -1,0 +1,451 @@
+// Integrators (velocity Verlet f32 / compensated, Langevin BAOAB), SETTLE
+// constraint hooks, hipGraph capture and replay, per-stage profiling, and
+// setup helpers (steepest descent, velocity generation).
+//
+// Mirrors src/integrator/mod.rs (StepPlan, run_step), velocity_verlet.rs
+// (:67-110), langevin_baoab.rs (:86-111), settle.rs (:310-404) and the
+// runner's graph capture / batched replay (src/runner.rs:2321-2470).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <map>
+
+#include "forcefield.h"
+
+namespace hmd {
+
+enum class Kind { VelocityVerlet, LangevinBaoab };
+
+struct Integrator {
+  Context& c;
+  Particles& p;
+  ForceField& ff;
+  Kind kind = Kind::VelocityVerlet;
+  float dt = 0;
+  bool compensated = false;
+  // SETTLE
+  uint32_t n_groups = 0;
+  float ra = 0, rb = 0, rc = 0, m_o = 0, m_h = 0;
+  DevBuf<uint32_t> group_atoms;
+  DevBuf<float> snapshot, constraint_virial;
+  // Langevin
+  float lan_a = 1.0f;
+  uint64_t seed = 0;
+  DevBuf<float> sqrt_kt_over_m;
+  DevBuf<unsigned long long> step_counter;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t steps = 0;
+  bool primed = false;
+
+  Integrator(Context& cc, Particles& pp, ForceField& f, const hmd_integrator_desc& d)
+      : c(cc), p(pp), ff(f) {
+    std::string k = d.kind ? d.kind : "velocity-verlet";
+    if (k == "velocity-verlet") kind = Kind::VelocityVerlet;
+    else if (k == "langevin-baoab") kind = Kind::LangevinBaoab;
+    else throw Error(HMD_ERR_INVALID, "unknown integrator kind '" + k + "'");
+    dt = (float)d.dt;
+    if (!(dt > 0.0f)) throw Error(HMD_ERR_INVALID, "dt must be positive");
+    compensated = d.compensated != 0;
+    if (compensated && kind != Kind::VelocityVerlet)
+      throw Error(HMD_ERR_UNSUPPORTED, "compensated mode is velocity-Verlet only");
+    hipStream_t s = c.stream;
+    n_groups = d.n_groups;
+    if (n_groups) {
+      // canonical geometry (src/integrator/settle.rs:83-93)
+      double total = d.m_o + 2.0 * d.m_h;
+      double h = std::sqrt(d.d_oh * d.d_oh - 0.25 * d.d_hh * d.d_hh);
+      ra = (float)(2.0 * d.m_h / total * h);
+      rb = (float)(d.m_o / total * h);
+      rc = (float)(0.5 * d.d_hh);
+      m_o = (float)d.m_o;
+      m_h = (float)d.m_h;
+      for (uint32_t k2 = 0; k2 < 3 * n_groups; ++k2)
+        if (d.group_atoms[k2] >= p.n) throw Error(HMD_ERR_INVALID, "SETTLE atom out of range");
+      group_atoms.alloc(3 * (size_t)n_groups, false);
+      group_atoms.upload(d.group_atoms, 3 * (size_t)n_groups, s);
+      snapshot.alloc(12 * (size_t)n_groups);
+      constraint_virial.alloc(3 * (size_t)n_groups);
+    }
+    if (kind == Kind::LangevinBaoab) {
+      lan_a = (float)std::exp(-d.gamma * d.dt);
+      seed = d.seed;
+      std::vector<float> sk(p.n);
+      for (uint32_t i = 0; i < p.n; ++i)
+        sk[i] = (float)std::sqrt(d.temperature / (double)p.host_mass[i]);
+      sqrt_kt_over_m.alloc(p.n, false);
+      sqrt_kt_over_m.upload(sk.data(), sk.size(), s);
+      step_counter.alloc(1);
+    }
+    check(hipMemsetAsync(p.pos_lo.ptr, 0, p.pos_lo.count * sizeof(double), s), "memset");
+    check(hipMemsetAsync(p.vel_lo.ptr, 0, p.vel_lo.count * sizeof(double), s), "memset");
+    check(hipStreamSynchronize(s), "sync");
+  }
+
+  ~Integrator() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+
+  VvParams vv() const {
+    VvParams v{};
+    v.posq = p.posq.ptr;
+    v.v = p.vel.ptr;
+    v.f = p.force.ptr;
+    v.inv_m = p.inv_mass.ptr;
+    v.images = p.images.ptr;
+    v.pos_lo = p.pos_lo.ptr;
+    v.vel_lo = p.vel_lo.ptr;
+    v.L = p.lattice;
+    v.dt = dt;
+    v.n = p.n;
+    v.compensated = compensated ? 1u : 0u;
+    return v;
+  }
+
+  SettleParams settle(float step_dt) const {
+    SettleParams s{};
+    s.posq = p.posq.ptr;
+    s.v = p.vel.ptr;
+    s.pos_lo = compensated ? p.pos_lo.ptr : nullptr;
+    s.vel_lo = compensated ? p.vel_lo.ptr : nullptr;
+    s.snapshot = snapshot.ptr;
+    s.group_atoms = group_atoms.ptr;
+    s.n_groups = n_groups;
+    s.n = p.n;
+    s.ra = ra; s.rb = rb; s.rc = rc; s.m_o = m_o; s.m_h = m_h;
+    s.L = p.lattice;
+    s.dt = step_dt;
+    s.constraint_virial = constraint_virial.ptr;
+    return s;
+  }
+
+  LangevinParams lan(float step_dt) const {
+    LangevinParams l{};
+    l.posq = p.posq.ptr;
+    l.v = p.vel.ptr;
+    l.inv_m = p.inv_mass.ptr;
+    l.sqrt_kt_over_m = sqrt_kt_over_m.ptr;
+    l.images = p.images.ptr;
+    l.L = p.lattice;
+    l.dt = step_dt;
+    l.a = lan_a;
+    l.seed_lo = (uint32_t)(seed & 0xFFFFFFFFull);
+    l.seed_hi = (uint32_t)(seed >> 32);
+    l.step = step_counter.ptr;
+    l.n = p.n;
+    return l;
+  }
+
+  void settle_snapshot() {
+    if (!n_groups) return;
+    launch(c, K_hmd_settle_snapshot, ceil_div(n_groups, 256), 256, 0, settle(dt));
+  }
+  void settle_positions(float step_dt) {
+    if (!n_groups) return;
+    launch(c, K_hmd_settle_positions, ceil_div(n_groups, 256), 256, 0, settle(step_dt));
+  }
+  void settle_velocities() {
+    if (!n_groups) return;
+    launch(c, K_hmd_settle_velocities, ceil_div(n_groups, 256), 256, 0, settle(dt));
+  }
+
+  // One step, enqueued only (capturable).
+  void enqueue_step() {
+    const uint32_t nb = ceil_div(p.n, 256);
+    if (kind == Kind::VelocityVerlet) {
+      settle_snapshot();                                        // BeforeDrift
+      launch(c, K_hmd_vv_kick_drift, nb, 256, 0, vv());         // KickDrift
+      settle_positions(dt);                                     // AfterDrift
+      c.mark("kick_drift_settle");
+      ff.compute(0);                                            // ForceEval
+      launch(c, K_hmd_vv_kick, nb, 256, 0, vv());               // KickHalf
+      settle_velocities();                                      // AfterKick
+      c.mark("kick_settle");
+    } else {
+      // B A O A (force) B, with SETTLE after each position and velocity
+      // update (the reference's BAOAB plan has no constraint points).
+      VvParams k = vv();
+      launch(c, K_hmd_vv_kick, nb, 256, 0, k);
+      settle_velocities();
+      settle_snapshot();
+      launch(c, K_hmd_lan_drift_half, nb, 256, 0, lan(dt));
+      settle_positions(0.5f * dt);
+      launch(c, K_hmd_lan_ou, nb, 256, 0, lan(dt));
+      settle_velocities();
+      settle_snapshot();
+      launch(c, K_hmd_lan_drift_half, nb, 256, 0, lan(dt));
+      settle_positions(0.5f * dt);
+      c.mark("baoa");
+      ff.compute(0);
+      launch(c, K_hmd_vv_kick, nb, 256, 0, k);
+      settle_velocities();
+      CounterParams cp{step_counter.ptr};
+      launch(c, K_hmd_counter_increment, 1, 64, 0, cp);
+      c.mark("kick_settle");
+    }
+  }
+
+  void prime() {
+    if (primed) return;
+    ff.compute(0);
+    check(hipStreamSynchronize(c.stream), "sync");
+    ff.check_status();
+    primed = true;
+  }
+
+  void step(uint32_t n) {
+    prime();
+    for (uint32_t i = 0; i < n; ++i) enqueue_step();
+    check(hipStreamSynchronize(c.stream), "sync");
+    steps += n;
+    ff.check_status();
+  }
+
+  void capture() {
+    prime();
+    if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+    if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+    check(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+      enqueue_step();
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(c.stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    check(hipStreamEndCapture(c.stream, &graph), "hipStreamEndCapture");
+    check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    ff.graph_invalid = false;
+  }
+
+  void replay(uint32_t n) {
+    if (!exec || ff.graph_invalid) capture();
+    for (uint32_t i = 0; i < n; ++i) check(hipGraphLaunch(exec, c.stream), "hipGraphLaunch");
+    check(hipStreamSynchronize(c.stream), "sync");
+    steps += n;
+    ff.check_status();
+  }
+};
+
+}  // namespace hmd
+
+struct hmd_integrator {
+  hmd::Integrator* it = nullptr;
+};
+
+using namespace hmd;
+
+// ------------------------------------------------------ velocity set-up ---
+// ChaCha block function (Bernstein 2008), `rounds` rounds on a raw state.
+static inline uint32_t rotl(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+static void chacha_block(const uint32_t in[16], uint32_t rounds, uint32_t out[16]) {
+  uint32_t x[16];
+  std::memcpy(x, in, sizeof x);
+#define QR(a, b, c, d)                   \
+  a += b; d ^= a; d = rotl(d, 16);       \
+  c += d; b ^= c; b = rotl(b, 12);       \
+  a += b; d ^= a; d = rotl(d, 8);        \
+  c += d; b ^= c; b = rotl(b, 7);
+  for (uint32_t r = 0; r < rounds; r += 2) {
+    QR(x[0], x[4], x[8], x[12]) QR(x[1], x[5], x[9], x[13])
+    QR(x[2], x[6], x[10], x[14]) QR(x[3], x[7], x[11], x[15])
+    QR(x[0], x[5], x[10], x[15]) QR(x[1], x[6], x[11], x[12])
+    QR(x[2], x[7], x[8], x[13]) QR(x[3], x[4], x[9], x[14])
+  }
+#undef QR
+  for (int i = 0; i < 16; ++i) out[i] = x[i] + in[i];
+}
+
+// ChaCha8 stream seeded like rand_core 0.6 `seed_from_u64` (a PCG32 pass
+// fills the 32-byte key) and read 64 bits at a time, low word first,
+// through a 4-block buffer (rand_chacha 0.3), with rand 0.8's f64 mapping
+// (u64 >> 11) * 2^-53.
+struct ChaCha8 {
+  uint32_t state[16];
+  uint32_t buf[64];
+  uint32_t idx = 64;
+  explicit ChaCha8(uint64_t seed) {
+    uint32_t key[8];
+    uint64_t st = seed;
+    for (int i = 0; i < 8; ++i) {
+      st = st * 6364136223846793005ull + 11634580027462260723ull;
+      uint32_t xorshifted = (uint32_t)(((st >> 18) ^ st) >> 27);
+      uint32_t rot = (uint32_t)(st >> 59);
+      key[i] = (xorshifted >> rot) | (xorshifted << ((32 - rot) & 31));
+    }
+    state[0] = 0x61707865u; state[1] = 0x3320646eu;
+    state[2] = 0x79622d32u; state[3] = 0x6b206574u;
+    for (int i = 0; i < 8; ++i) state[4 + i] = key[i];
+    state[12] = state[13] = state[14] = state[15] = 0u;
+  }
+  void refill() {
+    for (int b = 0; b < 4; ++b) {
+      chacha_block(state, 8, buf + 16 * b);
+      uint64_t ctr = ((uint64_t)state[13] << 32 | state[12]) + 1;
+      state[12] = (uint32_t)ctr;
+      state[13] = (uint32_t)(ctr >> 32);
+    }
+    idx = 0;
+  }
+  uint64_t next_u64() {
+    if (idx + 2 > 64) refill();
+    uint64_t lo = buf[idx], hi = buf[idx + 1];
+    idx += 2;
+    return (hi << 32) | lo;
+  }
+  double next_f64() { return (double)(next_u64() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+extern "C" {
+
+void hmd_chacha_block(const uint32_t state[16], uint32_t rounds, uint32_t out[16]) {
+  chacha_block(state, rounds, out);
+}
+
+// src/runner.rs:3108-3200
+hmd_status hmd_generate_velocities(uint32_t n, uint32_t n_constraints, double temperature,
+                                   uint64_t seed, const double* masses, float* vel) {
+  HMD_GUARD({
+    std::vector<float> v(3 * (size_t)n, 0.0f);
+    if (temperature != 0.0 && n != 0) {
+      ChaCha8 rng(seed);
+      for (uint32_t i = 0; i < n; ++i) {
+        double sigma = std::sqrt(temperature / masses[i]);
+        for (int a = 0; a < 3; ++a) {
+          double u1 = 1.0 - rng.next_f64();
+          double u2 = rng.next_f64();
+          double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+          v[3 * (size_t)i + a] = (float)(z * sigma);
+        }
+      }
+      double total = 0.0;
+      for (uint32_t i = 0; i < n; ++i) total += masses[i];
+      if (total > 0.0)
+        for (int a = 0; a < 3; ++a) {
+          double pm = 0.0;
+          for (uint32_t i = 0; i < n; ++i) pm += masses[i] * (double)v[3 * (size_t)i + a];
+          double vc = pm / total;
+          for (uint32_t i = 0; i < n; ++i)
+            v[3 * (size_t)i + a] = (float)((double)v[3 * (size_t)i + a] - vc);
+        }
+      int64_t dof = 3 * (int64_t)n - (int64_t)n_constraints - 3;
+      if (dof <= 0) {
+        std::fill(v.begin(), v.end(), 0.0f);
+      } else {
+        double ke = 0.0;
+        for (uint32_t i = 0; i < n; ++i) {
+          double s = 0.0;
+          for (int a = 0; a < 3; ++a) s += std::pow((double)v[3 * (size_t)i + a], 2);
+          ke += 0.5 * masses[i] * s;
+        }
+        if (ke > 0.0) {
+          double scale = std::sqrt(0.5 * (double)dof * temperature / ke);
+          for (auto& x : v) x = (float)((double)x * scale);
+        }
+      }
+    }
+    std::copy(v.begin(), v.end(), vel);
+  });
+}
+
+hmd_status hmd_integrator_create(hmd_ctx* h, hmd_particles* hp, hmd_forcefield* f,
+                                 const hmd_integrator_desc* d, hmd_integrator** out) {
+  HMD_GUARD({
+    if (!d) throw Error(HMD_ERR_INVALID, "null descriptor");
+    auto* w = new hmd_integrator();
+    try {
+      w->it = new Integrator(h->c, hp->p, *f->ff, *d);
+    } catch (...) {
+      delete w;
+      throw;
+    }
+    *out = w;
+  });
+}
+
+void hmd_integrator_destroy(hmd_integrator* w) {
+  if (!w) return;
+  delete w->it;
+  delete w;
+}
+
+hmd_status hmd_integrator_step(hmd_integrator* w, uint32_t n) { HMD_GUARD(w->it->step(n)); }
+hmd_status hmd_integrator_capture(hmd_integrator* w) { HMD_GUARD(w->it->capture()); }
+hmd_status hmd_integrator_replay(hmd_integrator* w, uint32_t n) { HMD_GUARD(w->it->replay(n)); }
+uint64_t hmd_integrator_steps(hmd_integrator* w) { return w->it->steps; }
+
+hmd_status hmd_integrator_profile(hmd_integrator* w, uint32_t n, double* us, int32_t max_stages,
+                                  int32_t* n_stages, char* names, int32_t names_len) {
+  HMD_GUARD({
+    Integrator& I = *w->it;
+    Context& c = I.c;
+    I.prime();
+    std::vector<std::string> order;
+    std::map<std::string, double> tot;
+    for (uint32_t s = 0; s < n; ++s) {
+      for (auto& m : c.marks) (void)hipEventDestroy(m.second);
+      c.marks.clear();
+      c.profiling = true;
+      try {
+        c.mark("begin");
+        I.enqueue_step();
+      } catch (...) {
+        c.profiling = false;
+        throw;
+      }
+      c.profiling = false;
+      check(hipStreamSynchronize(c.stream), "sync");
+      for (size_t k = 1; k < c.marks.size(); ++k) {
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, c.marks[k - 1].second, c.marks[k].second), "elapsed");
+        const std::string& nm = c.marks[k].first;
+        if (!tot.count(nm)) order.push_back(nm);
+        tot[nm] += 1000.0 * ms;
+      }
+    }
+    I.steps += n;
+    I.ff.check_status();
+    std::string all;
+    int32_t k = 0;
+    for (auto& nm : order) {
+      if (k < max_stages) us[k] = tot[nm] / n;
+      ++k;
+      all += nm + "\n";
+    }
+    *n_stages = std::min(k, max_stages);
+    if (names && names_len > 0) std::snprintf(names, (size_t)names_len, "%s", all.c_str());
+  });
+}
+
+hmd_status hmd_integrator_minimize(hmd_integrator* w, uint32_t n_iter, double max_step) {
+  HMD_GUARD({
+    Integrator& I = *w->it;
+    Context& c = I.c;
+    Particles& p = I.p;
+    std::vector<float> f(3 * (size_t)p.n);
+    for (uint32_t it = 0; it < n_iter; ++it) {
+      I.ff.compute(0);
+      p.force.download(f.data(), f.size(), c.stream);
+      I.ff.check_status();
+      double fmax = 0.0;
+      for (uint32_t i = 0; i < p.n; ++i) {
+        double s = 0.0;
+        for (int d = 0; d < 3; ++d) s += (double)f[(size_t)d * p.n + i] * f[(size_t)d * p.n + i];
+        fmax = std::max(fmax, std::sqrt(s));
+      }
+      if (!(fmax > 0.0) || !std::isfinite(fmax)) break;
+      SdParams sp{p.posq.ptr, p.force.ptr, p.n, p.lattice, (float)(max_step / fmax),
+                  (float)max_step};
+      I.settle_snapshot();
+      launch(c, K_hmd_sd_step, ceil_div(p.n, 256), 256, 0, sp);
+      I.settle_positions(0.0f);
+    }
+    check(hipStreamSynchronize(c.stream), "sync");
+    I.primed = false;
+  });
+}
+
+}  // extern "C"

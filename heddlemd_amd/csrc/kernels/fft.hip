@@ -1,0 +1,298 @@
+// LDS-resident 3D real<->complex FFT and fused influence-function solve for
+// SPME on gfx950.  Replaces the reference's closed-library R2C / C2R calls
+// and its separate influence kernels (src/forces/spme.rs:423-500,
+// kernels/spme_recip.cu:34-114, :296-402).
+//
+// The transform is five line passes over a complex mesh of shape
+// (n0, n1, n2/2 + 1) (last axis fastest, as the reference's R2C layout):
+//   1. c axis, real -> complex (an n2/2 complex FFT plus the standard
+//      even/odd split),
+//   2. b axis forward,
+//   3. a axis forward, multiply by the influence function G(k) while
+//      accumulating the Hermitian-weighted virial and energy, a axis
+//      inverse -- one kernel, the line never leaves LDS,
+//   4. b axis inverse,
+//   5. c axis, complex -> real (unnormalised, like the reference's C2R).
+// Each workgroup loads several lines into LDS with coalesced accesses, runs
+// a mixed-radix (2, 3, 4, 5) Stockham FFT there, and writes them back.
+#include "hmd_device.h"
+
+#define HMD_FFT_THREADS 256u
+
+__device__ inline float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+// multiply by sign * i
+__device__ inline float2 cmul_si(float2 a, int sign) {
+  return sign < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+// One radix-R butterfly: y_k = sum_r x_r e^{sign 2 pi i r k / R}.
+__device__ inline void hmd_dft(float2* v, uint32_t R, int sign) {
+  if (R == 2) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if (R == 4) {
+    float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    float2 t2 = cadd(v[1], v[3]), t3 = cmul_si(csub(v[1], v[3]), sign);
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  } else if (R == 3) {
+    const float s3 = 0.86602540378443865f;
+    float2 t = cadd(v[1], v[2]);
+    float2 u = cmul_si(csub(v[1], v[2]), sign);
+    float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+    v[0] = cadd(v[0], t);
+    v[1] = make_float2(m.x + s3 * u.x, m.y + s3 * u.y);
+    v[2] = make_float2(m.x - s3 * u.x, m.y - s3 * u.y);
+  } else {  // R == 5
+    const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
+    const float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;
+    float2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+    float2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+    float2 x0 = v[0];
+    float2 m1 = make_float2(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
+    float2 m2 = make_float2(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
+    float2 n1 = cmul_si(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y), sign);
+    float2 n2 = cmul_si(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y), sign);
+    v[0] = cadd(x0, cadd(a1, a2));
+    v[1] = cadd(m1, n1);
+    v[4] = csub(m1, n1);
+    v[2] = cadd(m2, n2);
+    v[3] = csub(m2, n2);
+  }
+}
+
+// Mixed-radix Stockham autosort FFT of `lines` lines of length n stored
+// line-major in LDS with line stride ls (n + 1: the pad spreads the strided
+// loads of the b and a passes over the LDS banks); buf1 is scratch.  Returns the buffer holding the
+// result.  Twiddles come from tw[m] = e^{-2 pi i m / n} (conjugated for the
+// inverse).
+__device__ float2* hmd_fft_lds(float2* buf0, float2* buf1, uint32_t n,
+                               uint32_t ls, uint32_t lines,
+                               const uint32_t* radix, uint32_t n_stages,
+                               const float2* tw, int sign) {
+  float2* src = buf0;
+  float2* dst = buf1;
+  uint32_t Ns = 1;
+  for (uint32_t st = 0; st < n_stages; ++st) {
+    const uint32_t R = radix[st];
+    const uint32_t nb = n / R;
+    const uint32_t step = n / (Ns * R);
+    for (uint32_t b = threadIdx.x; b < lines * nb; b += blockDim.x) {
+      const uint32_t l = b / nb, j = b - l * nb;
+      const uint32_t jm = j % Ns;
+      float2 v[5];
+      for (uint32_t r = 0; r < R; ++r) {
+        float2 x = src[l * ls + j + r * nb];
+        if (r > 0) {
+          float2 w = tw[(jm * r * step) % n];
+          if (sign > 0) w = cconj(w);
+          x = cmul(x, w);
+        }
+        v[r] = x;
+      }
+      hmd_dft(v, R, sign);
+      const uint32_t idx = (j / Ns) * Ns * R + jm;
+      for (uint32_t r = 0; r < R; ++r) dst[l * ls + idx + r * Ns] = v[r];
+    }
+    __syncthreads();
+    float2* t = src;
+    src = dst;
+    dst = t;
+    Ns *= R;
+  }
+  return src;
+}
+
+__device__ inline size_t hmd_line_base(const FftParams& p, uint32_t l) {
+  return (size_t)(l / p.line_minor) * p.major_stride +
+         (size_t)(l % p.line_minor) * p.minor_stride;
+}
+
+// Pass 1: real -> complex along c.  data = real mesh (n0*n1 lines of n2
+// reals), out = complex mesh.  n = n2/2.
+extern "C" __global__ void __launch_bounds__(256)
+hmd_fft_r2c_c(const FftParams p) {
+  extern __shared__ float2 s_buf[];
+  const uint32_t n = p.n, L = p.lines_per_block, ls = n + 1;
+  float2* b0 = s_buf;
+  float2* b1 = s_buf + L * ls;
+  const uint32_t l0 = blockIdx.x * L;
+  const uint32_t lines = min(L, p.n_lines - l0);
+  const float2* in = reinterpret_cast<const float2*>(p.data);
+  for (uint32_t k = threadIdx.x; k < lines * n; k += blockDim.x) {
+    const uint32_t l = k / n, m = k - l * n;
+    b0[l * ls + m] = in[(size_t)l0 * n + k];   // lines are contiguous
+  }
+  __syncthreads();
+  float2* z = hmd_fft_lds(b0, b1, n, ls, lines, p.radix, p.n_stages,
+                          reinterpret_cast<const float2*>(p.twiddle), -1);
+  const uint32_t nch = n + 1;
+  float2* out = reinterpret_cast<float2*>(p.out);
+  const float2* wr = reinterpret_cast<const float2*>(p.twiddle_r2c);
+  for (uint32_t k = threadIdx.x; k < lines * nch; k += blockDim.x) {
+    const uint32_t l = k / nch, m = k - l * nch;
+    float2 zk = z[l * ls + (m % n)];
+    float2 zc = cconj(z[l * ls + ((n - m) % n)]);
+    float2 e = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
+    float2 o = cmul_si(make_float2(0.5f * (zk.x - zc.x), 0.5f * (zk.y - zc.y)), -1);
+    out[(size_t)(l0 + l) * nch + m] = cadd(e, cmul(wr[m], o));
+  }
+}
+
+// Pass 5: complex -> real along c (unnormalised).  data = complex mesh,
+// out = real mesh.  n = n2/2.
+extern "C" __global__ void __launch_bounds__(256)
+hmd_fft_c2r_c(const FftParams p) {
+  extern __shared__ float2 s_buf[];
+  const uint32_t n = p.n, L = p.lines_per_block, ls = n + 1;
+  float2* b0 = s_buf;
+  float2* b1 = s_buf + L * ls;
+  const uint32_t l0 = blockIdx.x * L;
+  const uint32_t lines = min(L, p.n_lines - l0);
+  const uint32_t nch = n + 1;
+  const float2* in = reinterpret_cast<const float2*>(p.data);
+  const float2* wr = reinterpret_cast<const float2*>(p.twiddle_r2c);
+  for (uint32_t k = threadIdx.x; k < lines * n; k += blockDim.x) {
+    const uint32_t l = k / n, m = k - l * n;
+    const float2* x = in + (size_t)(l0 + l) * nch;
+    float2 xk = x[m], xc = cconj(x[n - m]);
+    float2 ee = cadd(xk, xc);
+    float2 oo = cmul(csub(xk, xc), cconj(wr[m]));
+    b0[l * ls + m] = cadd(ee, cmul_si(oo, +1));
+  }
+  __syncthreads();
+  float2* z = hmd_fft_lds(b0, b1, n, ls, lines, p.radix, p.n_stages,
+                          reinterpret_cast<const float2*>(p.twiddle), +1);
+  float2* out = reinterpret_cast<float2*>(p.out);
+  for (uint32_t k = threadIdx.x; k < lines * n; k += blockDim.x) {
+    const uint32_t l = k / n, m = k - l * n;
+    out[(size_t)l0 * n + k] = z[l * ls + m];
+  }
+}
+
+// Influence function G(k) and virial factor at mesh index (ka, kb, kc)
+// (kernels/spme_recip.cu:34-114), f32 like the reference.
+__device__ inline void hmd_influence(const FftParams& p, uint32_t ka, uint32_t kb,
+                                     uint32_t kc, float& G, float& vf) {
+  const HmdLattice L = p.L;
+  const float ilx = 1.0f / L.lx, ily = 1.0f / L.ly, ilz = 1.0f / L.lz;
+  const float ra_x = ilx, ra_y = -L.xy * ilx * ily,
+              ra_z = (L.xy * L.yz - L.xz * L.ly) * ilx * ily * ilz;
+  const float rb_y = ily, rb_z = -L.yz * ily * ilz, rc_z = ilz;
+  const float ma = (ka <= p.n0 / 2) ? (float)ka : (float)ka - (float)p.n0;
+  const float mb = (kb <= p.n1 / 2) ? (float)kb : (float)kb - (float)p.n1;
+  const float mc = (kc <= p.n2 / 2) ? (float)kc : (float)kc - (float)p.n2;
+  const float two_pi = 6.283185307179586f;
+  const float kx = two_pi * (ma * ra_x);
+  const float ky = two_pi * (ma * ra_y + mb * rb_y);
+  const float kz = two_pi * (ma * ra_z + mb * rb_z + mc * rc_z);
+  const float k2 = kx * kx + ky * ky + kz * kz;
+  if (k2 == 0.0f) {
+    G = 0.0f;
+    vf = 0.0f;
+    return;
+  }
+  const float vol = L.lx * L.ly * L.lz;
+  const float a2 = p.alpha * p.alpha;
+  G = (p.k_coulomb / vol) * (12.566370614359172f / k2) * expf(-k2 / (4.0f * a2)) *
+      p.bfac0[ka] * p.bfac1[kb] * p.bfac2[kc];
+  vf = G * (1.0f - k2 / (2.0f * a2));
+}
+
+// Passes 2-4: complex FFT along a strided axis (b or a).  With
+// apply_influence the a-axis line is transformed forward, multiplied by
+// G(k), and transformed back in place; per-block virial / energy partial
+// sums go to partials[block] in a fixed order (deterministic).
+extern "C" __global__ void __launch_bounds__(256)
+hmd_fft_axis(const FftParams p) {
+  extern __shared__ float2 s_buf[];
+  __shared__ double s_red[2][HMD_FFT_THREADS];
+  const uint32_t n = p.n, L = p.lines_per_block, ls = n + 1;
+  float2* b0 = s_buf;
+  float2* b1 = s_buf + L * ls;
+  const uint32_t l0 = blockIdx.x * L;
+  const uint32_t lines = min(L, p.n_lines - l0);
+  float2* data = reinterpret_cast<float2*>(p.data);
+  // element k of line l sits at base(l) + k * elem_stride; thread mapping
+  // (k outer, l inner) keeps consecutive threads on consecutive lines, which
+  // are adjacent in memory (minor_stride = 1).
+  for (uint32_t t = threadIdx.x; t < lines * n; t += blockDim.x) {
+    const uint32_t k = t / lines, l = t - k * lines;
+    b0[l * ls + k] = data[hmd_line_base(p, l0 + l) + (size_t)k * p.elem_stride];
+  }
+  __syncthreads();
+  const float2* tw = reinterpret_cast<const float2*>(p.twiddle);
+  float2* res = hmd_fft_lds(b0, b1, n, ls, lines, p.radix, p.n_stages, tw, p.sign);
+  if (p.apply_influence) {
+    double vir = 0.0, ene = 0.0;
+    for (uint32_t t = threadIdx.x; t < lines * n; t += blockDim.x) {
+      const uint32_t l = t / n, ka = t - l * n;
+      const uint32_t gl = l0 + l;                 // line = (kb, kc)
+      const uint32_t kb = gl / p.nch, kc = gl - kb * p.nch;
+      float G, vf;
+      hmd_influence(p, ka, kb, kc, G, vf);
+      float2 x = res[l * ls + ka];
+      const float hw = (kc == 0u || (p.n2 % 2u == 0u && 2u * kc == p.n2)) ? 1.0f : 2.0f;
+      const float r2 = x.x * x.x + x.y * x.y;
+      vir += (double)(hw * vf * r2);
+      ene += (double)(hw * G * r2);
+      res[l * ls + ka] = make_float2(G * x.x, G * x.y);
+    }
+    s_red[0][threadIdx.x] = vir;
+    s_red[1][threadIdx.x] = ene;
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) {
+        s_red[0][threadIdx.x] += s_red[0][threadIdx.x + s];
+        s_red[1][threadIdx.x] += s_red[1][threadIdx.x + s];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      p.partials[2 * blockIdx.x] = s_red[0][0];
+      p.partials[2 * blockIdx.x + 1] = s_red[1][0];
+    }
+    float2* other = (res == b0) ? b1 : b0;
+    __syncthreads();
+    res = hmd_fft_lds(res, other, n, ls, lines, p.radix, p.n_stages, tw, +1);
+  }
+  for (uint32_t t = threadIdx.x; t < lines * n; t += blockDim.x) {
+    const uint32_t k = t / lines, l = t - k * lines;
+    data[hmd_line_base(p, l0 + l) + (size_t)k * p.elem_stride] = res[l * ls + k];
+  }
+}
+
+// Fixed-order reduction of the per-block partials: out[0] = W_recip =
+// 0.5 * sum hw vf |rho_hat|^2, out[1] = U_recip = 0.5 * sum hw G |rho_hat|^2
+// (kernels/spme_recip.cu:376-402 with the 0.5 half-sum factor).
+extern "C" __global__ void __launch_bounds__(256)
+hmd_spme_reduce(const SpmeReduceParams p) {
+  __shared__ double s[2][256];
+  double a = 0.0, b = 0.0;
+  for (uint32_t i = threadIdx.x; i < p.n_partials; i += blockDim.x) {
+    a += p.partials[2 * i];
+    b += p.partials[2 * i + 1];
+  }
+  s[0][threadIdx.x] = a;
+  s[1][threadIdx.x] = b;
+  __syncthreads();
+  for (uint32_t st = blockDim.x / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      s[0][threadIdx.x] += s[0][threadIdx.x + st];
+      s[1][threadIdx.x] += s[1][threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    p.out[0] = 0.5 * s[0][0];
+    p.out[1] = 0.5 * s[1][0];
+  }
+}

@@ -1,0 +1,116 @@
+// Device helpers shared by every gfx950 kernel: periodic boundaries,
+// fixed-point conversion, wave64 primitives.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "hmd_params.h"
+
+#define HMD_WAVE 64
+
+// ------------------------------------------------------------------ PBC --
+// Cartesian -> fractional by back-substitution and minimum image, operation
+// for operation as kernels/pbc.cuh:17-43 (lower-triangular cell).  Fused
+// multiply-add contraction is disabled so the f32 result is one exactly
+// specified function (the CPU oracle evaluates the same sequence), which is
+// what makes the neighbour pair set bit-exact.
+__device__ inline void hmd_min_image(float& dx, float& dy, float& dz,
+                                     const HmdLattice& L) {
+#pragma clang fp contract(off)
+  float sc = dz / L.lz;
+  float sb = (dy - sc * L.yz) / L.ly;
+  float sa = (dx - sb * L.xy - sc * L.xz) / L.lx;
+  float ka = floorf(sa + 0.5f);
+  float kb = floorf(sb + 0.5f);
+  float kc = floorf(sc + 0.5f);
+  dx -= ka * L.lx + kb * L.xy + kc * L.xz;
+  dy -= kb * L.ly + kc * L.yz;
+  dz -= kc * L.lz;
+}
+
+// Wrap a position into the primary image and return the image triple
+// (kernels/pbc.cuh:74-92).
+__device__ inline void hmd_wrap(float& x, float& y, float& z, int& ka_out,
+                                int& kb_out, int& kc_out, const HmdLattice& L) {
+#pragma clang fp contract(off)
+  float sc = z / L.lz;
+  float sb = (y - sc * L.yz) / L.ly;
+  float sa = (x - sb * L.xy - sc * L.xz) / L.lx;
+  float ka = floorf(sa + 0.5f);
+  float kb = floorf(sb + 0.5f);
+  float kc = floorf(sc + 0.5f);
+  x -= ka * L.lx + kb * L.xy + kc * L.xz;
+  y -= kb * L.ly + kc * L.yz;
+  z -= kc * L.lz;
+  ka_out = (int)ka;
+  kb_out = (int)kb;
+  kc_out = (int)kc;
+}
+
+__device__ inline void hmd_cart_to_frac(float x, float y, float z,
+                                        const HmdLattice& L, float& sa,
+                                        float& sb, float& sc) {
+#pragma clang fp contract(off)
+  sc = z / L.lz;
+  sb = (y - sc * L.yz) / L.ly;
+  sa = (x - sb * L.xy - sc * L.xz) / L.lx;
+}
+
+// Exact f32 squared min-image distance (the neighbour pair-set test).
+__device__ inline float hmd_r2_exact(float xi, float yi, float zi, float xj,
+                                     float yj, float zj, const HmdLattice& L) {
+#pragma clang fp contract(off)
+  float dx = xi - xj, dy = yi - yj, dz = zi - zj;
+  hmd_min_image(dx, dy, dz, L);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// Orthorhombic variant that multiplies by the reciprocal box length instead
+// of dividing.  Its image index k can differ from the exact one only when
+// s = d/L lies within a rounding of a half-integer, i.e. |d| ~ L/2; for every
+// other d the subtraction d - k*L is the same operation on the same operands
+// and r^2 is bit-identical.  The caller uses it only when L/2 exceeds the
+// search radius with margin, so the |d| ~ L/2 cases are out of range under
+// either formula and the accept/reject decision is exact.
+__device__ inline float hmd_r2_ortho(float xi, float yi, float zi, float xj,
+                                     float yj, float zj, const HmdLattice& L,
+                                     float ilx, float ily, float ilz) {
+#pragma clang fp contract(off)
+  float dx = xi - xj, dy = yi - yj, dz = zi - zj;
+  float kc = floorf(dz * ilz + 0.5f);
+  float kb = floorf(dy * ily + 0.5f);
+  float ka = floorf(dx * ilx + 0.5f);
+  dx -= ka * L.lx + kb * L.xy + kc * L.xz;
+  dy -= kb * L.ly + kc * L.yz;
+  dz -= kc * L.lz;
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// ---------------------------------------------------------- fixed point --
+__device__ inline long long hmd_to_fixed(float v, int shift) {
+  return __float2ll_rn(ldexpf(v, shift));
+}
+__device__ inline float hmd_from_fixed(long long v, int shift) {
+  return (float)ldexp((double)v, -shift);
+}
+__device__ inline void hmd_atomic_add_fixed(long long* p, long long v) {
+  if (v != 0)
+    atomicAdd(reinterpret_cast<unsigned long long*>(p),
+              static_cast<unsigned long long>(v));
+}
+
+// ---------------------------------------------------------- wave helpers --
+__device__ inline uint32_t hmd_lane() { return __lane_id(); }
+
+__device__ inline float hmd_shfl_xor(float v, int mask) {
+  return __shfl_xor(v, mask, HMD_WAVE);
+}
+
+__device__ inline uint32_t hmd_readlane(uint32_t v, uint32_t lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+}
+
+// Device-side predicate for kernels captured into the per-step graph that
+// must run only when a rebuild is pending (status word bit set earlier in the
+// same step by another kernel on the same stream).
+__device__ inline bool hmd_gate(const uint32_t* status, uint32_t mask) {
+  return status == nullptr || (*(volatile const uint32_t*)status & mask) != 0u;
+}

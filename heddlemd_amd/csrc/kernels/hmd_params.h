@@ -1,0 +1,311 @@
+// Kernel parameter blocks shared by the host library and the gfx950 code
+// object.  Every kernel takes exactly one of these structs by value, so the
+// kernarg segment is the struct itself and the host and device agree on the
+// argument layout by construction (this replaces the reference's positional
+// argument tuples and their KernelArgSchema guard,
+// src/forces/jit_composed.rs:457-639).
+//
+// Plain C layout only: pointers, 32/64-bit integers, float, double.
+#pragma once
+#include <stdint.h>
+
+#define HMD_CLUSTER 64u             // atoms per cluster = lanes per wave
+#define HMD_ENTRY 64u               // j-atom slots per neighbour entry
+#define HMD_SENTINEL 0xFFFFFFFFu    // empty j slot
+#define HMD_FIXED_SHIFT 40          // pair accumulators: value * 2^40 in i64
+#define HMD_SPREAD_SHIFT 32         // mesh accumulators: value * 2^32 in i64
+
+// neighbour status word bits (kernels set them, the host reads them)
+#define HMD_STATUS_DISPLACED 1u     // an atom moved > r_skin/2 since rebuild
+#define HMD_STATUS_ENTRY_OVERFLOW 2u
+#define HMD_STATUS_CELL_OVERFLOW 4u
+#define HMD_STATUS_REBUILT 8u       // a rebuild ran since the host cleared it
+
+struct HmdLattice {
+  float lx, ly, lz, xy, xz, yz;
+};
+
+// ---------------------------------------------------------------- scan ---
+struct ScanParams {
+  const uint32_t* in;       // may alias out
+  uint32_t* out;
+  uint32_t* block_sums;     // [num_blocks]
+  uint32_t len;
+  const uint32_t* gate;     // optional: skip when (*gate & gate_mask) == 0
+  uint32_t gate_mask;
+  uint32_t total_slot;      // write out[len] = total when != 0 (single block)
+};
+
+// ----------------------------------------------------- neighbour list ---
+struct NblKeyParams {
+  const float* posq;        // canonical float4[n]
+  uint32_t n;
+  HmdLattice L;
+  uint32_t g0, g1, g2;      // fine cells per axis
+  uint32_t* atom_key;       // [n]
+  uint32_t* cell_count;     // [n_keys]  (zero on entry, zero again after scatter)
+  const uint32_t* cell_start;  // [n_keys+1] (scatter only)
+  uint32_t* sorted_id;      // [n_pad]   (scatter only)
+  const uint32_t* status;   // rebuild gate
+};
+
+struct NblSortParams {
+  const uint32_t* cell_start;
+  uint32_t n_keys;
+  uint32_t* sorted_id;
+  const uint32_t* status;
+};
+
+struct NblGatherParams {
+  const float* posq;        // canonical float4[n]
+  const uint32_t* type;     // canonical [n]
+  uint32_t n, n_pad;
+  const uint32_t* sorted_id;   // [n_pad] (pad entries written here)
+  uint32_t* slot_of;        // [n]
+  float* posq_sorted;       // float4[n_pad]
+  uint32_t* type_sorted;    // [n_pad]
+  float* ref_pos;           // float4[n] canonical positions at rebuild
+  const uint32_t* status;   // rebuild extras gated on the DISPLACED bit
+};
+
+struct NblBboxParams {
+  const float* posq_sorted;
+  uint32_t n;
+  uint32_t n_clusters;
+  float* bbox;              // float4[2*n_clusters]: (cx,cy,cz,-), (hx,hy,hz,-)
+  const uint32_t* status;
+};
+
+struct NblBuildParams {
+  const float* posq_sorted;
+  const uint32_t* sorted_id;
+  const float* bbox;
+  HmdLattice L;
+  uint32_t n, n_clusters;
+  float r_search_sq;
+  float r_search;
+  uint32_t orthorhombic_fast;  // 1: exact fast min-image valid (see kernel)
+  uint32_t cap_entries;        // entries per cluster
+  uint32_t* entry_j;           // [n_clusters*cap_entries*64]
+  uint64_t* entry_excl;        // [n_clusters*cap_entries*64]
+  uint32_t* entry_count;       // [n_clusters] (entries written / needed)
+  const uint32_t* excl_offsets;  // canonical CSR [n+1]
+  const uint32_t* excl_partners;
+  uint32_t* status;
+};
+
+struct NblWorkParams {
+  const uint32_t* entry_count;   // [n_clusters]
+  const uint32_t* entry_offset;  // [n_clusters+1] exclusive scan
+  uint32_t n_clusters, cap_entries;
+  uint32_t* work;                // [total]
+  uint32_t* n_work;              // [1]
+  uint32_t* status;              // [0] flags, [1] generation, [2] high water
+};
+
+struct NblDispParams {
+  const float* posq;
+  const float* ref_pos;
+  uint32_t n;
+  HmdLattice L;
+  float threshold_sq;
+  uint32_t* status;
+};
+
+struct NblPairsParams {        // decode the list into exact pair set
+  const float* posq_sorted;
+  const uint32_t* sorted_id;
+  const uint32_t* work;
+  const uint32_t* n_work;
+  const uint32_t* entry_j;
+  HmdLattice L;
+  uint32_t n;
+  uint32_t cap_entries;
+  float r_search_sq;
+  uint32_t* pairs;             // [cap*2] canonical (min,max)
+  unsigned long long* count;   // [1]
+  unsigned long long cap;
+  uint32_t* visits;            // optional [n*?] unused
+};
+
+// ------------------------------------------------------------- pair ----
+struct PairParams {
+  const float* posq_sorted;    // float4[n_pad]
+  const uint32_t* type_sorted; // [n_pad]
+  const uint32_t* work;
+  const uint32_t* n_work;
+  const uint32_t* entry_j;
+  const uint64_t* entry_excl;
+  const float* bbox;           // cluster centres + half extents
+  HmdLattice L;
+  uint32_t n, cap_entries;
+  uint32_t n_types;
+  const float* lj_c6c12;       // float2[n_types^2]: (4 eps s^6, 4 eps s^12)
+  uint32_t lj_on, coul_on;
+  float rc2;                   // max cutoff squared (mask)
+  float lj_rs2;                // switch start squared
+  float lj_inv_delta;          // 1/(rc^2 - rs^2); 0 disables the switch
+  float alpha, k_coulomb;
+  float shift_margin;          // r_cut + r_skin/2: per-cluster shift eligibility
+  long long* acc;              // [5][n_pad] fixed-point F_x,F_y,F_z,E,W (sorted)
+  uint32_t acc_stride;         // n_pad
+};
+
+struct PairFinalizeParams {
+  const long long* acc;
+  uint32_t acc_stride;
+  const uint32_t* slot_of;     // canonical -> sorted
+  const float* slow_f;         // [3][n] canonical SPME forces (may be null)
+  const float* slow_e;         // [n]
+  const float* slow_w;         // [n]
+  float* f;                    // [3][n] canonical total forces
+  float* e;                    // [n]  (fev only)
+  float* w;                    // [n]
+  uint32_t n;
+  uint32_t fev;
+  long long* acc_clear;        // accumulators zeroed after reading
+};
+
+// ------------------------------------------------------------- spme ----
+struct SpmeBinParams {
+  const float* posq;           // canonical
+  uint32_t n;
+  HmdLattice L;
+  uint32_t n0, n1, n2;         // mesh
+  uint32_t b0, b1, b2;         // brick dims (mesh points)
+  uint32_t nb0, nb1, nb2;      // bricks per axis
+  uint32_t* atom_bin;          // [n]
+  uint32_t* bin_count;         // [n_bins] zero on entry, zero after scatter
+  const uint32_t* bin_start;   // [n_bins+1]
+  uint32_t* bin_atoms;         // [n]
+};
+
+struct SpmeSpreadParams {
+  const float* posq;
+  uint32_t n;
+  HmdLattice L;
+  uint32_t n0, n1, n2;
+  uint32_t b0, b1, b2;
+  uint32_t nb0, nb1, nb2;
+  const uint32_t* bin_start;
+  const uint32_t* bin_atoms;
+  float* rho;                  // real mesh [n0*n1*n2]
+};
+
+struct SpmeGatherParams {
+  const float* posq;
+  uint32_t n;
+  HmdLattice L;
+  uint32_t n0, n1, n2;
+  uint32_t b0, b1, b2;
+  uint32_t nb0, nb1, nb2;
+  const uint32_t* bin_start;
+  const uint32_t* bin_atoms;
+  const float* V;              // real mesh
+  const float* u_self;         // [n]
+  const double* w_recip;       // [1] scalar reciprocal virial (total)
+  float* f;                    // [3][n] canonical, overwritten
+  float* e;                    // [n]
+  float* w;                    // [n]
+  uint32_t fev;
+};
+
+// one axis of the 3D FFT, LDS resident
+struct FftParams {
+  float* data;                 // complex mesh (float2), or real mesh for c-axis
+  float* out;                  // c-axis passes: the other array
+  uint32_t n;                  // transform length along the axis
+  uint32_t n_lines;            // lines in the pass
+  uint32_t lines_per_block;
+  uint32_t elem_stride;        // complex elements between successive points
+  uint32_t line_major, line_minor;  // line l -> (l / line_minor, l % line_minor)
+  uint64_t major_stride, minor_stride;  // complex elements
+  const float* twiddle;        // float2[n] e^{-2 pi i m/n}
+  const float* twiddle_r2c;    // float2[nc] e^{-2 pi i m/nc} (c-axis only)
+  uint32_t radix[8];
+  uint32_t n_stages;
+  int32_t sign;                // -1 forward, +1 inverse
+  // influence (a-axis pass only)
+  uint32_t apply_influence;
+  uint32_t n0, n1, n2, nch;
+  HmdLattice L;
+  float alpha, k_coulomb;
+  const float* bfac0;
+  const float* bfac1;
+  const float* bfac2;
+  double* partials;            // [grid blocks][2]: virial, energy
+};
+
+struct SpmeReduceParams {
+  const double* partials;
+  uint32_t n_partials;
+  double* out;                 // [2]: W_recip, U_recip (half sums applied)
+};
+
+// ------------------------------------------------------- integrators ---
+struct VvParams {
+  float* posq;                 // canonical float4
+  float* v;                    // [3][n]
+  const float* f;              // [3][n]
+  const float* inv_m;          // [n]
+  int32_t* images;             // [3][n]
+  double* pos_lo;              // [3][n] compensated mode
+  double* vel_lo;              // [3][n]
+  HmdLattice L;
+  float dt;
+  uint32_t n;
+  uint32_t compensated;
+};
+
+struct SettleParams {
+  float* posq;
+  float* v;                    // [3][n]
+  double* pos_lo;              // compensated (may be null)
+  double* vel_lo;
+  float* snapshot;             // float4 per atom slot [3*n_groups]
+  const uint32_t* group_atoms; // [3*n_groups] O, H1, H2
+  uint32_t n_groups;
+  uint32_t n;
+  float ra, rb, rc, m_o, m_h;
+  HmdLattice L;
+  float dt;
+  float* constraint_virial;    // [3*n_groups]
+};
+
+struct LangevinParams {
+  float* posq;
+  float* v;
+  const float* inv_m;
+  const float* sqrt_kt_over_m; // [n]
+  int32_t* images;
+  HmdLattice L;
+  float dt;
+  float a;                     // exp(-gamma dt)
+  uint32_t seed_lo, seed_hi;
+  const unsigned long long* step;  // device step counter
+  uint32_t n;
+};
+
+struct CounterParams {
+  unsigned long long* counter;
+};
+
+struct ReduceParams {
+  const float* x;              // [comps][n] or null
+  const float* v;              // velocities [3][n] (KE mode)
+  const float* m;              // masses (KE mode)
+  uint32_t n;
+  uint32_t mode;               // 0 = sum x, 1 = kinetic energy
+  double* partials;            // [blocks]
+  double* out;                 // [1]
+  uint32_t n_blocks;
+};
+
+struct SdParams {              // capped steepest-descent move (setup only)
+  float* posq;
+  const float* f;              // [3][n]
+  uint32_t n;
+  HmdLattice L;
+  float h;                     // step per unit force
+  float max_step;              // cap on |dx| per atom
+};

@@ -1,0 +1,361 @@
+// Cell list + packed cluster-pair neighbour list for gfx950.
+//
+// Reference behaviour (taylor-a-barnes/heddlemd): kernels/neighbor.cu and
+// src/forces/neighbor_list.rs.  The pair set is the same -- every unordered
+// pair whose exact f32 minimum-image squared distance is <= (r_c + r_skin)^2
+// (kernels/neighbor.cu:536-553) -- but the construction is wave64-native:
+//
+//  * atoms are sorted by the Morton order of a fine cell grid (about 0.25 nm
+//    cells), ties broken by atom id, so 64 consecutive sorted atoms form a
+//    spatially compact cluster and one wave owns one cluster (one atom per
+//    lane);
+//  * candidate j-clusters come from a bounding-box prune of clusters with a
+//    higher index (half list, Newton's third law in the force kernel);
+//  * j-atoms are filtered per atom against the 64 i-atoms and compacted in
+//    increasing sorted order with a wave ballot + prefix popcount, so each
+//    cluster's list -- and every entry of 64 j-slots -- is deterministic;
+//  * each entry carries, per i-lane, a 64-bit mask of excluded j-slots built
+//    from the exclusion CSR, so the force kernel needs no per-pair search.
+//
+// Every rebuild kernel is captured into the per-step graph and gated on the
+// device status word, so a displacement-triggered rebuild happens in the step
+// that needs it without a host round trip.
+#include "hmd_device.h"
+
+// Spread 10 bits of x so that consecutive bits are 3 apart (Morton code).
+__device__ inline uint32_t hmd_spread3(uint32_t x) {
+  x &= 0x3ffu;
+  x = (x | (x << 16)) & 0x030000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+// Fine cell of a (wrapped) position, clamped like
+// parallelepiped_cell_indices (kernels/neighbor.cu:11-36).
+__device__ inline void hmd_fine_cell(float x, float y, float z,
+                                     const HmdLattice& L, uint32_t g0,
+                                     uint32_t g1, uint32_t g2, uint32_t& c0,
+                                     uint32_t& c1, uint32_t& c2) {
+  int ia, ib, ic;
+  hmd_wrap(x, y, z, ia, ib, ic, L);
+  float sa, sb, sc;
+  hmd_cart_to_frac(x, y, z, L, sa, sb, sc);
+  int a = (int)floorf((sa + 0.5f) * (float)g0);
+  int b = (int)floorf((sb + 0.5f) * (float)g1);
+  int c = (int)floorf((sc + 0.5f) * (float)g2);
+  a = a < 0 ? 0 : (a >= (int)g0 ? (int)g0 - 1 : a);
+  b = b < 0 ? 0 : (b >= (int)g1 ? (int)g1 - 1 : b);
+  c = c < 0 ? 0 : (c >= (int)g2 ? (int)g2 - 1 : c);
+  c0 = (uint32_t)a;
+  c1 = (uint32_t)b;
+  c2 = (uint32_t)c;
+}
+
+// Skin trigger (kernels/neighbor.cu:50-76): any atom whose min-image
+// displacement from its rebuild-time position exceeds r_skin/2 sets bit 0.
+extern "C" __global__ void __launch_bounds__(256)
+hmd_nbl_displacement(const NblDispParams p) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.n) return;
+  float4 a = reinterpret_cast<const float4*>(p.posq)[i];
+  float4 r = reinterpret_cast<const float4*>(p.ref_pos)[i];
+  float dx = a.x - r.x, dy = a.y - r.y, dz = a.z - r.z;
+  hmd_min_image(dx, dy, dz, p.L);
+  float d2 = dx * dx + dy * dy + dz * dz;
+  if (d2 > p.threshold_sq) atomicOr(p.status, HMD_STATUS_DISPLACED);
+}
+
+// Fine-cell Morton key + histogram.
+extern "C" __global__ void __launch_bounds__(256)
+hmd_nbl_keys(const NblKeyParams p) {
+  if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.n) return;
+  float4 a = reinterpret_cast<const float4*>(p.posq)[i];
+  uint32_t c0, c1, c2;
+  hmd_fine_cell(a.x, a.y, a.z, p.L, p.g0, p.g1, p.g2, c0, c1, c2);
+  uint32_t key = (hmd_spread3(c0) << 2) | (hmd_spread3(c1) << 1) | hmd_spread3(c2);
+  p.atom_key[i] = key;
+  atomicAdd(&p.cell_count[key], 1u);
+}
+
+// Scatter into cells.  The count array doubles as the cursor and is driven
+// back to zero, ready for the next rebuild.  Order inside a cell is fixed
+// afterwards by hmd_nbl_sort_cells.
+extern "C" __global__ void __launch_bounds__(256)
+hmd_nbl_scatter(const NblKeyParams p) {
+  if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.n) return;
+  uint32_t key = p.atom_key[i];
+  uint32_t k = atomicSub(&p.cell_count[key], 1u) - 1u;
+  p.sorted_id[p.cell_start[key] + k] = i;
+}
+
+// Canonical (cell, atom id) order: insertion sort inside each fine cell
+// (a cell holds a couple of atoms; compare kernels/neighbor.cu:218-238).
+extern "C" __global__ void __launch_bounds__(256)
+hmd_nbl_sort_cells(const NblSortParams p) {
+  if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= p.n_keys) return;
+  uint32_t s = p.cell_start[c], e = p.cell_start[c + 1];
+  for (uint32_t k = s + 1; k < e; ++k) {
+    uint32_t key = p.sorted_id[k];
+    uint32_t pos = k;
+    while (pos > s && p.sorted_id[pos - 1] > key) {
+      p.sorted_id[pos] = p.sorted_id[pos - 1];
+      --pos;
+    }
+    p.sorted_id[pos] = key;
+  }
+}
+
+// Sorted views.  Every step it refreshes the sorted positions
+// (kernels/neighbor.cu:251-285 does the same refresh every step in
+// ForceField::run); in a rebuilding step it also writes the inverse map, the
+// sorted types, the rebuild-time reference positions and the padding lanes.
+extern "C" __global__ void __launch_bounds__(256)
+hmd_nbl_gather(const NblGatherParams p) {
+  const bool rebuild = hmd_gate(p.status, HMD_STATUS_DISPLACED);
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= p.n_pad) return;
+  float4* ps = reinterpret_cast<float4*>(p.posq_sorted);
+  if (s >= p.n) {
+    if (rebuild) {
+      // Padding lanes: far away and uncharged; their slots never pass the
+      // validity test of the force kernel.
+      ps[s] = make_float4(1.0e18f, 1.0e18f, 1.0e18f, 0.0f);
+      p.type_sorted[s] = 0u;
+    }
+    return;
+  }
+  uint32_t id = p.sorted_id[s];
+  float4 a = reinterpret_cast<const float4*>(p.posq)[id];
+  ps[s] = a;
+  if (rebuild) {
+    p.slot_of[id] = s;
+    p.type_sorted[s] = p.type[id];
+    reinterpret_cast<float4*>(p.ref_pos)[id] = a;
+  }
+}
+
+// Per-cluster axis-aligned bounding box (centre and half extents) of the
+// valid atoms, one wave per cluster (compare compute_block_bbox,
+// kernels/neighbor.cu:291-361).
+extern "C" __global__ void __launch_bounds__(256)
+hmd_nbl_bbox(const NblBboxParams p) {
+  if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
+  uint32_t c = blockIdx.x * (blockDim.x / HMD_WAVE) + threadIdx.x / HMD_WAVE;
+  if (c >= p.n_clusters) return;
+  uint32_t lane = hmd_lane();
+  uint32_t s = c * HMD_CLUSTER + lane;
+  bool valid = s < p.n;
+  float4 a = reinterpret_cast<const float4*>(p.posq_sorted)[s];
+  float lo[3] = {valid ? a.x : 3.0e38f, valid ? a.y : 3.0e38f, valid ? a.z : 3.0e38f};
+  float hi[3] = {valid ? a.x : -3.0e38f, valid ? a.y : -3.0e38f, valid ? a.z : -3.0e38f};
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = fminf(lo[d], hmd_shfl_xor(lo[d], off));
+      hi[d] = fmaxf(hi[d], hmd_shfl_xor(hi[d], off));
+    }
+  }
+  if (lane == 0) {
+    float4* b = reinterpret_cast<float4*>(p.bbox);
+    b[2 * c] = make_float4(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]),
+                           0.5f * (lo[2] + hi[2]), 0.0f);
+    b[2 * c + 1] = make_float4(0.5f * (hi[0] - lo[0]), 0.5f * (hi[1] - lo[1]),
+                               0.5f * (hi[2] - lo[2]), 0.0f);
+  }
+}
+
+// Conservative min-image gap between two boxes (compare
+// packed_block_bbox_dist_sq, kernels/neighbor.cu:365-388).
+__device__ inline float hmd_box_gap2(float4 ca, float4 ha, float4 cb, float4 hb,
+                                     const HmdLattice& L) {
+  float dx = ca.x - cb.x, dy = ca.y - cb.y, dz = ca.z - cb.z;
+  hmd_min_image(dx, dy, dz, L);
+  float ex = fmaxf(fabsf(dx) - (ha.x + hb.x), 0.0f);
+  float ey = fmaxf(fabsf(dy) - (ha.y + hb.y), 0.0f);
+  float ez = fmaxf(fabsf(dz) - (ha.z + hb.z), 0.0f);
+  return ex * ex + ey * ey + ez * ez;
+}
+
+// Build the packed list of one i-cluster per wave.
+//
+// Candidate j-clusters are those with index >= c whose boxes lie within the
+// search radius.  For each candidate, lane k tests j-atom k against the 64
+// i-atoms (exact f32 distance) and keeps it when some i-atom with a lower
+// sorted index is within r_search.  Kept atoms are appended in increasing
+// sorted order to 64-slot entries; lane k of the wave owns slot k of the
+// current entry, and every lane records in its 64-bit mask which slots hold
+// one of its atom's exclusion partners.
+extern "C" __global__ void __launch_bounds__(256)
+hmd_nbl_build(const NblBuildParams p) {
+  if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
+  __shared__ float4 s_i[4][HMD_CLUSTER];
+  const uint32_t wave = threadIdx.x / HMD_WAVE;
+  const uint32_t c = blockIdx.x * 4u + wave;
+  const uint32_t lane = hmd_lane();
+  const bool active_wave = c < p.n_clusters;
+  const float4* ps = reinterpret_cast<const float4*>(p.posq_sorted);
+  const float4* bb = reinterpret_cast<const float4*>(p.bbox);
+
+  uint32_t si = c * HMD_CLUSTER + lane;
+  float4 pi = active_wave ? ps[si] : make_float4(0, 0, 0, 0);
+  s_i[wave][lane] = pi;
+  __syncthreads();
+  if (!active_wave) return;
+
+  const float ilx = 1.0f / p.L.lx, ily = 1.0f / p.L.ly, ilz = 1.0f / p.L.lz;
+  // This lane's exclusion partners (canonical ids), read on demand.
+  uint32_t my_id = si < p.n ? p.sorted_id[si] : HMD_SENTINEL;
+  uint32_t ex_b = 0, ex_e = 0;
+  if (my_id != HMD_SENTINEL && p.excl_offsets) {
+    ex_b = p.excl_offsets[my_id];
+    ex_e = p.excl_offsets[my_id + 1];
+  }
+
+  float4 ci = bb[2 * c], hi = bb[2 * c + 1];
+  const uint32_t base_entry = c * p.cap_entries;
+  uint32_t n_entries = 0;       // entries completed (uniform)
+  uint32_t fill = 0;            // slots used in the current entry (uniform)
+  uint32_t my_slot_j = HMD_SENTINEL;
+  uint64_t my_mask = 0;
+  bool overflow = false;
+
+  for (uint32_t jb = c; jb < p.n_clusters; jb += HMD_WAVE) {
+    uint32_t jc = jb + lane;
+    bool cand = false;
+    if (jc < p.n_clusters) {
+      float4 cj = bb[2 * jc], hj = bb[2 * jc + 1];
+      cand = hmd_box_gap2(ci, hi, cj, hj, p.L) <= p.r_search_sq;
+    }
+    uint64_t cmask = __ballot(cand);
+    while (cmask) {
+      uint32_t bit = (uint32_t)__builtin_ctzll(cmask);
+      cmask &= cmask - 1;
+      uint32_t jcl = jb + bit;
+      uint32_t sj = jcl * HMD_CLUSTER + lane;
+      bool hit = false;
+      if (sj < p.n) {
+        float4 pj = ps[sj];
+        // Cheap prune: distance from the j-atom to the i-cluster box.
+        float dx = pj.x - ci.x, dy = pj.y - ci.y, dz = pj.z - ci.z;
+        hmd_min_image(dx, dy, dz, p.L);
+        float ex = fmaxf(fabsf(dx) - hi.x, 0.0f);
+        float ey = fmaxf(fabsf(dy) - hi.y, 0.0f);
+        float ez = fmaxf(fabsf(dz) - hi.z, 0.0f);
+        if (ex * ex + ey * ey + ez * ez <= p.r_search_sq) {
+          uint32_t ilim = (jcl == c) ? lane : HMD_CLUSTER;  // s_i < s_j
+          uint32_t imax = p.n - c * HMD_CLUSTER;
+          if (ilim > imax) ilim = imax;
+          for (uint32_t k = 0; k < ilim && !hit; ++k) {
+            float4 a = s_i[wave][k];
+            float r2 = p.orthorhombic_fast
+                           ? hmd_r2_ortho(a.x, a.y, a.z, pj.x, pj.y, pj.z, p.L, ilx, ily, ilz)
+                           : hmd_r2_exact(a.x, a.y, a.z, pj.x, pj.y, pj.z, p.L);
+            hit = r2 <= p.r_search_sq;
+          }
+        }
+      }
+      uint64_t hmask = __ballot(hit);
+      while (hmask) {
+        uint32_t hb = (uint32_t)__builtin_ctzll(hmask);
+        hmask &= hmask - 1;
+        uint32_t sj_h = jcl * HMD_CLUSTER + hb;          // uniform
+        uint32_t jid = p.sorted_id[sj_h];                // uniform
+        if (lane == fill) my_slot_j = sj_h;
+        for (uint32_t m = ex_b; m < ex_e; ++m)
+          if (p.excl_partners[m] == jid) my_mask |= (1ull << fill);
+        ++fill;
+        if (fill == HMD_ENTRY) {
+          if (n_entries < p.cap_entries) {
+            size_t o = (size_t)(base_entry + n_entries) * HMD_ENTRY + lane;
+            p.entry_j[o] = my_slot_j;
+            p.entry_excl[o] = my_mask;
+          } else {
+            overflow = true;
+          }
+          ++n_entries;
+          fill = 0;
+          my_slot_j = HMD_SENTINEL;
+          my_mask = 0;
+        }
+      }
+    }
+  }
+  if (fill > 0) {
+    if (n_entries < p.cap_entries) {
+      size_t o = (size_t)(base_entry + n_entries) * HMD_ENTRY + lane;
+      p.entry_j[o] = my_slot_j;
+      p.entry_excl[o] = my_mask;
+    } else {
+      overflow = true;
+    }
+    ++n_entries;
+  }
+  if (lane == 0) {
+    p.entry_count[c] = min(n_entries, p.cap_entries);
+    atomicMax(&p.status[2], n_entries);   // high-water mark: the host resizes
+    if (overflow) atomicOr(p.status, HMD_STATUS_ENTRY_OVERFLOW);
+  }
+}
+
+// Flatten (cluster, entry) into the work list the force kernel walks, and
+// close the rebuild (single block of 1024 threads; entry_offset is the
+// exclusive scan of entry_count with the total at [n_clusters]).
+extern "C" __global__ void __launch_bounds__(1024)
+hmd_nbl_worklist(const NblWorkParams p) {
+  if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
+  for (uint32_t c = threadIdx.x; c < p.n_clusters; c += blockDim.x) {
+    uint32_t cnt = p.entry_count[c];
+    uint32_t off = p.entry_offset[c];
+    for (uint32_t k = 0; k < cnt; ++k) p.work[off + k] = c * p.cap_entries + k;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    p.n_work[0] = p.entry_offset[p.n_clusters];
+    uint32_t* st = p.status;
+    uint32_t s = st[0];
+    st[0] = (s & ~HMD_STATUS_DISPLACED) | HMD_STATUS_REBUILT;
+    st[1] += 1u;   // rebuild generation
+  }
+}
+
+// Decode the list into the exact pair set (canonical (min, max) ids of every
+// listed pair whose f32 min-image distance is <= r_search), one wave per
+// work item.  This is the PairSnapshot of src/forces/neighbor_list.rs:
+// 1358-1445 restricted to pairs inside the search radius.
+extern "C" __global__ void __launch_bounds__(256)
+hmd_nbl_pairs(const NblPairsParams p) {
+  uint32_t w = blockIdx.x * (blockDim.x / HMD_WAVE) + threadIdx.x / HMD_WAVE;
+  if (w >= *p.n_work) return;
+  uint32_t lane = hmd_lane();
+  uint32_t slot = p.work[w];
+  uint32_t c = slot / p.cap_entries;
+  uint32_t si = c * HMD_CLUSTER + lane;
+  bool vi = si < p.n;
+  const float4* ps = reinterpret_cast<const float4*>(p.posq_sorted);
+  float4 a = ps[si];
+  uint32_t ida = vi ? p.sorted_id[si] : 0u;
+  for (uint32_t k = 0; k < HMD_ENTRY; ++k) {
+    uint32_t sj = p.entry_j[(size_t)slot * HMD_ENTRY + k];
+    if (sj == HMD_SENTINEL) break;
+    float4 b = ps[sj];
+    bool ok = vi && si < sj &&
+              hmd_r2_exact(a.x, a.y, a.z, b.x, b.y, b.z, p.L) <= p.r_search_sq;
+    if (ok) {
+      uint32_t idb = p.sorted_id[sj];
+      unsigned long long o = atomicAdd(p.count, 1ull);
+      if (o < p.cap) {
+        p.pairs[2 * o] = min(ida, idb);
+        p.pairs[2 * o + 1] = max(ida, idb);
+      }
+    }
+  }
+}

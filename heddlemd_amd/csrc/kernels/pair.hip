@@ -1,0 +1,259 @@
+// Fused LJ 12-6 (+C1 switch) + real-space Ewald pair kernel for gfx950.
+//
+// Physics is the reference's composed pair kernel: LjPairFunctor
+// (src/forces/lj.rs:395-445), SpmeRealPairFunctor with the Abramowitz &
+// Stegun 7.1.26 erfc (src/forces/spme.rs:712-760), exclusion scale 0 for
+// listed partners, branch-free max-cutoff mask, Newton's third law, and the
+// half/half energy and virial split (src/forces/jit_composed.rs:1302-1643).
+//
+// Mapping (wave64, VALU FP32; no MFMA -- there is no dense contraction):
+//  * one wave = one 64-atom i-cluster, one i-atom per lane, held in VGPRs;
+//  * the entry's 64 j-atoms are walked in order; j data are wave-uniform
+//    (scalar loads), so each step evaluates 64 pairs with no shuffles;
+//  * the j-side reaction forces of 8 consecutive j-atoms are summed across
+//    the wave by a transposed butterfly (8 values -> 1 per lane group), then
+//    8 lanes issue the fixed-point atomics;
+//  * i-forces stay in registers across every entry of the same cluster in the
+//    wave's contiguous share of the work list and are flushed once;
+//  * accumulation is i64 fixed point (2^40), so the per-atom totals do not
+//    depend on the order in which waves finish: runs are bit-reproducible.
+#include "hmd_device.h"
+
+#define HMD_MAX_TYPES 16u
+#define HMD_TWO_OVER_SQRT_PI 1.1283791670955126f
+#define HMD_LOG2E 1.4426950408889634f
+
+// Sum v[0..7] over the wave; lane l returns the total of v[l >> 3].
+__device__ inline float hmd_transpose_reduce8(const float v[8], uint32_t lane) {
+  const bool b5 = lane & 32u, b4 = lane & 16u, b3 = lane & 8u;
+  float a[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    float send = b5 ? v[t] : v[t + 4];
+    float keep = b5 ? v[t + 4] : v[t];
+    a[t] = keep + hmd_shfl_xor(send, 32);
+  }
+  float b[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    float send = b4 ? a[t] : a[t + 2];
+    float keep = b4 ? a[t + 2] : a[t];
+    b[t] = keep + hmd_shfl_xor(send, 16);
+  }
+  float send = b3 ? b[0] : b[1];
+  float keep = b3 ? b[1] : b[0];
+  float c = keep + hmd_shfl_xor(send, 8);
+  c += hmd_shfl_xor(c, 4);
+  c += hmd_shfl_xor(c, 2);
+  c += hmd_shfl_xor(c, 1);
+  return c;
+}
+
+template <bool FEV>
+__device__ inline void hmd_pair_body(const PairParams& p) {
+  __shared__ float2 s_lj[HMD_MAX_TYPES * HMD_MAX_TYPES];
+  const uint32_t nt2 = p.n_types * p.n_types;
+  for (uint32_t k = threadIdx.x; k < nt2; k += blockDim.x)
+    s_lj[k] = reinterpret_cast<const float2*>(p.lj_c6c12)[k];
+  __syncthreads();
+
+  const uint32_t lane = hmd_lane();
+  const uint32_t waves_per_block = blockDim.x / HMD_WAVE;
+  const uint32_t n_waves = gridDim.x * waves_per_block;
+  const uint32_t gw = blockIdx.x * waves_per_block + threadIdx.x / HMD_WAVE;
+  const uint32_t nw = *p.n_work;
+  const uint32_t per = (nw + n_waves - 1) / n_waves;
+  const uint32_t w0 = gw * per;
+  const uint32_t w1 = min(w0 + per, nw);
+  if (w0 >= w1) return;
+
+  const float4* ps = reinterpret_cast<const float4*>(p.posq_sorted);
+  const float4* bb = reinterpret_cast<const float4*>(p.bbox);
+  const HmdLattice L = p.L;
+  const bool ortho = (L.xy == 0.0f) && (L.xz == 0.0f) && (L.yz == 0.0f);
+  const float ilx = 1.0f / L.lx, ily = 1.0f / L.ly, ilz = 1.0f / L.lz;
+  const float two_alpha_sqrtpi = p.alpha * HMD_TWO_OVER_SQRT_PI;
+  long long* acc = p.acc;
+  const uint32_t stride = p.acc_stride;
+
+  uint32_t cur_c = HMD_SENTINEL;
+  uint32_t si = 0, ti = 0;
+  bool vi = false, shift = false;
+  float xi = 0, yi = 0, zi = 0, qi = 0, cx = 0, cy = 0, cz = 0;
+  float fix = 0, fiy = 0, fiz = 0, eiv = 0, wiv = 0;
+
+  for (uint32_t w = w0; w < w1; ++w) {
+    const uint32_t slot = p.work[w];
+    const uint32_t c = slot / p.cap_entries;
+    if (c != cur_c) {
+      if (cur_c != HMD_SENTINEL && vi) {
+        hmd_atomic_add_fixed(&acc[si], hmd_to_fixed(fix, HMD_FIXED_SHIFT));
+        hmd_atomic_add_fixed(&acc[stride + si], hmd_to_fixed(fiy, HMD_FIXED_SHIFT));
+        hmd_atomic_add_fixed(&acc[2 * stride + si], hmd_to_fixed(fiz, HMD_FIXED_SHIFT));
+        if (FEV) {
+          hmd_atomic_add_fixed(&acc[3 * stride + si], hmd_to_fixed(eiv, HMD_FIXED_SHIFT));
+          hmd_atomic_add_fixed(&acc[4 * stride + si], hmd_to_fixed(wiv, HMD_FIXED_SHIFT));
+        }
+      }
+      cur_c = c;
+      si = c * HMD_CLUSTER + lane;
+      vi = si < p.n;
+      float4 a = ps[si];
+      xi = a.x; yi = a.y; zi = a.z; qi = a.w;
+      ti = p.type_sorted[si];
+      float4 cb = bb[2 * c], hb = bb[2 * c + 1];
+      // Single-image fast path (compare jit_composed.rs:1396-1430): valid
+      // when every atom that can interact with the cluster lies within half
+      // a box length of its centre, so shifting i and j to the image nearest
+      // the centre yields the minimum-image displacement of every in-cutoff
+      // pair.  The margin covers motion since the rebuild.
+      shift = ortho && (hb.x + p.shift_margin < 0.5f * L.lx) &&
+              (hb.y + p.shift_margin < 0.5f * L.ly) &&
+              (hb.z + p.shift_margin < 0.5f * L.lz);
+      cx = cb.x; cy = cb.y; cz = cb.z;
+      if (shift) {
+        xi -= L.lx * floorf((xi - cx) * ilx + 0.5f);
+        yi -= L.ly * floorf((yi - cy) * ily + 0.5f);
+        zi -= L.lz * floorf((zi - cz) * ilz + 0.5f);
+      }
+      fix = fiy = fiz = eiv = wiv = 0.0f;
+    }
+
+    const size_t ebase = (size_t)slot * HMD_ENTRY;
+    const uint32_t my_j = p.entry_j[ebase + lane];
+    const uint64_t my_ex = p.entry_excl[ebase + lane];
+
+    for (uint32_t kb = 0; kb < HMD_ENTRY; kb += 8) {
+      if (hmd_readlane(my_j, kb) == HMD_SENTINEL) break;
+      float jfx[8], jfy[8], jfz[8], jfe[8], jfw[8];
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u) {
+        const uint32_t j = hmd_readlane(my_j, kb + u);     // wave-uniform
+        const bool jvalid = j != HMD_SENTINEL;
+        const uint32_t jj = jvalid ? j : 0u;
+        const float4 b = ps[jj];
+        const uint32_t tj = p.type_sorted[jj];
+        float xj = b.x, yj = b.y, zj = b.z;
+        float dx, dy, dz;
+        if (shift) {
+          xj -= L.lx * floorf((xj - cx) * ilx + 0.5f);
+          yj -= L.ly * floorf((yj - cy) * ily + 0.5f);
+          zj -= L.lz * floorf((zj - cz) * ilz + 0.5f);
+          dx = xi - xj; dy = yi - yj; dz = zi - zj;
+        } else {
+          dx = xi - xj; dy = yi - yj; dz = zi - zj;
+          hmd_min_image(dx, dy, dz, L);
+        }
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const bool excluded = (my_ex >> (kb + u)) & 1ull;
+        const bool ok = jvalid && vi && (si < j) && (r2 <= p.rc2) && !excluded;
+        const float r2s = ok ? r2 : 1.0f;
+        const float inv_r = __builtin_amdgcn_rsqf(r2s);
+        const float inv_r2 = inv_r * inv_r;
+        float factor = 0.0f, energy = 0.0f;
+        if (p.coul_on) {
+          const float r = r2s * inv_r;
+          const float qq = p.k_coulomb * qi * b.w;
+          const float ar = p.alpha * r;
+          const float gauss = __builtin_amdgcn_exp2f(-(ar * ar) * HMD_LOG2E);
+          const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ar);
+          const float erfc_ar =
+              ((((1.061405429f * t - 1.453152027f) * t + 1.421413741f) * t -
+                0.284496736f) * t + 0.254829592f) * t * gauss;
+          const float qq_ir = qq * inv_r;
+          factor = qq_ir * inv_r2 * erfc_ar + qq * inv_r2 * two_alpha_sqrtpi * gauss;
+          energy = qq_ir * erfc_ar;
+        }
+        if (p.lj_on) {
+          const float2 cc = s_lj[ti * p.n_types + tj];
+          const float ir6 = inv_r2 * inv_r2 * inv_r2;
+          float e_lj = (cc.y * ir6 - cc.x) * ir6;
+          float f_lj = (12.0f * cc.y * ir6 - 6.0f * cc.x) * ir6 * inv_r2;
+          float tau = (r2s - p.lj_rs2) * p.lj_inv_delta;
+          tau = (r2s > p.lj_rs2) ? tau : 0.0f;
+          const float omt = 1.0f - tau;
+          const float s = omt * omt * (1.0f + 2.0f * tau);
+          const float chain = 12.0f * tau * omt * p.lj_inv_delta;
+          f_lj = s * f_lj + chain * e_lj;
+          e_lj = s * e_lj;
+          factor += f_lj;
+          energy += e_lj;
+        }
+        factor = ok ? factor : 0.0f;
+        const float fx = factor * dx, fy = factor * dy, fz = factor * dz;
+        fix += fx; fiy += fy; fiz += fz;
+        jfx[u] = -fx; jfy[u] = -fy; jfz[u] = -fz;
+        if (FEV) {
+          const float he = ok ? 0.5f * energy : 0.0f;
+          const float hw = 0.5f * factor * r2s;
+          eiv += he; wiv += hw;
+          jfe[u] = he; jfw[u] = hw;
+        }
+      }
+      const float rx = hmd_transpose_reduce8(jfx, lane);
+      const float ry = hmd_transpose_reduce8(jfy, lane);
+      const float rz = hmd_transpose_reduce8(jfz, lane);
+      float re = 0.0f, rw = 0.0f;
+      if (FEV) {
+        re = hmd_transpose_reduce8(jfe, lane);
+        rw = hmd_transpose_reduce8(jfw, lane);
+      }
+      const uint32_t jt = __shfl(my_j, (int)(kb + (lane >> 3)), HMD_WAVE);
+      if ((lane & 7u) == 0u && jt != HMD_SENTINEL) {
+        hmd_atomic_add_fixed(&acc[jt], hmd_to_fixed(rx, HMD_FIXED_SHIFT));
+        hmd_atomic_add_fixed(&acc[stride + jt], hmd_to_fixed(ry, HMD_FIXED_SHIFT));
+        hmd_atomic_add_fixed(&acc[2 * stride + jt], hmd_to_fixed(rz, HMD_FIXED_SHIFT));
+        if (FEV) {
+          hmd_atomic_add_fixed(&acc[3 * stride + jt], hmd_to_fixed(re, HMD_FIXED_SHIFT));
+          hmd_atomic_add_fixed(&acc[4 * stride + jt], hmd_to_fixed(rw, HMD_FIXED_SHIFT));
+        }
+      }
+    }
+  }
+  if (vi) {
+    hmd_atomic_add_fixed(&acc[si], hmd_to_fixed(fix, HMD_FIXED_SHIFT));
+    hmd_atomic_add_fixed(&acc[stride + si], hmd_to_fixed(fiy, HMD_FIXED_SHIFT));
+    hmd_atomic_add_fixed(&acc[2 * stride + si], hmd_to_fixed(fiz, HMD_FIXED_SHIFT));
+    if (FEV) {
+      hmd_atomic_add_fixed(&acc[3 * stride + si], hmd_to_fixed(eiv, HMD_FIXED_SHIFT));
+      hmd_atomic_add_fixed(&acc[4 * stride + si], hmd_to_fixed(wiv, HMD_FIXED_SHIFT));
+    }
+  }
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+hmd_pair_force_f(const PairParams p) { hmd_pair_body<false>(p); }
+
+extern "C" __global__ void __launch_bounds__(256)
+hmd_pair_force_fev(const PairParams p) { hmd_pair_body<true>(p); }
+
+// Fixed point -> f32, sorted -> canonical order, plus the slow-class (SPME)
+// forces: the reference's finalize_packed_forces + combine_class_totals
+// (kernels/neighbor.cu:728-760, kernels/forces.cu:13-41).  The pair
+// accumulators are cleared for the next evaluation as they are read.
+extern "C" __global__ void __launch_bounds__(256)
+hmd_pair_finalize(const PairFinalizeParams p) {
+  const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= p.n) return;
+  const uint32_t s = p.slot_of[a];
+  const uint32_t st = p.acc_stride;
+  long long* acc = p.acc_clear;
+  float f[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    f[d] = hmd_from_fixed(p.acc[d * st + s], HMD_FIXED_SHIFT);
+    acc[d * st + s] = 0;
+    if (p.slow_f) f[d] += p.slow_f[d * p.n + a];
+    p.f[d * p.n + a] = f[d];
+  }
+  if (p.fev) {
+    float e = hmd_from_fixed(p.acc[3 * st + s], HMD_FIXED_SHIFT);
+    float w = hmd_from_fixed(p.acc[4 * st + s], HMD_FIXED_SHIFT);
+    acc[3 * st + s] = 0;
+    acc[4 * st + s] = 0;
+    if (p.slow_e) e += p.slow_e[a];
+    if (p.slow_w) w += p.slow_w[a];
+    p.e[a] = e;
+    p.w[a] = w;
+  }
+}

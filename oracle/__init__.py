@@ -1,0 +1,252 @@
+"""FP64 CPU oracle for the nonbonded hot path -- TEST INFRASTRUCTURE ONLY.
+
+This package is the checker.  Only ``tests/``, ``__graft_entry__.smoke()``
+and ``bench.py``'s ``cpu_baseline`` leg may import it; the product library
+(``heddlemd_amd``) never does.  Each wrapper calls one routine of
+``heddle_oracle.c``, whose comments cite the reference file:line restated.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    """Compile liboracle.so from heddle_oracle.c (gcc, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int)
+_up = ctypes.POINTER(ctypes.c_uint32)
+
+
+class PairParams(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int), ("pos", _dp), ("q", _dp), ("type", _ip), ("L", _dp),
+        ("n_types", ctypes.c_int), ("lj_sigma", _dp), ("lj_epsilon", _dp),
+        ("lj_cutoff", _dp), ("lj_switch", _dp), ("lj_on", ctypes.c_int),
+        ("coul_on", ctypes.c_int), ("k_coulomb", ctypes.c_double),
+        ("alpha", ctypes.c_double), ("r_cut_real", ctypes.c_double),
+        ("erfc_mode", ctypes.c_int),
+        ("excl_offsets", _ip), ("excl_partners", _ip),
+        ("excl_lj_scale", _dp), ("excl_coul_scale", _dp),
+        ("max_cutoff", ctypes.c_double),
+    ]
+
+
+def _declare(L):
+    L.orc_r2_f32.restype = ctypes.c_float
+    L.orc_r2_f32.argtypes = [_fp, _fp, _fp]
+    L.orc_pair_set_f32.restype = ctypes.c_int64
+    L.orc_pair_set_f32.argtypes = [ctypes.c_int, _fp, _fp, ctypes.c_float, _up, ctypes.c_int64]
+    L.orc_pair_forces.argtypes = [ctypes.POINTER(PairParams), _dp, _dp, _dp]
+    L.orc_pair_forces_cells.argtypes = [ctypes.POINTER(PairParams), _dp, _dp, _dp]
+    L.orc_ewald_recip_direct.restype = ctypes.c_double
+    L.orc_ewald_recip_direct.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_int, _dp, _dp]
+    L.orc_bspline.restype = ctypes.c_double
+    L.orc_bspline.argtypes = [ctypes.c_int, ctypes.c_double]
+    L.orc_b_factors.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+    L.orc_spme_spread.argtypes = [ctypes.c_int, _dp, _dp, _dp, _ip, ctypes.c_int, _dp]
+    L.orc_fft3d.argtypes = [_ip, ctypes.c_int, _dp, _dp]
+    L.orc_spme_recip.restype = ctypes.c_double
+    L.orc_spme_recip.argtypes = [ctypes.c_int, _dp, _dp, _dp, _ip, ctypes.c_int,
+                                 ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp, _dp]
+    L.orc_settle_geometry.argtypes = [ctypes.c_double] * 4 + [_dp]
+    L.orc_settle_positions_one.argtypes = [_dp, _dp, _dp] + [ctypes.c_double] * 6 + [_dp, _dp]
+    L.orc_settle_velocities_one.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double, _dp]
+    L.orc_philox4x32_10.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _up, _up]
+    L.orc_philox_gaussian.restype = ctypes.c_double
+    L.orc_philox_gaussian.argtypes = [ctypes.c_uint32] * 6
+    L.orc_kinetic_energy.restype = ctypes.c_double
+    L.orc_kinetic_energy.argtypes = [ctypes.c_int, _dp, _dp]
+    L.orc_num_threads.restype = ctypes.c_int
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def r2_f32(pi, pj, lattice):
+    pi = np.ascontiguousarray(pi, np.float32)
+    pj = np.ascontiguousarray(pj, np.float32)
+    la = np.ascontiguousarray(lattice, np.float32)
+    return lib().orc_r2_f32(_p(pi, _fp), _p(pj, _fp), _p(la, _fp))
+
+
+def pair_set_f32(posq, lattice, r_search_sq):
+    """Canonical (i<j) pairs within the exact f32 search radius: uint32[k,2]."""
+    posq = np.ascontiguousarray(posq, np.float32).reshape(-1, 4)
+    la = np.ascontiguousarray(lattice, np.float32)
+    n = posq.shape[0]
+    cap = 1 << 16
+    while True:
+        out = np.empty((cap, 2), np.uint32)
+        k = lib().orc_pair_set_f32(n, _p(posq, _fp), _p(la, _fp),
+                                   ctypes.c_float(r_search_sq), _p(out, _up), cap)
+        if k <= cap:
+            return out[:k].copy()
+        cap = int(k)
+
+
+class PairSystem:
+    """Holds the arrays a PairParams struct points to (keeps them alive)."""
+
+    def __init__(self, pos, q, types, lattice, lj=None, coulomb=None,
+                 exclusions=None, erfc_mode=0):
+        self.pos = _f64(pos).reshape(-1, 3)
+        n = self.pos.shape[0]
+        self.q = _f64(q)
+        self.types = np.ascontiguousarray(types, np.int32)
+        self.L = _f64(lattice)
+        p = PairParams()
+        p.n = n
+        p.pos = _p(self.pos, _dp)
+        p.q = _p(self.q, _dp)
+        p.type = _p(self.types, _ip)
+        p.L = _p(self.L, _dp)
+        cut = 0.0
+        if lj is not None:
+            self.lj = [_f64(lj[k]) for k in ("sigma", "epsilon", "cutoff", "switch")]
+            p.n_types = int(round(np.sqrt(self.lj[0].size)))
+            p.lj_sigma, p.lj_epsilon, p.lj_cutoff, p.lj_switch = [_p(a, _dp) for a in self.lj]
+            p.lj_on = 1
+            cut = max(cut, float(self.lj[2].max()))
+        if coulomb is not None:
+            p.coul_on = 1
+            p.k_coulomb = coulomb.get("k_coulomb", 1.0)
+            p.alpha = coulomb["alpha"]
+            p.r_cut_real = coulomb["r_cut"]
+            cut = max(cut, p.r_cut_real)
+        p.erfc_mode = erfc_mode
+        if exclusions is not None:
+            off, par, slj, scl = exclusions
+            self.ex = [np.ascontiguousarray(off, np.int32), np.ascontiguousarray(par, np.int32),
+                       _f64(slj), _f64(scl)]
+            p.excl_offsets = _p(self.ex[0], _ip)
+            p.excl_partners = _p(self.ex[1], _ip)
+            p.excl_lj_scale = _p(self.ex[2], _dp)
+            p.excl_coul_scale = _p(self.ex[3], _dp)
+        p.max_cutoff = cut
+        self.params = p
+        self.n = n
+
+    def forces(self, cells=False):
+        n = self.n
+        F = np.zeros((n, 3)); E = np.zeros(n); W = np.zeros(n)
+        fn = lib().orc_pair_forces_cells if cells else lib().orc_pair_forces
+        fn(ctypes.byref(self.params), _p(F, _dp), _p(E, _dp), _p(W, _dp))
+        return F, E, W
+
+
+def ewald_recip_direct(pos, q, lattice, alpha, m_max, k_coulomb=1.0):
+    pos = _f64(pos).reshape(-1, 3); q = _f64(q); la = _f64(lattice)
+    n = pos.shape[0]
+    F = np.zeros((n, 3)); E = np.zeros(n)
+    u = lib().orc_ewald_recip_direct(n, _p(pos, _dp), _p(q, _dp), _p(la, _dp), alpha,
+                                     k_coulomb, m_max, _p(F, _dp), _p(E, _dp))
+    return u, F, E
+
+
+def bspline(p, x):
+    return lib().orc_bspline(p, float(x))
+
+
+def b_factors(n, p):
+    out = np.zeros(n)
+    lib().orc_b_factors(n, p, _p(out, _dp))
+    return out
+
+
+def spme_spread(pos, q, lattice, grid, order):
+    pos = _f64(pos).reshape(-1, 3); q = _f64(q); la = _f64(lattice)
+    nk = np.ascontiguousarray(grid, np.int32)
+    rho = np.zeros(int(np.prod(grid)))
+    lib().orc_spme_spread(pos.shape[0], _p(pos, _dp), _p(q, _dp), _p(la, _dp),
+                          _p(nk, _ip), order, _p(rho, _dp))
+    return rho.reshape(tuple(grid))
+
+
+def fft3d(re, im, sign):
+    re = _f64(re).copy(); im = _f64(im).copy()
+    nk = np.ascontiguousarray(re.shape, np.int32)
+    lib().orc_fft3d(_p(nk, _ip), sign, _p(re, _dp), _p(im, _dp))
+    return re, im
+
+
+def spme_recip(pos, q, lattice, grid, order, alpha, k_coulomb=1.0, want_grids=False):
+    """Returns (U_recip, F[n,3], E[n] (incl. -u_self), W_recip[, rho, V])."""
+    pos = _f64(pos).reshape(-1, 3); q = _f64(q); la = _f64(lattice)
+    nk = np.ascontiguousarray(grid, np.int32)
+    n = pos.shape[0]
+    F = np.zeros((n, 3)); E = np.zeros(n); w = np.zeros(1)
+    M = int(np.prod(grid))
+    rho = np.zeros(M) if want_grids else None
+    V = np.zeros(M) if want_grids else None
+    u = lib().orc_spme_recip(n, _p(pos, _dp), _p(q, _dp), _p(la, _dp), _p(nk, _ip), order,
+                             alpha, k_coulomb, _p(F, _dp), _p(E, _dp), _p(w, _dp),
+                             _p(rho, _dp) if want_grids else None,
+                             _p(V, _dp) if want_grids else None)
+    if want_grids:
+        return u, F, E, float(w[0]), rho.reshape(tuple(grid)), V.reshape(tuple(grid))
+    return u, F, E, float(w[0])
+
+
+def settle_geometry(d_oh, d_hh, m_o, m_h):
+    out = np.zeros(3)
+    lib().orc_settle_geometry(d_oh, d_hh, m_o, m_h, _p(out, _dp))
+    return out
+
+
+def settle_positions_one(snap, cur, v, m_o, m_h, geom, dt, lattice):
+    snap = _f64(snap).reshape(9); cur = _f64(cur).reshape(9).copy()
+    v = _f64(v).reshape(9).copy(); la = _f64(lattice); vir = np.zeros(3)
+    lib().orc_settle_positions_one(_p(snap, _dp), _p(cur, _dp), _p(v, _dp), m_o, m_h,
+                                   geom[0], geom[1], geom[2], dt, _p(la, _dp), _p(vir, _dp))
+    return cur.reshape(3, 3), v.reshape(3, 3), vir
+
+
+def settle_velocities_one(pos, v, m_o, m_h, lattice):
+    pos = _f64(pos).reshape(9); v = _f64(v).reshape(9).copy(); la = _f64(lattice)
+    lib().orc_settle_velocities_one(_p(pos, _dp), _p(v, _dp), m_o, m_h, _p(la, _dp))
+    return v.reshape(3, 3)
+
+
+def philox4x32_10(key, ctr):
+    c = np.ascontiguousarray(ctr, np.uint32); o = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(int(key[0]), int(key[1]), _p(c, _up), _p(o, _up))
+    return o
+
+
+def philox_gaussian(seed_lo, seed_hi, step_lo, step_hi, pid, axis):
+    return lib().orc_philox_gaussian(seed_lo, seed_hi, step_lo, step_hi, pid, axis)
+
+
+def kinetic_energy(v, m):
+    v = _f64(v).reshape(-1, 3); m = _f64(m)
+    return lib().orc_kinetic_energy(v.shape[0], _p(v, _dp), _p(m, _dp))
+
+
+def num_threads():
+    return lib().orc_num_threads()

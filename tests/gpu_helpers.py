@@ -1,0 +1,37 @@
+"""Shared set-up for the GPU parity tests: build a system on the device and
+the matching FP64 oracle inputs."""
+import numpy as np
+
+import oracle
+from heddlemd_amd import api, systems
+from heddlemd_amd import units as U
+
+
+def water_case(ctx, grid=(12, 12, 9), seed=7, mesh=(32, 32, 32), order=4, relax=0,
+               r_skin_nm=0.1, spme=True):
+    sysw = systems.spc_water(grid=grid, seed=seed)
+    ff = systems.water_forcefield(grid=mesh if spme else None, order=order,
+                                  r_skin_nm=r_skin_nm, exclusions=sysw["exclusions"])
+    part = api.ParticleBuffers(ctx, sysw["posq"], sysw["mass"], sysw["lattice"],
+                               types=sysw["types"])
+    field = api.ForceField(ctx, part, ff)
+    integ = api.Integrator(ctx, part, field, dt=2.0 * U.FS, settle=sysw["settle"])
+    if relax:
+        integ.minimize(relax, 0.05 * U.ANGSTROM)
+    return sysw, ff, part, field, integ
+
+
+def oracle_pair(sysw, ff, posq, erfc_mode=1):
+    lj = ff["lj"]
+    off, par = ff["exclusions"]
+    ex = (off.astype(np.int32), par.astype(np.int32), np.zeros(par.size), np.zeros(par.size))
+    ps = oracle.PairSystem(posq[:, :3].astype(np.float64), posq[:, 3].astype(np.float64),
+                           sysw["types"].astype(np.int32), sysw["lattice"],
+                           lj=lj, coulomb=ff["coulomb"], exclusions=ex, erfc_mode=erfc_mode)
+    return ps.forces(cells=True)
+
+
+def rel_rms(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.sqrt(np.sum((a - b) ** 2)) / np.sqrt(np.sum(b ** 2)))
