@@ -29,7 +29,8 @@ EXPORTED = [
     "hmd_particles_kinetic_energy", "hmd_particles_energy_sums",
     "hmd_forcefield_create", "hmd_forcefield_destroy", "hmd_forcefield_compute",
     "hmd_forcefield_slots", "hmd_spme_scalars", "hmd_spme_grids", "hmd_spme_forces",
-    "hmd_nbl_request_rebuild", "hmd_nbl_pair_snapshot", "hmd_nbl_stats",
+    "hmd_nbl_request_rebuild", "hmd_nbl_pair_snapshot", "hmd_nbl_count_pairs",
+    "hmd_nbl_stats",
     "hmd_integrator_create", "hmd_integrator_destroy", "hmd_integrator_step",
     "hmd_integrator_capture", "hmd_integrator_replay", "hmd_integrator_steps",
     "hmd_integrator_profile", "hmd_integrator_minimize",
@@ -113,6 +114,7 @@ def _declare(L):
         "hmd_spme_forces": (st, [vp, _c_fp]),
         "hmd_nbl_request_rebuild": (st, [vp]),
         "hmd_nbl_pair_snapshot": (st, [vp, _c_u32p, ctypes.c_uint64, _c_u64p]),
+        "hmd_nbl_count_pairs": (st, [vp, ctypes.c_double, _c_u64p]),
         "hmd_nbl_stats": (st, [vp, _c_u32p]),
         "hmd_integrator_create": (st, [vp, vp, vp, ctypes.POINTER(IntegratorDesc), pvp]),
         "hmd_integrator_destroy": (None, [vp]),

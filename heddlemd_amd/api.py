@@ -207,6 +207,11 @@ class ForceField:
                 return pairs[order]
             cap = int(k.value)
 
+    def count_pairs(self, radius):
+        k = ctypes.c_uint64()
+        check(lib().hmd_nbl_count_pairs(self.h, radius, ctypes.byref(k)))
+        return int(k.value)
+
     def stats(self):
         out = np.zeros(8, np.uint32)
         check(lib().hmd_nbl_stats(self.h, _ptr(out, _u32p)))

@@ -134,6 +134,8 @@ ForceField::ForceField(Context& ctx, Particles& part, const hmd_forcefield_desc&
       double s6 = std::pow(sg, 6.0);
       c6c12[2 * k] = (float)(4.0 * ep * s6);
       c6c12[2 * k + 1] = (float)(4.0 * ep * s6 * s6);
+      if (c6c12[2 * k] != 0.0f || c6c12[2 * k + 1] != 0.0f)
+        lj_type_mask |= (1u << (k % n_types)) | (1u << (k / n_types));
       if (lj_cut < 0.0) { lj_cut = d.lj_cutoff[k]; lj_sw = d.lj_switch[k]; }
       if (d.lj_cutoff[k] != lj_cut || d.lj_switch[k] != lj_sw)
         throw Error(HMD_ERR_UNSUPPORTED,
@@ -164,8 +166,11 @@ ForceField::ForceField(Context& ctx, Particles& part, const hmd_forcefield_desc&
   std::vector<uint32_t> eoff(n + 1, 0u), epar;
   if (d.excl_offsets) {
     eoff.assign(d.excl_offsets, d.excl_offsets + n + 1);
-    for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t i = 0; i < n; ++i) {
       if (eoff[i + 1] < eoff[i]) throw Error(HMD_ERR_INVALID, "exclusion offsets not monotonic");
+      if (eoff[i + 1] - eoff[i] > 8)
+        throw Error(HMD_ERR_UNSUPPORTED, "more than 8 exclusion partners for one atom");
+    }
     epar.assign(d.excl_partners, d.excl_partners + eoff[n]);
     for (uint32_t m = 0; m < eoff[n]; ++m) {
       if ((d.excl_lj_scale && d.excl_lj_scale[m] != 0.0f) ||
@@ -246,6 +251,7 @@ void ForceField::init_neighbor_list(const hmd_forcefield_desc& d) {
   posq_sorted.alloc(4 * (size_t)n_pad);
   ref_pos.alloc(4 * (size_t)n);
   bbox.alloc(8 * (size_t)n_clusters);
+  sub_bbox.alloc(64 * (size_t)n_clusters);
   entry_count.alloc(n_clusters);
   entry_offset.alloc(n_clusters + 1);
   n_work.alloc(1);
@@ -371,12 +377,14 @@ void ForceField::rebuild_chain() {
   NblGatherParams gp{p.posq.ptr, p.type.ptr, n, n_pad, sorted_id.ptr, slot_of.ptr,
                      posq_sorted.ptr, type_sorted.ptr, ref_pos.ptr, gate};
   launch(c, K_hmd_nbl_gather, ceil_div(n_pad, 256), 256, 0, gp);
-  NblBboxParams bp{posq_sorted.ptr, n, n_clusters, bbox.ptr, gate};
+  NblBboxParams bp{posq_sorted.ptr, n, n_clusters, bbox.ptr, sub_bbox.ptr, gate};
   launch(c, K_hmd_nbl_bbox, ceil_div(n_clusters, 4), 256, 0, bp);
   NblBuildParams bd{};
   bd.posq_sorted = posq_sorted.ptr;
   bd.sorted_id = sorted_id.ptr;
+  bd.slot_of = slot_of.ptr;
   bd.bbox = bbox.ptr;
+  bd.sub_bbox = sub_bbox.ptr;
   bd.L = L;
   bd.n = n;
   bd.n_clusters = n_clusters;
@@ -420,8 +428,7 @@ void ForceField::pair_forces(bool fev) {
   pp.cap_entries = cap_entries;
   pp.n_types = n_types;
   pp.lj_c6c12 = lj_table.ptr;
-  pp.lj_on = lj_on;
-  pp.coul_on = coul_on;
+  pp.lj_type_mask = lj_type_mask;
   pp.rc2 = rc2;
   pp.lj_rs2 = lj_rs2;
   pp.lj_inv_delta = lj_inv_delta;
@@ -431,14 +438,17 @@ void ForceField::pair_forces(bool fev) {
   pp.acc = acc.ptr;
   pp.acc_stride = n_pad;
   const uint32_t blocks = std::max(1u, (uint32_t)c.num_cus * pair_blocks_per_cu);
-  launch(c, fev ? K_hmd_pair_force_fev : K_hmd_pair_force_f, blocks, 256, 0, pp);
+  KernelId k = lj_on && coul_on ? (fev ? K_hmd_pair_force_fev : K_hmd_pair_force_f)
+               : lj_on ? (fev ? K_hmd_pair_force_lj_fev : K_hmd_pair_force_lj_f)
+                       : (fev ? K_hmd_pair_force_coul_fev : K_hmd_pair_force_coul_f);
+  launch(c, k, blocks, 256, 0, pp);
   c.mark("pair_force");
 }
 
 void ForceField::spme_forces(bool fev) {
   const HmdLattice L = p.lattice;
   const uint32_t n0 = grid[0], n1 = grid[1], n2 = grid[2];
-  SpmeBinParams bp{p.posq.ptr, n, L, n0, n1, n2, brick[0], brick[1], brick[2],
+  SpmeBinParams bp{posq_sorted.ptr, sorted_id.ptr, n, L, n0, n1, n2, brick[0], brick[1], brick[2],
                    nb[0], nb[1], nb[2], atom_bin.ptr, bin_count.ptr, bin_start.ptr,
                    bin_atoms.ptr};
   launch(c, K_hmd_spme_bin, ceil_div(n, 256), 256, 0, bp);
@@ -690,6 +700,26 @@ hmd_status hmd_nbl_pair_snapshot(hmd_forcefield* f, uint32_t* pairs, uint64_t ca
     *n_pairs = total;
     uint64_t k = std::min<uint64_t>(total, cap);
     if (k) dpairs.download(pairs, 2 * k, c.stream);
+  });
+}
+
+hmd_status hmd_nbl_count_pairs(hmd_forcefield* f, double radius, uint64_t* n_pairs) {
+  HMD_GUARD({
+    ForceField& F = *f->ff;
+    Context& c = F.c;
+    const float r = (float)radius;
+    if (!(r > 0.0f) || r > F.r_search) throw Error(HMD_ERR_INVALID, "radius must be in (0, r_search]");
+    DevBuf<unsigned long long> cnt;
+    cnt.alloc(1);
+    uint32_t nw = 0;
+    F.n_work.download(&nw, 1, c.stream);
+    NblPairsParams pp{F.posq_sorted.ptr, F.sorted_id.ptr, F.work.ptr, F.n_work.ptr,
+                      F.entry_j.ptr, F.p.lattice, F.n, F.cap_entries, r * r,
+                      nullptr, cnt.ptr, 0ull, nullptr};
+    launch(c, K_hmd_nbl_pairs, ceil_div(nw, 4), 256, 0, pp);
+    unsigned long long total = 0;
+    cnt.download(&total, 1, c.stream);
+    *n_pairs = total;
   });
 }
 

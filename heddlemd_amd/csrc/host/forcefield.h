@@ -13,6 +13,7 @@ struct ForceField {
   // ---- fused pair slot
   bool lj_on = false, coul_on = false, spme_on = false;
   uint32_t n_types = 1;
+  uint32_t lj_type_mask = 0;
   float r_cut = 0, rc2 = 0, lj_rs2 = 0, lj_inv_delta = 0;
   float alpha = 0, k_coulomb = 1;
   DevBuf<float> lj_table;
@@ -26,7 +27,7 @@ struct ForceField {
   uint32_t n_keys = 1, n_clusters = 0, n_pad = 0, cap_entries = 0;
   DevBuf<uint32_t> status;          // [0] flags [1] generation [2] high water
   DevBuf<uint32_t> atom_key, cell_count, cell_start, sorted_id, slot_of, type_sorted;
-  DevBuf<float> posq_sorted, ref_pos, bbox;
+  DevBuf<float> posq_sorted, ref_pos, bbox, sub_bbox;
   DevBuf<uint32_t> entry_j, entry_count, entry_offset, work, n_work;
   DevBuf<uint64_t> entry_excl;
   DevBuf<long long> acc;            // [5][n_pad] fixed-point accumulators

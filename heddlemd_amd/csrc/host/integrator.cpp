@@ -422,26 +422,53 @@ hmd_status hmd_integrator_profile(hmd_integrator* w, uint32_t n, double* us, int
 
 hmd_status hmd_integrator_minimize(hmd_integrator* w, uint32_t n_iter, double max_step) {
   HMD_GUARD({
+    // Steepest descent with an adaptive step: move every atom by h F (the
+    // largest move is `step`), project with SETTLE, accept if the potential
+    // energy went down (step * 1.2), otherwise restore and halve the step.
     Integrator& I = *w->it;
     Context& c = I.c;
     Particles& p = I.p;
     std::vector<float> f(3 * (size_t)p.n);
-    for (uint32_t it = 0; it < n_iter; ++it) {
-      I.ff.compute(0);
-      p.force.download(f.data(), f.size(), c.stream);
+    DevBuf<float> backup;
+    backup.alloc(p.posq.count, false);
+    auto energy = [&]() {
+      I.ff.compute(1);
+      check(hipStreamSynchronize(c.stream), "sync");
       I.ff.check_status();
-      double fmax = 0.0;
+      return reduce_f64(c, p, p.energy.ptr, 0);
+    };
+    auto fmax_now = [&]() {
+      p.force.download(f.data(), f.size(), c.stream);
+      double fm = 0.0;
       for (uint32_t i = 0; i < p.n; ++i) {
-        double s = 0.0;
-        for (int d = 0; d < 3; ++d) s += (double)f[(size_t)d * p.n + i] * f[(size_t)d * p.n + i];
-        fmax = std::max(fmax, std::sqrt(s));
+        double s2 = 0.0;
+        for (int d = 0; d < 3; ++d) s2 += (double)f[(size_t)d * p.n + i] * f[(size_t)d * p.n + i];
+        fm = std::max(fm, std::sqrt(s2));
       }
-      if (!(fmax > 0.0) || !std::isfinite(fmax)) break;
-      SdParams sp{p.posq.ptr, p.force.ptr, p.n, p.lattice, (float)(max_step / fmax),
-                  (float)max_step};
+      return fm;
+    };
+    double e0 = energy();
+    double step = max_step;
+    for (uint32_t it = 0; it < n_iter; ++it) {
+      const double fm = fmax_now();
+      if (!(fm > 0.0) || !std::isfinite(fm)) break;
+      check(hipMemcpyAsync(backup.ptr, p.posq.ptr, p.posq.count * sizeof(float),
+                           hipMemcpyDeviceToDevice, c.stream), "D2D");
+      SdParams sp{p.posq.ptr, p.force.ptr, p.n, p.lattice, (float)(step / fm), (float)step};
       I.settle_snapshot();
       launch(c, K_hmd_sd_step, ceil_div(p.n, 256), 256, 0, sp);
       I.settle_positions(0.0f);
+      const double e1 = energy();
+      if (std::isfinite(e1) && e1 < e0) {
+        e0 = e1;
+        step = std::min(max_step, step * 1.2);
+      } else {
+        check(hipMemcpyAsync(p.posq.ptr, backup.ptr, p.posq.count * sizeof(float),
+                             hipMemcpyDeviceToDevice, c.stream), "D2D");
+        step *= 0.5;
+        e0 = energy();
+        if (step < 1e-6 * max_step) break;
+      }
     }
     check(hipStreamSynchronize(c.stream), "sync");
     I.primed = false;

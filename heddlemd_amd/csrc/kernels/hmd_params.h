@@ -73,13 +73,16 @@ struct NblBboxParams {
   uint32_t n;
   uint32_t n_clusters;
   float* bbox;              // float4[2*n_clusters]: (cx,cy,cz,-), (hx,hy,hz,-)
+  float* sub_bbox;          // float4[16*n_clusters]: 8 sub-cluster boxes
   const uint32_t* status;
 };
 
 struct NblBuildParams {
   const float* posq_sorted;
   const uint32_t* sorted_id;
+  const uint32_t* slot_of;
   const float* bbox;
+  const float* sub_bbox;
   HmdLattice L;
   uint32_t n, n_clusters;
   float r_search_sq;
@@ -141,7 +144,7 @@ struct PairParams {
   uint32_t n, cap_entries;
   uint32_t n_types;
   const float* lj_c6c12;       // float2[n_types^2]: (4 eps s^6, 4 eps s^12)
-  uint32_t lj_on, coul_on;
+  uint32_t lj_type_mask;       // bit t: type t has LJ with some type
   float rc2;                   // max cutoff squared (mask)
   float lj_rs2;                // switch start squared
   float lj_inv_delta;          // 1/(rc^2 - rs^2); 0 disables the switch
@@ -168,13 +171,14 @@ struct PairFinalizeParams {
 
 // ------------------------------------------------------------- spme ----
 struct SpmeBinParams {
-  const float* posq;           // canonical
+  const float* posq_sorted;    // neighbour-list (Morton) order
+  const uint32_t* sorted_id;   // sorted slot -> canonical id
   uint32_t n;
   HmdLattice L;
   uint32_t n0, n1, n2;         // mesh
   uint32_t b0, b1, b2;         // brick dims (mesh points)
   uint32_t nb0, nb1, nb2;      // bricks per axis
-  uint32_t* atom_bin;          // [n]
+  uint32_t* atom_bin;          // [n] bin of each sorted slot
   uint32_t* bin_count;         // [n_bins] zero on entry, zero after scatter
   const uint32_t* bin_start;   // [n_bins+1]
   uint32_t* bin_atoms;         // [n]

@@ -144,7 +144,9 @@ hmd_nbl_gather(const NblGatherParams p) {
 
 // Per-cluster axis-aligned bounding box (centre and half extents) of the
 // valid atoms, one wave per cluster (compare compute_block_bbox,
-// kernels/neighbor.cu:291-361).
+// kernels/neighbor.cu:291-361), plus the boxes of the cluster's eight
+// 8-atom sub-clusters (consecutive sorted atoms) used to prune the exact
+// distance tests of the list build.
 extern "C" __global__ void __launch_bounds__(256)
 hmd_nbl_bbox(const NblBboxParams p) {
   if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
@@ -156,12 +158,20 @@ hmd_nbl_bbox(const NblBboxParams p) {
   float4 a = reinterpret_cast<const float4*>(p.posq_sorted)[s];
   float lo[3] = {valid ? a.x : 3.0e38f, valid ? a.y : 3.0e38f, valid ? a.z : 3.0e38f};
   float hi[3] = {valid ? a.x : -3.0e38f, valid ? a.y : -3.0e38f, valid ? a.z : -3.0e38f};
+  float4* sub = reinterpret_cast<float4*>(p.sub_bbox);
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
+  for (int off = 1; off <= 32; off <<= 1) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       lo[d] = fminf(lo[d], hmd_shfl_xor(lo[d], off));
       hi[d] = fmaxf(hi[d], hmd_shfl_xor(hi[d], off));
+    }
+    if (off == 4 && (lane & 7u) == 0u) {   // 8-lane groups reduced
+      const uint32_t k = c * 8u + lane / 8u;
+      sub[2 * k] = make_float4(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]),
+                               0.5f * (lo[2] + hi[2]), 0.0f);
+      sub[2 * k + 1] = make_float4(0.5f * (hi[0] - lo[0]), 0.5f * (hi[1] - lo[1]),
+                                   0.5f * (hi[2] - lo[2]), 0.0f);
     }
   }
   if (lane == 0) {
@@ -174,11 +184,20 @@ hmd_nbl_bbox(const NblBboxParams p) {
 }
 
 // Conservative min-image gap between two boxes (compare
-// packed_block_bbox_dist_sq, kernels/neighbor.cu:365-388).
+// packed_block_bbox_dist_sq, kernels/neighbor.cu:365-388).  Used only as a
+// prune against r_search^2 with a relative margin, so the image may be
+// chosen with a reciprocal multiply.
 __device__ inline float hmd_box_gap2(float4 ca, float4 ha, float4 cb, float4 hb,
-                                     const HmdLattice& L) {
+                                     const HmdLattice& L, bool ortho, float ilx,
+                                     float ily, float ilz) {
   float dx = ca.x - cb.x, dy = ca.y - cb.y, dz = ca.z - cb.z;
-  hmd_min_image(dx, dy, dz, L);
+  if (ortho) {
+    dx -= L.lx * rintf(dx * ilx);
+    dy -= L.ly * rintf(dy * ily);
+    dz -= L.lz * rintf(dz * ilz);
+  } else {
+    hmd_min_image(dx, dy, dz, L);
+  }
   float ex = fmaxf(fabsf(dx) - (ha.x + hb.x), 0.0f);
   float ey = fmaxf(fabsf(dy) - (ha.y + hb.y), 0.0f);
   float ez = fmaxf(fabsf(dz) - (ha.z + hb.z), 0.0f);
@@ -190,14 +209,19 @@ __device__ inline float hmd_box_gap2(float4 ca, float4 ha, float4 cb, float4 hb,
 // Candidate j-clusters are those with index >= c whose boxes lie within the
 // search radius.  For each candidate, lane k tests j-atom k against the 64
 // i-atoms (exact f32 distance) and keeps it when some i-atom with a lower
-// sorted index is within r_search.  Kept atoms are appended in increasing
-// sorted order to 64-slot entries; lane k of the wave owns slot k of the
-// current entry, and every lane records in its 64-bit mask which slots hold
-// one of its atom's exclusion partners.
+// sorted index is within r_search.  The kept atoms of a candidate are placed
+// at consecutive entry positions in increasing sorted order by a ballot
+// prefix count, written straight to the entry array.  Each lane keeps the
+// sorted slots of its atom's exclusion partners (at most
+// HMD_MAX_INLINE_EXCL) in registers and, per candidate, sets the bit of the
+// entry position that holds a partner -- no per-hit loop, no dependent
+// memory access.
+#define HMD_MAX_INLINE_EXCL 8
 extern "C" __global__ void __launch_bounds__(256)
 hmd_nbl_build(const NblBuildParams p) {
   if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
   __shared__ float4 s_i[4][HMD_CLUSTER];
+  __shared__ float4 s_sub[4][16];
   const uint32_t wave = threadIdx.x / HMD_WAVE;
   const uint32_t c = blockIdx.x * 4u + wave;
   const uint32_t lane = hmd_lane();
@@ -205,95 +229,120 @@ hmd_nbl_build(const NblBuildParams p) {
   const float4* ps = reinterpret_cast<const float4*>(p.posq_sorted);
   const float4* bb = reinterpret_cast<const float4*>(p.bbox);
 
-  uint32_t si = c * HMD_CLUSTER + lane;
-  float4 pi = active_wave ? ps[si] : make_float4(0, 0, 0, 0);
-  s_i[wave][lane] = pi;
+  const uint32_t si = c * HMD_CLUSTER + lane;
+  s_i[wave][lane] = active_wave ? ps[si] : make_float4(0, 0, 0, 0);
+  if (lane < 16) s_sub[wave][lane] = active_wave
+      ? reinterpret_cast<const float4*>(p.sub_bbox)[16 * c + lane] : make_float4(0, 0, 0, 0);
   __syncthreads();
   if (!active_wave) return;
 
+  const bool ortho = p.orthorhombic_fast != 0;
   const float ilx = 1.0f / p.L.lx, ily = 1.0f / p.L.ly, ilz = 1.0f / p.L.lz;
-  // This lane's exclusion partners (canonical ids), read on demand.
-  uint32_t my_id = si < p.n ? p.sorted_id[si] : HMD_SENTINEL;
-  uint32_t ex_b = 0, ex_e = 0;
-  if (my_id != HMD_SENTINEL && p.excl_offsets) {
-    ex_b = p.excl_offsets[my_id];
-    ex_e = p.excl_offsets[my_id + 1];
+  const float rs2_prune = p.r_search_sq * 1.0002f;   // conservative prune
+  // Sorted slots of this lane's exclusion partners.
+  uint32_t ex_slot[HMD_MAX_INLINE_EXCL];
+  uint32_t n_ex = 0;
+  if (si < p.n && p.excl_offsets) {
+    const uint32_t id = p.sorted_id[si];
+    const uint32_t b = p.excl_offsets[id], e = p.excl_offsets[id + 1];
+    n_ex = min(e - b, (uint32_t)HMD_MAX_INLINE_EXCL);
+#pragma unroll
+    for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m)
+      ex_slot[m] = (m < n_ex) ? p.slot_of[p.excl_partners[b + m]] : HMD_SENTINEL;
+  } else {
+#pragma unroll
+    for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m) ex_slot[m] = HMD_SENTINEL;
   }
 
-  float4 ci = bb[2 * c], hi = bb[2 * c + 1];
-  const uint32_t base_entry = c * p.cap_entries;
-  uint32_t n_entries = 0;       // entries completed (uniform)
+  const float4 ci = bb[2 * c], hi = bb[2 * c + 1];
+  const size_t base_entry = (size_t)c * p.cap_entries;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t imax = min(p.n - c * HMD_CLUSTER, HMD_CLUSTER);
+  uint32_t n_entries = 0;       // completed entries (uniform)
   uint32_t fill = 0;            // slots used in the current entry (uniform)
-  uint32_t my_slot_j = HMD_SENTINEL;
-  uint64_t my_mask = 0;
+  uint64_t cur_mask = 0, next_mask = 0;
   bool overflow = false;
 
   for (uint32_t jb = c; jb < p.n_clusters; jb += HMD_WAVE) {
-    uint32_t jc = jb + lane;
+    const uint32_t jc = jb + lane;
     bool cand = false;
     if (jc < p.n_clusters) {
-      float4 cj = bb[2 * jc], hj = bb[2 * jc + 1];
-      cand = hmd_box_gap2(ci, hi, cj, hj, p.L) <= p.r_search_sq;
+      const float4 cj = bb[2 * jc], hj = bb[2 * jc + 1];
+      cand = hmd_box_gap2(ci, hi, cj, hj, p.L, ortho, ilx, ily, ilz) <= rs2_prune;
     }
     uint64_t cmask = __ballot(cand);
     while (cmask) {
-      uint32_t bit = (uint32_t)__builtin_ctzll(cmask);
+      const uint32_t jcl = jb + (uint32_t)__builtin_ctzll(cmask);
       cmask &= cmask - 1;
-      uint32_t jcl = jb + bit;
-      uint32_t sj = jcl * HMD_CLUSTER + lane;
-      bool hit = false;
+      const uint32_t sj = jcl * HMD_CLUSTER + lane;
+      // Sub-box mask: which 8-atom i-sub-clusters this j-atom may reach.
+      uint32_t smask = 0;
+      float4 pj = make_float4(0, 0, 0, 0);
       if (sj < p.n) {
-        float4 pj = ps[sj];
-        // Cheap prune: distance from the j-atom to the i-cluster box.
-        float dx = pj.x - ci.x, dy = pj.y - ci.y, dz = pj.z - ci.z;
-        hmd_min_image(dx, dy, dz, p.L);
-        float ex = fmaxf(fabsf(dx) - hi.x, 0.0f);
-        float ey = fmaxf(fabsf(dy) - hi.y, 0.0f);
-        float ez = fmaxf(fabsf(dz) - hi.z, 0.0f);
-        if (ex * ex + ey * ey + ez * ez <= p.r_search_sq) {
-          uint32_t ilim = (jcl == c) ? lane : HMD_CLUSTER;  // s_i < s_j
-          uint32_t imax = p.n - c * HMD_CLUSTER;
-          if (ilim > imax) ilim = imax;
-          for (uint32_t k = 0; k < ilim && !hit; ++k) {
-            float4 a = s_i[wave][k];
-            float r2 = p.orthorhombic_fast
-                           ? hmd_r2_ortho(a.x, a.y, a.z, pj.x, pj.y, pj.z, p.L, ilx, ily, ilz)
-                           : hmd_r2_exact(a.x, a.y, a.z, pj.x, pj.y, pj.z, p.L);
-            hit = r2 <= p.r_search_sq;
+        pj = ps[sj];
+        const float4 zero = make_float4(0, 0, 0, 0);
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) {
+          const float g2 = hmd_box_gap2(s_sub[wave][2 * k], s_sub[wave][2 * k + 1], pj, zero,
+                                        p.L, ortho, ilx, ily, ilz);
+          smask |= (g2 <= rs2_prune) ? (1u << k) : 0u;
+        }
+      }
+      // i-atoms allowed for this j: sorted index below j's, and valid.
+      const uint32_t ilim = (jcl == c) ? min(lane, imax) : imax;
+      bool hit = false;
+      for (uint32_t k = 0; k < 8; ++k) {
+        const bool want = !hit && ((smask >> k) & 1u);
+        if (__ballot(want) == 0) continue;
+        if (want) {
+          for (uint32_t t = 0; t < 8; ++t) {
+            const uint32_t i = 8 * k + t;
+            const float4 a = s_i[wave][i];
+            const float r2 = ortho
+                                 ? hmd_r2_ortho(a.x, a.y, a.z, pj.x, pj.y, pj.z, p.L, ilx, ily, ilz)
+                                 : hmd_r2_exact(a.x, a.y, a.z, pj.x, pj.y, pj.z, p.L);
+            hit = hit || (i < ilim && r2 <= p.r_search_sq);
           }
         }
       }
-      uint64_t hmask = __ballot(hit);
-      while (hmask) {
-        uint32_t hb = (uint32_t)__builtin_ctzll(hmask);
-        hmask &= hmask - 1;
-        uint32_t sj_h = jcl * HMD_CLUSTER + hb;          // uniform
-        uint32_t jid = p.sorted_id[sj_h];                // uniform
-        if (lane == fill) my_slot_j = sj_h;
-        for (uint32_t m = ex_b; m < ex_e; ++m)
-          if (p.excl_partners[m] == jid) my_mask |= (1ull << fill);
-        ++fill;
-        if (fill == HMD_ENTRY) {
-          if (n_entries < p.cap_entries) {
-            size_t o = (size_t)(base_entry + n_entries) * HMD_ENTRY + lane;
-            p.entry_j[o] = my_slot_j;
-            p.entry_excl[o] = my_mask;
-          } else {
-            overflow = true;
+      const uint64_t H = __ballot(hit);
+      if (H == 0) continue;
+      if (hit) {
+        const uint32_t pos = fill + (uint32_t)__builtin_popcountll(H & lt_mask);
+        const uint32_t e = n_entries + pos / HMD_ENTRY;
+        if (e < p.cap_entries) p.entry_j[(base_entry + e) * HMD_ENTRY + pos % HMD_ENTRY] = sj;
+      }
+#pragma unroll
+      for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m) {
+        const uint32_t es = ex_slot[m];
+        if (es != HMD_SENTINEL && es / HMD_CLUSTER == jcl) {
+          const uint32_t b = es % HMD_CLUSTER;
+          if ((H >> b) & 1ull) {
+            const uint64_t below = (b == 0) ? 0ull : (H & (~0ull >> (64 - b)));
+            const uint32_t pos = fill + (uint32_t)__builtin_popcountll(below);
+            if (pos < HMD_ENTRY) cur_mask |= 1ull << pos;
+            else next_mask |= 1ull << (pos - HMD_ENTRY);
           }
-          ++n_entries;
-          fill = 0;
-          my_slot_j = HMD_SENTINEL;
-          my_mask = 0;
         }
+      }
+      fill += (uint32_t)__builtin_popcountll(H);
+      if (fill >= HMD_ENTRY) {
+        if (n_entries < p.cap_entries)
+          p.entry_excl[(base_entry + n_entries) * HMD_ENTRY + lane] = cur_mask;
+        else
+          overflow = true;
+        ++n_entries;
+        fill -= HMD_ENTRY;
+        cur_mask = next_mask;
+        next_mask = 0;
       }
     }
   }
   if (fill > 0) {
     if (n_entries < p.cap_entries) {
-      size_t o = (size_t)(base_entry + n_entries) * HMD_ENTRY + lane;
-      p.entry_j[o] = my_slot_j;
-      p.entry_excl[o] = my_mask;
+      const size_t o = (base_entry + n_entries) * HMD_ENTRY + lane;
+      if (lane >= fill) p.entry_j[o] = HMD_SENTINEL;
+      p.entry_excl[o] = cur_mask;
     } else {
       overflow = true;
     }

@@ -8,8 +8,12 @@
 //
 // Mapping (wave64, VALU FP32; no MFMA -- there is no dense contraction):
 //  * one wave = one 64-atom i-cluster, one i-atom per lane, held in VGPRs;
-//  * the entry's 64 j-atoms are walked in order; j data are wave-uniform
-//    (scalar loads), so each step evaluates 64 pairs with no shuffles;
+//  * an entry's 64 j-atoms are gathered with one load per lane (and shifted
+//    to the image nearest the cluster centre), then broadcast one per step
+//    with readlane into scalar registers, so each step evaluates 64 pairs
+//    with no memory access and no shuffles;
+//  * slot combinations (LJ, Coulomb, energies) are compile-time variants,
+//    and LJ is skipped for j-types with no LJ partner (uniform branch);
 //  * the j-side reaction forces of 8 consecutive j-atoms are summed across
 //    the wave by a transposed butterfly (8 values -> 1 per lane group), then
 //    8 lanes issue the fixed-point atomics;
@@ -49,7 +53,138 @@ __device__ inline float hmd_transpose_reduce8(const float v[8], uint32_t lane) {
   return c;
 }
 
+__device__ inline float hmd_readlane_f(float v, uint32_t lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)lane));
+}
+
+struct HmdPairState {
+  float xi, yi, zi, kqi;
+  uint32_t si, ti;
+  bool vi;
+  float fix, fiy, fiz, eiv, wiv;
+};
+
+// One entry: the 64 j-atoms of the entry are gathered once (one lane per
+// j), shifted to the image nearest the cluster centre when SHIFT, and then
+// broadcast one per step with readlane.
+template <bool FEV, bool LJ, bool COUL, bool SHIFT>
+__device__ inline void hmd_pair_entry(const PairParams& p, const float2* s_lj,
+                                      HmdPairState& st, uint32_t slot, float cx,
+                                      float cy, float cz, uint32_t lane) {
+  const float4* ps = reinterpret_cast<const float4*>(p.posq_sorted);
+  const HmdLattice L = p.L;
+  const size_t ebase = (size_t)slot * HMD_ENTRY;
+  const uint32_t my_j = p.entry_j[ebase + lane];
+  const uint64_t my_ex = p.entry_excl[ebase + lane];
+  const uint32_t jj_l = (my_j != HMD_SENTINEL) ? my_j : 0u;
+  float4 pj = ps[jj_l];
+  const uint32_t tj_l = p.type_sorted[jj_l];
+  if (SHIFT) {
+    pj.x -= L.lx * floorf((pj.x - cx) * (1.0f / L.lx) + 0.5f);
+    pj.y -= L.ly * floorf((pj.y - cy) * (1.0f / L.ly) + 0.5f);
+    pj.z -= L.lz * floorf((pj.z - cz) * (1.0f / L.lz) + 0.5f);
+  }
+  const float c_gauss = -(p.alpha * p.alpha) * HMD_LOG2E;
+  const float two_alpha_sqrtpi = p.alpha * HMD_TWO_OVER_SQRT_PI;
+  long long* acc = p.acc;
+  const uint32_t stride = p.acc_stride;
+
+  for (uint32_t kb = 0; kb < HMD_ENTRY; kb += 8) {
+    if (hmd_readlane(my_j, kb) == HMD_SENTINEL) break;
+    float jfx[8], jfy[8], jfz[8], jfe[8], jfw[8];
+#pragma unroll
+    for (uint32_t u = 0; u < 8; ++u) {
+      const uint32_t k = kb + u;
+      const uint32_t j = hmd_readlane(my_j, k);              // wave-uniform
+      const float xj = hmd_readlane_f(pj.x, k);
+      const float yj = hmd_readlane_f(pj.y, k);
+      const float zj = hmd_readlane_f(pj.z, k);
+      float dx = st.xi - xj, dy = st.yi - yj, dz = st.zi - zj;
+      if (!SHIFT) hmd_min_image(dx, dy, dz, L);
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const bool excluded = (my_ex >> k) & 1ull;
+      const bool ok = (j != HMD_SENTINEL) && st.vi && (st.si < j) && (r2 <= p.rc2) && !excluded;
+      const float r2s = ok ? r2 : 1.0f;
+      const float inv_r = __builtin_amdgcn_rsqf(r2s);
+      const float inv_r2 = inv_r * inv_r;
+      float factor = 0.0f, energy = 0.0f;
+      if (COUL) {
+        const float qj = hmd_readlane_f(pj.w, k);
+        const float qq = st.kqi * qj;
+        const float ar = p.alpha * (r2s * inv_r);
+        const float gauss = __builtin_amdgcn_exp2f(r2s * c_gauss);
+        const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ar);
+        const float erfc_ar =
+            ((((1.061405429f * t - 1.453152027f) * t + 1.421413741f) * t -
+              0.284496736f) * t + 0.254829592f) * t * gauss;
+        const float qq_ir2 = qq * inv_r2;
+        factor = qq_ir2 * (erfc_ar * inv_r + two_alpha_sqrtpi * gauss);
+        if (FEV) energy = qq * inv_r * erfc_ar;
+      }
+      if (LJ) {
+        const uint32_t tj = hmd_readlane(tj_l, k);
+        if ((p.lj_type_mask >> tj) & 1u) {                  // uniform branch
+          const float2 cc = s_lj[st.ti * p.n_types + tj];
+          const float ir6 = inv_r2 * inv_r2 * inv_r2;
+          float e_lj = (cc.y * ir6 - cc.x) * ir6;
+          float f_lj = (12.0f * cc.y * ir6 - 6.0f * cc.x) * ir6 * inv_r2;
+          float tau = (r2s - p.lj_rs2) * p.lj_inv_delta;
+          tau = (r2s > p.lj_rs2) ? tau : 0.0f;
+          const float omt = 1.0f - tau;
+          const float sw = omt * omt * (1.0f + 2.0f * tau);
+          const float chain = 12.0f * tau * omt * p.lj_inv_delta;
+          f_lj = sw * f_lj + chain * e_lj;
+          factor += f_lj;
+          if (FEV) energy += sw * e_lj;
+        }
+      }
+      factor = ok ? factor : 0.0f;
+      const float fx = factor * dx, fy = factor * dy, fz = factor * dz;
+      st.fix += fx; st.fiy += fy; st.fiz += fz;
+      jfx[u] = -fx; jfy[u] = -fy; jfz[u] = -fz;
+      if (FEV) {
+        const float he = ok ? 0.5f * energy : 0.0f;
+        const float hw = 0.5f * factor * r2s;
+        st.eiv += he; st.wiv += hw;
+        jfe[u] = he; jfw[u] = hw;
+      }
+    }
+    const float rx = hmd_transpose_reduce8(jfx, lane);
+    const float ry = hmd_transpose_reduce8(jfy, lane);
+    const float rz = hmd_transpose_reduce8(jfz, lane);
+    float re = 0.0f, rw = 0.0f;
+    if (FEV) {
+      re = hmd_transpose_reduce8(jfe, lane);
+      rw = hmd_transpose_reduce8(jfw, lane);
+    }
+    const uint32_t jt = __shfl(my_j, (int)(kb + (lane >> 3)), HMD_WAVE);
+    if ((lane & 7u) == 0u && jt != HMD_SENTINEL) {
+      hmd_atomic_add_fixed(&acc[jt], hmd_to_fixed(rx, HMD_FIXED_SHIFT));
+      hmd_atomic_add_fixed(&acc[stride + jt], hmd_to_fixed(ry, HMD_FIXED_SHIFT));
+      hmd_atomic_add_fixed(&acc[2 * stride + jt], hmd_to_fixed(rz, HMD_FIXED_SHIFT));
+      if (FEV) {
+        hmd_atomic_add_fixed(&acc[3 * stride + jt], hmd_to_fixed(re, HMD_FIXED_SHIFT));
+        hmd_atomic_add_fixed(&acc[4 * stride + jt], hmd_to_fixed(rw, HMD_FIXED_SHIFT));
+      }
+    }
+  }
+}
+
 template <bool FEV>
+__device__ inline void hmd_flush_i(const PairParams& p, const HmdPairState& st) {
+  if (!st.vi) return;
+  long long* acc = p.acc;
+  const uint32_t stride = p.acc_stride, si = st.si;
+  hmd_atomic_add_fixed(&acc[si], hmd_to_fixed(st.fix, HMD_FIXED_SHIFT));
+  hmd_atomic_add_fixed(&acc[stride + si], hmd_to_fixed(st.fiy, HMD_FIXED_SHIFT));
+  hmd_atomic_add_fixed(&acc[2 * stride + si], hmd_to_fixed(st.fiz, HMD_FIXED_SHIFT));
+  if (FEV) {
+    hmd_atomic_add_fixed(&acc[3 * stride + si], hmd_to_fixed(st.eiv, HMD_FIXED_SHIFT));
+    hmd_atomic_add_fixed(&acc[4 * stride + si], hmd_to_fixed(st.wiv, HMD_FIXED_SHIFT));
+  }
+}
+
+template <bool FEV, bool LJ, bool COUL>
 __device__ inline void hmd_pair_body(const PairParams& p) {
   __shared__ float2 s_lj[HMD_MAX_TYPES * HMD_MAX_TYPES];
   const uint32_t nt2 = p.n_types * p.n_types;
@@ -71,37 +206,24 @@ __device__ inline void hmd_pair_body(const PairParams& p) {
   const float4* bb = reinterpret_cast<const float4*>(p.bbox);
   const HmdLattice L = p.L;
   const bool ortho = (L.xy == 0.0f) && (L.xz == 0.0f) && (L.yz == 0.0f);
-  const float ilx = 1.0f / L.lx, ily = 1.0f / L.ly, ilz = 1.0f / L.lz;
-  const float two_alpha_sqrtpi = p.alpha * HMD_TWO_OVER_SQRT_PI;
-  long long* acc = p.acc;
-  const uint32_t stride = p.acc_stride;
 
+  HmdPairState st{};
   uint32_t cur_c = HMD_SENTINEL;
-  uint32_t si = 0, ti = 0;
-  bool vi = false, shift = false;
-  float xi = 0, yi = 0, zi = 0, qi = 0, cx = 0, cy = 0, cz = 0;
-  float fix = 0, fiy = 0, fiz = 0, eiv = 0, wiv = 0;
-
+  bool shift = false;
+  float cx = 0, cy = 0, cz = 0;
   for (uint32_t w = w0; w < w1; ++w) {
     const uint32_t slot = p.work[w];
     const uint32_t c = slot / p.cap_entries;
     if (c != cur_c) {
-      if (cur_c != HMD_SENTINEL && vi) {
-        hmd_atomic_add_fixed(&acc[si], hmd_to_fixed(fix, HMD_FIXED_SHIFT));
-        hmd_atomic_add_fixed(&acc[stride + si], hmd_to_fixed(fiy, HMD_FIXED_SHIFT));
-        hmd_atomic_add_fixed(&acc[2 * stride + si], hmd_to_fixed(fiz, HMD_FIXED_SHIFT));
-        if (FEV) {
-          hmd_atomic_add_fixed(&acc[3 * stride + si], hmd_to_fixed(eiv, HMD_FIXED_SHIFT));
-          hmd_atomic_add_fixed(&acc[4 * stride + si], hmd_to_fixed(wiv, HMD_FIXED_SHIFT));
-        }
-      }
+      if (cur_c != HMD_SENTINEL) hmd_flush_i<FEV>(p, st);
       cur_c = c;
-      si = c * HMD_CLUSTER + lane;
-      vi = si < p.n;
-      float4 a = ps[si];
-      xi = a.x; yi = a.y; zi = a.z; qi = a.w;
-      ti = p.type_sorted[si];
-      float4 cb = bb[2 * c], hb = bb[2 * c + 1];
+      st.si = c * HMD_CLUSTER + lane;
+      st.vi = st.si < p.n;
+      const float4 a = ps[st.si];
+      st.xi = a.x; st.yi = a.y; st.zi = a.z;
+      st.kqi = p.k_coulomb * a.w;
+      st.ti = p.type_sorted[st.si];
+      const float4 cb = bb[2 * c], hb = bb[2 * c + 1];
       // Single-image fast path (compare jit_composed.rs:1396-1430): valid
       // when every atom that can interact with the cluster lies within half
       // a box length of its centre, so shifting i and j to the image nearest
@@ -112,120 +234,30 @@ __device__ inline void hmd_pair_body(const PairParams& p) {
               (hb.z + p.shift_margin < 0.5f * L.lz);
       cx = cb.x; cy = cb.y; cz = cb.z;
       if (shift) {
-        xi -= L.lx * floorf((xi - cx) * ilx + 0.5f);
-        yi -= L.ly * floorf((yi - cy) * ily + 0.5f);
-        zi -= L.lz * floorf((zi - cz) * ilz + 0.5f);
+        st.xi -= L.lx * floorf((st.xi - cx) * (1.0f / L.lx) + 0.5f);
+        st.yi -= L.ly * floorf((st.yi - cy) * (1.0f / L.ly) + 0.5f);
+        st.zi -= L.lz * floorf((st.zi - cz) * (1.0f / L.lz) + 0.5f);
       }
-      fix = fiy = fiz = eiv = wiv = 0.0f;
+      st.fix = st.fiy = st.fiz = st.eiv = st.wiv = 0.0f;
     }
-
-    const size_t ebase = (size_t)slot * HMD_ENTRY;
-    const uint32_t my_j = p.entry_j[ebase + lane];
-    const uint64_t my_ex = p.entry_excl[ebase + lane];
-
-    for (uint32_t kb = 0; kb < HMD_ENTRY; kb += 8) {
-      if (hmd_readlane(my_j, kb) == HMD_SENTINEL) break;
-      float jfx[8], jfy[8], jfz[8], jfe[8], jfw[8];
-#pragma unroll
-      for (uint32_t u = 0; u < 8; ++u) {
-        const uint32_t j = hmd_readlane(my_j, kb + u);     // wave-uniform
-        const bool jvalid = j != HMD_SENTINEL;
-        const uint32_t jj = jvalid ? j : 0u;
-        const float4 b = ps[jj];
-        const uint32_t tj = p.type_sorted[jj];
-        float xj = b.x, yj = b.y, zj = b.z;
-        float dx, dy, dz;
-        if (shift) {
-          xj -= L.lx * floorf((xj - cx) * ilx + 0.5f);
-          yj -= L.ly * floorf((yj - cy) * ily + 0.5f);
-          zj -= L.lz * floorf((zj - cz) * ilz + 0.5f);
-          dx = xi - xj; dy = yi - yj; dz = zi - zj;
-        } else {
-          dx = xi - xj; dy = yi - yj; dz = zi - zj;
-          hmd_min_image(dx, dy, dz, L);
-        }
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const bool excluded = (my_ex >> (kb + u)) & 1ull;
-        const bool ok = jvalid && vi && (si < j) && (r2 <= p.rc2) && !excluded;
-        const float r2s = ok ? r2 : 1.0f;
-        const float inv_r = __builtin_amdgcn_rsqf(r2s);
-        const float inv_r2 = inv_r * inv_r;
-        float factor = 0.0f, energy = 0.0f;
-        if (p.coul_on) {
-          const float r = r2s * inv_r;
-          const float qq = p.k_coulomb * qi * b.w;
-          const float ar = p.alpha * r;
-          const float gauss = __builtin_amdgcn_exp2f(-(ar * ar) * HMD_LOG2E);
-          const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ar);
-          const float erfc_ar =
-              ((((1.061405429f * t - 1.453152027f) * t + 1.421413741f) * t -
-                0.284496736f) * t + 0.254829592f) * t * gauss;
-          const float qq_ir = qq * inv_r;
-          factor = qq_ir * inv_r2 * erfc_ar + qq * inv_r2 * two_alpha_sqrtpi * gauss;
-          energy = qq_ir * erfc_ar;
-        }
-        if (p.lj_on) {
-          const float2 cc = s_lj[ti * p.n_types + tj];
-          const float ir6 = inv_r2 * inv_r2 * inv_r2;
-          float e_lj = (cc.y * ir6 - cc.x) * ir6;
-          float f_lj = (12.0f * cc.y * ir6 - 6.0f * cc.x) * ir6 * inv_r2;
-          float tau = (r2s - p.lj_rs2) * p.lj_inv_delta;
-          tau = (r2s > p.lj_rs2) ? tau : 0.0f;
-          const float omt = 1.0f - tau;
-          const float s = omt * omt * (1.0f + 2.0f * tau);
-          const float chain = 12.0f * tau * omt * p.lj_inv_delta;
-          f_lj = s * f_lj + chain * e_lj;
-          e_lj = s * e_lj;
-          factor += f_lj;
-          energy += e_lj;
-        }
-        factor = ok ? factor : 0.0f;
-        const float fx = factor * dx, fy = factor * dy, fz = factor * dz;
-        fix += fx; fiy += fy; fiz += fz;
-        jfx[u] = -fx; jfy[u] = -fy; jfz[u] = -fz;
-        if (FEV) {
-          const float he = ok ? 0.5f * energy : 0.0f;
-          const float hw = 0.5f * factor * r2s;
-          eiv += he; wiv += hw;
-          jfe[u] = he; jfw[u] = hw;
-        }
-      }
-      const float rx = hmd_transpose_reduce8(jfx, lane);
-      const float ry = hmd_transpose_reduce8(jfy, lane);
-      const float rz = hmd_transpose_reduce8(jfz, lane);
-      float re = 0.0f, rw = 0.0f;
-      if (FEV) {
-        re = hmd_transpose_reduce8(jfe, lane);
-        rw = hmd_transpose_reduce8(jfw, lane);
-      }
-      const uint32_t jt = __shfl(my_j, (int)(kb + (lane >> 3)), HMD_WAVE);
-      if ((lane & 7u) == 0u && jt != HMD_SENTINEL) {
-        hmd_atomic_add_fixed(&acc[jt], hmd_to_fixed(rx, HMD_FIXED_SHIFT));
-        hmd_atomic_add_fixed(&acc[stride + jt], hmd_to_fixed(ry, HMD_FIXED_SHIFT));
-        hmd_atomic_add_fixed(&acc[2 * stride + jt], hmd_to_fixed(rz, HMD_FIXED_SHIFT));
-        if (FEV) {
-          hmd_atomic_add_fixed(&acc[3 * stride + jt], hmd_to_fixed(re, HMD_FIXED_SHIFT));
-          hmd_atomic_add_fixed(&acc[4 * stride + jt], hmd_to_fixed(rw, HMD_FIXED_SHIFT));
-        }
-      }
-    }
+    if (shift)
+      hmd_pair_entry<FEV, LJ, COUL, true>(p, s_lj, st, slot, cx, cy, cz, lane);
+    else
+      hmd_pair_entry<FEV, LJ, COUL, false>(p, s_lj, st, slot, cx, cy, cz, lane);
   }
-  if (vi) {
-    hmd_atomic_add_fixed(&acc[si], hmd_to_fixed(fix, HMD_FIXED_SHIFT));
-    hmd_atomic_add_fixed(&acc[stride + si], hmd_to_fixed(fiy, HMD_FIXED_SHIFT));
-    hmd_atomic_add_fixed(&acc[2 * stride + si], hmd_to_fixed(fiz, HMD_FIXED_SHIFT));
-    if (FEV) {
-      hmd_atomic_add_fixed(&acc[3 * stride + si], hmd_to_fixed(eiv, HMD_FIXED_SHIFT));
-      hmd_atomic_add_fixed(&acc[4 * stride + si], hmd_to_fixed(wiv, HMD_FIXED_SHIFT));
-    }
-  }
+  hmd_flush_i<FEV>(p, st);
 }
 
-extern "C" __global__ void __launch_bounds__(256)
-hmd_pair_force_f(const PairParams p) { hmd_pair_body<false>(p); }
-
-extern "C" __global__ void __launch_bounds__(256)
-hmd_pair_force_fev(const PairParams p) { hmd_pair_body<true>(p); }
+#define HMD_PAIR_VARIANT(NAME, FEV, LJ, COUL)                                  \
+  extern "C" __global__ void __launch_bounds__(256) NAME(const PairParams p) { \
+    hmd_pair_body<FEV, LJ, COUL>(p);                                          \
+  }
+HMD_PAIR_VARIANT(hmd_pair_force_f, false, true, true)
+HMD_PAIR_VARIANT(hmd_pair_force_fev, true, true, true)
+HMD_PAIR_VARIANT(hmd_pair_force_lj_f, false, true, false)
+HMD_PAIR_VARIANT(hmd_pair_force_lj_fev, true, true, false)
+HMD_PAIR_VARIANT(hmd_pair_force_coul_f, false, false, true)
+HMD_PAIR_VARIANT(hmd_pair_force_coul_fev, true, false, true)
 
 // Fixed point -> f32, sorted -> canonical order, plus the slow-class (SPME)
 // forces: the reference's finalize_packed_forces + combine_class_totals

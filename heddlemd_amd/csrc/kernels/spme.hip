@@ -95,28 +95,60 @@ __device__ inline void hmd_spme_setup(float4 a, const HmdLattice& L,
   }
 }
 
-// Bin each atom by the brick holding its primary point; histogram.
-extern "C" __global__ void __launch_bounds__(256)
-hmd_spme_bin(const SpmeBinParams p) {
-  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= p.n) return;
-  float4 pq = reinterpret_cast<const float4*>(p.posq)[a];
+// Bin each atom by the brick holding its primary point.  Atoms are visited
+// in the neighbour list's Morton order, so a wave's 64 atoms fall into a few
+// bins: one atomic per (wave, bin) instead of one per atom.
+__device__ inline uint32_t hmd_spme_bin_of(const SpmeBinParams& p, float4 pq) {
   float t[3];
   int g[3];
   hmd_spme_setup(pq, p.L, p.n0, p.n1, p.n2, t, g);
-  uint32_t bin = ((uint32_t)g[0] / p.b0 * p.nb1 + (uint32_t)g[1] / p.b1) * p.nb2 +
-                 (uint32_t)g[2] / p.b2;
-  p.atom_bin[a] = bin;
-  atomicAdd(&p.bin_count[bin], 1u);
+  return ((uint32_t)g[0] / p.b0 * p.nb1 + (uint32_t)g[1] / p.b1) * p.nb2 +
+         (uint32_t)g[2] / p.b2;
 }
 
 extern "C" __global__ void __launch_bounds__(256)
+hmd_spme_bin(const SpmeBinParams p) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = s < p.n;
+  uint32_t bin = HMD_SENTINEL;
+  if (valid) {
+    bin = hmd_spme_bin_of(p, reinterpret_cast<const float4*>(p.posq_sorted)[s]);
+    p.atom_bin[s] = bin;
+  }
+  uint64_t pending = __ballot(valid);
+  while (pending) {
+    const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
+    const uint32_t b = hmd_readlane(bin, leader);
+    const uint64_t same = __ballot(valid && bin == b);
+    if (hmd_lane() == leader) atomicAdd(&p.bin_count[b], (uint32_t)__builtin_popcountll(same));
+    pending &= ~same;
+  }
+}
+
+// Scatter canonical atom ids into their bins.  The count array is driven
+// back to zero (one atomicSub per (wave, bin) claims a block of positions).
+extern "C" __global__ void __launch_bounds__(256)
 hmd_spme_bin_scatter(const SpmeBinParams p) {
-  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= p.n) return;
-  uint32_t bin = p.atom_bin[a];
-  uint32_t k = atomicSub(&p.bin_count[bin], 1u) - 1u;
-  p.bin_atoms[p.bin_start[bin] + k] = a;
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = s < p.n;
+  const uint32_t lane = hmd_lane();
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t bin = valid ? p.atom_bin[s] : HMD_SENTINEL;
+  uint64_t pending = __ballot(valid);
+  while (pending) {
+    const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
+    const uint32_t b = hmd_readlane(bin, leader);
+    const uint64_t same = __ballot(valid && bin == b);
+    const uint32_t cnt = (uint32_t)__builtin_popcountll(same);
+    uint32_t old = 0;
+    if (lane == leader) old = atomicSub(&p.bin_count[b], cnt);
+    old = hmd_readlane(old, leader);
+    if (valid && bin == b) {
+      const uint32_t rank = (uint32_t)__builtin_popcountll(same & lt_mask);
+      p.bin_atoms[p.bin_start[b] + old - cnt + rank] = p.sorted_id[s];
+    }
+    pending &= ~same;
+  }
 }
 
 template <int P>

@@ -151,6 +151,11 @@ hmd_status hmd_nbl_request_rebuild(hmd_forcefield* ff);
  * (unsorted).  *n_pairs receives the total even when it exceeds cap. */
 hmd_status hmd_nbl_pair_snapshot(hmd_forcefield* ff, uint32_t* pairs,
                                  uint64_t cap, uint64_t* n_pairs);
+/* Number of listed pairs whose exact f32 min-image distance is <= radius
+ * (radius <= r_c + r_skin); with radius = r_c this is the number of pairs
+ * the pair kernel evaluates inside the cutoff. */
+hmd_status hmd_nbl_count_pairs(hmd_forcefield* ff, double radius,
+                               uint64_t* n_pairs);
 /* out[0] entries in the work list, [1] rebuild generation, [2] status bits,
  * [3] high-water entries per cluster, [4] capacity per cluster,
  * [5] clusters, [6] fine cells per axis, [7] pad. */

@@ -1,0 +1,344 @@
+"""CPU tests that pin the FP64 oracle before it is trusted as the checker.
+
+Each check is independent of the oracle's own formulas: published
+known-answer vectors, closed forms, finite differences, invariants, numpy's
+FFT, and the reference's own fixtures (tests/spme_pipeline.rs,
+tests/forces_framework.rs).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from heddlemd_amd import units as U
+
+BOHR_PER_M = 1.0 / U.LENGTH_BOHR_TO_M
+
+
+# --------------------------------------------------------------- Philox ---
+@pytest.mark.parametrize("key,ctr,want", [
+    ((0, 0), (0, 0, 0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff, 0xffffffff), (0xffffffff,) * 4,
+     (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0xa4093822, 0x299f31d0), (0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+])
+def test_philox_random123_known_answers(key, ctr, want):
+    """Random123 kat_vectors for philox4x32 with 10 rounds."""
+    assert tuple(int(x) for x in oracle.philox4x32_10(key, ctr)) == want
+
+
+def test_philox_gaussian_moments():
+    xs = np.array([oracle.philox_gaussian(7, 0, s, 0, i, a)
+                   for s in range(4) for i in range(1500) for a in range(3)])
+    assert abs(xs.mean()) < 0.02
+    assert abs(xs.var() - 1.0) < 0.03
+    # Box-Muller maps the two leading words through (x + 1/2) 2^-32
+    o = oracle.philox4x32_10((7, 0), (3, 0, 11, 2))
+    u1, u2 = (o[0] + 0.5) / 2 ** 32, (o[1] + 0.5) / 2 ** 32
+    assert oracle.philox_gaussian(7, 0, 3, 0, 11, 2) == pytest.approx(
+        math.sqrt(-2 * math.log(u1)) * math.cos(2 * math.pi * u2), rel=1e-14)
+
+
+# ------------------------------------------------------------------ FFT ---
+@pytest.mark.parametrize("shape", [(6, 8, 10), (16, 16, 16), (24, 20, 30), (9, 5, 7)])
+def test_fft3d_matches_numpy(shape):
+    rng = np.random.default_rng(3)
+    a, b = rng.standard_normal(shape), rng.standard_normal(shape)
+    re, im = oracle.fft3d(a, b, -1)
+    ref = np.fft.fftn(a + 1j * b)
+    assert np.abs(re + 1j * im - ref).max() < 1e-10 * np.abs(ref).max()
+    re2, im2 = oracle.fft3d(re, im, +1)
+    assert np.abs((re2 + 1j * im2) / a.size - (a + 1j * b)).max() < 1e-12
+
+
+# ------------------------------------------------------------ B-splines ---
+@pytest.mark.parametrize("p", [4, 5, 6, 8])
+def test_bspline_partition_of_unity_and_symmetry(p):
+    for t in np.linspace(0, 0.999, 17):
+        assert sum(oracle.bspline(p, d + t) for d in range(p)) == pytest.approx(1.0, abs=1e-14)
+        assert oracle.bspline(p, t + 1.3) == pytest.approx(oracle.bspline(p, p - t - 1.3), abs=1e-14)
+
+
+def test_cubic_bspline_closed_form():
+    assert oracle.bspline(4, 1.0) == pytest.approx(1 / 6)
+    assert oracle.bspline(4, 2.0) == pytest.approx(2 / 3)
+    x = 0.37   # M_4(x) = x^3/6 on [0, 1)
+    assert oracle.bspline(4, x) == pytest.approx(x ** 3 / 6, abs=1e-15)
+
+
+def test_b_factors_cubic_closed_form():
+    """|sum_j M_4(j+1) e^{i theta j}|^2 = (2/3 + cos(theta)/3)^2 for p = 4."""
+    n = 16
+    got = oracle.b_factors(n, 4)
+    th = 2 * np.pi * np.arange(n) / n
+    assert np.allclose(got, 1.0 / (2 / 3 + np.cos(th) / 3) ** 2, rtol=1e-13)
+    assert got[0] == pytest.approx(1.0)
+
+
+def test_b_factors_odd_order_finite():
+    """Odd orders have a near-zero modulus at k = n/2; the neighbour average
+    keeps every factor finite (src/forces/spme.rs:545-554)."""
+    f = oracle.b_factors(16, 5)
+    assert np.all(np.isfinite(f)) and np.all(f > 0)
+
+
+# ------------------------------------------------------------------ SPME ---
+def _box(l_nm=1.0):
+    L = l_nm * 1e-9 * BOHR_PER_M
+    return L, np.array([L, L, L, 0, 0, 0])
+
+
+@pytest.mark.parametrize("positions,charges", [
+    ([[0.2e-9, 0, 0], [-0.2e-9, 0, 0]], [1.0, -1.0]),
+    ([[0.1e-9, 0, 0], [-0.1e-9, 0.2e-9, 0], [0, -0.2e-9, 0.15e-9],
+      [-0.15e-9, 0.1e-9, -0.1e-9]], [1.0, -1.0, 2.0, -2.0]),
+])
+@pytest.mark.parametrize("order", [4, 5, 6, 8])
+def test_spme_matches_explicit_ewald(positions, charges, order):
+    """tests/spme_pipeline.rs:101-189 and :875-913: L = 1 nm, alpha L = 4,
+    16^3 mesh, |m| <= 8, relative energy error < 5e-3."""
+    L, lat = _box()
+    alpha = 4.0e9 / BOHR_PER_M
+    pos = np.array(positions) * BOHR_PER_M
+    q = np.array(charges)
+    u, F, E, w = oracle.spme_recip(pos, q, lat, (16, 16, 16), order, alpha)
+    ud, Fd, Ed = oracle.ewald_recip_direct(pos, q, lat, alpha, 8)
+    assert abs(u - ud) / abs(ud) < 5e-3
+    assert np.abs(F - Fd).max() < 1e-2 * np.abs(Fd).max()
+
+
+def test_single_particle_spread_matches_bspline_closed_form():
+    """tests/spme_pipeline.rs:320-382."""
+    L, lat = _box()
+    n, p, q = 16, 4, 0.5
+    s = np.array([0.11, -0.18, 0.37])
+    rho = oracle.spme_spread((s * L)[None, :], [q], lat, (n, n, n), p)
+    u = (s + 0.5) * n
+    b = np.floor(u).astype(int)
+    t = u - b
+
+    def w(axis, g):
+        da = (b[axis] - g) % n
+        return oracle.bspline(p, da + t[axis]) if da < p else 0.0
+    ref = np.array([[[q * w(0, a) * w(1, bb) * w(2, c) for c in range(n)] for bb in range(n)]
+                    for a in range(n)])
+    assert np.abs(rho - ref).max() < 1e-14
+    assert rho.sum() == pytest.approx(q)
+
+
+def test_spread_zero_outside_support_and_conserves_charge():
+    """tests/spme_pipeline.rs:192-231 and :385-409."""
+    L, lat = _box()
+    rho = oracle.spme_spread(np.array([[0.2 * L, 0.2 * L, 0.2 * L]]), [1.0], lat, (16,) * 3, 4)
+    assert rho[0, 0, 0] == 0.0
+    pos = np.array([[0.1, 0.05, -0.2], [-0.1, 0.2, 0.15], [0.25, -0.1, 0.0]]) * L
+    q = [0.3, -0.5, 0.4]
+    assert oracle.spme_spread(pos, q, lat, (16,) * 3, 4).sum() == pytest.approx(sum(q), rel=1e-12)
+
+
+def test_direct_ewald_forces_are_energy_gradient():
+    rng = np.random.default_rng(5)
+    L, lat = _box()
+    alpha = 4.0e9 / BOHR_PER_M
+    pos = (rng.random((5, 3)) - 0.5) * L
+    q = np.array([1.0, -1.0, 0.5, -0.25, -0.25])
+    _, F, _ = oracle.ewald_recip_direct(pos, q, lat, alpha, 6)
+    h = 1e-5
+    for i, d in [(0, 0), (2, 1), (4, 2)]:
+        pp, pm = pos.copy(), pos.copy()
+        pp[i, d] += h
+        pm[i, d] -= h
+        ep = oracle.ewald_recip_direct(pp, q, lat, alpha, 6)[0]
+        em = oracle.ewald_recip_direct(pm, q, lat, alpha, 6)[0]
+        assert F[i, d] == pytest.approx(-(ep - em) / (2 * h), rel=1e-6, abs=1e-10)
+
+
+def test_spme_forces_are_energy_gradient():
+    rng = np.random.default_rng(6)
+    L, lat = _box(1.2)
+    alpha = 3.5 / (0.4e-9 * BOHR_PER_M)
+    pos = (rng.random((6, 3)) - 0.5) * L
+    q = np.array([1.0, -1.0, 0.5, -0.5, 0.3, -0.3])
+    _, F, _, _ = oracle.spme_recip(pos, q, lat, (20, 20, 20), 6, alpha)
+    h = 1e-5
+    for i, d in [(1, 0), (3, 2)]:
+        pp, pm = pos.copy(), pos.copy()
+        pp[i, d] += h
+        pm[i, d] -= h
+        ep = oracle.spme_recip(pp, q, lat, (20,) * 3, 6, alpha)[0]
+        em = oracle.spme_recip(pm, q, lat, (20,) * 3, 6, alpha)[0]
+        assert F[i, d] == pytest.approx(-(ep - em) / (2 * h), rel=1e-5, abs=1e-9)
+
+
+def test_spme_energy_is_half_sum_of_per_atom_shares():
+    rng = np.random.default_rng(8)
+    L, lat = _box()
+    alpha = 4.0e9 / BOHR_PER_M
+    pos = (rng.random((8, 3)) - 0.5) * L
+    q = rng.standard_normal(8)
+    q -= q.mean()
+    u, F, E, w = oracle.spme_recip(pos, q, lat, (16,) * 3, 4, alpha)
+    u_self = alpha / math.sqrt(math.pi) * np.sum(q ** 2)
+    assert E.sum() == pytest.approx(u - u_self, rel=1e-10)
+
+
+# ---------------------------------------------------------- pair forces ---
+def test_lj_cross_term_known_answer():
+    """tests/forces_framework.rs:590-627: r = 1.5, sigma = 1.3, eps = 2, no
+    switch; F = 24 eps / r (2 (s/r)^12 - (s/r)^6), equal and opposite."""
+    r, sig, eps = 1.5, 1.3, 2.0
+    ps = oracle.PairSystem([[0, 0, 0], [r, 0, 0]], [0, 0], [0, 1], [40, 40, 40, 0, 0, 0],
+                           lj={"sigma": [sig] * 4, "epsilon": [eps] * 4,
+                               "cutoff": [10.0] * 4, "switch": [10.0] * 4})
+    F, E, W = ps.forces()
+    want = 24 * eps / r * (2 * (sig / r) ** 12 - (sig / r) ** 6)
+    assert F[1, 0] == pytest.approx(want, rel=1e-12)
+    assert F[0, 0] == pytest.approx(-want, rel=1e-12)
+    assert E.sum() == pytest.approx(4 * eps * ((sig / r) ** 12 - (sig / r) ** 6), rel=1e-12)
+
+
+def _random_fluid(n=64, L=30.0, seed=1):
+    """Jittered simple-cubic lattice (no overlapping pairs, so the energy
+    finite differences are well conditioned)."""
+    rng = np.random.default_rng(seed)
+    k = int(round(n ** (1 / 3)))
+    g = np.stack(np.meshgrid(*[np.arange(k)] * 3, indexing="ij"), -1).reshape(-1, 3)[:n]
+    pos = (g + 0.5) * (L / k) - 0.5 * L + rng.uniform(-0.8, 0.8, (n, 3))
+    q = rng.choice([-0.8, 0.4], n)
+    types = rng.integers(0, 2, n)
+    lj = {"sigma": [3.0, 2.5, 2.5, 2.0], "epsilon": [1e-3, 5e-4, 5e-4, 0.0],
+          "cutoff": [9.0] * 4, "switch": [8.0] * 4}
+    return pos, q, types, [L, L, L, 0, 0, 0], lj
+
+
+def test_pair_forces_are_energy_gradient_with_switch_and_erfc():
+    pos, q, types, lat, lj = _random_fluid()
+    co = {"alpha": 0.35, "r_cut": 9.0}
+    h = 1e-6
+    base = oracle.PairSystem(pos, q, types, lat, lj=lj, coulomb=co)
+    F, _, _ = base.forces()
+    for i, d in [(0, 0), (7, 1), (13, 2)]:
+        pp, pm = pos.copy(), pos.copy()
+        pp[i, d] += h
+        pm[i, d] -= h
+        ep = oracle.PairSystem(pp, q, types, lat, lj=lj, coulomb=co).forces()[1].sum()
+        em = oracle.PairSystem(pm, q, types, lat, lj=lj, coulomb=co).forces()[1].sum()
+        assert F[i, d] == pytest.approx(-(ep - em) / (2 * h), rel=1e-5, abs=1e-9)
+
+
+def test_pair_forces_newton3_and_cells_match_brute_force():
+    pos, q, types, lat, lj = _random_fluid(n=343, L=32.0)
+    co = {"alpha": 0.35, "r_cut": 9.0}
+    ps = oracle.PairSystem(pos, q, types, lat, lj=lj, coulomb=co)
+    F, E, W = ps.forces()
+    Fc, Ec, Wc = ps.forces(cells=True)
+    assert np.abs(F.sum(axis=0)).max() < 1e-12 * np.abs(F).max() * len(F)
+    assert np.allclose(F, Fc, rtol=1e-12, atol=1e-15)
+    assert np.allclose(E, Ec, rtol=1e-12, atol=1e-15)
+
+
+def test_erfc_approximation_error_bound():
+    """A&S 7.1.26: |erfc_AS - erfc| <= 1.5e-7; the oracle's two modes differ
+    by at most that, scaled by the Coulomb factor."""
+    pos = np.array([[0, 0, 0], [0, 0, 0]], float)
+    for r in [0.5, 1.0, 2.0, 4.0, 8.0]:
+        pos[1, 0] = r
+        co = {"alpha": 0.4, "r_cut": 9.0}
+        e_as = oracle.PairSystem(pos, [1, -1], [0, 0], [40] * 3 + [0] * 3, coulomb=co,
+                                 erfc_mode=1).forces()[1].sum()
+        e_ex = oracle.PairSystem(pos, [1, -1], [0, 0], [40] * 3 + [0] * 3, coulomb=co,
+                                 erfc_mode=0).forces()[1].sum()
+        assert e_ex == pytest.approx(-math.erfc(0.4 * r) / r, rel=1e-13)
+        assert abs(e_as - e_ex) <= 1.5e-7 / r
+
+
+def test_exclusions_zero_the_pair():
+    """A listed partner contributes nothing (scale 0); every other pair is
+    untouched: the three-atom energy equals the two non-excluded two-atom
+    energies."""
+    pos = np.array([[0, 0, 0], [1.8, 0, 0], [0, 2.5, 0]], float)
+    q = [-0.8, 0.4, 0.4]
+    box = [40] * 3 + [0] * 3
+    co = {"alpha": 0.3, "r_cut": 9.0}
+    ex = ([0, 1, 2, 2], [1, 0], [0.0, 0.0], [0.0, 0.0])
+    F, E, W = oracle.PairSystem(pos, q, [0, 1, 1], box, coulomb=co, exclusions=ex).forces()
+
+    def two(a, b):
+        return oracle.PairSystem(pos[[a, b]], [q[a], q[b]], [0, 0], box, coulomb=co).forces()
+    F02, E02, _ = two(0, 2)
+    F12, E12, _ = two(1, 2)
+    assert E.sum() == pytest.approx(E02.sum() + E12.sum(), rel=1e-14)
+    assert np.allclose(F[0], F02[0], rtol=1e-14)
+    assert np.allclose(F[1], F12[0], rtol=1e-14)
+    assert np.allclose(F[2], F02[1] + F12[1], rtol=1e-14)
+
+
+# ---------------------------------------------------------- pair set -------
+def test_pair_set_agrees_with_f64_away_from_the_boundary():
+    rng = np.random.default_rng(2)
+    L = 30.0
+    posq = np.zeros((400, 4), np.float32)
+    posq[:, :3] = (rng.random((400, 3)) - 0.5) * L
+    lat = np.array([L, L, L, 0, 0, 0], np.float32)
+    rs = np.float32(9.5)
+    pairs = oracle.pair_set_f32(posq, lat, rs * rs)
+    got = set(map(tuple, pairs.tolist()))
+    p = posq[:, :3].astype(np.float64)
+    d = p[:, None, :] - p[None, :, :]
+    d -= L * np.floor(d / L + 0.5)
+    r2 = np.sum(d * d, axis=2)
+    i, j = np.triu_indices(400, 1)
+    margin = 1e-4 * float(rs) ** 2
+    inside = r2[i, j] < float(rs) ** 2 - margin
+    outside = r2[i, j] > float(rs) ** 2 + margin
+    assert all((a, b) in got for a, b in zip(i[inside], j[inside]))
+    assert not any((a, b) in got for a, b in zip(i[outside], j[outside]))
+    assert all(a < b for a, b in got)
+
+
+# ---------------------------------------------------------------- SETTLE --
+def _water(rng):
+    d_oh, ang = 1.8897261, math.radians(109.47)
+    loc = np.array([[0, 0, 0], [d_oh * math.sin(ang / 2), d_oh * math.cos(ang / 2), 0],
+                    [-d_oh * math.sin(ang / 2), d_oh * math.cos(ang / 2), 0]])
+    A = np.linalg.qr(rng.standard_normal((3, 3)))[0]
+    return loc @ A.T + rng.standard_normal(3) * 5, d_oh, 2 * d_oh * math.sin(ang / 2)
+
+
+def test_settle_position_reset_restores_rigidity_and_com():
+    rng = np.random.default_rng(4)
+    m_o, m_h = 15.9994 * U.AMU, 1.008 * U.AMU
+    lat = [60, 60, 60, 0, 0, 0]
+    for _ in range(5):
+        snap, d_oh, d_hh = _water(rng)
+        geom = oracle.settle_geometry(d_oh, d_hh, m_o, m_h)
+        cur = snap + rng.standard_normal((3, 3)) * 0.05
+        v = np.zeros((3, 3))
+        new, v2, vir = oracle.settle_positions_one(snap, cur, v, m_o, m_h, geom, 80.0, lat)
+        dist = [np.linalg.norm(new[1] - new[0]), np.linalg.norm(new[2] - new[0]),
+                np.linalg.norm(new[2] - new[1])]
+        assert dist == pytest.approx([d_oh, d_oh, d_hh], rel=1e-10)
+        m = np.array([m_o, m_h, m_h])[:, None]
+        assert np.allclose((m * new).sum(0), (m * cur).sum(0), rtol=1e-12)
+        # corrections change momentum by zero in total
+        assert np.abs((m * v2).sum(0)).max() < 1e-12
+
+
+def test_settle_velocity_projection_zeroes_bond_velocities():
+    rng = np.random.default_rng(9)
+    m_o, m_h = 15.9994 * U.AMU, 1.008 * U.AMU
+    pos, _, _ = _water(rng)
+    v = rng.standard_normal((3, 3)) * 1e-3
+    v2 = oracle.settle_velocities_one(pos, v, m_o, m_h, [60, 60, 60, 0, 0, 0])
+    for a, b in [(0, 1), (0, 2), (1, 2)]:
+        assert abs(np.dot(v2[a] - v2[b], pos[a] - pos[b])) < 1e-14
+    m = np.array([m_o, m_h, m_h])[:, None]
+    assert np.allclose((m * v2).sum(0), (m * v).sum(0), rtol=1e-12, atol=1e-18)
+
+
+def test_kinetic_energy_closed_form():
+    v = np.array([[1.0, 2.0, 3.0], [0.5, 0, 0]])
+    m = np.array([2.0, 4.0])
+    assert oracle.kinetic_energy(v, m) == pytest.approx(0.5 * 2 * 14 + 0.5 * 4 * 0.25)
