@@ -91,6 +91,9 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--eager", action="store_true",
+                    help="time eager launches instead of graph replays (for kernel "
+                         "tracers that do not follow graph launches)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -102,12 +105,14 @@ def main():
     t_setup = time.perf_counter()
     ctx, sysw, ff, part, field, integ = build_system(api, systems, U)
     log(f"setup {time.perf_counter() - t_setup:.1f} s")
-    integ.capture()
-    integ.replay(args.warmup)
+    run = integ.step if args.eager else integ.replay
+    if not args.eager:
+        integ.capture()
+    run(args.warmup)
     ctx.synchronize()
     gen0 = field.stats()["generation"]
     t0 = time.perf_counter()
-    integ.replay(args.steps)
+    run(args.steps)
     ctx.synchronize()
     el = time.perf_counter() - t0
     gen1 = field.stats()["generation"]
@@ -144,6 +149,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "plain SPME 128^3, 196,608-atom rigid SPC water, compensated "
                                "VV NVE 300 K, dt 2 fs (the NN-corrected 64^3 leg is not built yet)",
+                   "launch": "eager" if args.eager else "hipGraph replay",
                    "atoms": int(sysw["n"]), "mesh": 128, "r_cut_nm": 1.0, "r_skin_nm": 0.1,
                    "atom_steps_per_s": round(sysw["n"] * args.steps / el, 1),
                    "rebuilds": int(gen1 - gen0)},
