@@ -35,6 +35,11 @@ EXPORTED = [
     "hmd_integrator_capture", "hmd_integrator_replay", "hmd_integrator_steps",
     "hmd_integrator_profile", "hmd_integrator_minimize",
     "hmd_generate_velocities", "hmd_chacha_block",
+    "hmd_mlp_param_count", "hmd_mlp_create", "hmd_mlp_destroy", "hmd_mlp_set_params",
+    "hmd_mlp_get_params", "hmd_mlp_set_force_scale", "hmd_mlp_descriptors",
+    "hmd_mlp_forward", "hmd_mlp_loss_grad", "hmd_mlp_adam_step", "hmd_mlp_attach",
+    "hmd_mlp_dataset_reserve", "hmd_mlp_record_frame", "hmd_mlp_dataset_stats",
+    "hmd_mlp_train", "hmd_mlp_judge",
 ]
 
 
@@ -66,6 +71,18 @@ class IntegratorDesc(ctypes.Structure):
         ("group_atoms", _c_u32p),
         ("d_oh", ctypes.c_double), ("d_hh", ctypes.c_double),
         ("m_o", ctypes.c_double), ("m_h", ctypes.c_double),
+    ]
+
+
+class MlpDesc(ctypes.Structure):
+    _fields_ = [
+        ("r_cut", ctypes.c_double),
+        ("force_scale", ctypes.c_double),
+        ("n_types", ctypes.c_uint32),
+        ("species_of_type", _c_u32p),
+        ("seed", ctypes.c_uint64),
+        ("lr", ctypes.c_double), ("beta1", ctypes.c_double),
+        ("beta2", ctypes.c_double), ("eps", ctypes.c_double),
     ]
 
 
@@ -128,6 +145,22 @@ def _declare(L):
         "hmd_generate_velocities": (st, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
                                          ctypes.c_uint64, _c_dp, _c_fp]),
         "hmd_chacha_block": (None, [_c_u32p, ctypes.c_uint32, _c_u32p]),
+        "hmd_mlp_param_count": (ctypes.c_uint32, []),
+        "hmd_mlp_create": (st, [vp, ctypes.POINTER(MlpDesc), pvp]),
+        "hmd_mlp_destroy": (None, [vp]),
+        "hmd_mlp_set_params": (st, [vp, _c_fp]),
+        "hmd_mlp_get_params": (st, [vp, _c_fp]),
+        "hmd_mlp_set_force_scale": (st, [vp, ctypes.c_double]),
+        "hmd_mlp_descriptors": (st, [vp, vp, _c_fp]),
+        "hmd_mlp_forward": (st, [vp, vp, _c_fp]),
+        "hmd_mlp_loss_grad": (st, [vp, vp, _c_fp, _c_dp, _c_fp]),
+        "hmd_mlp_adam_step": (st, [vp, _c_fp]),
+        "hmd_mlp_attach": (st, [vp, vp]),
+        "hmd_mlp_dataset_reserve": (st, [vp, ctypes.c_uint32, ctypes.c_uint32]),
+        "hmd_mlp_record_frame": (st, [vp, vp, vp]),
+        "hmd_mlp_dataset_stats": (st, [vp, _c_dp]),
+        "hmd_mlp_train": (st, [vp, vp, ctypes.c_uint32, ctypes.c_uint32, _c_dp]),
+        "hmd_mlp_judge": (st, [vp, vp, _c_dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

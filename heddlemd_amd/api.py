@@ -5,7 +5,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import ForceFieldDesc, IntegratorDesc, check, lib
+from ._lib import ForceFieldDesc, IntegratorDesc, MlpDesc, check, lib
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -278,6 +278,99 @@ class Integrator:
         if self.h:
             lib().hmd_integrator_destroy(self.h)
             self.h = None
+
+
+class CorrectionNet:
+    """Pair-scalar network correcting a coarse SPME mesh toward a fine one
+    (the learned part of the hot path; no counterpart in the reference).
+    Parameters are 7553 floats: W1[64][51], b1, W2[64][64], b2, w3, b3."""
+
+    N_PARAMS = 7553
+
+    def __init__(self, ctx, r_cut, force_scale, species_of_type, seed=31, lr=1e-3):
+        sp = np.ascontiguousarray(species_of_type, np.uint32)
+        d = MlpDesc()
+        d.r_cut = r_cut
+        d.force_scale = force_scale
+        d.n_types = sp.size
+        d.species_of_type = _ptr(sp, _u32p)
+        d.seed = seed
+        d.lr = lr
+        h = ctypes.c_void_p()
+        check(lib().hmd_mlp_create(ctx.h, ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def params(self):
+        out = np.empty(self.N_PARAMS, np.float32)
+        check(lib().hmd_mlp_get_params(self.h, _ptr(out, _fp)))
+        return out
+
+    @params.setter
+    def params(self, values):
+        v = np.ascontiguousarray(values, np.float32)
+        assert v.size == self.N_PARAMS
+        check(lib().hmd_mlp_set_params(self.h, _ptr(v, _fp)))
+
+    def set_force_scale(self, s):
+        check(lib().hmd_mlp_set_force_scale(self.h, s))
+
+    def descriptors(self, ff):
+        out = np.empty((ff.particles.n, 16), np.float32)
+        check(lib().hmd_mlp_descriptors(self.h, ff.h, _ptr(out, _fp)))
+        return out
+
+    def forward(self, ff):
+        out = np.empty((ff.particles.n, 3), np.float32)
+        check(lib().hmd_mlp_forward(self.h, ff.h, _ptr(out, _fp)))
+        return out
+
+    def loss_grad(self, ff, target):
+        t = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+        loss = ctypes.c_double()
+        g = np.empty(self.N_PARAMS, np.float32)
+        check(lib().hmd_mlp_loss_grad(self.h, ff.h, _ptr(t, _fp), ctypes.byref(loss),
+                                      _ptr(g, _fp)))
+        return loss.value, g
+
+    def adam_step(self, grad):
+        g = np.ascontiguousarray(grad, np.float32)
+        check(lib().hmd_mlp_adam_step(self.h, _ptr(g, _fp)))
+
+    def attach(self, ff):
+        check(lib().hmd_mlp_attach(ff.h, self.h))
+
+    @staticmethod
+    def detach(ff):
+        check(lib().hmd_mlp_attach(ff.h, None))
+
+    def reserve(self, n_atoms, n_frames):
+        check(lib().hmd_mlp_dataset_reserve(self.h, n_atoms, n_frames))
+
+    def record_frame(self, fine, coarse):
+        check(lib().hmd_mlp_record_frame(self.h, fine.h, coarse.h))
+
+    def dataset_stats(self):
+        out = np.zeros(3)
+        check(lib().hmd_mlp_dataset_stats(self.h, _ptr(out, _dp)))
+        return {"frames": int(out[0]), "target_rms": out[1], "fine_recip_rms": out[2]}
+
+    def train(self, ff, n_steps, batch_frames=8):
+        losses = np.zeros(n_steps)
+        check(lib().hmd_mlp_train(self.h, ff.h, n_steps, batch_frames, _ptr(losses, _dp)))
+        return losses
+
+    def close(self):
+        if self.h:
+            lib().hmd_mlp_destroy(self.h)
+            self.h = None
+
+
+def judge(nn_ff, fine_ff):
+    """Fallback judge: ||F_nn - F_fine|| / ||F_fine|| at the current positions."""
+    r = ctypes.c_double()
+    check(lib().hmd_mlp_judge(nn_ff.h, fine_ff.h, ctypes.byref(r)))
+    return r.value
 
 
 def generate_velocities(masses, temperature, seed, n_constraints=0):

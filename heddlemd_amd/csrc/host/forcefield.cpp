@@ -567,6 +567,7 @@ void ForceField::compute(int level) {
   PairFinalizeParams fp{};
   pair_forces(fev);
   if (spme_on) spme_forces(fev);
+  if (mlp) mlp_correct(*mlp, *this);
   fp.acc = acc.ptr;
   fp.acc_stride = n_pad;
   fp.slot_of = slot_of.ptr;

@@ -4,6 +4,8 @@
 
 namespace hmd {
 
+struct Mlp;
+
 struct ForceField {
   Context& c;
   Particles& p;
@@ -46,6 +48,9 @@ struct ForceField {
   std::vector<uint32_t> rad_a, rad_b, rad_c;
   double u_self_total = 0.0;
 
+  // ---- learned correction of the reciprocal forces (optional)
+  Mlp* mlp = nullptr;
+
   ForceField(Context& ctx, Particles& part, const hmd_forcefield_desc& d);
 
   void init_neighbor_list(const hmd_forcefield_desc& d);
@@ -62,6 +67,9 @@ struct ForceField {
   void compute(int level);
   void check_status();
 };
+
+// Adds the network's correction to the slow-class forces (enqueue only).
+void mlp_correct(Mlp& m, ForceField& ff);
 
 }  // namespace hmd
 

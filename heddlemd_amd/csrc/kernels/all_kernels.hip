@@ -7,3 +7,4 @@
 #include "spme.hip"
 #include "fft.hip"
 #include "integrate.hip"
+#include "nn.hip"

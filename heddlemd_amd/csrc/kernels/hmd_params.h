@@ -313,3 +313,101 @@ struct SdParams {              // capped steepest-descent move (setup only)
   float h;                     // step per unit force
   float max_step;              // cap on |dx| per atom
 };
+
+// ------------------------------------------- learned mesh correction ----
+#define HMD_NN_RBF 16u               // radial basis functions
+#define HMD_NN_HID 64u               // hidden width
+#define HMD_NN_IN 51u                // 16 rbf + 3 one-hot + 2 x 16 descriptors
+#define HMD_NN_PARAMS 7553u          // W1 64x51, b1, W2 64x64, b2, w3, b3
+#define HMD_NN_DESC_SHIFT 32         // descriptor accumulators: value * 2^32
+#define HMD_NN_PARTIAL 11328u        // per-wave gradient partial (floats)
+#define HMD_NN_OPERAND_BYTES 8192u   // one packed 64x64 bf16 operand set
+
+// Packed network (written by hmd_nn_pack from the f32 master parameters):
+// three matrix-core operand sets of 4 x 2 x 64 lanes x 8 bf16, then f32
+// b1[64], b2[64], w3[64] (bf16-rounded values), b3.
+struct NnPackParams {
+  const float* master;         // [HMD_NN_PARAMS] canonical
+  uint16_t* ops;               // [3][4][2][64][8] bf16 bits: W1, W2, W2^T
+  float* vec;                  // [64*3 + 1]
+};
+
+// Shared by the descriptor, forward and backward passes: the packed
+// neighbour list of the force field plus the network's per-atom tables.
+struct NnPassParams {
+  const float* posq_sorted;    // float4[n_pad]
+  const uint32_t* type_sorted; // [n_pad]
+  const uint32_t* work;
+  const uint32_t* n_work;
+  const uint32_t* entry_j;
+  HmdLattice L;
+  uint32_t n, cap_entries, n_pad;
+  uint32_t species_mask;       // bit t = species (0/1) of type t
+  float r_cut2, inv_r_cut;     // r_nn^2, 1/r_nn
+  float force_scale;           // s_out
+  long long* desc_acc;         // [16][n_pad] fixed point (descriptor pass)
+  const float* desc;           // [n_pad][16] sorted (forward/backward)
+  const uint16_t* ops;         // packed operands (forward/backward)
+  const float* vec;            // b1, b2, w3, b3
+  long long* force_acc;        // [3][n_pad] fixed point dF (forward)
+  const float* resid;          // [3][n_pad] dL/d(dF), sorted (backward)
+  float* partials;             // [n_waves][HMD_NN_PARTIAL] (backward)
+};
+
+struct NnDescFinalizeParams {
+  long long* desc_acc;         // [16][n_pad], cleared after reading
+  uint32_t n_pad, n;
+  const uint32_t* sorted_id;   // sorted slot -> canonical id
+  float* desc;                 // [n_pad][16] sorted
+  float* desc_canon;           // [n][16] canonical (optional)
+};
+
+struct NnFinalizeParams {
+  long long* force_acc;        // [3][n_pad], cleared after reading
+  uint32_t n_pad, n;
+  const uint32_t* slot_of;     // canonical -> sorted
+  float* df;                   // [3][n] canonical
+  float* add_to;               // [3][n] canonical (optional): += dF
+};
+
+struct NnResidualParams {
+  const float* df;             // [3][n] canonical
+  const float* target;         // [3][n] canonical
+  const uint32_t* slot_of;
+  uint32_t n, n_pad;
+  float scale;                 // 2 / (3 n s_out^2 B)
+  double loss_scale;           // 1 / (3 n s_out^2 B)
+  float* resid;                // [3][n_pad] sorted
+  double* partials;            // [blocks]
+};
+
+struct NnGradReduceParams {
+  const float* partials;       // [n_waves][HMD_NN_PARTIAL]
+  uint32_t n_waves;
+  double* grad;                // [HMD_NN_PARAMS] accumulated (+=)
+};
+
+struct NnAdamParams {
+  float* master;               // [HMD_NN_PARAMS]
+  float* m;
+  float* v;
+  double* grad;                // cleared after the update
+  float lr, beta1, beta2, eps;
+  float bc1, bc2;              // 1 - beta^t
+};
+
+struct NnRecordParams {
+  const float* posq;           // canonical float4[n]
+  const float* fine_f;         // [3][n] fine-mesh reciprocal forces
+  const float* coarse_f;       // [3][n] coarse-mesh reciprocal forces
+  uint32_t n;
+  float* frame_posq;           // float4[n]
+  float* frame_target;         // [3][n]
+};
+
+struct NnDiffParams {          // sum |a - b|^2 and |b|^2 (FP64 partials)
+  const float* a;              // [3][n]
+  const float* b;              // [3][n]
+  uint32_t n;
+  double* partials;            // [2][blocks]: |a - b|^2, then |b|^2
+};

@@ -27,36 +27,6 @@
 #define HMD_TWO_OVER_SQRT_PI 1.1283791670955126f
 #define HMD_LOG2E 1.4426950408889634f
 
-// Sum v[0..7] over the wave; lane l returns the total of v[l >> 3].
-__device__ inline float hmd_transpose_reduce8(const float v[8], uint32_t lane) {
-  const bool b5 = lane & 32u, b4 = lane & 16u, b3 = lane & 8u;
-  float a[4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    float send = b5 ? v[t] : v[t + 4];
-    float keep = b5 ? v[t + 4] : v[t];
-    a[t] = keep + hmd_shfl_xor(send, 32);
-  }
-  float b[2];
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    float send = b4 ? a[t] : a[t + 2];
-    float keep = b4 ? a[t + 2] : a[t];
-    b[t] = keep + hmd_shfl_xor(send, 16);
-  }
-  float send = b3 ? b[0] : b[1];
-  float keep = b3 ? b[1] : b[0];
-  float c = keep + hmd_shfl_xor(send, 8);
-  c += hmd_shfl_xor(c, 4);
-  c += hmd_shfl_xor(c, 2);
-  c += hmd_shfl_xor(c, 1);
-  return c;
-}
-
-__device__ inline float hmd_readlane_f(float v, uint32_t lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)lane));
-}
-
 struct HmdPairState {
   float xi, yi, zi, kqi;
   uint32_t si, ti;

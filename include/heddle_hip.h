@@ -212,6 +212,70 @@ hmd_status hmd_generate_velocities(uint32_t n, uint32_t n_constraints,
 void hmd_chacha_block(const uint32_t state[16], uint32_t rounds,
                       uint32_t out[16]);
 
+/* ---- learned coarse-mesh correction --------------------------------------
+ * No counterpart in the reference: this is the hmd_mlp_{create, forward,
+ * backward, adam_step} part of the hot path's C ABI, for the pair-scalar
+ * network that corrects a coarse SPME mesh toward a fine one.  Model: for
+ * every pair within r_cut, x_ij = [16 Gaussian radial basis values of
+ * r_ij/r_cut, one-hot(species_i + species_j), D_i, D_j] with the per-atom
+ * charge-weighted radial descriptor D_i = sum_j q_j phi(r_ij) fc(r_ij);
+ * N = 51 -> 64 -> 64 -> 1 with SiLU (bf16 weights and activations on the
+ * matrix cores, f32 accumulation); g_ij = force_scale fc(r_ij)
+ * (N(x_ij) + N(x_ji))/2 and dF_i = sum_j g_ij (r_i - r_j)/r_ij.
+ * Parameters (hmd_mlp_param_count() = 7553 floats): W1[64][51], b1[64],
+ * W2[64][64], b2[64], w3[64], b3. */
+typedef struct hmd_mlp hmd_mlp;
+typedef struct {
+  double r_cut;                     /* pair range (Bohr), <= r_c + r_skin */
+  double force_scale;               /* output scale (Hartree/Bohr) */
+  uint32_t n_types;                 /* <= 32 */
+  const uint32_t* species_of_type;  /* 0 or 1 per type */
+  uint64_t seed;                    /* parameter init and frame order */
+  double lr, beta1, beta2, eps;     /* Adam; 0 = 1e-3, 0.9, 0.999, 1e-8 */
+} hmd_mlp_desc;
+
+uint32_t hmd_mlp_param_count(void);
+hmd_status hmd_mlp_create(hmd_ctx* ctx, const hmd_mlp_desc* desc,
+                          hmd_mlp** out);
+void hmd_mlp_destroy(hmd_mlp* mlp);
+hmd_status hmd_mlp_set_params(hmd_mlp* mlp, const float* params);
+hmd_status hmd_mlp_get_params(hmd_mlp* mlp, float* params);
+hmd_status hmd_mlp_set_force_scale(hmd_mlp* mlp, double force_scale);
+/* Descriptors ([n][16]) of ff's particles at their current positions. */
+hmd_status hmd_mlp_descriptors(hmd_mlp* mlp, hmd_forcefield* ff, float* out);
+/* Force correction ([n][3]) at the current positions. */
+hmd_status hmd_mlp_forward(hmd_mlp* mlp, hmd_forcefield* ff, float* dforce);
+/* Loss sum_i |dF_i - T_i|^2 / (3 n force_scale^2) of one frame (the current
+ * positions, target [n][3]) and its gradient (param_count floats). */
+hmd_status hmd_mlp_loss_grad(hmd_mlp* mlp, hmd_forcefield* ff,
+                             const float* target, double* loss, float* grad);
+/* One Adam step (bias-corrected) with the given gradient. */
+hmd_status hmd_mlp_adam_step(hmd_mlp* mlp, const float* grad);
+/* Add the correction to ff's reciprocal (slow-class) forces in every
+ * hmd_forcefield_compute, eager or graph-replayed; NULL detaches. */
+hmd_status hmd_mlp_attach(hmd_forcefield* ff, hmd_mlp* mlp);
+/* Training frames (device resident): positions and the target
+ * F_recip(fine) - F_recip(coarse) of two force fields on the same
+ * particles at their current positions. */
+hmd_status hmd_mlp_dataset_reserve(hmd_mlp* mlp, uint32_t n_atoms,
+                                   uint32_t n_frames);
+hmd_status hmd_mlp_record_frame(hmd_mlp* mlp, hmd_forcefield* fine,
+                                hmd_forcefield* coarse);
+/* out[0] frames, [1] RMS target component, [2] RMS fine reciprocal
+ * force component. */
+hmd_status hmd_mlp_dataset_stats(hmd_mlp* mlp, double out[3]);
+/* n_steps Adam steps over batches of recorded frames (seeded shuffle per
+ * epoch), forward and backward passes on the GPU; ff (the coarse force
+ * field) is the scratch system: its particle positions are overwritten.
+ * losses[n_steps] receives the batch losses (may be NULL). */
+hmd_status hmd_mlp_train(hmd_mlp* mlp, hmd_forcefield* ff, uint32_t n_steps,
+                         uint32_t batch_frames, double* losses);
+/* Fallback judge: total forces of nn_ff (with its network) and fine_ff at
+ * the current positions; *rel_rms = ||F_nn - F_fine|| / ||F_fine||.  The
+ * particle forces are left as nn_ff's. */
+hmd_status hmd_mlp_judge(hmd_forcefield* nn_ff, hmd_forcefield* fine_ff,
+                         double* rel_rms);
+
 #ifdef __cplusplus
 }
 #endif

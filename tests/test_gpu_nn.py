@@ -1,0 +1,221 @@
+"""GPU parity of the learned coarse-mesh correction: descriptors, the
+network's force correction, its loss gradient and the Adam step, through the
+C ABI against the oracle (oracle/nn.py); determinism, momentum conservation,
+the hook into the force field, and the gain of a trained network over the
+coarse mesh alone.
+
+Tolerances:
+* descriptors: relative RMS 1e-5 against FP64 (f32 basis functions and 2^-32
+  fixed-point sums);
+* force correction and gradient: relative RMS 1e-3 against the oracle in its
+  bf16 mode (same rounding points as the matrix-core operands, FP64
+  accumulation).  Differences come from operands that lie within an f32
+  rounding of a bf16 rounding boundary and round the other way; the exact
+  FP64 network is reported beside it;
+* Adam: relative 1e-5 (f32 update against FP64).
+"""
+import numpy as np
+import pytest
+
+from heddlemd_amd import api
+from heddlemd_amd import units as U
+from oracle import nn as N
+from tests.gpu_helpers import rel_rms, water_case
+
+pytestmark = pytest.mark.gpu
+
+R_NN = 1.0 * U.NM
+DESC_TOL = 1e-5
+NET_TOL = 1e-3
+
+
+@pytest.fixture(scope="module")
+def water(gpu_ctx):
+    case = water_case(gpu_ctx, relax=40)
+    yield case
+    for h in case[2:][::-1]:
+        h.close()
+
+
+def species(sysw):
+    return np.asarray(sysw["types"], np.int64)      # O = 0, H = 1
+
+
+def lattice_of(sysw):
+    return np.asarray(sysw["lattice"], np.float64)
+
+
+def make_net(ctx, scale=1e-3, seed=31):
+    return api.CorrectionNet(ctx, R_NN, scale, [0, 1], seed=seed)
+
+
+def test_descriptors_match_oracle(gpu_ctx, water):
+    sysw, ff, part, field, integ = water
+    net = make_net(gpu_ctx)
+    D = net.descriptors(field)
+    posq = part.download()["posq"].astype(np.float64)
+    Dref = N.descriptors(posq[:, :3], posq[:, 3], lattice_of(sysw), R_NN)
+    err = rel_rms(D, Dref)
+    print(f"descriptors rel RMS {err:.3e}, |D| rms {np.sqrt(np.mean(Dref ** 2)):.3e}")
+    assert err <= DESC_TOL
+    net.close()
+
+
+def test_forward_matches_oracle(gpu_ctx, water):
+    sysw, ff, part, field, integ = water
+    net = make_net(gpu_ctx, scale=2e-3)
+    params = net.params.astype(np.float64)
+    dF = net.forward(field)
+    D = net.descriptors(field).astype(np.float64)
+    posq = part.download()["posq"].astype(np.float64)
+    args = (posq[:, :3], posq[:, 3], species(sysw), lattice_of(sysw), R_NN, 2e-3)
+    ref_bf = N.forward(params, *args, D=D, bf16_mode=True)
+    ref_64 = N.forward(params, *args, D=D, bf16_mode=False)
+    err_bf, err_64 = rel_rms(dF, ref_bf), rel_rms(dF, ref_64)
+    print(f"dF rel RMS vs bf16 oracle {err_bf:.3e}, vs exact FP64 network {err_64:.3e}")
+    assert err_bf <= NET_TOL
+    assert err_64 <= 5e-2
+    net.close()
+
+
+def test_loss_grad_matches_oracle(gpu_ctx, water):
+    sysw, ff, part, field, integ = water
+    net = make_net(gpu_ctx, scale=2e-3, seed=5)
+    params = net.params.astype(np.float64)
+    rng = np.random.default_rng(3)
+    target = rng.normal(scale=2e-3, size=(part.n, 3)).astype(np.float32)
+    loss, g = net.loss_grad(field, target)
+    D = net.descriptors(field).astype(np.float64)
+    posq = part.download()["posq"].astype(np.float64)
+    args = (posq[:, :3], posq[:, 3], species(sysw), lattice_of(sysw), R_NN, 2e-3,
+            target.astype(np.float64))
+    l_ref, g_ref, _ = N.loss_grad(params, *args, D=D, bf16_mode=True)
+    l_64, g_64, _ = N.loss_grad(params, *args, D=D, bf16_mode=False)
+    err_l = abs(loss - l_ref) / abs(l_ref)
+    err_g, err_g64 = rel_rms(g, g_ref), rel_rms(g, g_64)
+    blocks = {"W1": slice(0, 3264), "b1": slice(3264, 3328), "W2": slice(3328, 7424),
+              "b2": slice(7424, 7488), "w3": slice(7488, 7552), "b3": slice(7552, 7553)}
+    per = {k: rel_rms(g[s], g_ref[s]) for k, s in blocks.items()}
+    print(f"loss {loss:.6e} rel err {err_l:.2e}; grad rel RMS vs bf16 oracle {err_g:.3e} "
+          f"(per block {per}), vs FP64 {err_g64:.3e}")
+    assert err_l <= 1e-4
+    assert err_g <= NET_TOL
+    for k, e in per.items():
+        assert e <= 10 * NET_TOL, k
+    net.close()
+
+
+def test_adam_step_matches_oracle(gpu_ctx):
+    net = make_net(gpu_ctx, seed=9)
+    p0 = net.params.astype(np.float64)
+    rng = np.random.default_rng(11)
+    g1 = rng.normal(size=p0.size).astype(np.float32)
+    g2 = rng.normal(size=p0.size).astype(np.float32)
+    net.adam_step(g1)
+    net.adam_step(g2)
+    m = v = np.zeros_like(p0)
+    p, m, v = N.adam_step(p0, g1.astype(np.float64), m, v, 1)
+    p, m, v = N.adam_step(p, g2.astype(np.float64), m, v, 2)
+    got = net.params.astype(np.float64)
+    err = np.abs((got - p0) - (p - p0)).max() / np.abs(p - p0).max()
+    print(f"Adam two steps: max rel deviation of the update {err:.2e}")
+    assert err <= 1e-5
+    net.close()
+
+
+def test_correction_is_deterministic_and_momentum_free(gpu_ctx, water):
+    sysw, ff, part, field, integ = water
+    net = make_net(gpu_ctx, scale=2e-3, seed=13)
+    a = net.forward(field)
+    field.request_rebuild()
+    b = net.forward(field)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    net_f = np.abs(a.astype(np.float64).sum(axis=0)).max()
+    print(f"net correction force {net_f:.3e} vs sum |dF| {np.abs(a).sum():.3e}")
+    assert net_f <= 1e-6 * np.abs(a).sum()
+    net.close()
+
+
+def test_attached_correction_adds_to_reciprocal_forces(gpu_ctx, water):
+    sysw, ff, part, field, integ = water
+    net = make_net(gpu_ctx, scale=2e-3, seed=17)
+    field.compute(0)
+    f0 = part.forces().astype(np.float64)
+    dF = net.forward(field).astype(np.float64)
+    net.attach(field)
+    field.compute(0)
+    f1 = part.forces().astype(np.float64)
+    api.CorrectionNet.detach(field)
+    err = rel_rms(f1 - f0, dF)
+    print(f"attached: (F_with - F_without) vs dF rel RMS {err:.3e}")
+    assert err <= 1e-4
+    net.close()
+
+
+def test_graph_replay_with_correction_matches_eager(gpu_ctx):
+    runs = []
+    for mode in ("eager", "graph", "graph"):
+        sysw, ff, part, field, integ = water_case(gpu_ctx, relax=30)
+        net = make_net(gpu_ctx, scale=2e-3, seed=19)
+        net.attach(field)
+        if mode == "eager":
+            integ.step(6)
+        else:
+            integ.replay(6)
+        runs.append(part.download()["posq"].copy())
+        for h in (integ, net, field, part):
+            h.close()
+    for k in (1, 2):
+        assert np.array_equal(runs[0].view(np.uint32), runs[k].view(np.uint32))
+
+
+def test_training_beats_the_coarse_mesh(gpu_ctx):
+    """Frames from plain fine-mesh (32^3) MD of the 3,888-atom box, target =
+    F_recip(32^3) - F_recip(16^3); after training, the corrected coarse mesh
+    is closer to the fine one than the coarse mesh alone on frames the
+    network has not seen."""
+    sysw, ff_f, part, fine, integ = water_case(gpu_ctx, relax=60, mesh=(32, 32, 32))
+    from heddlemd_amd import systems
+    ff_c = systems.water_forcefield(grid=(16, 16, 16), exclusions=sysw["exclusions"])
+    coarse = api.ForceField(gpu_ctx, part, ff_c)
+    v = api.generate_velocities(sysw["mass"], U.kelvin(300.0), 5,
+                                n_constraints=3 * sysw["n_mol"])
+    part.upload(vel=v)
+    net = make_net(gpu_ctx, scale=1e-3, seed=31)
+    n_train, n_test = 48, 8
+    net.reserve(part.n, n_train)
+    for _ in range(n_train):
+        integ.replay(10)
+        net.record_frame(fine, coarse)
+    st = net.dataset_stats()
+    net.set_force_scale(st["target_rms"])
+    held = []
+    for _ in range(n_test):
+        integ.replay(10)
+        held.append(part.download()["posq"].copy())
+
+    def errors(with_net):
+        e = []
+        for posq in held:
+            part.upload(posq=posq)
+            fine.compute(0)
+            f_fine = part.forces().astype(np.float64)
+            if with_net:
+                net.attach(coarse)
+            coarse.compute(0)
+            f_c = part.forces().astype(np.float64)
+            if with_net:
+                api.CorrectionNet.detach(coarse)
+            e.append(rel_rms(f_c, f_fine))
+        return float(np.mean(e))
+
+    before = errors(False)
+    losses = net.train(coarse, 120, batch_frames=4)
+    after = errors(True)
+    print(f"target RMS {st['target_rms']:.3e} (fine recip RMS {st['fine_recip_rms']:.3e}); "
+          f"loss {losses[0]:.4f} -> {losses[-10:].mean():.4f}; held-out rel RMS force error "
+          f"coarse {before:.3e} -> corrected {after:.3e}")
+    assert losses[-10:].mean() < losses[:10].mean()
+    assert after < before
+    for h in (net, coarse, integ, fine, part):
+        h.close()
