@@ -241,6 +241,8 @@ void ForceField::init_neighbor_list(const hmd_forcefield_desc& d) {
 
   n_clusters = ceil_div(n, HMD_CLUSTER);
   n_pad = n_clusters * HMD_CLUSTER;
+  if (n_clusters > 65536u)   // the list build ballots 64 x 1024 j-clusters per i-cluster
+    throw Error(HMD_ERR_UNSUPPORTED, "more than 65,536 clusters (4,194,304 atoms)");
   status.alloc(4);
   atom_key.alloc(n);
   cell_count.alloc(n_keys);
@@ -398,7 +400,7 @@ void ForceField::rebuild_chain() {
   bd.excl_offsets = excl_off.ptr;
   bd.excl_partners = excl_par.ptr;
   bd.status = status.ptr;
-  launch(c, K_hmd_nbl_build, ceil_div(n_clusters, 4), 256, 0, bd);
+  launch(c, K_hmd_nbl_build, n_clusters, 256, 0, bd);
   entry_scan.run(c, entry_count.ptr, entry_offset.ptr, gate, HMD_STATUS_DISPLACED);
   NblWorkParams wp{entry_count.ptr, entry_offset.ptr, n_clusters, cap_entries,
                    work.ptr, n_work.ptr, status.ptr};

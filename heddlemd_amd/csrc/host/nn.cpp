@@ -174,8 +174,9 @@ struct Mlp {
 void mlp_correct(Mlp& m, ForceField& ff) {
   m.ensure_size(ff);
   m.descriptors(ff, false);
+  ff.c.mark("nn_descriptors");
   m.forward(ff, ff.slow_f.ptr);
-  ff.c.mark("nn_correction");
+  ff.c.mark("nn_forward");
 }
 
 static void to_soa(const float* rows, uint32_t n, std::vector<float>& soa) {

@@ -204,87 +204,145 @@ __device__ inline float hmd_box_gap2(float4 ca, float4 ha, float4 cb, float4 hb,
   return ex * ex + ey * ey + ez * ez;
 }
 
-// Build the packed list of one i-cluster per wave.
+// Build the packed list of one i-cluster per workgroup of four waves.
 //
 // Candidate j-clusters are those with index >= c whose boxes lie within the
-// search radius.  For each candidate, lane k tests j-atom k against the 64
-// i-atoms (exact f32 distance) and keeps it when some i-atom with a lower
-// sorted index is within r_search.  The kept atoms of a candidate are placed
-// at consecutive entry positions in increasing sorted order by a ballot
-// prefix count, written straight to the entry array.  Each lane keeps the
-// sorted slots of its atom's exclusion partners (at most
-// HMD_MAX_INLINE_EXCL) in registers and, per candidate, sets the bit of the
-// entry position that holds a partner -- no per-hit loop, no dependent
-// memory access.
+// search radius (one ballot per 64 j-clusters, the four waves taking turns).
+// After a prefix over the per-chunk masks the candidates are listed in
+// increasing index order in LDS, and the four waves share them round robin:
+// lane k tests j-atom k of the candidate against the 64 i-atoms (a prune
+// against eight 8-atom i-sub-boxes, then the exact f32 distance) and keeps it
+// when some i-atom with a lower sorted index is within r_search; the next
+// candidate's j-atom is loaded while the current one is tested.  The 64-bit
+// hit mask of each candidate goes to LDS, and wave 0 then places the kept
+// atoms, candidate by candidate, at consecutive entry positions in
+// increasing sorted order (ballot prefix count), written straight to the
+// entry array.  Each lane of wave 0 keeps the sorted slots of its atom's
+// exclusion partners (at most HMD_MAX_INLINE_EXCL) in registers and, per
+// candidate, sets the bit of the entry position that holds a partner.  The
+// output depends only on the positions (deterministic).  More candidates than
+// fit in LDS are handled in rounds.
 #define HMD_MAX_INLINE_EXCL 8
-extern "C" __global__ void __launch_bounds__(256)
+#define HMD_NBL_MAX_CHUNKS 1024u    // 64 j-clusters per chunk: 65,536 clusters
+#define HMD_NBL_CAND_CAP 512u
+
+// Inclusive prefix sum over the 64 lanes.
+__device__ inline uint32_t hmd_wave_scan_incl(uint32_t v, uint32_t lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = (uint32_t)__shfl_up((int)v, (unsigned)o, HMD_WAVE);
+    if (lane >= (uint32_t)o) v += t;
+  }
+  return v;
+}
+
+extern "C" __global__ void __launch_bounds__(256, 3)
 hmd_nbl_build(const NblBuildParams p) {
   if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
-  __shared__ float4 s_i[4][HMD_CLUSTER];
-  __shared__ float4 s_sub[4][16];
+  __shared__ float4 s_i[HMD_CLUSTER];
+  __shared__ float4 s_sub[16];
+  __shared__ uint64_t s_mask[HMD_NBL_MAX_CHUNKS];
+  __shared__ uint32_t s_pre[HMD_NBL_MAX_CHUNKS + 1];
+  __shared__ uint32_t s_cand[HMD_NBL_CAND_CAP];
+  __shared__ uint64_t s_hit[HMD_NBL_CAND_CAP];
+  const uint32_t c = blockIdx.x;
   const uint32_t wave = threadIdx.x / HMD_WAVE;
-  const uint32_t c = blockIdx.x * 4u + wave;
   const uint32_t lane = hmd_lane();
-  const bool active_wave = c < p.n_clusters;
   const float4* ps = reinterpret_cast<const float4*>(p.posq_sorted);
   const float4* bb = reinterpret_cast<const float4*>(p.bbox);
-
-  const uint32_t si = c * HMD_CLUSTER + lane;
-  s_i[wave][lane] = active_wave ? ps[si] : make_float4(0, 0, 0, 0);
-  if (lane < 16) s_sub[wave][lane] = active_wave
-      ? reinterpret_cast<const float4*>(p.sub_bbox)[16 * c + lane] : make_float4(0, 0, 0, 0);
-  __syncthreads();
-  if (!active_wave) return;
-
+  if (wave == 0) {
+    s_i[lane] = ps[c * HMD_CLUSTER + lane];
+    if (lane < 16) s_sub[lane] = reinterpret_cast<const float4*>(p.sub_bbox)[16 * c + lane];
+  }
   const bool ortho = p.orthorhombic_fast != 0;
   const float ilx = 1.0f / p.L.lx, ily = 1.0f / p.L.ly, ilz = 1.0f / p.L.lz;
   const float rs2_prune = p.r_search_sq * 1.0002f;   // conservative prune
-  // Sorted slots of this lane's exclusion partners.
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t n_chunks = (p.n_clusters - c + HMD_WAVE - 1) / HMD_WAVE;
+
+  // candidate masks, 64 j-clusters per ballot
+  {
+    const float4 ci = bb[2 * c], hi = bb[2 * c + 1];
+    for (uint32_t ch = wave; ch < n_chunks; ch += 4) {
+      const uint32_t jc = c + ch * HMD_WAVE + lane;
+      bool cand = false;
+      if (jc < p.n_clusters) {
+        const float4 cj = bb[2 * jc], hj = bb[2 * jc + 1];
+        cand = hmd_box_gap2(ci, hi, cj, hj, p.L, ortho, ilx, ily, ilz) <= rs2_prune;
+      }
+      const uint64_t m = __ballot(cand);
+      if (lane == 0) s_mask[ch] = m;
+    }
+  }
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < n_chunks; b += HMD_WAVE) {
+      const uint32_t ch = b + lane;
+      const uint32_t cnt = ch < n_chunks ? (uint32_t)__builtin_popcountll(s_mask[ch]) : 0u;
+      const uint32_t incl = hmd_wave_scan_incl(cnt, lane);
+      if (ch < n_chunks) s_pre[ch] = carry + incl - cnt;
+      carry += hmd_readlane(incl, HMD_WAVE - 1);
+    }
+    if (lane == 0) s_pre[n_chunks] = carry;
+  }
+  __syncthreads();
+  const uint32_t total = s_pre[n_chunks];
+
+  // wave 0: exclusion slots and entry state
+  const uint32_t si = c * HMD_CLUSTER + lane;
   uint32_t ex_slot[HMD_MAX_INLINE_EXCL];
-  uint32_t n_ex = 0;
-  if (si < p.n && p.excl_offsets) {
+#pragma unroll
+  for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m) ex_slot[m] = HMD_SENTINEL;
+  if (wave == 0 && si < p.n && p.excl_offsets) {
     const uint32_t id = p.sorted_id[si];
     const uint32_t b = p.excl_offsets[id], e = p.excl_offsets[id + 1];
-    n_ex = min(e - b, (uint32_t)HMD_MAX_INLINE_EXCL);
+    const uint32_t n_ex = min(e - b, (uint32_t)HMD_MAX_INLINE_EXCL);
 #pragma unroll
     for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m)
-      ex_slot[m] = (m < n_ex) ? p.slot_of[p.excl_partners[b + m]] : HMD_SENTINEL;
-  } else {
-#pragma unroll
-    for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m) ex_slot[m] = HMD_SENTINEL;
+      if (m < n_ex) ex_slot[m] = p.slot_of[p.excl_partners[b + m]];
   }
-
-  const float4 ci = bb[2 * c], hi = bb[2 * c + 1];
   const size_t base_entry = (size_t)c * p.cap_entries;
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const uint32_t imax = min(p.n - c * HMD_CLUSTER, HMD_CLUSTER);
   uint32_t n_entries = 0;       // completed entries (uniform)
   uint32_t fill = 0;            // slots used in the current entry (uniform)
   uint64_t cur_mask = 0, next_mask = 0;
   bool overflow = false;
+  const float4 zero = make_float4(0, 0, 0, 0);
 
-  for (uint32_t jb = c; jb < p.n_clusters; jb += HMD_WAVE) {
-    const uint32_t jc = jb + lane;
-    bool cand = false;
-    if (jc < p.n_clusters) {
-      const float4 cj = bb[2 * jc], hj = bb[2 * jc + 1];
-      cand = hmd_box_gap2(ci, hi, cj, hj, p.L, ortho, ilx, ily, ilz) <= rs2_prune;
+  for (uint32_t base = 0; base < total; base += HMD_NBL_CAND_CAP) {
+    const uint32_t cnt = min(HMD_NBL_CAND_CAP, total - base);
+    for (uint32_t ch = wave; ch < n_chunks; ch += 4) {
+      const uint64_t m = s_mask[ch];
+      if ((m >> lane) & 1ull) {
+        const uint32_t pos = s_pre[ch] + (uint32_t)__builtin_popcountll(m & lt_mask);
+        if (pos >= base && pos < base + cnt) s_cand[pos - base] = c + ch * HMD_WAVE + lane;
+      }
     }
-    uint64_t cmask = __ballot(cand);
-    while (cmask) {
-      const uint32_t jcl = jb + (uint32_t)__builtin_ctzll(cmask);
-      cmask &= cmask - 1;
+    __syncthreads();
+
+    // hit masks, candidates shared round robin, next j-atom prefetched
+    uint32_t q = wave;
+    float4 pj_next = zero;
+    if (q < cnt) {
+      const uint32_t sj = s_cand[q] * HMD_CLUSTER + lane;
+      if (sj < p.n) pj_next = ps[sj];
+    }
+    for (; q < cnt; q += 4) {
+      const uint32_t jcl = s_cand[q];
       const uint32_t sj = jcl * HMD_CLUSTER + lane;
+      const float4 pj = pj_next;
+      if (q + 4 < cnt) {
+        const uint32_t sn = s_cand[q + 4] * HMD_CLUSTER + lane;
+        pj_next = sn < p.n ? ps[sn] : zero;
+      }
       // Sub-box mask: which 8-atom i-sub-clusters this j-atom may reach.
       uint32_t smask = 0;
-      float4 pj = make_float4(0, 0, 0, 0);
       if (sj < p.n) {
-        pj = ps[sj];
-        const float4 zero = make_float4(0, 0, 0, 0);
 #pragma unroll
         for (uint32_t k = 0; k < 8; ++k) {
-          const float g2 = hmd_box_gap2(s_sub[wave][2 * k], s_sub[wave][2 * k + 1], pj, zero,
-                                        p.L, ortho, ilx, ily, ilz);
+          const float g2 = hmd_box_gap2(s_sub[2 * k], s_sub[2 * k + 1], pj, zero, p.L, ortho,
+                                        ilx, ily, ilz);
           smask |= (g2 <= rs2_prune) ? (1u << k) : 0u;
         }
       }
@@ -295,9 +353,10 @@ hmd_nbl_build(const NblBuildParams p) {
         const bool want = !hit && ((smask >> k) & 1u);
         if (__ballot(want) == 0) continue;
         if (want) {
+#pragma unroll 4
           for (uint32_t t = 0; t < 8; ++t) {
             const uint32_t i = 8 * k + t;
-            const float4 a = s_i[wave][i];
+            const float4 a = s_i[i];
             const float r2 = ortho
                                  ? hmd_r2_ortho(a.x, a.y, a.z, pj.x, pj.y, pj.z, p.L, ilx, ily, ilz)
                                  : hmd_r2_exact(a.x, a.y, a.z, pj.x, pj.y, pj.z, p.L);
@@ -306,38 +365,51 @@ hmd_nbl_build(const NblBuildParams p) {
         }
       }
       const uint64_t H = __ballot(hit);
-      if (H == 0) continue;
-      if (hit) {
-        const uint32_t pos = fill + (uint32_t)__builtin_popcountll(H & lt_mask);
-        const uint32_t e = n_entries + pos / HMD_ENTRY;
-        if (e < p.cap_entries) p.entry_j[(base_entry + e) * HMD_ENTRY + pos % HMD_ENTRY] = sj;
-      }
+      if (lane == 0) s_hit[q] = H;
+    }
+    __syncthreads();
+
+    // placement, in candidate order (wave 0)
+    if (wave == 0) {
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const uint64_t H = s_hit[k];
+        if (H == 0) continue;
+        const uint32_t jcl = s_cand[k];
+        if ((H >> lane) & 1ull) {
+          const uint32_t pos = fill + (uint32_t)__builtin_popcountll(H & lt_mask);
+          const uint32_t e = n_entries + pos / HMD_ENTRY;
+          if (e < p.cap_entries)
+            p.entry_j[(base_entry + e) * HMD_ENTRY + pos % HMD_ENTRY] = jcl * HMD_CLUSTER + lane;
+        }
 #pragma unroll
-      for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m) {
-        const uint32_t es = ex_slot[m];
-        if (es != HMD_SENTINEL && es / HMD_CLUSTER == jcl) {
-          const uint32_t b = es % HMD_CLUSTER;
-          if ((H >> b) & 1ull) {
-            const uint64_t below = (b == 0) ? 0ull : (H & (~0ull >> (64 - b)));
-            const uint32_t pos = fill + (uint32_t)__builtin_popcountll(below);
-            if (pos < HMD_ENTRY) cur_mask |= 1ull << pos;
-            else next_mask |= 1ull << (pos - HMD_ENTRY);
+        for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m) {
+          const uint32_t es = ex_slot[m];
+          if (es != HMD_SENTINEL && es / HMD_CLUSTER == jcl) {
+            const uint32_t b = es % HMD_CLUSTER;
+            if ((H >> b) & 1ull) {
+              const uint64_t below = (b == 0) ? 0ull : (H & (~0ull >> (64 - b)));
+              const uint32_t pos = fill + (uint32_t)__builtin_popcountll(below);
+              if (pos < HMD_ENTRY) cur_mask |= 1ull << pos;
+              else next_mask |= 1ull << (pos - HMD_ENTRY);
+            }
           }
         }
-      }
-      fill += (uint32_t)__builtin_popcountll(H);
-      if (fill >= HMD_ENTRY) {
-        if (n_entries < p.cap_entries)
-          p.entry_excl[(base_entry + n_entries) * HMD_ENTRY + lane] = cur_mask;
-        else
-          overflow = true;
-        ++n_entries;
-        fill -= HMD_ENTRY;
-        cur_mask = next_mask;
-        next_mask = 0;
+        fill += (uint32_t)__builtin_popcountll(H);
+        if (fill >= HMD_ENTRY) {
+          if (n_entries < p.cap_entries)
+            p.entry_excl[(base_entry + n_entries) * HMD_ENTRY + lane] = cur_mask;
+          else
+            overflow = true;
+          ++n_entries;
+          fill -= HMD_ENTRY;
+          cur_mask = next_mask;
+          next_mask = 0;
+        }
       }
     }
+    __syncthreads();
   }
+  if (wave != 0) return;
   if (fill > 0) {
     if (n_entries < p.cap_entries) {
       const size_t o = (base_entry + n_entries) * HMD_ENTRY + lane;

@@ -57,6 +57,16 @@ __device__ inline uint16_t hmd_bf16_bits(float v) {
   return __builtin_bit_cast(uint16_t, b);
 }
 
+typedef uint32_t hmd_u32x4 __attribute__((ext_vector_type(4)));
+
+// Bits of element e of a bf16x8 operand, read from its packed 32-bit words
+// (element 2w is the low half of word w, element 2w + 1 the high half).
+__device__ inline uint16_t hmd_bf16x8_bits(hmd_bf16x8 v, int e) {
+  const hmd_u32x4 w = __builtin_bit_cast(hmd_u32x4, v);
+  const uint32_t d = w[e >> 1];
+  return (uint16_t)((e & 1) ? (d >> 16) : (d & 0xFFFFu));
+}
+
 // Internal input order of the 64 (51 used) network inputs: rbf 0..15, the
 // first atom's descriptor 16..31, the second's 32..47, one-hot 48..50.
 // Returns the canonical index (rbf 0..15, one-hot 16..18, D_first 19..34,
@@ -631,7 +641,7 @@ __device__ inline void hmd_nn_backward_batch(const NnPassParams& p, const HmdNnS
       for (int r = 0; r < 4; ++r) {
         const uint32_t hrow = 16u * t + 4u * g + r, col = 16u * m + pl;
         T0[hrow][col] = hmd_bf16_bits(z2[t][m][r]);
-        T1[hrow][col] = __builtin_bit_cast(uint16_t, H1[m][t >> 1][4 * (t & 1) + r]);
+        T1[hrow][col] = hmd_bf16x8_bits(H1[m][t >> 1], 4 * (t & 1) + r);
       }
   hmd_wave_sync();
 #pragma unroll
@@ -661,7 +671,7 @@ __device__ inline void hmd_nn_backward_batch(const NnPassParams& p, const HmdNnS
     for (int kap = 0; kap < 2; ++kap)
 #pragma unroll
       for (int e = 0; e < 8; ++e)
-        T1[32u * kap + 8u * g + e][16u * m + pl] = __builtin_bit_cast(uint16_t, P.B[m][kap][e]);
+        T1[32u * kap + 8u * g + e][16u * m + pl] = hmd_bf16x8_bits(P.B[m][kap], e);
   hmd_wave_sync();
 #pragma unroll
   for (int th = 0; th < 4; ++th) {
@@ -682,7 +692,7 @@ extern "C" __global__ void __launch_bounds__(256)
 hmd_nn_backward(const NnPassParams p) {
   __shared__ HmdNnShared S;
   __shared__ HmdNnWave Wv[4];
-  __shared__ uint16_t s_t[4][2][HMD_NN_HID][HMD_NN_TSTRIDE];
+  __shared__ __attribute__((aligned(16))) uint16_t s_t[4][2][HMD_NN_HID][HMD_NN_TSTRIDE];
   hmd_nn_load_shared(p, S, 3);
   const uint32_t lane = hmd_lane();
   const uint32_t wv = threadIdx.x / HMD_WAVE;
