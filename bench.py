@@ -22,8 +22,9 @@ Two legs, as the metric names them:
   leg is timed (untimed).  Inside the timed leg the plain 128^3 forces are
   recomputed every 50 steps as the judge; if the relative RMS force error
   exceeds 1e-3 the run reverts to the plain mesh for the rest of the leg.
-`value` is the NN-corrected leg's ns/day; the plain leg is reported beside
-it.
+`value` is the ns/day of the fastest leg whose forces meet dF <= 1e-3
+against the plain 128^3 forces: the NN leg only if every judge passed, else
+the plain leg; both legs are reported.
 
 Prints one JSON line (see the keys below).  --gpus N > 1 exits: this build
 runs on one GPU.
@@ -162,9 +163,14 @@ def main():
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                 "traffic": None, "pairs_in_cutoff": n_in_cut,
                 "kernel_us": round(pair_us, 2)}
+    # value: the fastest leg whose forces stay within dF <= 1e-3 of the plain
+    # 128^3 forces (the plain leg by definition; the NN leg only if every
+    # judge passed)
     value, ms_val, leg = ns_day, ms_per_step, "plain 128^3"
     if nn is not None:
-        value, ms_val, leg = nn["ns_day"], nn["ms_per_step"], "NN-corrected 64^3"
+        nn["meets_dF_bound"] = nn["reverted_at_step"] is None
+        if nn["meets_dF_bound"] and nn["ns_day"] > ns_day:
+            value, ms_val, leg = nn["ns_day"], nn["ms_per_step"], "NN-corrected 64^3"
     out = {
         "metric": "ns/day, 196k-atom SPC water: plain 128^3 SPME vs NN-corrected 64^3, dF<=1e-3",
         "value": round(value, 3),
@@ -176,7 +182,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+bf16" if nn is not None else "f32",
+        "dtype": "f32+bf16" if leg.startswith("NN") else "f32",
         "data": "synthetic",
         "config": {"workload": f"{leg} leg: 196,608-atom rigid SPC water, compensated VV NVE "
                                "300 K, dt 2 fs, r_c 1.0 nm, skin 0.1 nm",
@@ -187,9 +193,12 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    if nn is not None:   # the NN leg's dominant kernel carries the roofline
-        out["roofline"] = nn.pop("roofline")
-        out["roofline_pair"] = roofline
+    if nn is not None:
+        nn_roof = nn.pop("roofline")
+        if leg.startswith("NN"):   # that leg's dominant kernel carries the roofline
+            out["roofline"], out["roofline_pair"] = nn_roof, roofline
+        else:
+            out["roofline_nn"] = nn_roof
     print(json.dumps(out))
 
 
@@ -229,27 +238,28 @@ def nn_leg(api, systems, U, args, ctx, sysw, part, fine, integ_fine, run_fine):
         integ.capture()
     run(args.warmup)
     ctx.synchronize()
+    # The whole leg runs corrected, with the judge every 50 steps (its cost
+    # included); a judge result above the threshold is where a production run
+    # would revert to the plain mesh (recorded, not acted on, so that the
+    # corrected stepping rate is what is timed).
     judges, reverted_at = [], None
     t0 = time.perf_counter()
     done = 0
     while done < args.steps:
         k = min(50, args.steps - done)
-        if reverted_at is None:
-            run(k)
-            done += k
-            e = api.judge(coarse, fine)
-            judges.append(e)
-            if e > 1e-3:
-                reverted_at = done
-        else:
-            run_fine(k)
-            done += k
+        run(k)
+        done += k
+        e = api.judge(coarse, fine)
+        judges.append(e)
+        if e > 1e-3 and reverted_at is None:
+            reverted_at = done
     ctx.synchronize()
     el = time.perf_counter() - t0
     ms = el / args.steps * 1e3
     ns = U.ns_per_day(args.steps / el, DT_FS)
     log(f"NN leg: {args.steps} steps in {el:.3f} s: {ms:.3f} ms/step, {ns:.2f} ns/day; "
-        f"judge rel RMS {['%.2e' % j for j in judges]}; reverted at {reverted_at}")
+        f"judge rel RMS {['%.2e' % j for j in judges]}; first judge above 1e-3 at "
+        f"{reverted_at}")
     prof = integ.profile(10)
     log("NN stage us: " + ", ".join(f"{k} {v:.1f}" for k, v in prof.items()))
     n_nn = coarse.count_pairs(1.0 * U.NM)

@@ -89,6 +89,8 @@ struct Mlp {
     q.work = ff.work.ptr;
     q.n_work = ff.n_work.ptr;
     q.entry_j = ff.entry_j.ptr;
+    q.bbox = ff.bbox.ptr;
+    q.shift_margin = ff.r_cut + 0.5f * ff.r_skin;
     q.L = ff.p.lattice;
     q.n = ff.n;
     q.cap_entries = ff.cap_entries;

@@ -340,8 +340,10 @@ struct NnPassParams {
   const uint32_t* work;
   const uint32_t* n_work;
   const uint32_t* entry_j;
+  const float* bbox;           // cluster centres + half extents
   HmdLattice L;
   uint32_t n, cap_entries, n_pad;
+  float shift_margin;          // r_cut + r_skin/2: single-image shift eligibility
   uint32_t species_mask;       // bit t = species (0/1) of type t
   float r_cut2, inv_r_cut;     // r_nn^2, 1/r_nn
   float force_scale;           // s_out

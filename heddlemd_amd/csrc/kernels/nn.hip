@@ -249,13 +249,55 @@ struct HmdNnShared {
   float vec[HMD_NN_HID * 3 + 1];           // b1, b2, w3, b3
 };
 
-struct HmdNnWave {
-  float4 pos[2 * HMD_WAVE];                // i-cluster atoms, then entry j-atoms
-  float4 res[2 * HMD_WAVE];                // dL/d(dF) of the same (backward)
+// Per-wave tables of the current entry: its i-atoms (0..63) and j-atoms
+// (64..127), positions shifted to the image nearest the cluster centre when
+// the cluster qualifies, and the queue of in-range pairs.
+struct HmdNnTables {
+  float4 pos[2 * HMD_WAVE];
   uint32_t sp[2 * HMD_WAVE];               // species
   hmd_bf16x8 dsc[2 * HMD_WAVE][2];         // descriptors, bf16
   uint16_t queue[HMD_NN_QCAP];             // (i lane) | (j slot << 6)
 };
+
+struct HmdNnWave : HmdNnTables {           // backward pass
+  float4 res[2 * HMD_WAVE];                // dL/d(dF) of the same atoms
+};
+
+struct HmdNnFwdWave : HmdNnTables {        // forward pass: per-batch pair table
+  hmd_bf16x8 phi[32][2];                   // radial basis, bf16
+  float4 geo[32];                          // fc, unit vector i - j
+  uint32_t info[32];                       // i | j << 6 | one-hot << 12 | valid << 14
+};
+
+// Single-image shift of a cluster (compare jit_composed.rs:1396-1430, as in
+// the pair kernel): valid when every atom within r_cut + r_skin/2 of the
+// cluster lies within half a box of its centre; positions are then moved to
+// the image nearest the centre and pair differences need no minimum image.
+struct HmdNnShift {
+  bool on;
+  float cx, cy, cz;
+};
+
+__device__ inline HmdNnShift hmd_nn_cluster_shift(const NnPassParams& p, uint32_t c) {
+  const HmdLattice& L = p.L;
+  HmdNnShift sh{false, 0.0f, 0.0f, 0.0f};
+  if (L.xy != 0.0f || L.xz != 0.0f || L.yz != 0.0f) return sh;
+  const float4* bb = reinterpret_cast<const float4*>(p.bbox);
+  const float4 cb = bb[2 * c], hb = bb[2 * c + 1];
+  sh.on = (hb.x + p.shift_margin < 0.5f * L.lx) && (hb.y + p.shift_margin < 0.5f * L.ly) &&
+          (hb.z + p.shift_margin < 0.5f * L.lz);
+  sh.cx = cb.x; sh.cy = cb.y; sh.cz = cb.z;
+  return sh;
+}
+
+__device__ inline float4 hmd_nn_apply_shift(float4 a, const HmdNnShift& sh, const HmdLattice& L) {
+  if (sh.on) {
+    a.x -= L.lx * floorf((a.x - sh.cx) * (1.0f / L.lx) + 0.5f);
+    a.y -= L.ly * floorf((a.y - sh.cy) * (1.0f / L.ly) + 0.5f);
+    a.z -= L.lz * floorf((a.z - sh.cz) * (1.0f / L.lz) + 0.5f);
+  }
+  return a;
+}
 
 // Geometry and inputs of the two pairs lane l works on in a batch
 // (pairs l%16 and 16 + l%16 of the batch).
@@ -267,7 +309,7 @@ struct HmdNnPairs {
   hmd_bf16x8 B[4][2];                      // X^T operands: row tiles m, k-steps
 };
 
-__device__ inline void hmd_nn_gather_pairs(const NnPassParams& p, const HmdNnWave& W,
+__device__ inline void hmd_nn_gather_pairs(const NnPassParams& p, const HmdNnTables& W,
                                            uint32_t base, uint32_t cnt, uint32_t lane,
                                            HmdNnPairs& P) {
   const uint32_t g = lane >> 4, pl = lane & 15u;
@@ -387,11 +429,10 @@ __device__ inline void hmd_nn_output(const HmdNnShared& S, const float z2[4][4][
   }
 }
 
-// Load the wave tables of a new i-cluster (lane = i-atom).
-__device__ inline void hmd_nn_load_i(const NnPassParams& p, HmdNnWave& W, uint32_t si,
-                                     uint32_t lane, bool with_res) {
-  const float4* ps = reinterpret_cast<const float4*>(p.posq_sorted);
-  W.pos[lane] = ps[si];
+// Load the wave tables of a new i-cluster (lane = i-atom; a is shifted).
+__device__ inline void hmd_nn_load_i(const NnPassParams& p, HmdNnTables& W, uint32_t si,
+                                     float4 a, uint32_t lane) {
+  W.pos[lane] = a;
   W.sp[lane] = hmd_nn_species(p.species_mask, p.type_sorted[si]);
   const float* dr = p.desc + (size_t)si * 16;
   hmd_bf16x8 d0, d1;
@@ -399,14 +440,11 @@ __device__ inline void hmd_nn_load_i(const NnPassParams& p, HmdNnWave& W, uint32
   for (int e = 0; e < 8; ++e) { d0[e] = (__bf16)dr[e]; d1[e] = (__bf16)dr[8 + e]; }
   W.dsc[lane][0] = d0;
   W.dsc[lane][1] = d1;
-  if (with_res)
-    W.res[lane] = make_float4(p.resid[si], p.resid[p.n_pad + si], p.resid[2 * p.n_pad + si], 0.0f);
 }
 
-__device__ inline void hmd_nn_load_j(const NnPassParams& p, HmdNnWave& W, uint32_t jj,
-                                     uint32_t lane, bool with_res) {
-  const float4* ps = reinterpret_cast<const float4*>(p.posq_sorted);
-  W.pos[HMD_WAVE + lane] = ps[jj];
+__device__ inline void hmd_nn_load_j(const NnPassParams& p, HmdNnTables& W, uint32_t jj,
+                                     float4 b, uint32_t lane) {
+  W.pos[HMD_WAVE + lane] = b;
   W.sp[HMD_WAVE + lane] = hmd_nn_species(p.species_mask, p.type_sorted[jj]);
   const float* dr = p.desc + (size_t)jj * 16;
   hmd_bf16x8 d0, d1;
@@ -414,16 +452,17 @@ __device__ inline void hmd_nn_load_j(const NnPassParams& p, HmdNnWave& W, uint32
   for (int e = 0; e < 8; ++e) { d0[e] = (__bf16)dr[e]; d1[e] = (__bf16)dr[8 + e]; }
   W.dsc[HMD_WAVE + lane][0] = d0;
   W.dsc[HMD_WAVE + lane][1] = d1;
-  if (with_res)
-    W.res[HMD_WAVE + lane] =
-        make_float4(p.resid[jj], p.resid[p.n_pad + jj], p.resid[2 * p.n_pad + jj], 0.0f);
+}
+
+__device__ inline float4 hmd_nn_resid(const NnPassParams& p, uint32_t s) {
+  return make_float4(p.resid[s], p.resid[p.n_pad + s], p.resid[2 * p.n_pad + s], 0.0f);
 }
 
 // Append the in-range pairs of j-slots [kb, kb + 16) to the wave queue.
-__device__ inline uint32_t hmd_nn_compact(const NnPassParams& p, HmdNnWave& W, uint32_t my_j,
+__device__ inline uint32_t hmd_nn_compact(const NnPassParams& p, HmdNnTables& W, uint32_t my_j,
                                           float4 pj, float xi, float yi, float zi, bool vi,
-                                          uint32_t si, uint32_t c, uint32_t kb, uint32_t qn,
-                                          uint32_t lane) {
+                                          uint32_t si, uint32_t c, bool shift, uint32_t kb,
+                                          uint32_t qn, uint32_t lane) {
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll 4
   for (uint32_t u = 0; u < 16; ++u) {
@@ -433,7 +472,7 @@ __device__ inline uint32_t hmd_nn_compact(const NnPassParams& p, HmdNnWave& W, u
     float dx = xi - hmd_readlane_f(pj.x, k);
     float dy = yi - hmd_readlane_f(pj.y, k);
     float dz = zi - hmd_readlane_f(pj.z, k);
-    hmd_min_image(dx, dy, dz, p.L);
+    if (!shift) hmd_min_image(dx, dy, dz, p.L);
     const float r2 = dx * dx + dy * dy + dz * dz;
     const bool ok = vi && ((j >> 6) != c || si < j) && (r2 < p.r_cut2) && (r2 > 0.0f);
     const uint64_t m = __ballot(ok);
@@ -452,42 +491,128 @@ __device__ inline void hmd_nn_load_shared(const NnPassParams& p, HmdNnShared& S,
 }
 
 // Forward: dF into the fixed-point accumulators (sorted order).
+//
+// A batch is 32 pairs.  Its geometry and radial basis are computed once per
+// pair (lanes l and l + 32 take pair l % 32, each half of the basis) into a
+// per-wave LDS table; then the four row tiles (pairs 0-15 and 16-31, each in
+// both orientations) go through the two layers one tile at a time, so only
+// one tile's activations are live.
 __device__ inline void hmd_nn_forward_batch(const NnPassParams& p, const HmdNnShared& S,
-                                            const HmdNnWave& W, long long (*acc)[3][HMD_WAVE],
-                                            uint32_t base, uint32_t cnt, uint32_t lane) {
-  HmdNnPairs P;
-  hmd_nn_gather_pairs(p, W, base, cnt, lane, P);
-  float z1[4][4][4], z2[4][4][4];
-  hmd_bf16x8 H1[4][2];
-  hmd_nn_layers(S, P, lane, z1, H1, z2);
-  float y[4];
-  hmd_nn_output(S, z2, lane, y);
-  // lanes of group 0 apply pair l%16, lanes of group 1 pair 16 + l%16
-  const uint32_t g = lane >> 4;
-  const bool h1 = g == 1u;
-  const bool valid = (g < 2u) && (h1 ? P.valid[1] : P.valid[0]);
-  if (valid) {
-    const float fc = h1 ? P.fc[1] : P.fc[0];
-    const float ysum = h1 ? (y[1] + y[3]) : (y[0] + y[2]);
-    const float gp = p.force_scale * fc * 0.5f * ysum;
-    const uint32_t ia = h1 ? P.ia[1] : P.ia[0];
-    const uint32_t jk = (h1 ? P.jb[1] : P.jb[0]) - HMD_WAVE;
+                                            HmdNnFwdWave& W, long long (*acc)[3][HMD_WAVE],
+                                            uint32_t base, uint32_t cnt, bool shift,
+                                            const float (&bias1)[4][4], const float (&bias2)[4][4],
+                                            const float (&w3)[4][4], uint32_t lane) {
+  const uint32_t g = lane >> 4, pl = lane & 15u;
+  {
+    const uint32_t pr = lane & 31u, hf = lane >> 5;
+    const bool valid = pr < cnt;
+    const uint32_t q = valid ? W.queue[base + pr] : 0u;
+    const uint32_t il = q & 63u, jk = q >> 6;
+    const float4 a = W.pos[il], b = W.pos[HMD_WAVE + jk];
+    float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    if (!shift) hmd_min_image(dx, dy, dz, p.L);
+    const float r2 = valid ? dx * dx + dy * dy + dz * dz : 1.0f;
+    const float inv_r = __builtin_amdgcn_rsqf(r2);
+    const float sc = r2 * inv_r * p.inv_r_cut;
+    const float u15 = 15.0f * sc;
+    const float k0 = (float)(8u * hf);
+    hmd_bf16x8 ph;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const float rh = h1 ? P.rh[1][d] : P.rh[0][d];
-      const long long f = hmd_to_fixed(gp * rh, HMD_NN_FORCE_SHIFT);
-      atomicAdd(reinterpret_cast<unsigned long long*>(&acc[0][d][ia]),
-                static_cast<unsigned long long>(f));
-      atomicAdd(reinterpret_cast<unsigned long long*>(&acc[1][d][jk]),
-                static_cast<unsigned long long>(-f));
+    for (int e = 0; e < 8; ++e) ph[e] = (__bf16)hmd_nn_rbf(u15, k0 + (float)e);
+    W.phi[pr][hf] = ph;
+    if (hf == 0u) {
+      const float fc = valid ? hmd_nn_cutoff(sc) : 0.0f;
+      W.geo[pr] = make_float4(fc, dx * inv_r, dy * inv_r, dz * inv_r);
+      const uint32_t oh = W.sp[il] + W.sp[HMD_WAVE + jk];
+      W.info[pr] = il | (jk << 6) | (oh << 12) | ((valid ? 1u : 0u) << 14);
     }
   }
+  hmd_wave_sync();
+  const uint32_t info0 = W.info[pl], info1 = W.info[16 + pl];
+  float y[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const uint32_t info = (m & 1) ? info1 : info0;
+    const uint32_t pr = 16u * (m & 1) + pl;
+    const bool valid = (info >> 14) & 1u;
+    const uint32_t il = info & 63u, jrow = HMD_WAVE + ((info >> 6) & 63u);
+    const uint32_t oh = (info >> 12) & 3u;
+    const uint32_t rowA = (m < 2) ? il : jrow;
+    const uint32_t rowB = (m < 2) ? jrow : il;
+    hmd_bf16x8 b0, b1;
+    if (g < 2u) {
+      b0 = W.phi[pr][g];
+      b1 = W.dsc[rowB][g];
+    } else {
+      b0 = W.dsc[rowA][g - 2u];
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        b1[e] = (__bf16)((g == 2u && (uint32_t)e == oh) ? 1.0f : 0.0f);
+    }
+    if (!valid) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) { b0[e] = (__bf16)0.0f; b1[e] = (__bf16)0.0f; }
+    }
+    float h[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      hmd_f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+      c = hmd_mfma16(S.ops[0][2 * t][lane], b0, c);
+      c = hmd_mfma16(S.ops[0][2 * t + 1][lane], b1, c);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float z = c[r] + bias1[t][r];
+        h[t][r] = z * hmd_sigmoid(z);
+      }
+    }
+    hmd_bf16x8 H0, H1;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      H0[e] = (__bf16)h[e >> 2][e & 3];
+      H1[e] = (__bf16)h[2 + (e >> 2)][e & 3];
+    }
+    float acc_y = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      hmd_f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+      c = hmd_mfma16(S.ops[1][2 * t][lane], H0, c);
+      c = hmd_mfma16(S.ops[1][2 * t + 1][lane], H1, c);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float z = c[r] + bias2[t][r];
+        acc_y += w3[t][r] * (z * hmd_sigmoid(z));
+      }
+    }
+    acc_y += hmd_shfl_xor(acc_y, 16);
+    acc_y += hmd_shfl_xor(acc_y, 32);
+    y[m] = acc_y + S.vec[192];
+  }
+  // lanes of group 0 apply pair l%16, lanes of group 1 pair 16 + l%16
+  if (g < 2u) {
+    const uint32_t info = g ? info1 : info0;
+    if ((info >> 14) & 1u) {
+      const float4 geo = W.geo[16u * g + pl];
+      const float ysum = g ? (y[1] + y[3]) : (y[0] + y[2]);
+      const float gp = p.force_scale * geo.x * 0.5f * ysum;
+      const uint32_t ia = info & 63u, jk = (info >> 6) & 63u;
+      const float rh[3] = {geo.y, geo.z, geo.w};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const long long f = hmd_to_fixed(gp * rh[d], HMD_NN_FORCE_SHIFT);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&acc[0][d][ia]),
+                  static_cast<unsigned long long>(f));
+        atomicAdd(reinterpret_cast<unsigned long long*>(&acc[1][d][jk]),
+                  static_cast<unsigned long long>(-f));
+      }
+    }
+  }
+  hmd_wave_sync();
 }
 
 extern "C" __global__ void __launch_bounds__(256)
 hmd_nn_forward(const NnPassParams p) {
   __shared__ HmdNnShared S;
-  __shared__ HmdNnWave Wv[4];
+  __shared__ HmdNnFwdWave Wv[4];
   __shared__ long long s_acc[4][2][3][HMD_WAVE];
   hmd_nn_load_shared(p, S, 2);
   const uint32_t lane = hmd_lane();
@@ -499,21 +624,32 @@ hmd_nn_forward(const NnPassParams p) {
   const uint32_t w0 = gw * per;
   const uint32_t w1 = min(w0 + per, nw);
   if (w0 >= w1) return;
-  HmdNnWave& W = Wv[wv];
+  HmdNnFwdWave& W = Wv[wv];
   long long(*acc)[3][HMD_WAVE] = s_acc[wv];
   const float4* ps = reinterpret_cast<const float4*>(p.posq_sorted);
   const uint32_t stride = p.n_pad;
+  const uint32_t g = lane >> 4;
+  float bias1[4][4], bias2[4][4], w3[4][4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      bias1[t][r] = S.vec[16 * t + 4 * g + r];
+      bias2[t][r] = S.vec[64 + 16 * t + 4 * g + r];
+      w3[t][r] = S.vec[128 + 16 * t + 4 * g + r];
+    }
 #pragma unroll
   for (int d = 0; d < 3; ++d) { acc[0][d][lane] = 0; acc[1][d][lane] = 0; }
 
   uint32_t cur_c = HMD_SENTINEL, si = 0;
   bool vi = false;
-  float xi = 0, yi = 0, zi = 0;
+  HmdNnShift sh{false, 0.0f, 0.0f, 0.0f};
+  float4 ai = make_float4(0, 0, 0, 0);
   for (uint32_t w = w0; w < w1; ++w) {
     const uint32_t slot = p.work[w];
     const uint32_t c = slot / p.cap_entries;
+    hmd_wave_sync();
     if (c != cur_c) {
-      hmd_wave_sync();
       if (cur_c != HMD_SENTINEL) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -525,22 +661,23 @@ hmd_nn_forward(const NnPassParams p) {
       cur_c = c;
       si = c * HMD_CLUSTER + lane;
       vi = si < p.n;
-      hmd_nn_load_i(p, W, si, lane, false);
-      const float4 a = ps[si];
-      xi = a.x; yi = a.y; zi = a.z;
+      sh = hmd_nn_cluster_shift(p, c);
+      ai = hmd_nn_apply_shift(ps[si], sh, p.L);
+      hmd_nn_load_i(p, W, si, ai, lane);
     }
     const uint32_t my_j = p.entry_j[(size_t)slot * HMD_ENTRY + lane];
     const uint32_t jj = my_j != HMD_SENTINEL ? my_j : 0u;
-    hmd_nn_load_j(p, W, jj, lane, false);
-    const float4 pj = ps[jj];
+    const float4 pj = hmd_nn_apply_shift(ps[jj], sh, p.L);
+    hmd_nn_load_j(p, W, jj, pj, lane);
     hmd_wave_sync();
     uint32_t qn = 0;
     for (uint32_t kb = 0; kb < HMD_ENTRY; kb += 16) {
       if (hmd_readlane(my_j, kb) == HMD_SENTINEL) break;
-      qn = hmd_nn_compact(p, W, my_j, pj, xi, yi, zi, vi, si, c, kb, qn, lane);
+      qn = hmd_nn_compact(p, W, my_j, pj, ai.x, ai.y, ai.z, vi, si, c, sh.on, kb, qn, lane);
       hmd_wave_sync();
       uint32_t head = 0;
-      for (; qn - head >= 32u; head += 32u) hmd_nn_forward_batch(p, S, W, acc, head, 32u, lane);
+      for (; qn - head >= 32u; head += 32u)
+        hmd_nn_forward_batch(p, S, W, acc, head, 32u, sh.on, bias1, bias2, w3, lane);
       const uint32_t rem = qn - head;
       const uint16_t carry = lane < rem ? W.queue[head + lane] : (uint16_t)0;
       hmd_wave_sync();
@@ -548,7 +685,7 @@ hmd_nn_forward(const NnPassParams p) {
       qn = rem;
       hmd_wave_sync();
     }
-    if (qn > 0) hmd_nn_forward_batch(p, S, W, acc, 0, qn, lane);
+    if (qn > 0) hmd_nn_forward_batch(p, S, W, acc, 0, qn, sh.on, bias1, bias2, w3, lane);
     hmd_wave_sync();
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -718,7 +855,8 @@ hmd_nn_backward(const NnPassParams p) {
 
   uint32_t cur_c = HMD_SENTINEL, si = 0;
   bool vi = false;
-  float xi = 0, yi = 0, zi = 0;
+  HmdNnShift sh{false, 0.0f, 0.0f, 0.0f};
+  float4 ai = make_float4(0, 0, 0, 0);
   for (uint32_t w = w0; w < w1; ++w) {
     const uint32_t slot = p.work[w];
     const uint32_t c = slot / p.cap_entries;
@@ -727,20 +865,22 @@ hmd_nn_backward(const NnPassParams p) {
       cur_c = c;
       si = c * HMD_CLUSTER + lane;
       vi = si < p.n;
-      hmd_nn_load_i(p, W, si, lane, true);
-      const float4 a = ps[si];
-      xi = a.x; yi = a.y; zi = a.z;
+      sh = hmd_nn_cluster_shift(p, c);
+      ai = hmd_nn_apply_shift(ps[si], sh, p.L);
+      hmd_nn_load_i(p, W, si, ai, lane);
+      W.res[lane] = hmd_nn_resid(p, si);
     }
     const uint32_t my_j = p.entry_j[(size_t)slot * HMD_ENTRY + lane];
     const uint32_t jj = my_j != HMD_SENTINEL ? my_j : 0u;
     hmd_wave_sync();
-    hmd_nn_load_j(p, W, jj, lane, true);
-    const float4 pj = ps[jj];
+    const float4 pj = hmd_nn_apply_shift(ps[jj], sh, p.L);
+    hmd_nn_load_j(p, W, jj, pj, lane);
+    W.res[HMD_WAVE + lane] = hmd_nn_resid(p, jj);
     hmd_wave_sync();
     uint32_t qn = 0;
     for (uint32_t kb = 0; kb < HMD_ENTRY; kb += 16) {
       if (hmd_readlane(my_j, kb) == HMD_SENTINEL) break;
-      qn = hmd_nn_compact(p, W, my_j, pj, xi, yi, zi, vi, si, c, kb, qn, lane);
+      qn = hmd_nn_compact(p, W, my_j, pj, ai.x, ai.y, ai.z, vi, si, c, sh.on, kb, qn, lane);
       hmd_wave_sync();
       uint32_t head = 0;
       for (; qn - head >= 32u; head += 32u)

@@ -124,14 +124,22 @@ def test_adam_step_matches_oracle(gpu_ctx):
 
 
 def test_correction_is_deterministic_and_momentum_free(gpu_ctx, water):
+    """Same positions and list: bit-identical (fixed-point sums).  After a
+    forced rebuild the list is laid out differently, which regroups the f32
+    partial sums of the descriptors; the correction then agrees to bf16
+    rounding of its inputs."""
     sysw, ff, part, field, integ = water
     net = make_net(gpu_ctx, scale=2e-3, seed=13)
     a = net.forward(field)
-    field.request_rebuild()
     b = net.forward(field)
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    field.request_rebuild()
+    c = net.forward(field)
+    err = rel_rms(c, a)
     net_f = np.abs(a.astype(np.float64).sum(axis=0)).max()
-    print(f"net correction force {net_f:.3e} vs sum |dF| {np.abs(a).sum():.3e}")
+    print(f"after a forced rebuild rel RMS {err:.2e}; net correction force {net_f:.3e} "
+          f"vs sum |dF| {np.abs(a).sum():.3e}")
+    assert err <= NET_TOL
     assert net_f <= 1e-6 * np.abs(a).sum()
     net.close()
 
