@@ -226,6 +226,8 @@ __device__ inline float hmd_box_gap2(float4 ca, float4 ha, float4 cb, float4 hb,
 #define HMD_NBL_MAX_CHUNKS 1024u    // 64 j-clusters per chunk: 65,536 clusters
 #define HMD_NBL_CAND_CAP 512u
 
+typedef __attribute__((address_space(4))) const float4 hmd_const_f4;
+
 // Inclusive prefix sum over the 64 lanes.
 __device__ inline uint32_t hmd_wave_scan_incl(uint32_t v, uint32_t lane) {
 #pragma unroll
@@ -239,8 +241,6 @@ __device__ inline uint32_t hmd_wave_scan_incl(uint32_t v, uint32_t lane) {
 extern "C" __global__ void __launch_bounds__(256, 3)
 hmd_nbl_build(const NblBuildParams p) {
   if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
-  __shared__ float4 s_i[HMD_CLUSTER];
-  __shared__ float4 s_sub[16];
   __shared__ uint64_t s_mask[HMD_NBL_MAX_CHUNKS];
   __shared__ uint32_t s_pre[HMD_NBL_MAX_CHUNKS + 1];
   __shared__ uint32_t s_cand[HMD_NBL_CAND_CAP];
@@ -250,15 +250,16 @@ hmd_nbl_build(const NblBuildParams p) {
   const uint32_t lane = hmd_lane();
   const float4* ps = reinterpret_cast<const float4*>(p.posq_sorted);
   const float4* bb = reinterpret_cast<const float4*>(p.bbox);
-  if (wave == 0) {
-    s_i[lane] = ps[c * HMD_CLUSTER + lane];
-    if (lane < 16) s_sub[lane] = reinterpret_cast<const float4*>(p.sub_bbox)[16 * c + lane];
-  }
   const bool ortho = p.orthorhombic_fast != 0;
   const float ilx = 1.0f / p.L.lx, ily = 1.0f / p.L.ly, ilz = 1.0f / p.L.lz;
   const float rs2_prune = p.r_search_sq * 1.0002f;   // conservative prune
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const uint32_t n_chunks = (p.n_clusters - c + HMD_WAVE - 1) / HMD_WAVE;
+  // The i-cluster's atoms and sub-boxes are read at wave-uniform addresses
+  // in the hit tests: read them through the constant address space (scalar
+  // loads into SGPRs) rather than as LDS broadcasts.
+  const hmd_const_f4* ci4 = (const hmd_const_f4*)(ps + c * HMD_CLUSTER);
+  const hmd_const_f4* csub = (const hmd_const_f4*)(reinterpret_cast<const float4*>(p.sub_bbox) + 16 * c);
 
   // candidate masks, 64 j-clusters per ballot
   {
@@ -341,7 +342,7 @@ hmd_nbl_build(const NblBuildParams p) {
       if (sj < p.n) {
 #pragma unroll
         for (uint32_t k = 0; k < 8; ++k) {
-          const float g2 = hmd_box_gap2(s_sub[2 * k], s_sub[2 * k + 1], pj, zero, p.L, ortho,
+          const float g2 = hmd_box_gap2(csub[2 * k], csub[2 * k + 1], pj, zero, p.L, ortho,
                                         ilx, ily, ilz);
           smask |= (g2 <= rs2_prune) ? (1u << k) : 0u;
         }
@@ -356,7 +357,7 @@ hmd_nbl_build(const NblBuildParams p) {
 #pragma unroll 4
           for (uint32_t t = 0; t < 8; ++t) {
             const uint32_t i = 8 * k + t;
-            const float4 a = s_i[i];
+            const float4 a = ci4[i];
             const float r2 = ortho
                                  ? hmd_r2_ortho(a.x, a.y, a.z, pj.x, pj.y, pj.z, p.L, ilx, ily, ilz)
                                  : hmd_r2_exact(a.x, a.y, a.z, pj.x, pj.y, pj.z, p.L);
