@@ -107,6 +107,58 @@ static uint32_t lines_per_block(uint32_t n) {
   return std::max(1u, std::min(64u, L));
 }
 
+// Power-of-two line lengths with a compile-time kernel: index 0..4 for
+// 16, 32, 64, 128, 256 (else -1), and the second-stage radix N2.
+static int fft_p2_index(uint32_t n) {
+  switch (n) {
+    case 16: return 0;
+    case 32: return 1;
+    case 64: return 2;
+    case 128: return 3;
+    case 256: return 4;
+    default: return -1;
+  }
+}
+static const uint32_t kP2N2[5] = {4, 4, 8, 8, 16};
+static const KernelId kP2Axis[5] = {K_hmd_fft_axis_16, K_hmd_fft_axis_32, K_hmd_fft_axis_64,
+                                    K_hmd_fft_axis_128, K_hmd_fft_axis_256};
+static const KernelId kP2R2c[5] = {K_hmd_fft_r2c_16, K_hmd_fft_r2c_32, K_hmd_fft_r2c_64,
+                                   K_hmd_fft_r2c_128, K_hmd_fft_r2c_256};
+static const KernelId kP2C2r[5] = {K_hmd_fft_c2r_16, K_hmd_fft_c2r_32, K_hmd_fft_c2r_64,
+                                   K_hmd_fft_c2r_128, K_hmd_fft_c2r_256};
+
+// Strided axis of length n over n_outer slabs of `cols` adjacent lines:
+// column tiles of width W (stage-1 capacity 2 x 256 / N2, 64 KiB of LDS),
+// at least about one block per compute unit.
+static ForceField::FftPass plan_axis(uint32_t n, uint32_t n_outer, uint32_t cols, uint32_t cus) {
+  ForceField::FftPass ps;
+  const int ix = fft_p2_index(n);
+  if (ix < 0) return ps;
+  const uint32_t wmax = std::min(512u / kP2N2[ix], 8192u / n);
+  uint32_t tiles = std::max(ceil_div(cols, wmax), ceil_div(cus, n_outer));
+  tiles = std::min(tiles, cols);
+  ps.p2 = true;
+  ps.k = kP2Axis[ix];
+  ps.W = ceil_div(cols, tiles);
+  ps.grid = n_outer * ceil_div(cols, ps.W);
+  ps.shmem = (size_t)n * ps.W * 8;
+  return ps;
+}
+
+// Contiguous lines (the c axis, N = n2 / 2 complex points each).
+static ForceField::FftPass plan_lines(uint32_t N, uint32_t n_lines, uint32_t cus, bool r2c) {
+  ForceField::FftPass ps;
+  const int ix = fft_p2_index(N);
+  if (ix < 0) return ps;
+  const uint32_t wmax = std::min(512u / kP2N2[ix], 8192u / N);
+  ps.p2 = true;
+  ps.k = r2c ? kP2R2c[ix] : kP2C2r[ix];
+  ps.W = std::max(1u, std::min(wmax, ceil_div(n_lines, cus)));
+  ps.grid = ceil_div(n_lines, ps.W);
+  ps.shmem = (size_t)N * ps.W * 8;
+  return ps;
+}
+
 // ---------------------------------------------------------- construction --
 ForceField::ForceField(Context& ctx, Particles& part, const hmd_forcefield_desc& d)
     : c(ctx), p(part) {
@@ -332,8 +384,17 @@ void ForceField::init_spme(const hmd_forcefield_desc& d) {
     u_self_total += u;
   }
   up(u_self, us);
-  const uint32_t La = lines_per_block(n0);
-  partial_blocks = ceil_div((size_t)n1 * nch, La);
+  const uint32_t cus = (uint32_t)c.num_cus;
+  pass_r2c = plan_lines(n2 / 2, n0 * n1, cus, true);
+  pass_c2r = plan_lines(n2 / 2, n0 * n1, cus, false);
+  pass_b = plan_axis(n1, n0, nch, cus);
+  pass_a = plan_axis(n0, n1, nch, cus);
+  if (pass_a.p2) {
+    partial_blocks = pass_a.grid;
+  } else {
+    const uint32_t La = lines_per_block(n0);
+    partial_blocks = ceil_div((size_t)n1 * nch, La);
+  }
   partials.alloc(2 * (size_t)partial_blocks);
 }
 
@@ -491,8 +552,13 @@ void ForceField::spme_forces(bool fev) {
     FftParams& r = q;
     r.n_stages = (uint32_t)rad_c.size();
     for (size_t k = 0; k < rad_c.size(); ++k) r.radix[k] = rad_c[k];
-    launch(c, K_hmd_fft_r2c_c, ceil_div(q.n_lines, q.lines_per_block), 256,
-           2 * (size_t)q.lines_per_block * (q.n + 1) * 8, q);
+    if (pass_r2c.p2) {
+      q.lines_per_block = pass_r2c.W;
+      launch(c, pass_r2c.k, pass_r2c.grid, 256, pass_r2c.shmem, q);
+    } else {
+      launch(c, K_hmd_fft_r2c_c, ceil_div(q.n_lines, q.lines_per_block), 256,
+             2 * (size_t)q.lines_per_block * (q.n + 1) * 8, q);
+    }
   }
   // 2-4. b forward, a forward * G * a inverse, b inverse
   FftParams fb = f;
@@ -509,7 +575,16 @@ void ForceField::spme_forces(bool fev) {
   fb.n_stages = (uint32_t)rad_b.size();
   for (size_t k = 0; k < rad_b.size(); ++k) fb.radix[k] = rad_b[k];
   const size_t sh_b = 2 * (size_t)fb.lines_per_block * (n1 + 1) * 8;
-  launch(c, K_hmd_fft_axis, ceil_div(fb.n_lines, fb.lines_per_block), 256, sh_b, fb);
+  auto launch_b = [&]() {
+    if (pass_b.p2) {
+      FftParams q = fb;
+      q.lines_per_block = pass_b.W;
+      launch(c, pass_b.k, pass_b.grid, 256, pass_b.shmem, q);
+    } else {
+      launch(c, K_hmd_fft_axis, ceil_div(fb.n_lines, fb.lines_per_block), 256, sh_b, fb);
+    }
+  };
+  launch_b();
   FftParams fa = f;
   fa.data = mesh_c.ptr;
   fa.n = n0;
@@ -526,10 +601,15 @@ void ForceField::spme_forces(bool fev) {
   set_radix(rad_a);
   fa.n_stages = f.n_stages;
   for (int k = 0; k < 8; ++k) fa.radix[k] = f.radix[k];
-  launch(c, K_hmd_fft_axis, ceil_div(fa.n_lines, fa.lines_per_block), 256,
-         2 * (size_t)fa.lines_per_block * (n0 + 1) * 8, fa);
+  if (pass_a.p2) {
+    fa.lines_per_block = pass_a.W;
+    launch(c, pass_a.k, pass_a.grid, 256, pass_a.shmem, fa);
+  } else {
+    launch(c, K_hmd_fft_axis, ceil_div(fa.n_lines, fa.lines_per_block), 256,
+           2 * (size_t)fa.lines_per_block * (n0 + 1) * 8, fa);
+  }
   fb.sign = +1;
-  launch(c, K_hmd_fft_axis, ceil_div(fb.n_lines, fb.lines_per_block), 256, sh_b, fb);
+  launch_b();
   // 5. c axis, C2R
   {
     FftParams q = f;
@@ -543,8 +623,13 @@ void ForceField::spme_forces(bool fev) {
     q.sign = +1;
     q.n_stages = (uint32_t)rad_c.size();
     for (size_t k = 0; k < rad_c.size(); ++k) q.radix[k] = rad_c[k];
-    launch(c, K_hmd_fft_c2r_c, ceil_div(q.n_lines, q.lines_per_block), 256,
-           2 * (size_t)q.lines_per_block * (q.n + 1) * 8, q);
+    if (pass_c2r.p2) {
+      q.lines_per_block = pass_c2r.W;
+      launch(c, pass_c2r.k, pass_c2r.grid, 256, pass_c2r.shmem, q);
+    } else {
+      launch(c, K_hmd_fft_c2r_c, ceil_div(q.n_lines, q.lines_per_block), 256,
+             2 * (size_t)q.lines_per_block * (q.n + 1) * 8, q);
+    }
   }
   c.mark("spme_fft_solve");
   SpmeReduceParams rp{partials.ptr, partial_blocks, spme_out.ptr};

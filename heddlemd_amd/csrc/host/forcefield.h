@@ -46,6 +46,15 @@ struct ForceField {
   DevBuf<float> slow_f, slow_e, slow_w;
   DevBuf<double> partials, spme_out;
   std::vector<uint32_t> rad_a, rad_b, rad_c;
+  // Launch shape of one FFT pass: compile-time power-of-two kernel (p2) or
+  // the generic mixed-radix one.
+  struct FftPass {
+    bool p2 = false;
+    KernelId k = K_COUNT;
+    uint32_t W = 0, grid = 0;
+    size_t shmem = 0;
+  };
+  FftPass pass_r2c, pass_c2r, pass_b, pass_a;
   double u_self_total = 0.0;
 
   // ---- learned correction of the reciprocal forces (optional)

@@ -29,6 +29,11 @@
   X(hmd_spme_spread_p6) X(hmd_spme_gather_p6)                                \
   X(hmd_spme_spread_p8) X(hmd_spme_gather_p8)                                \
   X(hmd_fft_r2c_c) X(hmd_fft_c2r_c) X(hmd_fft_axis) X(hmd_spme_reduce)       \
+  X(hmd_fft_axis_16) X(hmd_fft_r2c_16) X(hmd_fft_c2r_16)                     \
+  X(hmd_fft_axis_32) X(hmd_fft_r2c_32) X(hmd_fft_c2r_32)                     \
+  X(hmd_fft_axis_64) X(hmd_fft_r2c_64) X(hmd_fft_c2r_64)                     \
+  X(hmd_fft_axis_128) X(hmd_fft_r2c_128) X(hmd_fft_c2r_128)                  \
+  X(hmd_fft_axis_256) X(hmd_fft_r2c_256) X(hmd_fft_c2r_256)                  \
   X(hmd_vv_kick_drift) X(hmd_vv_kick)                                        \
   X(hmd_settle_snapshot) X(hmd_settle_positions) X(hmd_settle_velocities)    \
   X(hmd_lan_drift_half) X(hmd_lan_ou) X(hmd_counter_increment)               \

@@ -145,7 +145,7 @@ struct Mlp {
     NnPassParams q = pass_params(ff);
     launch(c, K_hmd_nn_backward, (uint32_t)c.num_cus, 256, 0, q);
     NnGradReduceParams gp{partials.ptr, n_waves, grad.ptr};
-    launch(c, K_hmd_nn_grad_reduce, ceil_div(HMD_NN_PARAMS, 256), 256, 0, gp);
+    launch(c, K_hmd_nn_grad_reduce, HMD_NN_PARAMS, 256, 0, gp);
   }
 
   void adam() {

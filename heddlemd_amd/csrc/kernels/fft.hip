@@ -296,3 +296,268 @@ hmd_spme_reduce(const SpmeReduceParams p) {
     p.out[1] = 0.5 * s[1][0];
   }
 }
+
+// ---------------------------------------------------------------------------
+// Power-of-two line lengths n = N1 N2 (16 ... 256): compile-time two-stage
+// transform, n = N2 n1 + n2 -> k = k1 + N1 k2:
+//   stage 1: for each n2, an N1-point DFT over n1 in registers, times the
+//            twiddle e^{s 2 pi i n2 k1 / n}, stored at N1 n2 + k1;
+//   stage 2: for each k1, an N2-point DFT over n2 in registers, in place,
+//            giving X[k1 + N1 k2] in natural order.
+// The block works on a tile of W lines ("columns") of one slab of the mesh,
+// tile[k * W + w], loaded and stored with coalesced accesses; no runtime
+// radix, no integer division in the butterflies, no scratch arrays.
+
+template <int SG>
+__device__ inline float2 hmd_mul_si(float2 a) {   // a * (SG i)
+  return SG < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+template <int SG>
+__device__ inline void hmd_dft4(float2* v) {
+  const float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+  const float2 t2 = cadd(v[1], v[3]), t3 = hmd_mul_si<SG>(csub(v[1], v[3]));
+  v[0] = cadd(t0, t2);
+  v[2] = csub(t0, t2);
+  v[1] = cadd(t1, t3);
+  v[3] = csub(t1, t3);
+}
+
+template <int SG>
+__device__ inline void hmd_dft8(float2* v) {
+  float2 e[4] = {v[0], v[2], v[4], v[6]};
+  float2 o[4] = {v[1], v[3], v[5], v[7]};
+  hmd_dft4<SG>(e);
+  hmd_dft4<SG>(o);
+  const float h = 0.70710678118654752f;   // cos(pi/4)
+  const float2 o1 = cmul(o[1], make_float2(h, SG * h));
+  const float2 o2 = hmd_mul_si<SG>(o[2]);
+  const float2 o3 = cmul(o[3], make_float2(-h, SG * h));
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);
+  v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);
+  v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);
+  v[7] = csub(e[3], o3);
+}
+
+template <int SG>
+__device__ inline void hmd_dft16(float2* v) {
+  float2 e[8], o[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    e[k] = v[2 * k];
+    o[k] = v[2 * k + 1];
+  }
+  hmd_dft8<SG>(e);
+  hmd_dft8<SG>(o);
+  // cos and sin of 2 pi k / 16
+  const float C[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
+                      0.0f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f};
+  const float S[8] = {0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f,
+                      1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const float2 t = cmul(o[k], make_float2(C[k], SG * S[k]));
+    v[k] = cadd(e[k], t);
+    v[k + 8] = csub(e[k], t);
+  }
+}
+
+template <int N, int SG>
+__device__ inline void hmd_dft(float2* v) {
+  if constexpr (N == 4) hmd_dft4<SG>(v);
+  else if constexpr (N == 8) hmd_dft8<SG>(v);
+  else hmd_dft16<SG>(v);
+}
+
+// In-place transform of the W columns of tile[k * W + w] (k < N1 N2);
+// requires W * N2 <= 2 * blockDim.x.  tw[m] = e^{-2 pi i m / n}.
+template <int N1, int N2, int SG>
+__device__ inline void hmd_fft_cols(float2* tile, uint32_t W, const float2* tw) {
+  constexpr uint32_t N = (uint32_t)(N1 * N2);
+  float2 v[2][N1];
+  const uint32_t ntask1 = W * (uint32_t)N2;
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const uint32_t task = threadIdx.x + (uint32_t)a * blockDim.x;
+    if (task < ntask1) {
+      const uint32_t w = task % W, n2 = task / W;
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) v[a][n1] = tile[((uint32_t)N2 * n1 + n2) * W + w];
+      hmd_dft<N1, SG>(v[a]);
+#pragma unroll
+      for (int k1 = 1; k1 < N1; ++k1) {
+        float2 t = tw[(n2 * (uint32_t)k1) & (N - 1u)];
+        if (SG > 0) t = cconj(t);
+        v[a][k1] = cmul(v[a][k1], t);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const uint32_t task = threadIdx.x + (uint32_t)a * blockDim.x;
+    if (task < ntask1) {
+      const uint32_t w = task % W, n2 = task / W;
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) tile[((uint32_t)N1 * n2 + k1) * W + w] = v[a][k1];
+    }
+  }
+  __syncthreads();
+  const uint32_t ntask2 = W * (uint32_t)N1;
+  for (uint32_t task = threadIdx.x; task < ntask2; task += blockDim.x) {
+    const uint32_t w = task % W, k1 = task / W;
+    float2 u[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) u[n2] = tile[((uint32_t)N1 * n2 + k1) * W + w];
+    hmd_dft<N2, SG>(u);
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) tile[(k1 + (uint32_t)N1 * k2) * W + w] = u[k2];
+  }
+  __syncthreads();
+}
+
+// Passes 2-4 for power-of-two axis lengths.  Block = (outer index, column
+// tile): columns are the line_minor lines of one slab that are adjacent in
+// memory (the c index), elem_stride steps along the transformed axis,
+// major_stride along the outer axis.  With apply_influence (a axis: outer =
+// b, column = c) the line is multiplied by G(k) between the forward and the
+// inverse transform, with fixed-order virial / energy block partials.
+template <int N1, int N2>
+__device__ inline void hmd_fft_axis_slab(const FftParams& p) {
+  extern __shared__ float2 s_tile[];
+  __shared__ double s_red[2][HMD_FFT_THREADS];
+  constexpr uint32_t N = (uint32_t)(N1 * N2);
+  const uint32_t W = p.lines_per_block;
+  const uint32_t n_tiles = (p.line_minor + W - 1) / W;
+  const uint32_t outer = blockIdx.x / n_tiles;
+  const uint32_t c0 = (blockIdx.x - outer * n_tiles) * W;
+  const uint32_t Wc = min(W, p.line_minor - c0);
+  float2* data = reinterpret_cast<float2*>(p.data);
+  const size_t base = (size_t)outer * p.major_stride + c0;
+  for (uint32_t t = threadIdx.x; t < N * W; t += blockDim.x) {
+    const uint32_t k = t / W, w = t - k * W;
+    s_tile[t] = (w < Wc) ? data[base + (size_t)k * p.elem_stride + w] : make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  const float2* tw = reinterpret_cast<const float2*>(p.twiddle);
+  if (p.sign < 0)
+    hmd_fft_cols<N1, N2, -1>(s_tile, W, tw);
+  else
+    hmd_fft_cols<N1, N2, +1>(s_tile, W, tw);
+  if (p.apply_influence) {
+    double vir = 0.0, ene = 0.0;
+    for (uint32_t t = threadIdx.x; t < N * W; t += blockDim.x) {
+      const uint32_t ka = t / W, w = t - ka * W;
+      if (w >= Wc) continue;
+      const uint32_t kb = outer, kc = c0 + w;
+      float G, vf;
+      hmd_influence(p, ka, kb, kc, G, vf);
+      const float2 x = s_tile[t];
+      const float hw = (kc == 0u || (p.n2 % 2u == 0u && 2u * kc == p.n2)) ? 1.0f : 2.0f;
+      const float r2 = x.x * x.x + x.y * x.y;
+      vir += (double)(hw * vf * r2);
+      ene += (double)(hw * G * r2);
+      s_tile[t] = make_float2(G * x.x, G * x.y);
+    }
+    s_red[0][threadIdx.x] = vir;
+    s_red[1][threadIdx.x] = ene;
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) {
+        s_red[0][threadIdx.x] += s_red[0][threadIdx.x + s];
+        s_red[1][threadIdx.x] += s_red[1][threadIdx.x + s];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      p.partials[2 * blockIdx.x] = s_red[0][0];
+      p.partials[2 * blockIdx.x + 1] = s_red[1][0];
+    }
+    hmd_fft_cols<N1, N2, +1>(s_tile, W, tw);
+  }
+  for (uint32_t t = threadIdx.x; t < N * W; t += blockDim.x) {
+    const uint32_t k = t / W, w = t - k * W;
+    if (w < Wc) data[base + (size_t)k * p.elem_stride + w] = s_tile[t];
+  }
+}
+
+// Pass 1 for n2 = 2N: W contiguous real lines per block viewed as N complex
+// values each, transformed, then split into the n2/2 + 1 half spectrum.
+template <int N1, int N2>
+__device__ inline void hmd_fft_r2c_slab(const FftParams& p) {
+  extern __shared__ float2 s_tile[];
+  constexpr uint32_t N = (uint32_t)(N1 * N2);
+  const uint32_t W = p.lines_per_block;
+  const uint32_t l0 = blockIdx.x * W;
+  const uint32_t Wc = min(W, p.n_lines - l0);
+  const float2* in = reinterpret_cast<const float2*>(p.data);
+  for (uint32_t t = threadIdx.x; t < N * W; t += blockDim.x) {
+    const uint32_t w = t / N, m = t - w * N;
+    s_tile[m * W + w] = (w < Wc) ? in[(size_t)(l0 + w) * N + m] : make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  hmd_fft_cols<N1, N2, -1>(s_tile, W, reinterpret_cast<const float2*>(p.twiddle));
+  const uint32_t nch = N + 1;
+  float2* out = reinterpret_cast<float2*>(p.out);
+  const float2* wr = reinterpret_cast<const float2*>(p.twiddle_r2c);
+  for (uint32_t t = threadIdx.x; t < Wc * nch; t += blockDim.x) {
+    const uint32_t w = t / nch, m = t - w * nch;
+    const float2 zk = s_tile[(m & (N - 1u)) * W + w];
+    const float2 zc = cconj(s_tile[((N - m) & (N - 1u)) * W + w]);
+    const float2 e = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
+    const float2 o = cmul_si(make_float2(0.5f * (zk.x - zc.x), 0.5f * (zk.y - zc.y)), -1);
+    out[(size_t)(l0 + w) * nch + m] = cadd(e, cmul(wr[m], o));
+  }
+}
+
+// Pass 5 for n2 = 2N (unnormalised complex -> real).
+template <int N1, int N2>
+__device__ inline void hmd_fft_c2r_slab(const FftParams& p) {
+  extern __shared__ float2 s_tile[];
+  constexpr uint32_t N = (uint32_t)(N1 * N2);
+  const uint32_t W = p.lines_per_block;
+  const uint32_t l0 = blockIdx.x * W;
+  const uint32_t Wc = min(W, p.n_lines - l0);
+  const uint32_t nch = N + 1;
+  const float2* in = reinterpret_cast<const float2*>(p.data);
+  const float2* wr = reinterpret_cast<const float2*>(p.twiddle_r2c);
+  for (uint32_t t = threadIdx.x; t < N * W; t += blockDim.x) {
+    const uint32_t w = t / N, m = t - w * N;
+    float2 z = make_float2(0.0f, 0.0f);
+    if (w < Wc) {
+      const float2* x = in + (size_t)(l0 + w) * nch;
+      const float2 xk = x[m], xc = cconj(x[N - m]);
+      const float2 ee = cadd(xk, xc);
+      const float2 oo = cmul(csub(xk, xc), cconj(wr[m]));
+      z = cadd(ee, cmul_si(oo, +1));
+    }
+    s_tile[m * W + w] = z;
+  }
+  __syncthreads();
+  hmd_fft_cols<N1, N2, +1>(s_tile, W, reinterpret_cast<const float2*>(p.twiddle));
+  float2* out = reinterpret_cast<float2*>(p.out);
+  for (uint32_t t = threadIdx.x; t < Wc * N; t += blockDim.x) {
+    const uint32_t w = t / N, m = t - w * N;
+    out[(size_t)(l0 + w) * N + m] = s_tile[m * W + w];
+  }
+}
+
+#define HMD_FFT_P2(N, N1, N2)                                                       \
+  extern "C" __global__ void __launch_bounds__(256) hmd_fft_axis_##N(const FftParams p) { \
+    hmd_fft_axis_slab<N1, N2>(p);                                                   \
+  }                                                                                 \
+  extern "C" __global__ void __launch_bounds__(256) hmd_fft_r2c_##N(const FftParams p) { \
+    hmd_fft_r2c_slab<N1, N2>(p);                                                    \
+  }                                                                                 \
+  extern "C" __global__ void __launch_bounds__(256) hmd_fft_c2r_##N(const FftParams p) { \
+    hmd_fft_c2r_slab<N1, N2>(p);                                                    \
+  }
+HMD_FFT_P2(16, 4, 4)
+HMD_FFT_P2(32, 8, 4)
+HMD_FFT_P2(64, 8, 8)
+HMD_FFT_P2(128, 16, 8)
+HMD_FFT_P2(256, 16, 16)

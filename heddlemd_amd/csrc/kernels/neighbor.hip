@@ -228,6 +228,24 @@ __device__ inline float hmd_box_gap2(float4 ca, float4 ha, float4 cb, float4 hb,
 
 typedef __attribute__((address_space(4))) const float4 hmd_const_f4;
 
+// The same two measures without image arithmetic, for cluster pairs whose
+// differences are known to need none (image index 0 on every axis): the
+// difference is then bit-identical to the one the minimum-image sequence
+// returns.
+__device__ inline float hmd_box_gap2_direct(float4 ca, float4 ha, float4 pb) {
+  const float ex = fmaxf(fabsf(ca.x - pb.x) - ha.x, 0.0f);
+  const float ey = fmaxf(fabsf(ca.y - pb.y) - ha.y, 0.0f);
+  const float ez = fmaxf(fabsf(ca.z - pb.z) - ha.z, 0.0f);
+  return ex * ex + ey * ey + ez * ez;
+}
+
+__device__ inline float hmd_r2_direct(float xi, float yi, float zi, float xj, float yj,
+                                      float zj) {
+#pragma clang fp contract(off)
+  const float dx = xi - xj, dy = yi - yj, dz = zi - zj;
+  return dx * dx + dy * dy + dz * dz;
+}
+
 // Inclusive prefix sum over the 64 lanes.
 __device__ inline uint32_t hmd_wave_scan_incl(uint32_t v, uint32_t lane) {
 #pragma unroll
@@ -260,6 +278,8 @@ hmd_nbl_build(const NblBuildParams p) {
   // loads into SGPRs) rather than as LDS broadcasts.
   const hmd_const_f4* ci4 = (const hmd_const_f4*)(ps + c * HMD_CLUSTER);
   const hmd_const_f4* csub = (const hmd_const_f4*)(reinterpret_cast<const float4*>(p.sub_bbox) + 16 * c);
+  const hmd_const_f4* cbox = (const hmd_const_f4*)bb;
+  const float4 cib = cbox[2 * c], hib = cbox[2 * c + 1];
 
   // candidate masks, 64 j-clusters per ballot
   {
@@ -337,13 +357,22 @@ hmd_nbl_build(const NblBuildParams p) {
         const uint32_t sn = s_cand[q + 4] * HMD_CLUSTER + lane;
         pj_next = sn < p.n ? ps[sn] : zero;
       }
+      // No wrap: every atom pair of the two clusters is closer than 0.49 L
+      // along each axis (orthorhombic box), so the minimum image of each
+      // difference is the difference itself (image index 0) and the exact
+      // f32 distance needs no image arithmetic.
+      const float4 cjb = cbox[2 * jcl], hjb = cbox[2 * jcl + 1];
+      const bool nowrap = ortho && fabsf(cjb.x - cib.x) + hjb.x + hib.x < 0.49f * p.L.lx &&
+                          fabsf(cjb.y - cib.y) + hjb.y + hib.y < 0.49f * p.L.ly &&
+                          fabsf(cjb.z - cib.z) + hjb.z + hib.z < 0.49f * p.L.lz;
       // Sub-box mask: which 8-atom i-sub-clusters this j-atom may reach.
       uint32_t smask = 0;
       if (sj < p.n) {
 #pragma unroll
         for (uint32_t k = 0; k < 8; ++k) {
-          const float g2 = hmd_box_gap2(csub[2 * k], csub[2 * k + 1], pj, zero, p.L, ortho,
-                                        ilx, ily, ilz);
+          const float g2 = nowrap ? hmd_box_gap2_direct(csub[2 * k], csub[2 * k + 1], pj)
+                                  : hmd_box_gap2(csub[2 * k], csub[2 * k + 1], pj, zero, p.L,
+                                                 ortho, ilx, ily, ilz);
           smask |= (g2 <= rs2_prune) ? (1u << k) : 0u;
         }
       }
@@ -354,14 +383,24 @@ hmd_nbl_build(const NblBuildParams p) {
         const bool want = !hit && ((smask >> k) & 1u);
         if (__ballot(want) == 0) continue;
         if (want) {
+          if (nowrap) {
+#pragma unroll
+            for (uint32_t t = 0; t < 8; ++t) {
+              const uint32_t i = 8 * k + t;
+              const float4 a = ci4[i];
+              const float r2 = hmd_r2_direct(a.x, a.y, a.z, pj.x, pj.y, pj.z);
+              hit = hit || (i < ilim && r2 <= p.r_search_sq);
+            }
+          } else {
 #pragma unroll 4
-          for (uint32_t t = 0; t < 8; ++t) {
-            const uint32_t i = 8 * k + t;
-            const float4 a = ci4[i];
-            const float r2 = ortho
-                                 ? hmd_r2_ortho(a.x, a.y, a.z, pj.x, pj.y, pj.z, p.L, ilx, ily, ilz)
-                                 : hmd_r2_exact(a.x, a.y, a.z, pj.x, pj.y, pj.z, p.L);
-            hit = hit || (i < ilim && r2 <= p.r_search_sq);
+            for (uint32_t t = 0; t < 8; ++t) {
+              const uint32_t i = 8 * k + t;
+              const float4 a = ci4[i];
+              const float r2 =
+                  ortho ? hmd_r2_ortho(a.x, a.y, a.z, pj.x, pj.y, pj.z, p.L, ilx, ily, ilz)
+                        : hmd_r2_exact(a.x, a.y, a.z, pj.x, pj.y, pj.z, p.L);
+              hit = hit || (i < ilim && r2 <= p.r_search_sq);
+            }
           }
         }
       }

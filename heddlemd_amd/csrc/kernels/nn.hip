@@ -916,12 +916,13 @@ hmd_nn_backward(const NnPassParams p) {
   out[11264 + lane] = G.b3;
 }
 
-// Canonical gradient from the per-wave partials: one thread per parameter,
-// a fixed summation order over waves (and lanes), FP64; added to grad.
+// Canonical gradient from the per-wave partials: one workgroup per
+// parameter; threads take waves w = tid, tid + 256, ... (and the lanes that
+// hold the parameter), then a fixed LDS tree: FP64, a fixed summation order.
 extern "C" __global__ void __launch_bounds__(256)
 hmd_nn_grad_reduce(const NnGradReduceParams p) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= HMD_NN_PARAMS) return;
+  __shared__ double s_sum[256];
+  const uint32_t k = blockIdx.x;
   uint32_t off = 0, n_lanes = 1;
   if (k < 64u * 51u) {                                   // W1[h][cin]
     const uint32_t h = k / 51u, cin = k % 51u;
@@ -948,12 +949,18 @@ hmd_nn_grad_reduce(const NnGradReduceParams p) {
     off = 11264u;
     n_lanes = 64;
   }
-  double s = 0.0;
-  for (uint32_t w = 0; w < p.n_waves; ++w) {
+  double acc = 0.0;
+  for (uint32_t w = threadIdx.x; w < p.n_waves; w += blockDim.x) {
     const float* src = p.partials + (size_t)w * HMD_NN_PARTIAL + off;
-    for (uint32_t l = 0; l < n_lanes; ++l) s += (double)src[l];
+    for (uint32_t l = 0; l < n_lanes; ++l) acc += (double)src[l];
   }
-  p.grad[k] += s;
+  s_sum[threadIdx.x] = acc;
+  __syncthreads();
+  for (uint32_t st = blockDim.x / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s_sum[threadIdx.x] += s_sum[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) p.grad[k] += s_sum[0];
 }
 
 // ------------------------------------------------------------ helpers ---

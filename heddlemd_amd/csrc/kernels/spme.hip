@@ -179,27 +179,42 @@ __device__ inline void hmd_spread_body(const SpmeSpreadParams& p) {
       float t[3];
       int g[3];
       hmd_spme_setup(pq, p.L, p.n0, p.n1, p.n2, t, g);
+      // Which of the P support planes along each axis fall in this brick;
+      // an atom of a neighbour bin usually reaches it along no plane of
+      // some axis, and is then skipped before its weights are computed.
+      uint32_t ma = 0, mb = 0, mc = 0;
+#pragma unroll
+      for (int d = 0; d < P; ++d) {
+        int la = g[0] - d - o0, lb = g[1] - d - o1, lc = g[2] - d - o2;
+        la += (la < 0) ? (int)p.n0 : 0;
+        lb += (lb < 0) ? (int)p.n1 : 0;
+        lc += (lc < 0) ? (int)p.n2 : 0;
+        ma |= (la < (int)p.b0) ? (1u << d) : 0u;
+        mb |= (lb < (int)p.b1) ? (1u << d) : 0u;
+        mc |= (lc < (int)p.b2) ? (1u << d) : 0u;
+      }
+      if (ma == 0u || mb == 0u || mc == 0u) continue;
       float wa[P], wb[P], wc[P];
       hmd_bspline<P>(t[0], wa);
       hmd_bspline<P>(t[1], wb);
       hmd_bspline<P>(t[2], wc);
 #pragma unroll
       for (int da = 0; da < P; ++da) {
+        if (!((ma >> da) & 1u)) continue;
         int la = g[0] - da - o0;
         la += (la < 0) ? (int)p.n0 : 0;
-        if (la >= (int)p.b0) continue;
         const float qa = pq.w * wa[da];
 #pragma unroll
         for (int db = 0; db < P; ++db) {
+          if (!((mb >> db) & 1u)) continue;
           int lb = g[1] - db - o1;
           lb += (lb < 0) ? (int)p.n1 : 0;
-          if (lb >= (int)p.b1) continue;
           const float qab = qa * wb[db];
 #pragma unroll
           for (int dc = 0; dc < P; ++dc) {
+            if (!((mc >> dc) & 1u)) continue;
             int lc = g[2] - dc - o2;
             lc += (lc < 0) ? (int)p.n2 : 0;
-            if (lc >= (int)p.b2) continue;
             long long v = hmd_to_fixed(qab * wc[dc], HMD_SPREAD_SHIFT);
             if (v != 0)
               atomicAdd(reinterpret_cast<unsigned long long*>(
