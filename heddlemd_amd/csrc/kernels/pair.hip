@@ -9,9 +9,9 @@
 // Mapping (wave64, VALU FP32; no MFMA -- there is no dense contraction):
 //  * one wave = one 64-atom i-cluster, one i-atom per lane, held in VGPRs;
 //  * an entry's 64 j-atoms are gathered with one load per lane (and shifted
-//    to the image nearest the cluster centre), then broadcast one per step
-//    with readlane into scalar registers, so each step evaluates 64 pairs
-//    with no memory access and no shuffles;
+//    to the image nearest the cluster centre) into a per-wave LDS table,
+//    then broadcast one per step by a wave-uniform LDS read, so each step
+//    evaluates 64 pairs with no global memory access and no shuffles;
 //  * slot combinations (LJ, Coulomb, energies) are compile-time variants,
 //    and LJ is skipped for j-types with no LJ partner (uniform branch);
 //  * the j-side reaction forces of 8 consecutive j-atoms are summed across
@@ -39,8 +39,9 @@ struct HmdPairState {
 // broadcast one per step with readlane.
 template <bool FEV, bool LJ, bool COUL, bool SHIFT>
 __device__ inline void hmd_pair_entry(const PairParams& p, const float2* s_lj,
-                                      HmdPairState& st, uint32_t slot, float cx,
-                                      float cy, float cz, uint32_t lane) {
+                                      float4* s_jp, uint2* s_jt, HmdPairState& st,
+                                      uint32_t slot, float cx, float cy, float cz,
+                                      uint32_t lane) {
   const float4* ps = reinterpret_cast<const float4*>(p.posq_sorted);
   const HmdLattice L = p.L;
   const size_t ebase = (size_t)slot * HMD_ENTRY;
@@ -58,6 +59,14 @@ __device__ inline void hmd_pair_entry(const PairParams& p, const float2* s_lj,
   const float two_alpha_sqrtpi = p.alpha * HMD_TWO_OVER_SQRT_PI;
   long long* acc = p.acc;
   const uint32_t stride = p.acc_stride;
+  // The entry's j-atoms go to a per-wave LDS table; each step reads one
+  // record at a wave-uniform address (an LDS broadcast, no VALU work).
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  s_jp[lane] = pj;
+  s_jt[lane] = make_uint2(my_j, tj_l);
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
 
   for (uint32_t kb = 0; kb < HMD_ENTRY; kb += 8) {
     if (hmd_readlane(my_j, kb) == HMD_SENTINEL) break;
@@ -65,10 +74,10 @@ __device__ inline void hmd_pair_entry(const PairParams& p, const float2* s_lj,
 #pragma unroll
     for (uint32_t u = 0; u < 8; ++u) {
       const uint32_t k = kb + u;
-      const uint32_t j = hmd_readlane(my_j, k);              // wave-uniform
-      const float xj = hmd_readlane_f(pj.x, k);
-      const float yj = hmd_readlane_f(pj.y, k);
-      const float zj = hmd_readlane_f(pj.z, k);
+      const float4 bj = s_jp[k];
+      const uint2 jt2 = s_jt[k];
+      const uint32_t j = __builtin_amdgcn_readfirstlane(jt2.x);   // wave-uniform
+      const float xj = bj.x, yj = bj.y, zj = bj.z;
       float dx = st.xi - xj, dy = st.yi - yj, dz = st.zi - zj;
       if (!SHIFT) hmd_min_image(dx, dy, dz, L);
       const float r2 = dx * dx + dy * dy + dz * dz;
@@ -79,7 +88,7 @@ __device__ inline void hmd_pair_entry(const PairParams& p, const float2* s_lj,
       const float inv_r2 = inv_r * inv_r;
       float factor = 0.0f, energy = 0.0f;
       if (COUL) {
-        const float qj = hmd_readlane_f(pj.w, k);
+        const float qj = bj.w;
         const float qq = st.kqi * qj;
         const float ar = p.alpha * (r2s * inv_r);
         const float gauss = __builtin_amdgcn_exp2f(r2s * c_gauss);
@@ -92,7 +101,7 @@ __device__ inline void hmd_pair_entry(const PairParams& p, const float2* s_lj,
         if (FEV) energy = qq * inv_r * erfc_ar;
       }
       if (LJ) {
-        const uint32_t tj = hmd_readlane(tj_l, k);
+        const uint32_t tj = __builtin_amdgcn_readfirstlane(jt2.y);
         if ((p.lj_type_mask >> tj) & 1u) {                  // uniform branch
           const float2 cc = s_lj[st.ti * p.n_types + tj];
           const float ir6 = inv_r2 * inv_r2 * inv_r2;
@@ -157,6 +166,8 @@ __device__ inline void hmd_flush_i(const PairParams& p, const HmdPairState& st) 
 template <bool FEV, bool LJ, bool COUL>
 __device__ inline void hmd_pair_body(const PairParams& p) {
   __shared__ float2 s_lj[HMD_MAX_TYPES * HMD_MAX_TYPES];
+  __shared__ float4 s_jp[4][HMD_ENTRY];
+  __shared__ uint2 s_jt[4][HMD_ENTRY];
   const uint32_t nt2 = p.n_types * p.n_types;
   for (uint32_t k = threadIdx.x; k < nt2; k += blockDim.x)
     s_lj[k] = reinterpret_cast<const float2*>(p.lj_c6c12)[k];
@@ -210,10 +221,13 @@ __device__ inline void hmd_pair_body(const PairParams& p) {
       }
       st.fix = st.fiy = st.fiz = st.eiv = st.wiv = 0.0f;
     }
+    const uint32_t wv = threadIdx.x / HMD_WAVE;
     if (shift)
-      hmd_pair_entry<FEV, LJ, COUL, true>(p, s_lj, st, slot, cx, cy, cz, lane);
+      hmd_pair_entry<FEV, LJ, COUL, true>(p, s_lj, s_jp[wv], s_jt[wv], st, slot, cx, cy, cz,
+                                          lane);
     else
-      hmd_pair_entry<FEV, LJ, COUL, false>(p, s_lj, st, slot, cx, cy, cz, lane);
+      hmd_pair_entry<FEV, LJ, COUL, false>(p, s_lj, s_jp[wv], s_jt[wv], st, slot, cx, cy, cz,
+                                           lane);
   }
   hmd_flush_i<FEV>(p, st);
 }
