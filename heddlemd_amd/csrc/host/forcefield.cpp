@@ -277,17 +277,19 @@ void ForceField::init_neighbor_list(const hmd_forcefield_desc& d) {
   ortho_fast = orthorhombic && 0.5 * L.lx > 1.001 * r_search &&
                0.5 * L.ly > 1.001 * r_search && 0.5 * L.lz > 1.001 * r_search;
 
-  // Fine Morton grid: ~0.47 Bohr^-3 (water) -> ~1.5 atoms per cell.
+  // Fine ordering grid of 2^bits cells per axis (the Hilbert curve needs a
+  // power of two), at least as fine as ~1.6 atoms per cell.
   const double vol = (double)L.lx * L.ly * L.lz;
   const double cell = std::cbrt(1.6 * vol / n);
   uint32_t gmax = 1;
   for (int k = 0; k < 3; ++k) {
     double w = k == 0 ? L.lx : (k == 1 ? L.ly : L.lz);
-    g[k] = (uint32_t)std::max(1.0, std::min(1024.0, std::floor(w / cell)));
-    gmax = std::max(gmax, g[k]);
+    gmax = std::max(gmax, (uint32_t)std::max(1.0, std::min(1024.0, std::floor(w / cell))));
   }
-  uint32_t bits = 0;
-  while ((1u << bits) < gmax) ++bits;
+  grid_bits = 1;
+  while ((1u << grid_bits) < gmax) ++grid_bits;
+  for (int k = 0; k < 3; ++k) g[k] = 1u << grid_bits;
+  const uint32_t bits = grid_bits;
   n_keys = 1u << (3 * bits);
   if (n_keys > (1u << 24)) throw Error(HMD_ERR_UNSUPPORTED, "fine grid too large");
 
@@ -430,7 +432,7 @@ void ForceField::request_rebuild() {
 void ForceField::rebuild_chain() {
   const uint32_t* gate = status.ptr;
   const HmdLattice L = p.lattice;
-  NblKeyParams kp{p.posq.ptr, n, L, g[0], g[1], g[2], atom_key.ptr, cell_count.ptr,
+  NblKeyParams kp{p.posq.ptr, n, L, g[0], g[1], g[2], grid_bits, atom_key.ptr, cell_count.ptr,
                   cell_start.ptr, sorted_id.ptr, gate};
   launch(c, K_hmd_nbl_keys, ceil_div(n, 256), 256, 0, kp);
   cell_scan.run(c, cell_count.ptr, cell_start.ptr, gate, HMD_STATUS_DISPLACED);

@@ -26,6 +26,7 @@ struct ForceField {
   float r_skin = 0, r_search = 0, r_search_sq = 0;
   bool orthorhombic = true, ortho_fast = false;
   uint32_t g[3] = {1, 1, 1};
+  uint32_t grid_bits = 1;
   uint32_t n_keys = 1, n_clusters = 0, n_pad = 0, cap_entries = 0;
   DevBuf<uint32_t> status;          // [0] flags [1] generation [2] high water
   DevBuf<uint32_t> atom_key, cell_count, cell_start, sorted_id, slot_of, type_sorted;

@@ -41,7 +41,8 @@ struct NblKeyParams {
   const float* posq;        // canonical float4[n]
   uint32_t n;
   HmdLattice L;
-  uint32_t g0, g1, g2;      // fine cells per axis
+  uint32_t g0, g1, g2;      // fine cells per axis (2^bits each)
+  uint32_t bits;
   uint32_t* atom_key;       // [n]
   uint32_t* cell_count;     // [n_keys]  (zero on entry, zero again after scatter)
   const uint32_t* cell_start;  // [n_keys+1] (scatter only)

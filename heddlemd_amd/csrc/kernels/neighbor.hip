@@ -32,6 +32,39 @@ __device__ inline uint32_t hmd_spread3(uint32_t x) {
   return x;
 }
 
+// Position of cell (x, y, z) along the 3D Hilbert curve of a 2^bits grid:
+// the coordinates are transformed into the curve's "transpose" form
+// (Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707, 2004)
+// and their bits interleaved.  Unlike the Morton (Z) order, consecutive
+// cells are always face neighbours, so runs of 64 atoms form compact
+// clusters with no long jumps.
+__device__ inline uint32_t hmd_hilbert3(uint32_t x, uint32_t y, uint32_t z, uint32_t bits) {
+  uint32_t X[3] = {x, y, z};
+  const uint32_t M = 1u << (bits - 1u);
+  for (uint32_t Q = M; Q > 1u; Q >>= 1) {
+    const uint32_t P = Q - 1u;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      if (X[i] & Q) {
+        X[0] ^= P;
+      } else {
+        const uint32_t t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  X[1] ^= X[0];
+  X[2] ^= X[1];
+  uint32_t t = 0;
+  for (uint32_t Q = M; Q > 1u; Q >>= 1)
+    if (X[2] & Q) t ^= Q - 1u;
+  X[0] ^= t;
+  X[1] ^= t;
+  X[2] ^= t;
+  return (hmd_spread3(X[0]) << 2) | (hmd_spread3(X[1]) << 1) | hmd_spread3(X[2]);
+}
+
 // Fine cell of a (wrapped) position, clamped like
 // parallelepiped_cell_indices (kernels/neighbor.cu:11-36).
 __device__ inline void hmd_fine_cell(float x, float y, float z,
@@ -76,7 +109,7 @@ hmd_nbl_keys(const NblKeyParams p) {
   float4 a = reinterpret_cast<const float4*>(p.posq)[i];
   uint32_t c0, c1, c2;
   hmd_fine_cell(a.x, a.y, a.z, p.L, p.g0, p.g1, p.g2, c0, c1, c2);
-  uint32_t key = (hmd_spread3(c0) << 2) | (hmd_spread3(c1) << 1) | hmd_spread3(c2);
+  const uint32_t key = hmd_hilbert3(c0, c1, c2, p.bits);
   p.atom_key[i] = key;
   atomicAdd(&p.cell_count[key], 1u);
 }
@@ -260,6 +293,7 @@ extern "C" __global__ void __launch_bounds__(256, 3)
 hmd_nbl_build(const NblBuildParams p) {
   if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
   __shared__ uint64_t s_mask[HMD_NBL_MAX_CHUNKS];
+  __shared__ uint64_t s_nowrap[HMD_NBL_MAX_CHUNKS];
   __shared__ uint32_t s_pre[HMD_NBL_MAX_CHUNKS + 1];
   __shared__ uint32_t s_cand[HMD_NBL_CAND_CAP];
   __shared__ uint64_t s_hit[HMD_NBL_CAND_CAP];
@@ -271,6 +305,7 @@ hmd_nbl_build(const NblBuildParams p) {
   const bool ortho = p.orthorhombic_fast != 0;
   const float ilx = 1.0f / p.L.lx, ily = 1.0f / p.L.ly, ilz = 1.0f / p.L.lz;
   const float rs2_prune = p.r_search_sq * 1.0002f;   // conservative prune
+  const float rs2_accept = p.r_search_sq * 0.9996f;  // conservative accept
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const uint32_t n_chunks = (p.n_clusters - c + HMD_WAVE - 1) / HMD_WAVE;
   // The i-cluster's atoms and sub-boxes are read at wave-uniform addresses
@@ -278,21 +313,30 @@ hmd_nbl_build(const NblBuildParams p) {
   // loads into SGPRs) rather than as LDS broadcasts.
   const hmd_const_f4* ci4 = (const hmd_const_f4*)(ps + c * HMD_CLUSTER);
   const hmd_const_f4* csub = (const hmd_const_f4*)(reinterpret_cast<const float4*>(p.sub_bbox) + 16 * c);
-  const hmd_const_f4* cbox = (const hmd_const_f4*)bb;
-  const float4 cib = cbox[2 * c], hib = cbox[2 * c + 1];
 
   // candidate masks, 64 j-clusters per ballot
   {
     const float4 ci = bb[2 * c], hi = bb[2 * c + 1];
     for (uint32_t ch = wave; ch < n_chunks; ch += 4) {
       const uint32_t jc = c + ch * HMD_WAVE + lane;
-      bool cand = false;
+      bool cand = false, nowrap = false;
       if (jc < p.n_clusters) {
         const float4 cj = bb[2 * jc], hj = bb[2 * jc + 1];
         cand = hmd_box_gap2(ci, hi, cj, hj, p.L, ortho, ilx, ily, ilz) <= rs2_prune;
+        // No wrap: every atom pair of the two clusters is closer than 0.49 L
+        // along each axis (orthorhombic box), so the minimum image of each
+        // difference is the difference itself (image index 0) and the exact
+        // f32 distance needs no image arithmetic.
+        nowrap = ortho && fabsf(cj.x - ci.x) + hj.x + hi.x < 0.49f * p.L.lx &&
+                 fabsf(cj.y - ci.y) + hj.y + hi.y < 0.49f * p.L.ly &&
+                 fabsf(cj.z - ci.z) + hj.z + hi.z < 0.49f * p.L.lz;
       }
       const uint64_t m = __ballot(cand);
-      if (lane == 0) s_mask[ch] = m;
+      const uint64_t nw = __ballot(nowrap);
+      if (lane == 0) {
+        s_mask[ch] = m;
+        s_nowrap[ch] = nw;
+      }
     }
   }
   __syncthreads();
@@ -325,6 +369,7 @@ hmd_nbl_build(const NblBuildParams p) {
   }
   const size_t base_entry = (size_t)c * p.cap_entries;
   const uint32_t imax = min(p.n - c * HMD_CLUSTER, HMD_CLUSTER);
+  const uint32_t full_sub = (imax >= 64u) ? 0xFFu : ((1u << (imax / 8u)) - 1u);
   uint32_t n_entries = 0;       // completed entries (uniform)
   uint32_t fill = 0;            // slots used in the current entry (uniform)
   uint64_t cur_mask = 0, next_mask = 0;
@@ -337,7 +382,9 @@ hmd_nbl_build(const NblBuildParams p) {
       const uint64_t m = s_mask[ch];
       if ((m >> lane) & 1ull) {
         const uint32_t pos = s_pre[ch] + (uint32_t)__builtin_popcountll(m & lt_mask);
-        if (pos >= base && pos < base + cnt) s_cand[pos - base] = c + ch * HMD_WAVE + lane;
+        if (pos >= base && pos < base + cnt)
+          s_cand[pos - base] = (c + ch * HMD_WAVE + lane) |
+                               ((uint32_t)((s_nowrap[ch] >> lane) & 1ull) << 31);
       }
     }
     __syncthreads();
@@ -346,39 +393,51 @@ hmd_nbl_build(const NblBuildParams p) {
     uint32_t q = wave;
     float4 pj_next = zero;
     if (q < cnt) {
-      const uint32_t sj = s_cand[q] * HMD_CLUSTER + lane;
+      const uint32_t sj = (s_cand[q] & 0x7FFFFFFFu) * HMD_CLUSTER + lane;
       if (sj < p.n) pj_next = ps[sj];
     }
     for (; q < cnt; q += 4) {
-      const uint32_t jcl = s_cand[q];
+      const uint32_t cw = s_cand[q];
+      const uint32_t jcl = cw & 0x7FFFFFFFu;
+      const bool nowrap = (cw >> 31) != 0u;
       const uint32_t sj = jcl * HMD_CLUSTER + lane;
       const float4 pj = pj_next;
       if (q + 4 < cnt) {
-        const uint32_t sn = s_cand[q + 4] * HMD_CLUSTER + lane;
+        const uint32_t sn = (s_cand[q + 4] & 0x7FFFFFFFu) * HMD_CLUSTER + lane;
         pj_next = sn < p.n ? ps[sn] : zero;
       }
-      // No wrap: every atom pair of the two clusters is closer than 0.49 L
-      // along each axis (orthorhombic box), so the minimum image of each
-      // difference is the difference itself (image index 0) and the exact
-      // f32 distance needs no image arithmetic.
-      const float4 cjb = cbox[2 * jcl], hjb = cbox[2 * jcl + 1];
-      const bool nowrap = ortho && fabsf(cjb.x - cib.x) + hjb.x + hib.x < 0.49f * p.L.lx &&
-                          fabsf(cjb.y - cib.y) + hjb.y + hib.y < 0.49f * p.L.ly &&
-                          fabsf(cjb.z - cib.z) + hjb.z + hib.z < 0.49f * p.L.lz;
-      // Sub-box mask: which 8-atom i-sub-clusters this j-atom may reach.
-      uint32_t smask = 0;
+      // Sub-box masks: which 8-atom i-sub-clusters this j-atom may reach
+      // (nearest point of the box within the search radius), and which it
+      // surely reaches (farthest corner within it, with a relative margin of
+      // 2e-4 that exceeds any f32 rounding of the exact test, so that every
+      // atom of the sub-cluster passes the exact test as well).
+      uint32_t smask = 0, amask = 0;
       if (sj < p.n) {
+        if (nowrap) {
 #pragma unroll
-        for (uint32_t k = 0; k < 8; ++k) {
-          const float g2 = nowrap ? hmd_box_gap2_direct(csub[2 * k], csub[2 * k + 1], pj)
-                                  : hmd_box_gap2(csub[2 * k], csub[2 * k + 1], pj, zero, p.L,
-                                                 ortho, ilx, ily, ilz);
-          smask |= (g2 <= rs2_prune) ? (1u << k) : 0u;
+          for (uint32_t k = 0; k < 8; ++k) {
+            const float4 cs = csub[2 * k], hs = csub[2 * k + 1];
+            const float dx = fabsf(cs.x - pj.x), dy = fabsf(cs.y - pj.y), dz = fabsf(cs.z - pj.z);
+            const float ex = fmaxf(dx - hs.x, 0.0f), ey = fmaxf(dy - hs.y, 0.0f),
+                        ez = fmaxf(dz - hs.z, 0.0f);
+            const float fx = dx + hs.x, fy = dy + hs.y, fz = dz + hs.z;
+            smask |= (ex * ex + ey * ey + ez * ez <= rs2_prune) ? (1u << k) : 0u;
+            amask |= (fx * fx + fy * fy + fz * fz <= rs2_accept) ? (1u << k) : 0u;
+          }
+        } else {
+#pragma unroll
+          for (uint32_t k = 0; k < 8; ++k) {
+            const float g2 = hmd_box_gap2(csub[2 * k], csub[2 * k + 1], pj, zero, p.L, ortho,
+                                          ilx, ily, ilz);
+            smask |= (g2 <= rs2_prune) ? (1u << k) : 0u;
+          }
         }
       }
       // i-atoms allowed for this j: sorted index below j's, and valid.
       const uint32_t ilim = (jcl == c) ? min(lane, imax) : imax;
-      bool hit = false;
+      // a sure sub-cluster of another cluster decides the hit (its eight
+      // atoms are valid and all within r_search)
+      bool hit = (jcl != c) && (amask & full_sub) != 0u;
       for (uint32_t k = 0; k < 8; ++k) {
         const bool want = !hit && ((smask >> k) & 1u);
         if (__ballot(want) == 0) continue;
@@ -414,7 +473,7 @@ hmd_nbl_build(const NblBuildParams p) {
       for (uint32_t k = 0; k < cnt; ++k) {
         const uint64_t H = s_hit[k];
         if (H == 0) continue;
-        const uint32_t jcl = s_cand[k];
+        const uint32_t jcl = s_cand[k] & 0x7FFFFFFFu;
         if ((H >> lane) & 1ull) {
           const uint32_t pos = fill + (uint32_t)__builtin_popcountll(H & lt_mask);
           const uint32_t e = n_entries + pos / HMD_ENTRY;
