@@ -73,6 +73,29 @@ __device__ inline void hmd_bspline_deriv(float x, float w[P], float dw[P]) {
   }
 }
 
+// Order 4 in closed form (the cubic cardinal B-spline on its four unit
+// intervals; the same function as the recursion, fewer operations):
+// w[d] = M_4(d + t), dw[d] = M_4'(d + t).
+template <>
+__device__ inline void hmd_bspline<4>(float t, float w[4]) {
+  const float t2 = t * t, t3 = t2 * t, u = 1.0f - t;
+  const float s6 = 1.0f / 6.0f;
+  w[0] = t3 * s6;
+  w[1] = (1.0f + 3.0f * t + 3.0f * t2 - 3.0f * t3) * s6;
+  w[2] = (4.0f - 6.0f * t2 + 3.0f * t3) * s6;
+  w[3] = u * u * u * s6;
+}
+
+template <>
+__device__ inline void hmd_bspline_deriv<4>(float t, float w[4], float dw[4]) {
+  hmd_bspline<4>(t, w);
+  const float t2 = t * t, u = 1.0f - t;
+  dw[0] = 0.5f * t2;
+  dw[1] = 0.5f * (1.0f + 2.0f * t - 3.0f * t2);
+  dw[2] = 0.5f * (3.0f * t2 - 4.0f * t);
+  dw[3] = -0.5f * u * u;
+}
+
 // u = (s + 1/2) n, primary point g = floor(u) mod n, t = u - floor(u)
 // (spme_spread_gather.cuh:90-126).
 __device__ inline void hmd_spme_setup(float4 a, const HmdLattice& L,
@@ -151,6 +174,18 @@ hmd_spme_bin_scatter(const SpmeBinParams p) {
   }
 }
 
+// A contribution v (in units of 2^-24 e, |v| < 2^31) as the i64 count of
+// 2^-32 e rounded to nearest, from two i32 roundings: r1 = rint(v), the
+// remainder v - r1 (exact in f32, |.| <= 1/2) scaled by 2^8 and rounded,
+// and r1 * 2^8 + r2 -- the same value as rounding v * 2^8 directly, without
+// the long f32 -> i64 conversion sequence.
+__device__ inline long long hmd_fixed32_of24(float v) {
+  const float f1 = rintf(v);
+  const int r1 = (int)f1;
+  const int r2 = (int)rintf((v - f1) * 256.0f);
+  return ((long long)r1 << 8) + (long long)r2;
+}
+
 template <int P>
 __device__ inline void hmd_spread_body(const SpmeSpreadParams& p) {
   extern __shared__ long long s_acc[];
@@ -198,28 +233,31 @@ __device__ inline void hmd_spread_body(const SpmeSpreadParams& p) {
       hmd_bspline<P>(t[0], wa);
       hmd_bspline<P>(t[1], wb);
       hmd_bspline<P>(t[2], wc);
+      int lc_of[P];
+#pragma unroll
+      for (int dc = 0; dc < P; ++dc) {
+        int lc = g[2] - dc - o2;
+        lc_of[dc] = lc + ((lc < 0) ? (int)p.n2 : 0);
+      }
+      const float q24 = pq.w * 16777216.0f;   // charge in 2^-24 units
 #pragma unroll
       for (int da = 0; da < P; ++da) {
         if (!((ma >> da) & 1u)) continue;
         int la = g[0] - da - o0;
         la += (la < 0) ? (int)p.n0 : 0;
-        const float qa = pq.w * wa[da];
+        const float qa = q24 * wa[da];
 #pragma unroll
         for (int db = 0; db < P; ++db) {
           if (!((mb >> db) & 1u)) continue;
           int lb = g[1] - db - o1;
           lb += (lb < 0) ? (int)p.n1 : 0;
           const float qab = qa * wb[db];
+          long long* row = s_acc + ((uint32_t)la * p.b1 + (uint32_t)lb) * p.b2;
 #pragma unroll
           for (int dc = 0; dc < P; ++dc) {
             if (!((mc >> dc) & 1u)) continue;
-            int lc = g[2] - dc - o2;
-            lc += (lc < 0) ? (int)p.n2 : 0;
-            long long v = hmd_to_fixed(qab * wc[dc], HMD_SPREAD_SHIFT);
-            if (v != 0)
-              atomicAdd(reinterpret_cast<unsigned long long*>(
-                            &s_acc[((uint32_t)la * p.b1 + (uint32_t)lb) * p.b2 + (uint32_t)lc]),
-                        (unsigned long long)v);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&row[lc_of[dc]]),
+                      (unsigned long long)hmd_fixed32_of24(qab * wc[dc]));
           }
         }
       }
