@@ -159,6 +159,51 @@ static ForceField::FftPass plan_lines(uint32_t N, uint32_t n_lines, uint32_t cus
   return ps;
 }
 
+// The reference's A&S 7.1.26 erfc (src/forces/spme.rs:712-760), in FP64.
+static double erfc_as(double x) {
+  const double t = 1.0 / (1.0 + 0.3275911 * x);
+  const double poly =
+      t * (0.254829592 + t * (-0.284496736 + t * (1.421413741 + t * (-1.453152027 +
+                                                                        t * 1.061405429))));
+  return poly * std::exp(-x * x);
+}
+
+// Real-space Ewald kernel on HMD_COUL_BINS bins of width h = r_c/(bins - 1)
+// (bin b covers [b h, (b + 1) h); the last one lies beyond the cutoff): per
+// bin the cubic Hermite interpolant, in t = r/h - b, of the force factor
+// G(r) = [erfc(a r)/r + 2a/sqrt(pi) exp(-a^2 r^2)]/r^2 and of the energy
+// E(r) = erfc(a r)/r, from FP64 values and derivatives at both bin ends
+// (derivatives by a fourth-order central difference).
+void ForceField::build_coulomb_table(double a, double rc, hipStream_t s) {
+  const uint32_t nb = HMD_COUL_BINS;
+  const double h = rc / (nb - 1);
+  auto G = [a](double r) {
+    return (erfc_as(a * r) / r + 2.0 * a / std::sqrt(M_PI) * std::exp(-a * a * r * r)) / (r * r);
+  };
+  auto E = [a](double r) { return erfc_as(a * r) / r; };
+  std::vector<float> tab(2 * 4 * nb);
+  for (int f = 0; f < 2; ++f) {
+    auto F = [&](double r) { return f == 0 ? G(r) : E(r); };
+    std::vector<double> v(nb + 1), dv(nb + 1);
+    for (uint32_t k = 0; k <= nb; ++k) {
+      const double r = std::max((double)k, 0.5) * h;   // bin 0: no pair is that close
+      const double d = 1e-3 * h;
+      v[k] = F(r);
+      dv[k] = h * (F(r - 2 * d) - 8 * F(r - d) + 8 * F(r + d) - F(r + 2 * d)) / (12 * d);
+    }
+    for (uint32_t k = 0; k < nb; ++k) {
+      float* c = &tab[4 * (f * nb + k)];
+      c[0] = (float)v[k];
+      c[1] = (float)dv[k];
+      c[2] = (float)(3.0 * (v[k + 1] - v[k]) - 2.0 * dv[k] - dv[k + 1]);
+      c[3] = (float)(2.0 * (v[k] - v[k + 1]) + dv[k] + dv[k + 1]);
+    }
+  }
+  coul_table.alloc(tab.size(), false);
+  coul_table.upload(tab.data(), tab.size(), s);
+  coul_inv_h = (float)(1.0 / h);
+}
+
 // ---------------------------------------------------------- construction --
 ForceField::ForceField(Context& ctx, Particles& part, const hmd_forcefield_desc& d)
     : c(ctx), p(part) {
@@ -213,6 +258,7 @@ ForceField::ForceField(Context& ctx, Particles& part, const hmd_forcefield_desc&
   }
   lj_table.alloc(c6c12.size(), false);
   lj_table.upload(c6c12.data(), c6c12.size(), s);
+  if (coul_on) build_coulomb_table(d.alpha, cut, s);
 
   // ---- exclusions
   std::vector<uint32_t> eoff(n + 1, 0u), epar;
@@ -499,6 +545,8 @@ void ForceField::pair_forces(bool fev) {
   pp.lj_inv_delta = lj_inv_delta;
   pp.alpha = alpha;
   pp.k_coulomb = k_coulomb;
+  pp.coul_table = coul_table.ptr;
+  pp.coul_inv_h = coul_inv_h;
   pp.shift_margin = r_cut + 0.5f * r_skin;
   pp.acc = acc.ptr;
   pp.acc_stride = n_pad;

@@ -19,8 +19,14 @@ struct ForceField {
   float r_cut = 0, rc2 = 0, lj_rs2 = 0, lj_inv_delta = 0;
   float alpha = 0, k_coulomb = 1;
   DevBuf<float> lj_table;
+  DevBuf<float> coul_table;    // float4[2][HMD_COUL_BINS]
+  float coul_inv_h = 0;
+  void build_coulomb_table(double alpha, double r_cut, hipStream_t s);
   DevBuf<uint32_t> excl_off, excl_par;
-  uint32_t pair_blocks_per_cu = 8;
+#ifndef HMD_PAIR_BLOCKS_PER_CU
+#define HMD_PAIR_BLOCKS_PER_CU 24
+#endif
+  uint32_t pair_blocks_per_cu = HMD_PAIR_BLOCKS_PER_CU;
 
   // ---- neighbour list
   float r_skin = 0, r_search = 0, r_search_sq = 0;

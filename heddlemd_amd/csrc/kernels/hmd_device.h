@@ -134,6 +134,56 @@ __device__ inline float hmd_transpose_reduce8(const float v[8], uint32_t lane) {
   return c;
 }
 
+// DPP row operations (rows of 16 lanes): lane l receives the value of lane
+// l + n (row_shl:n), l - n (row_shr:n), or (l + n) mod 16 within its row
+// (row_ror:n).  Lanes whose source lies outside the row, or is disabled in
+// EXEC, receive 0: callers execute them with every lane enabled.
+template <int CTRL>
+__device__ inline float hmd_dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+#define HMD_DPP_ROW_SHL(n) (0x100 + (n))
+#define HMD_DPP_ROW_SHR(n) (0x110 + (n))
+#define HMD_DPP_ROW_ROR(n) (0x120 + (n))
+
+// Sum v[0..7] over the wave; the total of v[m] is left in lane 8m + 7 (other
+// lanes hold partial sums).  A transposed butterfly: every exchange stage
+// halves the number of values a lane carries.  The half-wave and row
+// exchanges are gfx950's v_permlane32_swap / v_permlane16_swap, which trade
+// the upper half (odd rows) of one register for the lower half (even rows) of
+// another in one instruction; the in-row stages are DPP operands of the adds.
+__device__ inline float hmd_wave_sum8(const float v[8], uint32_t lane) {
+  float a[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    // lanes 0-31: v[t] of lanes l and l+32; lanes 32-63: v[t+4] likewise
+    const auto s = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[t]),
+                                                    __float_as_uint(v[t + 4]), false, false);
+    a[t] = __uint_as_float(s[0]) + __uint_as_float(s[1]);
+  }
+  float b[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    // even rows: a[t] of rows 2q and 2q+1; odd rows: a[t+2] likewise
+    const auto s = __builtin_amdgcn_permlane16_swap(__float_as_uint(a[t]),
+                                                    __float_as_uint(a[t + 2]), false, false);
+    b[t] = __uint_as_float(s[0]) + __uint_as_float(s[1]);
+  }
+  // lanes with bit 3 clear keep b[0] and take the partner's b[0] (lane l^8)
+  const bool b3 = (lane & 8u) != 0u;
+  const float send = b3 ? b[0] : b[1];
+  const float keep = b3 ? b[1] : b[0];
+  float c = keep + hmd_dpp<HMD_DPP_ROW_ROR(8)>(send);
+  c += hmd_dpp<HMD_DPP_ROW_SHR(1)>(c);
+  c += hmd_dpp<HMD_DPP_ROW_SHR(2)>(c);
+  c += hmd_dpp<HMD_DPP_ROW_SHR(4)>(c);
+  // DPP reads a disabled source lane as 0: an opaque use here keeps every
+  // stage above in the converged code of the caller, so that none of them is
+  // sunk into a branch that only the lanes 8m + 7 take
+  asm volatile("" : "+v"(c));
+  return c;
+}
+
 __device__ inline float hmd_readlane_f(float v, uint32_t lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)lane));
 }

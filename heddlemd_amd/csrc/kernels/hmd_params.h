@@ -13,6 +13,7 @@
 #define HMD_ENTRY 64u               // j-atom slots per neighbour entry
 #define HMD_SENTINEL 0xFFFFFFFFu    // empty j slot
 #define HMD_FIXED_SHIFT 40          // pair accumulators: value * 2^40 in i64
+#define HMD_COUL_BINS 512u          // real-space Ewald table bins
 #define HMD_SPREAD_SHIFT 32         // mesh accumulators: value * 2^32 in i64
 
 // neighbour status word bits (kernels set them, the host reads them)
@@ -150,6 +151,11 @@ struct PairParams {
   float lj_rs2;                // switch start squared
   float lj_inv_delta;          // 1/(rc^2 - rs^2); 0 disables the switch
   float alpha, k_coulomb;
+  // real-space Ewald kernel, tabulated: float4[2][HMD_COUL_BINS] cubic
+  // coefficients per bin of width r_cut / (HMD_COUL_BINS - 1), force factor
+  // G(r) = [erfc(a r)/r + 2a/sqrt(pi) exp(-a^2 r^2)] / r^2 then energy erfc(a r)/r
+  const float* coul_table;
+  float coul_inv_h;            // 1 / bin width
   float shift_margin;          // r_cut + r_skin/2: per-cluster shift eligibility
   long long* acc;              // [5][n_pad] fixed-point F_x,F_y,F_z,E,W (sorted)
   uint32_t acc_stride;         // n_pad

@@ -5,6 +5,11 @@
 // Stegun 7.1.26 erfc (src/forces/spme.rs:712-760), exclusion scale 0 for
 // listed partners, branch-free max-cutoff mask, Newton's third law, and the
 // half/half energy and virial split (src/forces/jit_composed.rs:1302-1643).
+// The real-space Ewald force and energy functions of r (the A&S erfc
+// formula, evaluated in FP64 on the host) are read from a 512-bin cubic
+// Hermite table in LDS (relative interpolation error below 3e-7 for
+// r >= 0.09 nm), which replaces the exp, reciprocal and polynomial of the
+// closed form by one LDS gather and three FMAs per pair.
 //
 // Mapping (wave64, VALU FP32; no MFMA -- there is no dense contraction):
 //  * one wave = one 64-atom i-cluster, one i-atom per lane, held in VGPRs;
@@ -14,9 +19,15 @@
 //    evaluates 64 pairs with no global memory access and no shuffles;
 //  * slot combinations (LJ, Coulomb, energies) are compile-time variants,
 //    and LJ is skipped for j-types with no LJ partner (uniform branch);
+//  * entries whose j-atoms all lie in later clusters and carry no exclusion
+//    (all but the first one or two of a cluster) run a step variant with the
+//    cutoff test alone;
 //  * the j-side reaction forces of 8 consecutive j-atoms are summed across
-//    the wave by a transposed butterfly (8 values -> 1 per lane group), then
-//    8 lanes issue the fixed-point atomics;
+//    the wave by a transposed butterfly on the gfx950 lane swaps
+//    (v_permlane32_swap, v_permlane16_swap) and DPP row operations; the 3
+//    (with energy and virial 5) components of the 8 j-atoms are then packed
+//    into 24 (40) lanes that convert to fixed point and issue their atomics
+//    together;
 //  * i-forces stay in registers across every entry of the same cluster in the
 //    wave's contiguous share of the work list and are flushed once;
 //  * accumulation is i64 fixed point (2^40), so the per-atom totals do not
@@ -24,129 +35,169 @@
 #include "hmd_device.h"
 
 #define HMD_MAX_TYPES 16u
-#define HMD_TWO_OVER_SQRT_PI 1.1283791670955126f
-#define HMD_LOG2E 1.4426950408889634f
 
 struct HmdPairState {
   float xi, yi, zi, kqi;
-  uint32_t si, ti;
+  uint32_t si, ti_row;      // sorted index; type * n_types (LJ table row)
   bool vi;
   float fix, fiy, fiz, eiv, wiv;
 };
 
-// One entry: the 64 j-atoms of the entry are gathered once (one lane per
-// j), shifted to the image nearest the cluster centre when SHIFT, and then
-// broadcast one per step with readlane.
+// Steps over the j-atoms of one entry (64 pairs per step).  PLAIN entries
+// (every valid j-atom in a later cluster, no exclusion bit in any lane) need
+// only the cutoff test: i < j holds by construction, the i-atoms are all
+// valid (only the last cluster is partial, and it has no later cluster), and
+// empty slots carry a far-away position so that they fail the cutoff test.
+template <bool FEV, bool LJ, bool COUL, bool SHIFT, bool PLAIN>
+__device__ inline void hmd_pair_steps(const PairParams& p, const float4* s_lj,
+                                      const float4* s_ct, const float4* s_jp,
+                                      const uint2* s_jt,
+                                      HmdPairState& st, uint32_t my_j, uint64_t my_ex,
+                                      uint64_t lj_bits, uint32_t lane) {
+  const HmdLattice L = p.L;
+  const float x_max = (float)HMD_COUL_BINS - 0.01f;   // last bin: beyond r_cut, masked
+  const float lj_rs2_d = p.lj_rs2 * p.lj_inv_delta;
+  const float lj_12d = 12.0f * p.lj_inv_delta;
+  constexpr uint32_t NC = FEV ? 5u : 3u;        // reaction components per j
+  const uint32_t grp = lane & 7u;
+  const uint32_t comp = 7u - grp;               // component packed into this lane
+  long long* acc = p.acc;
+  const uint32_t stride = p.acc_stride;
+
+  for (uint32_t kb = 0; kb < HMD_ENTRY; kb += 8) {
+    if (hmd_readlane(my_j, kb) == HMD_SENTINEL) break;
+    // jr[c][u]: this lane's contribution to component c of j-atom kb + u
+    // (forces with the i-side sign; the j-side reaction is their negation)
+    float jr[NC][8];
+#pragma unroll
+    for (uint32_t u = 0; u < 8; ++u) {
+      const uint32_t k = kb + u;
+      const float4 bj = s_jp[k];
+      float dx = st.xi - bj.x, dy = st.yi - bj.y, dz = st.zi - bj.z;
+      if (!SHIFT) hmd_min_image(dx, dy, dz, L);
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      bool ok = r2 <= p.rc2;
+      if (!PLAIN) {
+        const uint32_t j = __builtin_amdgcn_readfirstlane(s_jt[k].x);   // wave-uniform
+        const bool excluded = (my_ex >> k) & 1ull;
+        ok = ok && (j != HMD_SENTINEL) && st.vi && (st.si < j) && !excluded;
+      }
+      // PLAIN: every pair has r2 > 0 (distinct atoms; empty slots far away),
+      // so the masked-out pairs evaluate to finite values without a guard
+      const float r2s = PLAIN ? r2 : (ok ? r2 : 1.0f);
+      const float inv_r = __builtin_amdgcn_rsqf(r2s);
+      const float inv_r2 = inv_r * inv_r;
+      float factor = 0.0f, energy = 0.0f;
+      if (COUL) {
+        // tabulated erfc kernel: bin b = floor(r/h), cubic in t = r/h - b
+        const float qq = st.kqi * bj.w;
+        const float x = fminf(r2s * inv_r * p.coul_inv_h, x_max);
+        const float t = __builtin_amdgcn_fractf(x);
+        const uint32_t b = (uint32_t)x;
+        const float4 g = s_ct[b];
+        factor = qq * (((g.w * t + g.z) * t + g.y) * t + g.x);
+        if (FEV) {
+          const float4 e = s_ct[HMD_COUL_BINS + b];
+          energy = qq * (((e.w * t + e.z) * t + e.y) * t + e.x);
+        }
+      }
+      if (LJ) {
+        if ((lj_bits >> k) & 1ull) {                        // uniform branch
+          // (c6, c12, 6 c6, 12 c12)
+          const float4 cc = s_lj[st.ti_row + s_jt[k].y];
+          const float ir6 = inv_r2 * inv_r2 * inv_r2;
+          const float e_lj = (cc.y * ir6 - cc.x) * ir6;
+          const float f_lj = (cc.w * ir6 - cc.z) * ir6 * inv_r2;
+          // C1 switch: tau = (r^2 - rs^2)/(rc^2 - rs^2) clamped at 0,
+          // S = (1 - tau)^2 (1 + 2 tau), force gains -dS/dr / r * E
+          const float tau = fmaxf(r2s * p.lj_inv_delta - lj_rs2_d, 0.0f);
+          const float omt = 1.0f - tau;
+          const float sw = omt * omt * (1.0f + 2.0f * tau);
+          const float chain = (lj_12d * tau) * omt;
+          factor += sw * f_lj + chain * e_lj;
+          if (FEV) energy += sw * e_lj;
+        }
+      }
+      factor = ok ? factor : 0.0f;
+      // the i-side sums use their own FMAs, so that the j-side values are
+      // dead once the lane swaps consume them (no copies)
+      st.fix = fmaf(factor, dx, st.fix);
+      st.fiy = fmaf(factor, dy, st.fiy);
+      st.fiz = fmaf(factor, dz, st.fiz);
+      jr[0][u] = factor * dx; jr[1][u] = factor * dy; jr[2][u] = factor * dz;
+      if (FEV) {
+        const float he = ok ? 0.5f * energy : 0.0f;
+        const float hw = 0.5f * factor * r2s;
+        st.eiv += he; st.wiv += hw;
+        jr[3][u] = he; jr[4][u] = hw;
+      }
+    }
+    // Wave sums of the 8 j-atoms' components (totals in lanes 8m + 7), then
+    // component c of j-atom kb + m moves to lane 8m + 7 - c, so that one
+    // conversion and one atomic per lane cover every component.
+    float tot[NC];
+#pragma unroll
+    for (uint32_t c = 0; c < NC; ++c) tot[c] = hmd_wave_sum8(jr[c], lane);
+    float sh[NC];
+    sh[0] = tot[0];
+    sh[1] = hmd_dpp<HMD_DPP_ROW_SHL(1)>(tot[1]);
+    sh[2] = hmd_dpp<HMD_DPP_ROW_SHL(2)>(tot[2]);
+    if (FEV) {
+      sh[3 % NC] = hmd_dpp<HMD_DPP_ROW_SHL(3)>(tot[3 % NC]);
+      sh[4 % NC] = hmd_dpp<HMD_DPP_ROW_SHL(4)>(tot[4 % NC]);
+    }
+    // the moves run with all lanes enabled (an opaque use pins them here,
+    // ahead of the branch below, where a disabled source lane would read 0)
+#pragma unroll
+    for (uint32_t c = 0; c < NC; ++c) asm volatile("" : "+v"(sh[c]));
+    float v = sh[0];
+#pragma unroll
+    for (uint32_t c = 1; c < NC; ++c) v = (comp == c) ? sh[c] : v;
+    const uint32_t jt = s_jt[kb + (lane >> 3)].x;
+    if (comp < NC && jt != HMD_SENTINEL) {
+      v = (comp < 3u) ? -v : v;                    // forces react; E, W halves add
+      hmd_atomic_add_fixed(&acc[comp * stride + jt], hmd_to_fixed(v, HMD_FIXED_SHIFT));
+    }
+  }
+}
+
+// One entry: the 64 j-atoms are gathered once (one lane per j), shifted to
+// the image nearest the cluster centre when SHIFT, and stored in a per-wave
+// LDS table that the steps read at wave-uniform addresses (LDS broadcasts).
 template <bool FEV, bool LJ, bool COUL, bool SHIFT>
-__device__ inline void hmd_pair_entry(const PairParams& p, const float2* s_lj,
-                                      float4* s_jp, uint2* s_jt, HmdPairState& st,
+__device__ inline void hmd_pair_entry(const PairParams& p, const float4* s_lj,
+                                      const float4* s_ct, float4* s_jp, uint2* s_jt, HmdPairState& st,
                                       uint32_t slot, float cx, float cy, float cz,
-                                      uint32_t lane) {
+                                      uint32_t c_end, uint32_t lane) {
   const float4* ps = reinterpret_cast<const float4*>(p.posq_sorted);
   const HmdLattice L = p.L;
   const size_t ebase = (size_t)slot * HMD_ENTRY;
   const uint32_t my_j = p.entry_j[ebase + lane];
   const uint64_t my_ex = p.entry_excl[ebase + lane];
-  const uint32_t jj_l = (my_j != HMD_SENTINEL) ? my_j : 0u;
+  const bool valid_j = my_j != HMD_SENTINEL;
+  const uint32_t jj_l = valid_j ? my_j : 0u;
   float4 pj = ps[jj_l];
   const uint32_t tj_l = p.type_sorted[jj_l];
   if (SHIFT) {
     pj.x -= L.lx * floorf((pj.x - cx) * (1.0f / L.lx) + 0.5f);
     pj.y -= L.ly * floorf((pj.y - cy) * (1.0f / L.ly) + 0.5f);
     pj.z -= L.lz * floorf((pj.z - cz) * (1.0f / L.lz) + 0.5f);
+    if (!valid_j) pj.x = pj.y = pj.z = 1.0e18f;   // fails every cutoff test
   }
-  const float c_gauss = -(p.alpha * p.alpha) * HMD_LOG2E;
-  const float two_alpha_sqrtpi = p.alpha * HMD_TWO_OVER_SQRT_PI;
-  long long* acc = p.acc;
-  const uint32_t stride = p.acc_stride;
-  // The entry's j-atoms go to a per-wave LDS table; each step reads one
-  // record at a wave-uniform address (an LDS broadcast, no VALU work).
+  const uint64_t lj_bits = LJ ? __ballot(valid_j && ((p.lj_type_mask >> tj_l) & 1u)) : 0ull;
+  const bool plain = SHIFT && __ballot(my_ex != 0ull || my_j < c_end) == 0ull;
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
   s_jp[lane] = pj;
   s_jt[lane] = make_uint2(my_j, tj_l);
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
-
-  for (uint32_t kb = 0; kb < HMD_ENTRY; kb += 8) {
-    if (hmd_readlane(my_j, kb) == HMD_SENTINEL) break;
-    float jfx[8], jfy[8], jfz[8], jfe[8], jfw[8];
-#pragma unroll
-    for (uint32_t u = 0; u < 8; ++u) {
-      const uint32_t k = kb + u;
-      const float4 bj = s_jp[k];
-      const uint2 jt2 = s_jt[k];
-      const uint32_t j = __builtin_amdgcn_readfirstlane(jt2.x);   // wave-uniform
-      const float xj = bj.x, yj = bj.y, zj = bj.z;
-      float dx = st.xi - xj, dy = st.yi - yj, dz = st.zi - zj;
-      if (!SHIFT) hmd_min_image(dx, dy, dz, L);
-      const float r2 = dx * dx + dy * dy + dz * dz;
-      const bool excluded = (my_ex >> k) & 1ull;
-      const bool ok = (j != HMD_SENTINEL) && st.vi && (st.si < j) && (r2 <= p.rc2) && !excluded;
-      const float r2s = ok ? r2 : 1.0f;
-      const float inv_r = __builtin_amdgcn_rsqf(r2s);
-      const float inv_r2 = inv_r * inv_r;
-      float factor = 0.0f, energy = 0.0f;
-      if (COUL) {
-        const float qj = bj.w;
-        const float qq = st.kqi * qj;
-        const float ar = p.alpha * (r2s * inv_r);
-        const float gauss = __builtin_amdgcn_exp2f(r2s * c_gauss);
-        const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ar);
-        const float erfc_ar =
-            ((((1.061405429f * t - 1.453152027f) * t + 1.421413741f) * t -
-              0.284496736f) * t + 0.254829592f) * t * gauss;
-        const float qq_ir2 = qq * inv_r2;
-        factor = qq_ir2 * (erfc_ar * inv_r + two_alpha_sqrtpi * gauss);
-        if (FEV) energy = qq * inv_r * erfc_ar;
-      }
-      if (LJ) {
-        const uint32_t tj = __builtin_amdgcn_readfirstlane(jt2.y);
-        if ((p.lj_type_mask >> tj) & 1u) {                  // uniform branch
-          const float2 cc = s_lj[st.ti * p.n_types + tj];
-          const float ir6 = inv_r2 * inv_r2 * inv_r2;
-          float e_lj = (cc.y * ir6 - cc.x) * ir6;
-          float f_lj = (12.0f * cc.y * ir6 - 6.0f * cc.x) * ir6 * inv_r2;
-          float tau = (r2s - p.lj_rs2) * p.lj_inv_delta;
-          tau = (r2s > p.lj_rs2) ? tau : 0.0f;
-          const float omt = 1.0f - tau;
-          const float sw = omt * omt * (1.0f + 2.0f * tau);
-          const float chain = 12.0f * tau * omt * p.lj_inv_delta;
-          f_lj = sw * f_lj + chain * e_lj;
-          factor += f_lj;
-          if (FEV) energy += sw * e_lj;
-        }
-      }
-      factor = ok ? factor : 0.0f;
-      const float fx = factor * dx, fy = factor * dy, fz = factor * dz;
-      st.fix += fx; st.fiy += fy; st.fiz += fz;
-      jfx[u] = -fx; jfy[u] = -fy; jfz[u] = -fz;
-      if (FEV) {
-        const float he = ok ? 0.5f * energy : 0.0f;
-        const float hw = 0.5f * factor * r2s;
-        st.eiv += he; st.wiv += hw;
-        jfe[u] = he; jfw[u] = hw;
-      }
-    }
-    const float rx = hmd_transpose_reduce8(jfx, lane);
-    const float ry = hmd_transpose_reduce8(jfy, lane);
-    const float rz = hmd_transpose_reduce8(jfz, lane);
-    float re = 0.0f, rw = 0.0f;
-    if (FEV) {
-      re = hmd_transpose_reduce8(jfe, lane);
-      rw = hmd_transpose_reduce8(jfw, lane);
-    }
-    const uint32_t jt = __shfl(my_j, (int)(kb + (lane >> 3)), HMD_WAVE);
-    if ((lane & 7u) == 0u && jt != HMD_SENTINEL) {
-      hmd_atomic_add_fixed(&acc[jt], hmd_to_fixed(rx, HMD_FIXED_SHIFT));
-      hmd_atomic_add_fixed(&acc[stride + jt], hmd_to_fixed(ry, HMD_FIXED_SHIFT));
-      hmd_atomic_add_fixed(&acc[2 * stride + jt], hmd_to_fixed(rz, HMD_FIXED_SHIFT));
-      if (FEV) {
-        hmd_atomic_add_fixed(&acc[3 * stride + jt], hmd_to_fixed(re, HMD_FIXED_SHIFT));
-        hmd_atomic_add_fixed(&acc[4 * stride + jt], hmd_to_fixed(rw, HMD_FIXED_SHIFT));
-      }
-    }
-  }
+  if (SHIFT && plain)
+    hmd_pair_steps<FEV, LJ, COUL, true, true>(p, s_lj, s_ct, s_jp, s_jt, st, my_j, my_ex,
+                                               lj_bits, lane);
+  else
+    hmd_pair_steps<FEV, LJ, COUL, SHIFT, false>(p, s_lj, s_ct, s_jp, s_jt, st, my_j, my_ex,
+                                                 lj_bits, lane);
 }
 
 template <bool FEV>
@@ -165,12 +216,21 @@ __device__ inline void hmd_flush_i(const PairParams& p, const HmdPairState& st) 
 
 template <bool FEV, bool LJ, bool COUL>
 __device__ inline void hmd_pair_body(const PairParams& p) {
-  __shared__ float2 s_lj[HMD_MAX_TYPES * HMD_MAX_TYPES];
+  __shared__ float4 s_lj[HMD_MAX_TYPES * HMD_MAX_TYPES];   // (c6, c12, 6 c6, 12 c12)
+  // erfc table: force coefficients, then (energy variants) energy coefficients
+  __shared__ float4 s_ct[COUL ? (FEV ? 2 * HMD_COUL_BINS : HMD_COUL_BINS) : 1];
   __shared__ float4 s_jp[4][HMD_ENTRY];
   __shared__ uint2 s_jt[4][HMD_ENTRY];
   const uint32_t nt2 = p.n_types * p.n_types;
-  for (uint32_t k = threadIdx.x; k < nt2; k += blockDim.x)
-    s_lj[k] = reinterpret_cast<const float2*>(p.lj_c6c12)[k];
+  for (uint32_t k = threadIdx.x; k < nt2; k += blockDim.x) {
+    const float2 cc = reinterpret_cast<const float2*>(p.lj_c6c12)[k];
+    s_lj[k] = make_float4(cc.x, cc.y, 6.0f * cc.x, 12.0f * cc.y);
+  }
+  if (COUL) {
+    const float4* ct = reinterpret_cast<const float4*>(p.coul_table);
+    for (uint32_t k = threadIdx.x; k < (FEV ? 2u : 1u) * HMD_COUL_BINS; k += blockDim.x)
+      s_ct[k] = ct[k];
+  }
   __syncthreads();
 
   const uint32_t lane = hmd_lane();
@@ -203,7 +263,7 @@ __device__ inline void hmd_pair_body(const PairParams& p) {
       const float4 a = ps[st.si];
       st.xi = a.x; st.yi = a.y; st.zi = a.z;
       st.kqi = p.k_coulomb * a.w;
-      st.ti = p.type_sorted[st.si];
+      st.ti_row = p.type_sorted[st.si] * p.n_types;
       const float4 cb = bb[2 * c], hb = bb[2 * c + 1];
       // Single-image fast path (compare jit_composed.rs:1396-1430): valid
       // when every atom that can interact with the cluster lies within half
@@ -223,17 +283,22 @@ __device__ inline void hmd_pair_body(const PairParams& p) {
     }
     const uint32_t wv = threadIdx.x / HMD_WAVE;
     if (shift)
-      hmd_pair_entry<FEV, LJ, COUL, true>(p, s_lj, s_jp[wv], s_jt[wv], st, slot, cx, cy, cz,
-                                          lane);
+      hmd_pair_entry<FEV, LJ, COUL, true>(p, s_lj, s_ct, s_jp[wv], s_jt[wv], st, slot, cx, cy, cz,
+                                          (c + 1u) * HMD_CLUSTER, lane);
     else
-      hmd_pair_entry<FEV, LJ, COUL, false>(p, s_lj, s_jp[wv], s_jt[wv], st, slot, cx, cy, cz,
-                                           lane);
+      hmd_pair_entry<FEV, LJ, COUL, false>(p, s_lj, s_ct, s_jp[wv], s_jt[wv], st, slot, cx, cy, cz,
+                                           (c + 1u) * HMD_CLUSTER, lane);
   }
   hmd_flush_i<FEV>(p, st);
 }
 
+// Minimum waves per SIMD the register allocation must allow (1: no limit).
+#ifndef HMD_PAIR_WAVES_PER_EU
+#define HMD_PAIR_WAVES_PER_EU 1
+#endif
 #define HMD_PAIR_VARIANT(NAME, FEV, LJ, COUL)                                  \
-  extern "C" __global__ void __launch_bounds__(256) NAME(const PairParams p) { \
+  extern "C" __global__ void __launch_bounds__(256, HMD_PAIR_WAVES_PER_EU)     \
+  NAME(const PairParams p) {                                                   \
     hmd_pair_body<FEV, LJ, COUL>(p);                                          \
   }
 HMD_PAIR_VARIANT(hmd_pair_force_f, false, true, true)
