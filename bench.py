@@ -50,6 +50,24 @@ PEAK_BF16_TFLOPS = 2500.0     # MI355X published dense bf16 matrix peak
 FLOP_PER_PAIR = 59
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "traffic_spc196k_plain128.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed hardware-counter
+    summary (two rocprofv3 --pmc runs, FETCH_SIZE then WRITE_SIZE, each scaled
+    by the reading of a copy of known size); None when it holds no figure."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes" and parts[1] == kernel:
+                    return int(float(parts[2]))
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -161,7 +179,9 @@ def main():
     roofline = {"bound": "valu", "kernel": "hmd_pair_force_f",
                 "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                "traffic": None, "pairs_in_cutoff": n_in_cut,
+                "traffic": profiled_traffic("hmd_pair_force_f"),
+                "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT),
+                "pairs_in_cutoff": n_in_cut,
                 "kernel_us": round(pair_us, 2)}
     # value: the fastest leg whose forces stay within dF <= 1e-3 of the plain
     # 128^3 forces (the plain leg by definition; the NN leg only if every

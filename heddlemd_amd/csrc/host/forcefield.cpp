@@ -73,7 +73,7 @@ static double cardinal_bspline(int p, double x) {
   return v[0];
 }
 
-static std::vector<float> b_factors(uint32_t n, uint32_t p) {
+static std::vector<double> b_factors(uint32_t n, uint32_t p) {
   std::vector<double> mod(n);
   for (uint32_t k = 0; k < n; ++k) {
     double th = 2.0 * M_PI * k / n, re = 0.0, im = 0.0;
@@ -86,8 +86,8 @@ static std::vector<float> b_factors(uint32_t n, uint32_t p) {
   }
   for (uint32_t k = 0; k < n; ++k)
     if (mod[k] < 1.0e-7) mod[k] = 0.5 * (mod[(k + n - 1) % n] + mod[(k + 1) % n]);
-  std::vector<float> out(n);
-  for (uint32_t k = 0; k < n; ++k) out[k] = mod[k] > 0.0 ? (float)(1.0 / mod[k]) : 0.0f;
+  std::vector<double> out(n);
+  for (uint32_t k = 0; k < n; ++k) out[k] = mod[k] > 0.0 ? 1.0 / mod[k] : 0.0;
   return out;
 }
 
@@ -352,6 +352,7 @@ void ForceField::init_neighbor_list(const hmd_forcefield_desc& d) {
   type_sorted.alloc(n_pad);
   posq_sorted.alloc(4 * (size_t)n_pad);
   ref_pos.alloc(4 * (size_t)n);
+  image_sorted.alloc(4 * (size_t)n_pad);
   bbox.alloc(8 * (size_t)n_clusters);
   sub_bbox.alloc(64 * (size_t)n_clusters);
   entry_count.alloc(n_clusters);
@@ -409,9 +410,11 @@ void ForceField::init_spme(const hmd_forcefield_desc& d) {
     b.alloc(v.size(), false);
     b.upload(v.data(), v.size(), s);
   };
-  up(bfac[0], b_factors(n0, order));
-  up(bfac[1], b_factors(n1, order));
-  up(bfac[2], b_factors(n2, order));
+  for (int d = 0; d < 3; ++d) {
+    const std::vector<double> b = b_factors(grid[d], order);
+    bfac[d].alloc(b.size(), false);
+    bfac[d].upload(b.data(), b.size(), s);
+  }
   twiddles(tw_a, n0, s);
   twiddles(tw_b, n1, s);
   twiddles(tw_c, n2 / 2, s);
@@ -486,7 +489,8 @@ void ForceField::rebuild_chain() {
   NblSortParams sp{cell_start.ptr, n_keys, sorted_id.ptr, gate};
   launch(c, K_hmd_nbl_sort_cells, ceil_div(n_keys, 256), 256, 0, sp);
   NblGatherParams gp{p.posq.ptr, p.type.ptr, n, n_pad, sorted_id.ptr, slot_of.ptr,
-                     posq_sorted.ptr, type_sorted.ptr, ref_pos.ptr, gate};
+                     posq_sorted.ptr, type_sorted.ptr, ref_pos.ptr, gate, p.lattice,
+                     image_sorted.ptr};
   launch(c, K_hmd_nbl_gather, ceil_div(n_pad, 256), 256, 0, gp);
   NblBboxParams bp{posq_sorted.ptr, n, n_clusters, bbox.ptr, sub_bbox.ptr, gate};
   launch(c, K_hmd_nbl_bbox, ceil_div(n_clusters, 4), 256, 0, bp);

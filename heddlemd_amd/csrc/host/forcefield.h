@@ -37,6 +37,7 @@ struct ForceField {
   DevBuf<uint32_t> status;          // [0] flags [1] generation [2] high water
   DevBuf<uint32_t> atom_key, cell_count, cell_start, sorted_id, slot_of, type_sorted;
   DevBuf<float> posq_sorted, ref_pos, bbox, sub_bbox;
+  DevBuf<int32_t> image_sorted;   // int4[n_pad]
   DevBuf<uint32_t> entry_j, entry_count, entry_offset, work, n_work;
   DevBuf<uint64_t> entry_excl;
   DevBuf<long long> acc;            // [5][n_pad] fixed-point accumulators
@@ -49,7 +50,8 @@ struct ForceField {
   size_t M = 0;
   DevBuf<uint32_t> atom_bin, bin_count, bin_start, bin_atoms;
   Scanner bin_scan;
-  DevBuf<float> rho, V, mesh_c, bfac[3], tw_a, tw_b, tw_c, tw_r2c, u_self;
+  DevBuf<float> rho, V, mesh_c, tw_a, tw_b, tw_c, tw_r2c, u_self;
+  DevBuf<double> bfac[3];
   DevBuf<float> slow_f, slow_e, slow_w;
   DevBuf<double> partials, spme_out;
   std::vector<uint32_t> rad_a, rad_b, rad_c;

@@ -431,11 +431,21 @@ hmd_status hmd_integrator_minimize(hmd_integrator* w, uint32_t n_iter, double ma
     std::vector<float> f(3 * (size_t)p.n);
     DevBuf<float> backup;
     backup.alloc(p.posq.count, false);
+    // A list overflow (the relaxing structure can outgrow the capacity probed
+    // on the starting one) grows the capacity and marks a rebuild: the
+    // evaluation is then repeated at the same positions.
     auto energy = [&]() {
-      I.ff.compute(1);
-      check(hipStreamSynchronize(c.stream), "sync");
-      I.ff.check_status();
-      return reduce_f64(c, p, p.energy.ptr, 0);
+      for (int attempt = 0;; ++attempt) {
+        I.ff.compute(1);
+        check(hipStreamSynchronize(c.stream), "sync");
+        try {
+          I.ff.check_status();
+        } catch (const Error& e) {
+          if (e.code != HMD_ERR_NBL_OVERFLOW || attempt >= 3) throw;
+          continue;
+        }
+        return reduce_f64(c, p, p.energy.ptr, 0);
+      }
     };
     auto fmax_now = [&]() {
       p.force.download(f.data(), f.size(), c.stream);

@@ -179,32 +179,35 @@ hmd_fft_c2r_c(const FftParams p) {
 }
 
 // Influence function G(k) and virial factor at mesh index (ka, kb, kc)
-// (kernels/spme_recip.cu:34-114), f32 like the reference.
+// (kernels/spme_recip.cu:34-114).  Evaluated in FP64: exp(-k^2/4a^2) turns
+// the relative rounding of its argument into a relative error of G that
+// grows with k^2/4a^2, which in f32 dominated the reciprocal energy error;
+// the rest of the mesh pipeline is f32.
 __device__ inline void hmd_influence(const FftParams& p, uint32_t ka, uint32_t kb,
-                                     uint32_t kc, float& G, float& vf) {
+                                     uint32_t kc, double& G, double& vf) {
   const HmdLattice L = p.L;
-  const float ilx = 1.0f / L.lx, ily = 1.0f / L.ly, ilz = 1.0f / L.lz;
-  const float ra_x = ilx, ra_y = -L.xy * ilx * ily,
-              ra_z = (L.xy * L.yz - L.xz * L.ly) * ilx * ily * ilz;
-  const float rb_y = ily, rb_z = -L.yz * ily * ilz, rc_z = ilz;
-  const float ma = (ka <= p.n0 / 2) ? (float)ka : (float)ka - (float)p.n0;
-  const float mb = (kb <= p.n1 / 2) ? (float)kb : (float)kb - (float)p.n1;
-  const float mc = (kc <= p.n2 / 2) ? (float)kc : (float)kc - (float)p.n2;
-  const float two_pi = 6.283185307179586f;
-  const float kx = two_pi * (ma * ra_x);
-  const float ky = two_pi * (ma * ra_y + mb * rb_y);
-  const float kz = two_pi * (ma * ra_z + mb * rb_z + mc * rc_z);
-  const float k2 = kx * kx + ky * ky + kz * kz;
-  if (k2 == 0.0f) {
-    G = 0.0f;
-    vf = 0.0f;
+  const double lx = L.lx, ly = L.ly, lz = L.lz, xy = L.xy, xz = L.xz, yz = L.yz;
+  const double ilx = 1.0 / lx, ily = 1.0 / ly, ilz = 1.0 / lz;
+  const double ra_x = ilx, ra_y = -xy * ilx * ily, ra_z = (xy * yz - xz * ly) * ilx * ily * ilz;
+  const double rb_y = ily, rb_z = -yz * ily * ilz, rc_z = ilz;
+  const double ma = (ka <= p.n0 / 2) ? (double)ka : (double)ka - (double)p.n0;
+  const double mb = (kb <= p.n1 / 2) ? (double)kb : (double)kb - (double)p.n1;
+  const double mc = (kc <= p.n2 / 2) ? (double)kc : (double)kc - (double)p.n2;
+  const double two_pi = 6.283185307179586;
+  const double kx = two_pi * (ma * ra_x);
+  const double ky = two_pi * (ma * ra_y + mb * rb_y);
+  const double kz = two_pi * (ma * ra_z + mb * rb_z + mc * rc_z);
+  const double k2 = kx * kx + ky * ky + kz * kz;
+  if (k2 == 0.0) {
+    G = 0.0;
+    vf = 0.0;
     return;
   }
-  const float vol = L.lx * L.ly * L.lz;
-  const float a2 = p.alpha * p.alpha;
-  G = (p.k_coulomb / vol) * (12.566370614359172f / k2) * expf(-k2 / (4.0f * a2)) *
+  const double vol = lx * ly * lz;
+  const double a2 = (double)p.alpha * (double)p.alpha;
+  G = ((double)p.k_coulomb / vol) * (12.566370614359172 / k2) * exp(-k2 / (4.0 * a2)) *
       p.bfac0[ka] * p.bfac1[kb] * p.bfac2[kc];
-  vf = G * (1.0f - k2 / (2.0f * a2));
+  vf = G * (1.0 - k2 / (2.0 * a2));
 }
 
 // Passes 2-4: complex FFT along a strided axis (b or a).  With
@@ -237,14 +240,15 @@ hmd_fft_axis(const FftParams p) {
       const uint32_t l = t / n, ka = t - l * n;
       const uint32_t gl = l0 + l;                 // line = (kb, kc)
       const uint32_t kb = gl / p.nch, kc = gl - kb * p.nch;
-      float G, vf;
+      double G, vf;
       hmd_influence(p, ka, kb, kc, G, vf);
       float2 x = res[l * ls + ka];
-      const float hw = (kc == 0u || (p.n2 % 2u == 0u && 2u * kc == p.n2)) ? 1.0f : 2.0f;
-      const float r2 = x.x * x.x + x.y * x.y;
-      vir += (double)(hw * vf * r2);
-      ene += (double)(hw * G * r2);
-      res[l * ls + ka] = make_float2(G * x.x, G * x.y);
+      const double hw = (kc == 0u || (p.n2 % 2u == 0u && 2u * kc == p.n2)) ? 1.0 : 2.0;
+      const double r2 = (double)x.x * x.x + (double)x.y * x.y;
+      vir += hw * vf * r2;
+      ene += hw * G * r2;
+      const float Gf = (float)G;
+      res[l * ls + ka] = make_float2(Gf * x.x, Gf * x.y);
     }
     s_red[0][threadIdx.x] = vir;
     s_red[1][threadIdx.x] = ene;
@@ -454,14 +458,15 @@ __device__ inline void hmd_fft_axis_slab(const FftParams& p) {
       const uint32_t ka = t / W, w = t - ka * W;
       if (w >= Wc) continue;
       const uint32_t kb = outer, kc = c0 + w;
-      float G, vf;
+      double G, vf;
       hmd_influence(p, ka, kb, kc, G, vf);
       const float2 x = s_tile[t];
-      const float hw = (kc == 0u || (p.n2 % 2u == 0u && 2u * kc == p.n2)) ? 1.0f : 2.0f;
-      const float r2 = x.x * x.x + x.y * x.y;
-      vir += (double)(hw * vf * r2);
-      ene += (double)(hw * G * r2);
-      s_tile[t] = make_float2(G * x.x, G * x.y);
+      const double hw = (kc == 0u || (p.n2 % 2u == 0u && 2u * kc == p.n2)) ? 1.0 : 2.0;
+      const double r2 = (double)x.x * x.x + (double)x.y * x.y;
+      vir += hw * vf * r2;
+      ene += hw * G * r2;
+      const float Gf = (float)G;
+      s_tile[t] = make_float2(Gf * x.x, Gf * x.y);
     }
     s_red[0][threadIdx.x] = vir;
     s_red[1][threadIdx.x] = ene;

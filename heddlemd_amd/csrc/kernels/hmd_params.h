@@ -68,6 +68,8 @@ struct NblGatherParams {
   uint32_t* type_sorted;    // [n_pad]
   float* ref_pos;           // float4[n] canonical positions at rebuild
   const uint32_t* status;   // rebuild extras gated on the DISPLACED bit
+  HmdLattice L;
+  int32_t* image;           // int4[n_pad]: lattice images removed at the rebuild
 };
 
 struct NblBboxParams {
@@ -241,9 +243,9 @@ struct FftParams {
   uint32_t n0, n1, n2, nch;
   HmdLattice L;
   float alpha, k_coulomb;
-  const float* bfac0;
-  const float* bfac1;
-  const float* bfac2;
+  const double* bfac0;         // |b(k)|^2 moduli per axis, FP64
+  const double* bfac1;
+  const double* bfac2;
   double* partials;            // [grid blocks][2]: virial, energy
 };
 

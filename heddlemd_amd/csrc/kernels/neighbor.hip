@@ -1,19 +1,22 @@
 // Cell list + packed cluster-pair neighbour list for gfx950.
 //
 // Reference behaviour (taylor-a-barnes/heddlemd): kernels/neighbor.cu and
-// src/forces/neighbor_list.rs.  The pair set is the same -- every unordered
-// pair whose exact f32 minimum-image squared distance is <= (r_c + r_skin)^2
-// (kernels/neighbor.cu:536-553) -- but the construction is wave64-native:
+// src/forces/neighbor_list.rs.  The list holds every unordered pair whose
+// exact f32 minimum-image squared distance is <= (r_c + r_skin)^2
+// (kernels/neighbor.cu:536-553; hmd_nbl_pairs decodes exactly that set), plus
+// some farther pairs that the force kernels mask by their cutoff tests.  The
+// construction is wave64-native:
 //
-//  * atoms are sorted by the Morton order of a fine cell grid (about 0.25 nm
-//    cells), ties broken by atom id, so 64 consecutive sorted atoms form a
-//    spatially compact cluster and one wave owns one cluster (one atom per
-//    lane);
+//  * atoms are sorted along a Hilbert curve over a power-of-two grid of
+//    cells (about 0.2-0.4 nm), ties broken by atom id, so 64 consecutive
+//    sorted atoms form a spatially compact cluster and one wave owns one
+//    cluster (one atom per lane);
 //  * candidate j-clusters come from a bounding-box prune of clusters with a
 //    higher index (half list, Newton's third law in the force kernel);
-//  * j-atoms are filtered per atom against the 64 i-atoms and compacted in
-//    increasing sorted order with a wave ballot + prefix popcount, so each
-//    cluster's list -- and every entry of 64 j-slots -- is deterministic;
+//  * j-atoms are filtered per atom against the i-cluster's eight 8-atom
+//    sub-boxes and compacted in increasing sorted order with a wave ballot +
+//    prefix popcount, so each cluster's list -- and every entry of 64
+//    j-slots -- is deterministic;
 //  * each entry carries, per i-lane, a 64-bit mask of excluded j-slots built
 //    from the exclusion CSR, so the force kernel needs no per-pair search.
 //
@@ -150,8 +153,15 @@ hmd_nbl_sort_cells(const NblSortParams p) {
 // (kernels/neighbor.cu:251-285 does the same refresh every step in
 // ForceField::run); in a rebuilding step it also writes the inverse map, the
 // sorted types, the rebuild-time reference positions and the padding lanes.
+// The sorted positions carry the lattice images that the rebuild's cell
+// assignment removed (hmd_wrap of the rebuild-time position, kept per slot
+// and applied every step with the same operations): at the rebuild they are
+// the wrapped positions the cells were taken from, so every cluster's box is
+// compact even for atoms that a constraint step left just outside the primary
+// cell, and between rebuilds they move continuously.
 extern "C" __global__ void __launch_bounds__(256)
 hmd_nbl_gather(const NblGatherParams p) {
+#pragma clang fp contract(off)
   const bool rebuild = hmd_gate(p.status, HMD_STATUS_DISPLACED);
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= p.n_pad) return;
@@ -166,20 +176,33 @@ hmd_nbl_gather(const NblGatherParams p) {
     return;
   }
   uint32_t id = p.sorted_id[s];
-  float4 a = reinterpret_cast<const float4*>(p.posq)[id];
-  ps[s] = a;
+  const float4 a = reinterpret_cast<const float4*>(p.posq)[id];
+  int4* img = reinterpret_cast<int4*>(p.image);
+  int4 im;
   if (rebuild) {
+    float x = a.x, y = a.y, z = a.z;
+    hmd_wrap(x, y, z, im.x, im.y, im.z, p.L);
+    im.w = 0;
+    img[s] = im;
     p.slot_of[id] = s;
     p.type_sorted[s] = p.type[id];
     reinterpret_cast<float4*>(p.ref_pos)[id] = a;
+  } else {
+    im = img[s];
   }
+  const float ka = (float)im.x, kb = (float)im.y, kc = (float)im.z;
+  float4 w = a;
+  w.x -= ka * p.L.lx + kb * p.L.xy + kc * p.L.xz;
+  w.y -= kb * p.L.ly + kc * p.L.yz;
+  w.z -= kc * p.L.lz;
+  ps[s] = w;
 }
 
 // Per-cluster axis-aligned bounding box (centre and half extents) of the
 // valid atoms, one wave per cluster (compare compute_block_bbox,
 // kernels/neighbor.cu:291-361), plus the boxes of the cluster's eight
-// 8-atom sub-clusters (consecutive sorted atoms) used to prune the exact
-// distance tests of the list build.
+// 8-atom sub-clusters (consecutive sorted atoms) against which the list build
+// tests each j-atom.
 extern "C" __global__ void __launch_bounds__(256)
 hmd_nbl_bbox(const NblBboxParams p) {
   if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
@@ -243,10 +266,10 @@ __device__ inline float hmd_box_gap2(float4 ca, float4 ha, float4 cb, float4 hb,
 // search radius (one ballot per 64 j-clusters, the four waves taking turns).
 // After a prefix over the per-chunk masks the candidates are listed in
 // increasing index order in LDS, and the four waves share them round robin:
-// lane k tests j-atom k of the candidate against the 64 i-atoms (a prune
-// against eight 8-atom i-sub-boxes, then the exact f32 distance) and keeps it
-// when some i-atom with a lower sorted index is within r_search; the next
-// candidate's j-atom is loaded while the current one is tested.  The 64-bit
+// lane k tests j-atom k of the candidate against the i-cluster's eight 8-atom
+// sub-boxes and keeps it when a sub-box holding an i-atom with a lower sorted
+// index lies within r_search of it; the next candidate's j-atom is loaded
+// while the current one is tested.  The 64-bit
 // hit mask of each candidate goes to LDS, and wave 0 then places the kept
 // atoms, candidate by candidate, at consecutive entry positions in
 // increasing sorted order (ballot prefix count), written straight to the
@@ -260,24 +283,6 @@ __device__ inline float hmd_box_gap2(float4 ca, float4 ha, float4 cb, float4 hb,
 #define HMD_NBL_CAND_CAP 512u
 
 typedef __attribute__((address_space(4))) const float4 hmd_const_f4;
-
-// The same two measures without image arithmetic, for cluster pairs whose
-// differences are known to need none (image index 0 on every axis): the
-// difference is then bit-identical to the one the minimum-image sequence
-// returns.
-__device__ inline float hmd_box_gap2_direct(float4 ca, float4 ha, float4 pb) {
-  const float ex = fmaxf(fabsf(ca.x - pb.x) - ha.x, 0.0f);
-  const float ey = fmaxf(fabsf(ca.y - pb.y) - ha.y, 0.0f);
-  const float ez = fmaxf(fabsf(ca.z - pb.z) - ha.z, 0.0f);
-  return ex * ex + ey * ey + ez * ez;
-}
-
-__device__ inline float hmd_r2_direct(float xi, float yi, float zi, float xj, float yj,
-                                      float zj) {
-#pragma clang fp contract(off)
-  const float dx = xi - xj, dy = yi - yj, dz = zi - zj;
-  return dx * dx + dy * dy + dz * dz;
-}
 
 // Inclusive prefix sum over the 64 lanes.
 __device__ inline uint32_t hmd_wave_scan_incl(uint32_t v, uint32_t lane) {
@@ -305,13 +310,11 @@ hmd_nbl_build(const NblBuildParams p) {
   const bool ortho = p.orthorhombic_fast != 0;
   const float ilx = 1.0f / p.L.lx, ily = 1.0f / p.L.ly, ilz = 1.0f / p.L.lz;
   const float rs2_prune = p.r_search_sq * 1.0002f;   // conservative prune
-  const float rs2_accept = p.r_search_sq * 0.9996f;  // conservative accept
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const uint32_t n_chunks = (p.n_clusters - c + HMD_WAVE - 1) / HMD_WAVE;
-  // The i-cluster's atoms and sub-boxes are read at wave-uniform addresses
-  // in the hit tests: read them through the constant address space (scalar
-  // loads into SGPRs) rather than as LDS broadcasts.
-  const hmd_const_f4* ci4 = (const hmd_const_f4*)(ps + c * HMD_CLUSTER);
+  // The i-cluster's sub-boxes are read at wave-uniform addresses in the hit
+  // tests: read them through the constant address space (scalar loads into
+  // SGPRs) rather than as LDS broadcasts.
   const hmd_const_f4* csub = (const hmd_const_f4*)(reinterpret_cast<const float4*>(p.sub_bbox) + 16 * c);
 
   // candidate masks, 64 j-clusters per ballot
@@ -368,8 +371,6 @@ hmd_nbl_build(const NblBuildParams p) {
       if (m < n_ex) ex_slot[m] = p.slot_of[p.excl_partners[b + m]];
   }
   const size_t base_entry = (size_t)c * p.cap_entries;
-  const uint32_t imax = min(p.n - c * HMD_CLUSTER, HMD_CLUSTER);
-  const uint32_t full_sub = (imax >= 64u) ? 0xFFu : ((1u << (imax / 8u)) - 1u);
   uint32_t n_entries = 0;       // completed entries (uniform)
   uint32_t fill = 0;            // slots used in the current entry (uniform)
   uint64_t cur_mask = 0, next_mask = 0;
@@ -406,23 +407,24 @@ hmd_nbl_build(const NblBuildParams p) {
         const uint32_t sn = (s_cand[q + 4] & 0x7FFFFFFFu) * HMD_CLUSTER + lane;
         pj_next = sn < p.n ? ps[sn] : zero;
       }
-      // Sub-box masks: which 8-atom i-sub-clusters this j-atom may reach
-      // (nearest point of the box within the search radius), and which it
-      // surely reaches (farthest corner within it, with a relative margin of
-      // 2e-4 that exceeds any f32 rounding of the exact test, so that every
-      // atom of the sub-cluster passes the exact test as well).
-      uint32_t smask = 0, amask = 0;
+      // Sub-box mask: which 8-atom i-sub-clusters this j-atom may reach (the
+      // nearest point of the sub-box within the search radius, with a relative
+      // margin of 2e-4).  The j-atom is listed when it may reach one that holds
+      // an i-atom below it in sorted order: every pair within r_search is
+      // listed, together with some farther ones (about 7% more j-atoms than an
+      // exact per-atom test would keep), which the force kernels mask by their
+      // own cutoff tests.  Empty sub-boxes (past the last atom) have negative
+      // extents and never pass.
+      uint32_t smask = 0;
       if (sj < p.n) {
         if (nowrap) {
 #pragma unroll
           for (uint32_t k = 0; k < 8; ++k) {
             const float4 cs = csub[2 * k], hs = csub[2 * k + 1];
-            const float dx = fabsf(cs.x - pj.x), dy = fabsf(cs.y - pj.y), dz = fabsf(cs.z - pj.z);
-            const float ex = fmaxf(dx - hs.x, 0.0f), ey = fmaxf(dy - hs.y, 0.0f),
-                        ez = fmaxf(dz - hs.z, 0.0f);
-            const float fx = dx + hs.x, fy = dy + hs.y, fz = dz + hs.z;
+            const float ex = fmaxf(fabsf(cs.x - pj.x) - hs.x, 0.0f);
+            const float ey = fmaxf(fabsf(cs.y - pj.y) - hs.y, 0.0f);
+            const float ez = fmaxf(fabsf(cs.z - pj.z) - hs.z, 0.0f);
             smask |= (ex * ex + ey * ey + ez * ez <= rs2_prune) ? (1u << k) : 0u;
-            amask |= (fx * fx + fy * fy + fz * fz <= rs2_accept) ? (1u << k) : 0u;
           }
         } else {
 #pragma unroll
@@ -433,36 +435,9 @@ hmd_nbl_build(const NblBuildParams p) {
           }
         }
       }
-      // i-atoms allowed for this j: sorted index below j's, and valid.
-      const uint32_t ilim = (jcl == c) ? min(lane, imax) : imax;
-      // a sure sub-cluster of another cluster decides the hit (its eight
-      // atoms are valid and all within r_search)
-      bool hit = (jcl != c) && (amask & full_sub) != 0u;
-      for (uint32_t k = 0; k < 8; ++k) {
-        const bool want = !hit && ((smask >> k) & 1u);
-        if (__ballot(want) == 0) continue;
-        if (want) {
-          if (nowrap) {
-#pragma unroll
-            for (uint32_t t = 0; t < 8; ++t) {
-              const uint32_t i = 8 * k + t;
-              const float4 a = ci4[i];
-              const float r2 = hmd_r2_direct(a.x, a.y, a.z, pj.x, pj.y, pj.z);
-              hit = hit || (i < ilim && r2 <= p.r_search_sq);
-            }
-          } else {
-#pragma unroll 4
-            for (uint32_t t = 0; t < 8; ++t) {
-              const uint32_t i = 8 * k + t;
-              const float4 a = ci4[i];
-              const float r2 =
-                  ortho ? hmd_r2_ortho(a.x, a.y, a.z, pj.x, pj.y, pj.z, p.L, ilx, ily, ilz)
-                        : hmd_r2_exact(a.x, a.y, a.z, pj.x, pj.y, pj.z, p.L);
-              hit = hit || (i < ilim && r2 <= p.r_search_sq);
-            }
-          }
-        }
-      }
+      // own cluster: only sub-clusters holding an i-atom below this lane's j
+      const uint32_t lim = (jcl == c) ? ((1u << ((lane + 7u) / 8u)) - 1u) : 0xFFu;
+      const bool hit = (smask & lim) != 0u;
       const uint64_t H = __ballot(hit);
       if (lane == 0) s_hit[q] = H;
     }
