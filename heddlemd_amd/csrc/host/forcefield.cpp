@@ -130,12 +130,18 @@ static const KernelId kP2C2r[5] = {K_hmd_fft_c2r_16, K_hmd_fft_c2r_32, K_hmd_fft
 // Strided axis of length n over n_outer slabs of `cols` adjacent lines:
 // column tiles of width W (stage-1 capacity 2 x 256 / N2, 64 KiB of LDS),
 // at least about one block per compute unit.
+// Workgroups per CU the FFT passes aim for (several, so that one group's
+// loads overlap another's transform).
+#ifndef HMD_FFT_BLOCKS_PER_CU
+#define HMD_FFT_BLOCKS_PER_CU 2
+#endif
+
 static ForceField::FftPass plan_axis(uint32_t n, uint32_t n_outer, uint32_t cols, uint32_t cus) {
   ForceField::FftPass ps;
   const int ix = fft_p2_index(n);
   if (ix < 0) return ps;
   const uint32_t wmax = std::min(512u / kP2N2[ix], 8192u / n);
-  uint32_t tiles = std::max(ceil_div(cols, wmax), ceil_div(cus, n_outer));
+  uint32_t tiles = std::max(ceil_div(cols, wmax), ceil_div(cus * HMD_FFT_BLOCKS_PER_CU, n_outer));
   tiles = std::min(tiles, cols);
   ps.p2 = true;
   ps.k = kP2Axis[ix];
@@ -153,7 +159,7 @@ static ForceField::FftPass plan_lines(uint32_t N, uint32_t n_lines, uint32_t cus
   const uint32_t wmax = std::min(512u / kP2N2[ix], 8192u / N);
   ps.p2 = true;
   ps.k = r2c ? kP2R2c[ix] : kP2C2r[ix];
-  ps.W = std::max(1u, std::min(wmax, ceil_div(n_lines, cus)));
+  ps.W = std::max(1u, std::min(wmax, ceil_div(n_lines, cus * HMD_FFT_BLOCKS_PER_CU)));
   ps.grid = ceil_div(n_lines, ps.W);
   ps.shmem = (size_t)N * ps.W * 8;
   return ps;
