@@ -296,7 +296,7 @@ ForceField::ForceField(Context& ctx, Particles& part, const hmd_forcefield_desc&
 }
 
 // compute_cell_layout analogue (src/forces/neighbor_list.rs:1563-1583):
-// the reference needs >= 3 cells of edge r_search per axis; the fine Morton
+// the reference needs >= 3 cells of edge r_search per axis; the fine Hilbert
 // grid here only orders atoms, but the same box requirement keeps the
 // minimum-image pair set well defined.
 void ForceField::init_neighbor_list(const hmd_forcefield_desc& d) {

@@ -180,7 +180,7 @@ struct PairFinalizeParams {
 
 // ------------------------------------------------------------- spme ----
 struct SpmeBinParams {
-  const float* posq_sorted;    // neighbour-list (Morton) order
+  const float* posq_sorted;    // neighbour-list (Hilbert) order
   const uint32_t* sorted_id;   // sorted slot -> canonical id
   uint32_t n;
   HmdLattice L;

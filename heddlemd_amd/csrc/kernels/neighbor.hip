@@ -103,7 +103,7 @@ hmd_nbl_displacement(const NblDispParams p) {
   if (d2 > p.threshold_sq) atomicOr(p.status, HMD_STATUS_DISPLACED);
 }
 
-// Fine-cell Morton key + histogram.
+// Fine-cell Hilbert key + histogram.
 extern "C" __global__ void __launch_bounds__(256)
 hmd_nbl_keys(const NblKeyParams p) {
   if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;

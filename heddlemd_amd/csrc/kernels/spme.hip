@@ -119,7 +119,7 @@ __device__ inline void hmd_spme_setup(float4 a, const HmdLattice& L,
 }
 
 // Bin each atom by the brick holding its primary point.  Atoms are visited
-// in the neighbour list's Morton order, so a wave's 64 atoms fall into a few
+// in the neighbour list's Hilbert order, so a wave's 64 atoms fall into a few
 // bins: one atomic per (wave, bin) instead of one per atom.
 __device__ inline uint32_t hmd_spme_bin_of(const SpmeBinParams& p, float4 pq) {
   float t[3];
