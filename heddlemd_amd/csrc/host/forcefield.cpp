@@ -136,18 +136,21 @@ static const KernelId kP2C2r[5] = {K_hmd_fft_c2r_16, K_hmd_fft_c2r_32, K_hmd_fft
 #define HMD_FFT_BLOCKS_PER_CU 2
 #endif
 
-static ForceField::FftPass plan_axis(uint32_t n, uint32_t n_outer, uint32_t cols, uint32_t cus) {
+// `tiles` LDS tiles of n x W complex values per workgroup (the a-axis pass
+// stages the influence table beside the data), at most 64 KiB in all.
+static ForceField::FftPass plan_axis(uint32_t n, uint32_t n_outer, uint32_t cols, uint32_t cus,
+                                     uint32_t tiles_lds = 1) {
   ForceField::FftPass ps;
   const int ix = fft_p2_index(n);
   if (ix < 0) return ps;
-  const uint32_t wmax = std::min(512u / kP2N2[ix], 8192u / n);
+  const uint32_t wmax = std::min(512u / kP2N2[ix], 8192u / (n * tiles_lds));
   uint32_t tiles = std::max(ceil_div(cols, wmax), ceil_div(cus * HMD_FFT_BLOCKS_PER_CU, n_outer));
   tiles = std::min(tiles, cols);
   ps.p2 = true;
   ps.k = kP2Axis[ix];
   ps.W = ceil_div(cols, tiles);
   ps.grid = n_outer * ceil_div(cols, ps.W);
-  ps.shmem = (size_t)n * ps.W * 8;
+  ps.shmem = (size_t)n * ps.W * 8 * tiles_lds;
   return ps;
 }
 
@@ -445,7 +448,7 @@ void ForceField::init_spme(const hmd_forcefield_desc& d) {
   pass_r2c = plan_lines(n2 / 2, n0 * n1, cus, true);
   pass_c2r = plan_lines(n2 / 2, n0 * n1, cus, false);
   pass_b = plan_axis(n1, n0, nch, cus);
-  pass_a = plan_axis(n0, n1, nch, cus);
+  pass_a = plan_axis(n0, n1, nch, cus, 2);
   if (pass_a.p2) {
     partial_blocks = pass_a.grid;
   } else {
@@ -453,6 +456,19 @@ void ForceField::init_spme(const hmd_forcefield_desc& d) {
     partial_blocks = ceil_div((size_t)n1 * nch, La);
   }
   partials.alloc(2 * (size_t)partial_blocks);
+  // influence table for the power-of-two a-axis pass (the box is fixed)
+  if (pass_a.p2) {
+    const size_t m_c = (size_t)n0 * n1 * nch;
+    gv.alloc(2 * m_c, false);
+    FftParams g{};
+    g.L = p.lattice;
+    g.alpha = alpha;
+    g.k_coulomb = k_coulomb;
+    g.n0 = n0; g.n1 = n1; g.n2 = n2; g.nch = nch;
+    g.bfac0 = bfac[0].ptr; g.bfac1 = bfac[1].ptr; g.bfac2 = bfac[2].ptr;
+    g.gv = gv.ptr;
+    launch(c, K_hmd_spme_gv_table, (uint32_t)ceil_div(m_c, (size_t)256), 256, 0, g);
+  }
 }
 
 // Probe rebuild: size the per-cluster entry capacity from the measured high
@@ -594,6 +610,7 @@ void ForceField::spme_forces(bool fev) {
   f.k_coulomb = k_coulomb;
   f.n0 = n0; f.n1 = n1; f.n2 = n2; f.nch = nch;
   f.bfac0 = bfac[0].ptr; f.bfac1 = bfac[1].ptr; f.bfac2 = bfac[2].ptr;
+  f.gv = gv.ptr;
   auto set_radix = [&](const std::vector<uint32_t>& r) {
     f.n_stages = (uint32_t)r.size();
     for (size_t k = 0; k < r.size(); ++k) f.radix[k] = r[k];
@@ -663,7 +680,7 @@ void ForceField::spme_forces(bool fev) {
   for (int k = 0; k < 8; ++k) fa.radix[k] = f.radix[k];
   if (pass_a.p2) {
     fa.lines_per_block = pass_a.W;
-    launch(c, pass_a.k, pass_a.grid, 256, pass_a.shmem, fa);
+    launch(c, pass_a.k, pass_a.grid, 256, pass_a.shmem, fa);   // data + (G, vf) tiles
   } else {
     launch(c, K_hmd_fft_axis, ceil_div(fa.n_lines, fa.lines_per_block), 256,
            2 * (size_t)fa.lines_per_block * (n0 + 1) * 8, fa);

@@ -52,6 +52,7 @@ struct ForceField {
   Scanner bin_scan;
   DevBuf<float> rho, V, mesh_c, tw_a, tw_b, tw_c, tw_r2c, u_self;
   DevBuf<double> bfac[3];
+  DevBuf<float> gv;            // float2 (G, vf) per complex mesh point (p2 a-axis pass)
   DevBuf<float> slow_f, slow_e, slow_w;
   DevBuf<double> partials, spme_out;
   std::vector<uint32_t> rad_a, rad_b, rad_c;

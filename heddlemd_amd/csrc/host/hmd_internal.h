@@ -34,6 +34,7 @@
   X(hmd_fft_axis_64) X(hmd_fft_r2c_64) X(hmd_fft_c2r_64)                     \
   X(hmd_fft_axis_128) X(hmd_fft_r2c_128) X(hmd_fft_c2r_128)                  \
   X(hmd_fft_axis_256) X(hmd_fft_r2c_256) X(hmd_fft_c2r_256)                  \
+  X(hmd_spme_gv_table)                                                        \
   X(hmd_vv_kick_drift) X(hmd_vv_kick)                                        \
   X(hmd_settle_snapshot) X(hmd_settle_positions) X(hmd_settle_velocities)    \
   X(hmd_lan_drift_half) X(hmd_lan_ou) X(hmd_counter_increment)               \

@@ -184,7 +184,7 @@ hmd_fft_c2r_c(const FftParams p) {
 // grows with k^2/4a^2, which in f32 dominated the reciprocal energy error;
 // the rest of the mesh pipeline is f32.
 __device__ inline void hmd_influence(const FftParams& p, uint32_t ka, uint32_t kb,
-                                     uint32_t kc, double& G, double& vf) {
+                                     uint32_t kc, double bprod, double& G, double& vf) {
   const HmdLattice L = p.L;
   const double lx = L.lx, ly = L.ly, lz = L.lz, xy = L.xy, xz = L.xz, yz = L.yz;
   const double ilx = 1.0 / lx, ily = 1.0 / ly, ilz = 1.0 / lz;
@@ -205,9 +205,25 @@ __device__ inline void hmd_influence(const FftParams& p, uint32_t ka, uint32_t k
   }
   const double vol = lx * ly * lz;
   const double a2 = (double)p.alpha * (double)p.alpha;
-  G = ((double)p.k_coulomb / vol) * (12.566370614359172 / k2) * exp(-k2 / (4.0 * a2)) *
-      p.bfac0[ka] * p.bfac1[kb] * p.bfac2[kc];
+  G = ((double)p.k_coulomb / vol) * (12.566370614359172 / k2) * exp(-k2 / (4.0 * a2)) * bprod;
   vf = G * (1.0 - k2 / (2.0 * a2));
+}
+
+// Influence table: (G, vf) of every complex mesh point, evaluated in FP64 and
+// rounded once to f32, in the layout of the complex mesh.  The box is fixed
+// for the life of a force field, so the a-axis pass reads this table instead
+// of evaluating exp per point every step.
+extern "C" __global__ void __launch_bounds__(256)
+hmd_spme_gv_table(const FftParams p) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)p.n0 * p.n1 * p.nch;
+  if (i >= total) return;
+  const uint32_t kc = (uint32_t)(i % p.nch);
+  const uint32_t kb = (uint32_t)((i / p.nch) % p.n1);
+  const uint32_t ka = (uint32_t)(i / ((size_t)p.nch * p.n1));
+  double G, vf;
+  hmd_influence(p, ka, kb, kc, p.bfac0[ka] * p.bfac1[kb] * p.bfac2[kc], G, vf);
+  reinterpret_cast<float2*>(const_cast<float*>(p.gv))[i] = make_float2((float)G, (float)vf);
 }
 
 // Passes 2-4: complex FFT along a strided axis (b or a).  With
@@ -241,7 +257,7 @@ hmd_fft_axis(const FftParams p) {
       const uint32_t gl = l0 + l;                 // line = (kb, kc)
       const uint32_t kb = gl / p.nch, kc = gl - kb * p.nch;
       double G, vf;
-      hmd_influence(p, ka, kb, kc, G, vf);
+      hmd_influence(p, ka, kb, kc, p.bfac0[ka] * p.bfac1[kb] * p.bfac2[kc], G, vf);
       float2 x = res[l * ls + ka];
       const double hw = (kc == 0u || (p.n2 % 2u == 0u && 2u * kc == p.n2)) ? 1.0 : 2.0;
       const double r2 = (double)x.x * x.x + (double)x.y * x.y;
@@ -424,6 +440,28 @@ __device__ inline void hmd_fft_cols(float2* tile, uint32_t W, const float2* tw) 
   __syncthreads();
 }
 
+// Tile fill with the loads of HMD_FFT_BATCH elements per thread issued before
+// any of them is waited for: a plain strided loop waits for each load before
+// its LDS store, which serialises the memory latency (~10 round trips per
+// tile).  src(t) returns element t, dst(t) its LDS index.
+#define HMD_FFT_BATCH 8
+template <typename Src, typename Dst>
+__device__ inline void hmd_fill_tile(float2* tile, uint32_t total, Src src, Dst dst) {
+  for (uint32_t t0 = threadIdx.x; t0 < total; t0 += HMD_FFT_BATCH * blockDim.x) {
+    float2 v[HMD_FFT_BATCH];
+#pragma unroll
+    for (int u = 0; u < HMD_FFT_BATCH; ++u) {
+      const uint32_t t = t0 + (uint32_t)u * blockDim.x;
+      v[u] = (t < total) ? src(t) : make_float2(0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int u = 0; u < HMD_FFT_BATCH; ++u) {
+      const uint32_t t = t0 + (uint32_t)u * blockDim.x;
+      if (t < total) tile[dst(t)] = v[u];
+    }
+  }
+}
+
 // Passes 2-4 for power-of-two axis lengths.  Block = (outer index, column
 // tile): columns are the line_minor lines of one slab that are adjacent in
 // memory (the c index), elem_stride steps along the transformed axis,
@@ -442,9 +480,24 @@ __device__ inline void hmd_fft_axis_slab(const FftParams& p) {
   const uint32_t Wc = min(W, p.line_minor - c0);
   float2* data = reinterpret_cast<float2*>(p.data);
   const size_t base = (size_t)outer * p.major_stride + c0;
-  for (uint32_t t = threadIdx.x; t < N * W; t += blockDim.x) {
-    const uint32_t k = t / W, w = t - k * W;
-    s_tile[t] = (w < Wc) ? data[base + (size_t)k * p.elem_stride + w] : make_float2(0.0f, 0.0f);
+  hmd_fill_tile(
+      s_tile, N * W,
+      [&](uint32_t t) {
+        const uint32_t k = t / W, w = t - k * W;
+        return (w < Wc) ? data[base + (size_t)k * p.elem_stride + w] : make_float2(0.0f, 0.0f);
+      },
+      [](uint32_t t) { return t; });
+  // (G, vf) of this tile from the influence table, behind the data tile
+  float2* s_gv = s_tile + N * W;
+  if (p.apply_influence) {
+    const float2* gv = reinterpret_cast<const float2*>(p.gv);
+    hmd_fill_tile(
+        s_gv, N * W,
+        [&](uint32_t t) {
+          const uint32_t k = t / W, w = t - k * W;
+          return (w < Wc) ? gv[base + (size_t)k * p.elem_stride + w] : make_float2(0.0f, 0.0f);
+        },
+        [](uint32_t t) { return t; });
   }
   __syncthreads();
   const float2* tw = reinterpret_cast<const float2*>(p.twiddle);
@@ -455,18 +508,16 @@ __device__ inline void hmd_fft_axis_slab(const FftParams& p) {
   if (p.apply_influence) {
     double vir = 0.0, ene = 0.0;
     for (uint32_t t = threadIdx.x; t < N * W; t += blockDim.x) {
-      const uint32_t ka = t / W, w = t - ka * W;
+      const uint32_t w = t % W;
       if (w >= Wc) continue;
-      const uint32_t kb = outer, kc = c0 + w;
-      double G, vf;
-      hmd_influence(p, ka, kb, kc, G, vf);
+      const uint32_t kc = c0 + w;
+      const float2 g = s_gv[t];          // (G, vf)
       const float2 x = s_tile[t];
       const double hw = (kc == 0u || (p.n2 % 2u == 0u && 2u * kc == p.n2)) ? 1.0 : 2.0;
       const double r2 = (double)x.x * x.x + (double)x.y * x.y;
-      vir += hw * vf * r2;
-      ene += hw * G * r2;
-      const float Gf = (float)G;
-      s_tile[t] = make_float2(Gf * x.x, Gf * x.y);
+      vir += hw * (double)g.y * r2;
+      ene += hw * (double)g.x * r2;
+      s_tile[t] = make_float2(g.x * x.x, g.x * x.y);
     }
     s_red[0][threadIdx.x] = vir;
     s_red[1][threadIdx.x] = ene;
@@ -500,10 +551,16 @@ __device__ inline void hmd_fft_r2c_slab(const FftParams& p) {
   const uint32_t l0 = blockIdx.x * W;
   const uint32_t Wc = min(W, p.n_lines - l0);
   const float2* in = reinterpret_cast<const float2*>(p.data);
-  for (uint32_t t = threadIdx.x; t < N * W; t += blockDim.x) {
-    const uint32_t w = t / N, m = t - w * N;
-    s_tile[m * W + w] = (w < Wc) ? in[(size_t)(l0 + w) * N + m] : make_float2(0.0f, 0.0f);
-  }
+  hmd_fill_tile(
+      s_tile, N * W,
+      [&](uint32_t t) {
+        const uint32_t w = t / N, m = t - w * N;
+        return (w < Wc) ? in[(size_t)(l0 + w) * N + m] : make_float2(0.0f, 0.0f);
+      },
+      [&](uint32_t t) {
+        const uint32_t w = t / N, m = t - w * N;
+        return m * W + w;
+      });
   __syncthreads();
   hmd_fft_cols<N1, N2, -1>(s_tile, W, reinterpret_cast<const float2*>(p.twiddle));
   const uint32_t nch = N + 1;
@@ -530,18 +587,24 @@ __device__ inline void hmd_fft_c2r_slab(const FftParams& p) {
   const uint32_t nch = N + 1;
   const float2* in = reinterpret_cast<const float2*>(p.data);
   const float2* wr = reinterpret_cast<const float2*>(p.twiddle_r2c);
-  for (uint32_t t = threadIdx.x; t < N * W; t += blockDim.x) {
-    const uint32_t w = t / N, m = t - w * N;
-    float2 z = make_float2(0.0f, 0.0f);
-    if (w < Wc) {
-      const float2* x = in + (size_t)(l0 + w) * nch;
-      const float2 xk = x[m], xc = cconj(x[N - m]);
-      const float2 ee = cadd(xk, xc);
-      const float2 oo = cmul(csub(xk, xc), cconj(wr[m]));
-      z = cadd(ee, cmul_si(oo, +1));
-    }
-    s_tile[m * W + w] = z;
-  }
+  hmd_fill_tile(
+      s_tile, N * W,
+      [&](uint32_t t) {
+        const uint32_t w = t / N, m = t - w * N;
+        float2 z = make_float2(0.0f, 0.0f);
+        if (w < Wc) {
+          const float2* x = in + (size_t)(l0 + w) * nch;
+          const float2 xk = x[m], xc = cconj(x[N - m]);
+          const float2 ee = cadd(xk, xc);
+          const float2 oo = cmul(csub(xk, xc), cconj(wr[m]));
+          z = cadd(ee, cmul_si(oo, +1));
+        }
+        return z;
+      },
+      [&](uint32_t t) {
+        const uint32_t w = t / N, m = t - w * N;
+        return m * W + w;
+      });
   __syncthreads();
   hmd_fft_cols<N1, N2, +1>(s_tile, W, reinterpret_cast<const float2*>(p.twiddle));
   float2* out = reinterpret_cast<float2*>(p.out);

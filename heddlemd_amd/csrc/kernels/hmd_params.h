@@ -247,6 +247,7 @@ struct FftParams {
   const double* bfac1;
   const double* bfac2;
   double* partials;            // [grid blocks][2]: virial, energy
+  const float* gv;             // float2[n0][n1][nch]: (G, vf) in f32, or null
 };
 
 struct SpmeReduceParams {
