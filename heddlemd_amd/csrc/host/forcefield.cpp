@@ -539,7 +539,8 @@ void ForceField::rebuild_chain() {
   entry_scan.run(c, entry_count.ptr, entry_offset.ptr, gate, HMD_STATUS_DISPLACED);
   NblWorkParams wp{entry_count.ptr, entry_offset.ptr, n_clusters, cap_entries,
                    work.ptr, n_work.ptr, status.ptr};
-  launch(c, K_hmd_nbl_worklist, 1, 1024, 0, wp);
+  launch(c, K_hmd_nbl_worklist, ceil_div(n_clusters, 4u), 256, 0, wp);
+  launch(c, K_hmd_nbl_close, 1, 64, 0, wp);
 }
 
 void ForceField::neighbor_pre_step() {

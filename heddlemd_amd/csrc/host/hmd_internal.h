@@ -19,7 +19,7 @@
   X(hmd_scan_tiles) X(hmd_scan_add)                                          \
   X(hmd_nbl_displacement) X(hmd_nbl_keys) X(hmd_nbl_scatter)                 \
   X(hmd_nbl_sort_cells) X(hmd_nbl_gather) X(hmd_nbl_bbox) X(hmd_nbl_build)   \
-  X(hmd_nbl_worklist) X(hmd_nbl_pairs)                                       \
+  X(hmd_nbl_worklist) X(hmd_nbl_close) X(hmd_nbl_pairs)                     \
   X(hmd_pair_force_f) X(hmd_pair_force_fev) X(hmd_pair_force_lj_f)           \
   X(hmd_pair_force_lj_fev) X(hmd_pair_force_coul_f) X(hmd_pair_force_coul_fev)\
   X(hmd_pair_finalize)                                                       \

@@ -302,6 +302,8 @@ hmd_nbl_build(const NblBuildParams p) {
   __shared__ uint32_t s_pre[HMD_NBL_MAX_CHUNKS + 1];
   __shared__ uint32_t s_cand[HMD_NBL_CAND_CAP];
   __shared__ uint64_t s_hit[HMD_NBL_CAND_CAP];
+  __shared__ uint32_t s_off[HMD_NBL_CAND_CAP];
+  __shared__ uint32_t s_fill;
   const uint32_t c = blockIdx.x;
   const uint32_t wave = threadIdx.x / HMD_WAVE;
   const uint32_t lane = hmd_lane();
@@ -317,28 +319,40 @@ hmd_nbl_build(const NblBuildParams p) {
   // SGPRs) rather than as LDS broadcasts.
   const hmd_const_f4* csub = (const hmd_const_f4*)(reinterpret_cast<const float4*>(p.sub_bbox) + 16 * c);
 
-  // candidate masks, 64 j-clusters per ballot
+  // candidate masks, 64 j-clusters per ballot; each wave loads the boxes of
+  // four chunks before testing them, so that the loads' latencies overlap
   {
     const float4 ci = bb[2 * c], hi = bb[2 * c + 1];
-    for (uint32_t ch = wave; ch < n_chunks; ch += 4) {
-      const uint32_t jc = c + ch * HMD_WAVE + lane;
-      bool cand = false, nowrap = false;
-      if (jc < p.n_clusters) {
-        const float4 cj = bb[2 * jc], hj = bb[2 * jc + 1];
-        cand = hmd_box_gap2(ci, hi, cj, hj, p.L, ortho, ilx, ily, ilz) <= rs2_prune;
-        // No wrap: every atom pair of the two clusters is closer than 0.49 L
-        // along each axis (orthorhombic box), so the minimum image of each
-        // difference is the difference itself (image index 0) and the exact
-        // f32 distance needs no image arithmetic.
-        nowrap = ortho && fabsf(cj.x - ci.x) + hj.x + hi.x < 0.49f * p.L.lx &&
-                 fabsf(cj.y - ci.y) + hj.y + hi.y < 0.49f * p.L.ly &&
-                 fabsf(cj.z - ci.z) + hj.z + hi.z < 0.49f * p.L.lz;
+    for (uint32_t ch0 = wave; ch0 < n_chunks; ch0 += 16) {
+      float4 cj[4], hj[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        const uint32_t jc = c + (ch0 + 4 * u) * HMD_WAVE + lane;
+        const bool in = ch0 + 4 * u < n_chunks && jc < p.n_clusters;
+        cj[u] = in ? bb[2 * jc] : make_float4(0, 0, 0, 0);
+        hj[u] = in ? bb[2 * jc + 1] : make_float4(-1.0f, -1.0f, -1.0f, 0);
       }
-      const uint64_t m = __ballot(cand);
-      const uint64_t nw = __ballot(nowrap);
-      if (lane == 0) {
-        s_mask[ch] = m;
-        s_nowrap[ch] = nw;
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        const uint32_t ch = ch0 + 4 * u;
+        if (ch >= n_chunks) break;
+        const uint32_t jc = c + ch * HMD_WAVE + lane;
+        bool cand = false, nowrap = false;
+        if (jc < p.n_clusters) {
+          cand = hmd_box_gap2(ci, hi, cj[u], hj[u], p.L, ortho, ilx, ily, ilz) <= rs2_prune;
+          // No wrap: every atom pair of the two clusters is closer than 0.49 L
+          // along each axis (orthorhombic box), so the minimum image of each
+          // difference is the difference itself (image index 0).
+          nowrap = ortho && fabsf(cj[u].x - ci.x) + hj[u].x + hi.x < 0.49f * p.L.lx &&
+                   fabsf(cj[u].y - ci.y) + hj[u].y + hi.y < 0.49f * p.L.ly &&
+                   fabsf(cj[u].z - ci.z) + hj[u].z + hi.z < 0.49f * p.L.lz;
+        }
+        const uint64_t m = __ballot(cand);
+        const uint64_t nw = __ballot(nowrap);
+        if (lane == 0) {
+          s_mask[ch] = m;
+          s_nowrap[ch] = nw;
+        }
       }
     }
   }
@@ -352,16 +366,23 @@ hmd_nbl_build(const NblBuildParams p) {
       if (ch < n_chunks) s_pre[ch] = carry + incl - cnt;
       carry += hmd_readlane(incl, HMD_WAVE - 1);
     }
-    if (lane == 0) s_pre[n_chunks] = carry;
+    if (lane == 0) {
+      s_pre[n_chunks] = carry;
+      s_fill = 0;
+    }
   }
   __syncthreads();
   const uint32_t total = s_pre[n_chunks];
 
-  // wave 0: exclusion slots and entry state
+  // wave 0: the sorted slots of each i-atom's exclusion partners, and their
+  // positions in the cluster's list once placed
   const uint32_t si = c * HMD_CLUSTER + lane;
-  uint32_t ex_slot[HMD_MAX_INLINE_EXCL];
+  uint32_t ex_slot[HMD_MAX_INLINE_EXCL], ex_pos[HMD_MAX_INLINE_EXCL];
 #pragma unroll
-  for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m) ex_slot[m] = HMD_SENTINEL;
+  for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m) {
+    ex_slot[m] = HMD_SENTINEL;
+    ex_pos[m] = HMD_SENTINEL;
+  }
   if (wave == 0 && si < p.n && p.excl_offsets) {
     const uint32_t id = p.sorted_id[si];
     const uint32_t b = p.excl_offsets[id], e = p.excl_offsets[id + 1];
@@ -371,10 +392,7 @@ hmd_nbl_build(const NblBuildParams p) {
       if (m < n_ex) ex_slot[m] = p.slot_of[p.excl_partners[b + m]];
   }
   const size_t base_entry = (size_t)c * p.cap_entries;
-  uint32_t n_entries = 0;       // completed entries (uniform)
-  uint32_t fill = 0;            // slots used in the current entry (uniform)
-  uint64_t cur_mask = 0, next_mask = 0;
-  bool overflow = false;
+  const uint32_t cap_slots = p.cap_entries * HMD_ENTRY;
   const float4 zero = make_float4(0, 0, 0, 0);
 
   for (uint32_t base = 0; base < total; base += HMD_NBL_CAND_CAP) {
@@ -443,80 +461,104 @@ hmd_nbl_build(const NblBuildParams p) {
     }
     __syncthreads();
 
-    // placement, in candidate order (wave 0)
+    // list positions of each candidate's hits: exclusive scan of the hit
+    // counts in candidate order (wave 0)
     if (wave == 0) {
-      for (uint32_t k = 0; k < cnt; ++k) {
-        const uint64_t H = s_hit[k];
-        if (H == 0) continue;
-        const uint32_t jcl = s_cand[k] & 0x7FFFFFFFu;
-        if ((H >> lane) & 1ull) {
-          const uint32_t pos = fill + (uint32_t)__builtin_popcountll(H & lt_mask);
-          const uint32_t e = n_entries + pos / HMD_ENTRY;
-          if (e < p.cap_entries)
-            p.entry_j[(base_entry + e) * HMD_ENTRY + pos % HMD_ENTRY] = jcl * HMD_CLUSTER + lane;
-        }
+      uint32_t carry = s_fill;
+      for (uint32_t b = 0; b < cnt; b += HMD_WAVE) {
+        const uint32_t k = b + lane;
+        const uint32_t h = k < cnt ? (uint32_t)__builtin_popcountll(s_hit[k]) : 0u;
+        const uint32_t incl = hmd_wave_scan_incl(h, lane);
+        if (k < cnt) s_off[k] = carry + incl - h;
+        carry += hmd_readlane(incl, HMD_WAVE - 1);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (lane == 0) s_fill = carry;
+      // exclusion partners in this round's candidates: the candidates are in
+      // increasing cluster order, so a binary search finds a partner's one
 #pragma unroll
-        for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m) {
-          const uint32_t es = ex_slot[m];
-          if (es != HMD_SENTINEL && es / HMD_CLUSTER == jcl) {
-            const uint32_t b = es % HMD_CLUSTER;
-            if ((H >> b) & 1ull) {
-              const uint64_t below = (b == 0) ? 0ull : (H & (~0ull >> (64 - b)));
-              const uint32_t pos = fill + (uint32_t)__builtin_popcountll(below);
-              if (pos < HMD_ENTRY) cur_mask |= 1ull << pos;
-              else next_mask |= 1ull << (pos - HMD_ENTRY);
-            }
-          }
+      for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m) {
+        const uint32_t es = ex_slot[m];
+        if (es == HMD_SENTINEL) continue;
+        const uint32_t pc = es / HMD_CLUSTER, pb = es % HMD_CLUSTER;
+        uint32_t lo = 0, hi = cnt;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) / 2;
+          if ((s_cand[mid] & 0x7FFFFFFFu) < pc) lo = mid + 1;
+          else hi = mid;
         }
-        fill += (uint32_t)__builtin_popcountll(H);
-        if (fill >= HMD_ENTRY) {
-          if (n_entries < p.cap_entries)
-            p.entry_excl[(base_entry + n_entries) * HMD_ENTRY + lane] = cur_mask;
-          else
-            overflow = true;
-          ++n_entries;
-          fill -= HMD_ENTRY;
-          cur_mask = next_mask;
-          next_mask = 0;
+        if (lo < cnt && (s_cand[lo] & 0x7FFFFFFFu) == pc) {
+          const uint64_t H = s_hit[lo];
+          if ((H >> pb) & 1ull) {
+            const uint64_t below = (pb == 0) ? 0ull : (H & (~0ull >> (64 - pb)));
+            ex_pos[m] = s_off[lo] + (uint32_t)__builtin_popcountll(below);
+          }
         }
       }
     }
     __syncthreads();
-  }
-  if (wave != 0) return;
-  if (fill > 0) {
-    if (n_entries < p.cap_entries) {
-      const size_t o = (base_entry + n_entries) * HMD_ENTRY + lane;
-      if (lane >= fill) p.entry_j[o] = HMD_SENTINEL;
-      p.entry_excl[o] = cur_mask;
-    } else {
-      overflow = true;
+
+    // placement, all waves: lane k of candidate q writes its j-atom at list
+    // position s_off[q] + (hits of q below lane k)
+    for (uint32_t k = wave; k < cnt; k += 4) {
+      const uint64_t H = s_hit[k];
+      if ((H >> lane) & 1ull) {
+        const uint32_t pos = s_off[k] + (uint32_t)__builtin_popcountll(H & lt_mask);
+        if (pos < cap_slots)
+          p.entry_j[base_entry * HMD_ENTRY + pos] =
+              (s_cand[k] & 0x7FFFFFFFu) * HMD_CLUSTER + lane;
+      }
     }
-    ++n_entries;
+    __syncthreads();
   }
-  if (lane == 0) {
-    p.entry_count[c] = min(n_entries, p.cap_entries);
-    atomicMax(&p.status[2], n_entries);   // high-water mark: the host resizes
-    if (overflow) atomicOr(p.status, HMD_STATUS_ENTRY_OVERFLOW);
+  const uint32_t fill = s_fill;
+  const uint32_t n_entries = (fill + HMD_ENTRY - 1) / HMD_ENTRY;
+  const uint32_t n_written = min(n_entries, p.cap_entries);
+  // empty slots of the last entry, and the exclusion masks of every entry
+  if (wave == 1 && n_entries > 0 && n_entries <= p.cap_entries) {
+    const uint32_t last = (n_entries - 1) * HMD_ENTRY + lane;
+    if (last >= fill) p.entry_j[base_entry * HMD_ENTRY + last] = HMD_SENTINEL;
+  }
+  if (wave == 0) {
+    for (uint32_t e = 0; e < n_written; ++e) {
+      uint64_t mask = 0;
+#pragma unroll
+      for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m)
+        if (ex_pos[m] != HMD_SENTINEL && ex_pos[m] / HMD_ENTRY == e)
+          mask |= 1ull << (ex_pos[m] % HMD_ENTRY);
+      p.entry_excl[(base_entry + e) * HMD_ENTRY + lane] = mask;
+    }
+    if (lane == 0) {
+      p.entry_count[c] = n_written;
+      atomicMax(&p.status[2], n_entries);   // high-water mark: the host resizes
+      if (n_entries > p.cap_entries) atomicOr(p.status, HMD_STATUS_ENTRY_OVERFLOW);
+    }
   }
 }
 
-// Flatten (cluster, entry) into the work list the force kernel walks, and
-// close the rebuild (single block of 1024 threads; entry_offset is the
-// exclusive scan of entry_count with the total at [n_clusters]).
-extern "C" __global__ void __launch_bounds__(1024)
+// Flatten (cluster, entry) into the work list the force kernel walks, one
+// wave per cluster (entry_offset is the exclusive scan of entry_count with
+// the total at [n_clusters]).
+extern "C" __global__ void __launch_bounds__(256)
 hmd_nbl_worklist(const NblWorkParams p) {
   if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
-  for (uint32_t c = threadIdx.x; c < p.n_clusters; c += blockDim.x) {
-    uint32_t cnt = p.entry_count[c];
-    uint32_t off = p.entry_offset[c];
-    for (uint32_t k = 0; k < cnt; ++k) p.work[off + k] = c * p.cap_entries + k;
-  }
-  __syncthreads();
+  const uint32_t c = blockIdx.x * (blockDim.x / HMD_WAVE) + threadIdx.x / HMD_WAVE;
+  if (c >= p.n_clusters) return;
+  const uint32_t cnt = p.entry_count[c];
+  const uint32_t off = p.entry_offset[c];
+  for (uint32_t k = hmd_lane(); k < cnt; k += HMD_WAVE) p.work[off + k] = c * p.cap_entries + k;
+}
+
+// Close the rebuild: work-list length, status flags and generation (a kernel
+// of its own, after every block of the work-list kernel has read the gate).
+extern "C" __global__ void __launch_bounds__(64)
+hmd_nbl_close(const NblWorkParams p) {
+  if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
   if (threadIdx.x == 0) {
     p.n_work[0] = p.entry_offset[p.n_clusters];
     uint32_t* st = p.status;
-    uint32_t s = st[0];
+    const uint32_t s = st[0];
     st[0] = (s & ~HMD_STATUS_DISPLACED) | HMD_STATUS_REBUILT;
     st[1] += 1u;   // rebuild generation
   }
