@@ -535,7 +535,10 @@ void ForceField::rebuild_chain() {
   bd.excl_offsets = excl_off.ptr;
   bd.excl_partners = excl_par.ptr;
   bd.status = status.ptr;
-  launch(c, K_hmd_nbl_build, n_clusters, 256, 0, bd);
+#ifndef HMD_NBL_WAVES
+#define HMD_NBL_WAVES 4u
+#endif
+  launch(c, K_hmd_nbl_build, n_clusters, 64 * HMD_NBL_WAVES, 0, bd);
   entry_scan.run(c, entry_count.ptr, entry_offset.ptr, gate, HMD_STATUS_DISPLACED);
   NblWorkParams wp{entry_count.ptr, entry_offset.ptr, n_clusters, cap_entries,
                    work.ptr, n_work.ptr, status.ptr};

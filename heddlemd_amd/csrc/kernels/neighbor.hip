@@ -281,6 +281,9 @@ __device__ inline float hmd_box_gap2(float4 ca, float4 ha, float4 cb, float4 hb,
 #define HMD_MAX_INLINE_EXCL 8
 #define HMD_NBL_MAX_CHUNKS 1024u    // 64 j-clusters per chunk: 65,536 clusters
 #define HMD_NBL_CAND_CAP 512u
+#ifndef HMD_NBL_PREFETCH
+#define HMD_NBL_PREFETCH 3       // candidates' j-atoms loaded ahead per wave
+#endif
 
 typedef __attribute__((address_space(4))) const float4 hmd_const_f4;
 
@@ -294,7 +297,16 @@ __device__ inline uint32_t hmd_wave_scan_incl(uint32_t v, uint32_t lane) {
   return v;
 }
 
-extern "C" __global__ void __launch_bounds__(256, 3)
+#ifndef HMD_NBL_BUILD_WAVES_PER_EU
+#define HMD_NBL_BUILD_WAVES_PER_EU 3
+#endif
+#ifndef HMD_NBL_WAVES
+#define HMD_NBL_WAVES 4u              // waves per i-cluster workgroup
+#endif
+#ifndef HMD_NBL_BOX_BATCH
+#define HMD_NBL_BOX_BATCH 4u          // candidate chunks loaded together
+#endif
+extern "C" __global__ void __launch_bounds__(64 * HMD_NBL_WAVES, HMD_NBL_BUILD_WAVES_PER_EU)
 hmd_nbl_build(const NblBuildParams p) {
   if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
   __shared__ uint64_t s_mask[HMD_NBL_MAX_CHUNKS];
@@ -319,22 +331,40 @@ hmd_nbl_build(const NblBuildParams p) {
   // SGPRs) rather than as LDS broadcasts.
   const hmd_const_f4* csub = (const hmd_const_f4*)(reinterpret_cast<const float4*>(p.sub_bbox) + 16 * c);
 
+  // wave 0: the sorted slots of each i-atom's exclusion partners (loaded
+  // first, so that the chain of dependent loads overlaps the candidate
+  // search), and their positions in the cluster's list once placed
+  const uint32_t si = c * HMD_CLUSTER + lane;
+  uint32_t ex_slot[HMD_MAX_INLINE_EXCL], ex_pos[HMD_MAX_INLINE_EXCL];
+#pragma unroll
+  for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m) {
+    ex_slot[m] = HMD_SENTINEL;
+    ex_pos[m] = HMD_SENTINEL;
+  }
+  if (wave == 0 && si < p.n && p.excl_offsets) {
+    const uint32_t id = p.sorted_id[si];
+    const uint32_t b = p.excl_offsets[id], e = p.excl_offsets[id + 1];
+    const uint32_t n_ex = min(e - b, (uint32_t)HMD_MAX_INLINE_EXCL);
+#pragma unroll
+    for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m)
+      if (m < n_ex) ex_slot[m] = p.slot_of[p.excl_partners[b + m]];
+  }
   // candidate masks, 64 j-clusters per ballot; each wave loads the boxes of
   // four chunks before testing them, so that the loads' latencies overlap
   {
     const float4 ci = bb[2 * c], hi = bb[2 * c + 1];
-    for (uint32_t ch0 = wave; ch0 < n_chunks; ch0 += 16) {
-      float4 cj[4], hj[4];
+    for (uint32_t ch0 = wave; ch0 < n_chunks; ch0 += HMD_NBL_BOX_BATCH * HMD_NBL_WAVES) {
+      float4 cj[HMD_NBL_BOX_BATCH], hj[HMD_NBL_BOX_BATCH];
 #pragma unroll
-      for (uint32_t u = 0; u < 4; ++u) {
-        const uint32_t jc = c + (ch0 + 4 * u) * HMD_WAVE + lane;
-        const bool in = ch0 + 4 * u < n_chunks && jc < p.n_clusters;
+      for (uint32_t u = 0; u < HMD_NBL_BOX_BATCH; ++u) {
+        const uint32_t jc = c + (ch0 + HMD_NBL_WAVES * u) * HMD_WAVE + lane;
+        const bool in = ch0 + HMD_NBL_WAVES * u < n_chunks && jc < p.n_clusters;
         cj[u] = in ? bb[2 * jc] : make_float4(0, 0, 0, 0);
         hj[u] = in ? bb[2 * jc + 1] : make_float4(-1.0f, -1.0f, -1.0f, 0);
       }
 #pragma unroll
-      for (uint32_t u = 0; u < 4; ++u) {
-        const uint32_t ch = ch0 + 4 * u;
+      for (uint32_t u = 0; u < HMD_NBL_BOX_BATCH; ++u) {
+        const uint32_t ch = ch0 + HMD_NBL_WAVES * u;
         if (ch >= n_chunks) break;
         const uint32_t jc = c + ch * HMD_WAVE + lane;
         bool cand = false, nowrap = false;
@@ -374,30 +404,13 @@ hmd_nbl_build(const NblBuildParams p) {
   __syncthreads();
   const uint32_t total = s_pre[n_chunks];
 
-  // wave 0: the sorted slots of each i-atom's exclusion partners, and their
-  // positions in the cluster's list once placed
-  const uint32_t si = c * HMD_CLUSTER + lane;
-  uint32_t ex_slot[HMD_MAX_INLINE_EXCL], ex_pos[HMD_MAX_INLINE_EXCL];
-#pragma unroll
-  for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m) {
-    ex_slot[m] = HMD_SENTINEL;
-    ex_pos[m] = HMD_SENTINEL;
-  }
-  if (wave == 0 && si < p.n && p.excl_offsets) {
-    const uint32_t id = p.sorted_id[si];
-    const uint32_t b = p.excl_offsets[id], e = p.excl_offsets[id + 1];
-    const uint32_t n_ex = min(e - b, (uint32_t)HMD_MAX_INLINE_EXCL);
-#pragma unroll
-    for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m)
-      if (m < n_ex) ex_slot[m] = p.slot_of[p.excl_partners[b + m]];
-  }
   const size_t base_entry = (size_t)c * p.cap_entries;
   const uint32_t cap_slots = p.cap_entries * HMD_ENTRY;
   const float4 zero = make_float4(0, 0, 0, 0);
 
   for (uint32_t base = 0; base < total; base += HMD_NBL_CAND_CAP) {
     const uint32_t cnt = min(HMD_NBL_CAND_CAP, total - base);
-    for (uint32_t ch = wave; ch < n_chunks; ch += 4) {
+    for (uint32_t ch = wave; ch < n_chunks; ch += HMD_NBL_WAVES) {
       const uint64_t m = s_mask[ch];
       if ((m >> lane) & 1ull) {
         const uint32_t pos = s_pre[ch] + (uint32_t)__builtin_popcountll(m & lt_mask);
@@ -409,21 +422,33 @@ hmd_nbl_build(const NblBuildParams p) {
     __syncthreads();
 
     // hit masks, candidates shared round robin, next j-atom prefetched
+    // (j-atoms of the next HMD_NBL_PREFETCH candidates of this wave in flight)
     uint32_t q = wave;
-    float4 pj_next = zero;
-    if (q < cnt) {
-      const uint32_t sj = (s_cand[q] & 0x7FFFFFFFu) * HMD_CLUSTER + lane;
-      if (sj < p.n) pj_next = ps[sj];
+    float4 pj_pre[HMD_NBL_PREFETCH];
+#pragma unroll
+    for (uint32_t d = 0; d < HMD_NBL_PREFETCH; ++d) {
+      const uint32_t qq = q + HMD_NBL_WAVES * d;
+      pj_pre[d] = zero;
+      if (qq < cnt) {
+        const uint32_t sj = (s_cand[qq] & 0x7FFFFFFFu) * HMD_CLUSTER + lane;
+        if (sj < p.n) pj_pre[d] = ps[sj];
+      }
     }
-    for (; q < cnt; q += 4) {
+    for (; q < cnt; q += HMD_NBL_WAVES) {
       const uint32_t cw = s_cand[q];
       const uint32_t jcl = cw & 0x7FFFFFFFu;
       const bool nowrap = (cw >> 31) != 0u;
       const uint32_t sj = jcl * HMD_CLUSTER + lane;
-      const float4 pj = pj_next;
-      if (q + 4 < cnt) {
-        const uint32_t sn = (s_cand[q + 4] & 0x7FFFFFFFu) * HMD_CLUSTER + lane;
-        pj_next = sn < p.n ? ps[sn] : zero;
+      const float4 pj = pj_pre[0];
+#pragma unroll
+      for (uint32_t d = 0; d + 1 < HMD_NBL_PREFETCH; ++d) pj_pre[d] = pj_pre[d + 1];
+      {
+        const uint32_t qn = q + HMD_NBL_WAVES * HMD_NBL_PREFETCH;
+        pj_pre[HMD_NBL_PREFETCH - 1] = zero;
+        if (qn < cnt) {
+          const uint32_t sn = (s_cand[qn] & 0x7FFFFFFFu) * HMD_CLUSTER + lane;
+          if (sn < p.n) pj_pre[HMD_NBL_PREFETCH - 1] = ps[sn];
+        }
       }
       // Sub-box mask: which 8-atom i-sub-clusters this j-atom may reach (the
       // nearest point of the sub-box within the search radius, with a relative
@@ -501,7 +526,7 @@ hmd_nbl_build(const NblBuildParams p) {
 
     // placement, all waves: lane k of candidate q writes its j-atom at list
     // position s_off[q] + (hits of q below lane k)
-    for (uint32_t k = wave; k < cnt; k += 4) {
+    for (uint32_t k = wave; k < cnt; k += HMD_NBL_WAVES) {
       const uint64_t H = s_hit[k];
       if ((H >> lane) & 1ull) {
         const uint32_t pos = s_off[k] + (uint32_t)__builtin_popcountll(H & lt_mask);
