@@ -18,7 +18,14 @@ _c_u64p = ctypes.POINTER(ctypes.c_uint64)
 
 HMD_OK = 0
 STATUS_NAMES = {1: "HMD_ERR_GPU", 2: "HMD_ERR_INVALID", 3: "HMD_ERR_NBL_OVERFLOW",
-                4: "HMD_ERR_BOX_TOO_SMALL", 5: "HMD_ERR_UNSUPPORTED", 6: "HMD_ERR_NO_DEVICE"}
+                4: "HMD_ERR_BOX_TOO_SMALL", 5: "HMD_ERR_UNSUPPORTED", 6: "HMD_ERR_NO_DEVICE",
+                7: "HMD_ERR_OUTPUT_EXISTS", 8: "HMD_ERR_IO"}
+HMD_ERR_OUTPUT_EXISTS = 7
+UNITS = {"si": 0, "atomic": 1}
+# Dimension (src/units/mod.rs), in the order of include/heddle_hip.h
+DIMENSIONS = ["dimensionless", "length", "inverse_length", "mass", "charge", "energy", "time",
+              "inverse_time", "force", "pressure", "inverse_pressure", "temperature", "velocity",
+              "stiffness"]
 
 # Every symbol include/heddle_hip.h declares (checked by the CPU tests).
 EXPORTED = [
@@ -40,6 +47,8 @@ EXPORTED = [
     "hmd_mlp_forward", "hmd_mlp_loss_grad", "hmd_mlp_adam_step", "hmd_mlp_attach",
     "hmd_mlp_dataset_reserve", "hmd_mlp_record_frame", "hmd_mlp_dataset_stats",
     "hmd_mlp_train", "hmd_mlp_judge",
+    "hmd_traj_open", "hmd_traj_write_frame", "hmd_traj_flush", "hmd_traj_close",
+    "hmd_log_open", "hmd_log_write_row", "hmd_log_flush", "hmd_log_close",
 ]
 
 
@@ -161,6 +170,19 @@ def _declare(L):
         "hmd_mlp_dataset_stats": (st, [vp, _c_dp]),
         "hmd_mlp_train": (st, [vp, vp, ctypes.c_uint32, ctypes.c_uint32, _c_dp]),
         "hmd_mlp_judge": (st, [vp, vp, _c_dp]),
+        "hmd_traj_open": (st, [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32,
+                               ctypes.POINTER(ctypes.c_char_p), ctypes.c_uint32, pvp]),
+        "hmd_traj_write_frame": (st, [vp, ctypes.c_uint64, ctypes.c_double, _c_fp, ctypes.c_uint32,
+                                      _c_u32p, _c_fp, _c_fp, _c_fp, _c_fp, _c_fp, _c_fp,
+                                      _c_i32p, _c_i32p, _c_i32p]),
+        "hmd_traj_flush": (st, [vp]),
+        "hmd_traj_close": (None, [vp]),
+        "hmd_log_open": (st, [ctypes.c_char_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_char_p),
+                              _c_u32p, ctypes.c_uint32, pvp]),
+        "hmd_log_write_row": (st, [vp, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, _c_dp]),
+        "hmd_log_flush": (st, [vp]),
+        "hmd_log_close": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -386,3 +386,82 @@ def chacha_block(state, rounds):
     out = np.empty(16, np.uint32)
     lib().hmd_chacha_block(_ptr(s, _u32p), rounds, _ptr(out, _u32p))
     return out
+
+
+class TrajectoryWriter:
+    """Extended-XYZ trajectory writer (src/io/trajectory.rs:39-176), byte for
+    byte: `units` "si" or "atomic"; frames take atomic-unit values."""
+
+    def __init__(self, path, units="si", include_velocities=False, include_images=False,
+                 type_names=("X",)):
+        from ._lib import UNITS
+        names = (ctypes.c_char_p * len(type_names))(*[n.encode() for n in type_names])
+        h = ctypes.c_void_p()
+        check(lib().hmd_traj_open(str(path).encode(), UNITS[units], int(include_velocities),
+                                  int(include_images), names, len(type_names),
+                                  ctypes.byref(h)))
+        self.h = h
+        self.velocities, self.images = include_velocities, include_images
+
+    def write_frame(self, step, dt, lattice, types, positions, velocities=None, images=None):
+        """positions, velocities: (n, 3) in atomic units (cast to f32 like the
+        engine's Real); images: (n, 3) int; lattice {lx, ly, lz, xy, xz, yz}."""
+        pos = np.asarray(positions, np.float32).reshape(-1, 3)
+        n = pos.shape[0]
+        cols = [np.ascontiguousarray(pos[:, k]) for k in range(3)]
+        t = np.ascontiguousarray(types, np.uint32)
+        lat = np.ascontiguousarray(lattice, np.float32)
+        vel = [None] * 3
+        img = [None] * 3
+        if velocities is not None:
+            v = np.asarray(velocities, np.float32).reshape(-1, 3)
+            vel = [np.ascontiguousarray(v[:, k]) for k in range(3)]
+        if images is not None:
+            m = np.asarray(images, np.int32).reshape(-1, 3)
+            img = [np.ascontiguousarray(m[:, k]) for k in range(3)]
+        fp = lambda a: _ptr(a, _fp) if a is not None else None
+        ip = lambda a: _ptr(a, _i32p) if a is not None else None
+        check(lib().hmd_traj_write_frame(self.h, int(step), float(dt), _ptr(lat, _fp), n,
+                                         _ptr(t, _u32p), *[fp(c) for c in cols],
+                                         *[fp(c) for c in vel], *[ip(c) for c in img]))
+
+    def flush(self):
+        check(lib().hmd_traj_flush(self.h))
+
+    def close(self):
+        if self.h:
+            lib().hmd_traj_close(self.h)
+            self.h = None
+
+
+class LogWriter:
+    """CSV log writer (src/io/log_output.rs:36-110), byte for byte; extra
+    columns are (name, dimension) with dimension a name from
+    heddlemd_amd._lib.DIMENSIONS."""
+
+    def __init__(self, path, units="si", extra_columns=()):
+        from ._lib import UNITS, DIMENSIONS
+        names = (ctypes.c_char_p * max(1, len(extra_columns)))(
+            *[c[0].encode() for c in extra_columns])
+        dims = np.ascontiguousarray([DIMENSIONS.index(c[1]) for c in extra_columns] or [0],
+                                    np.uint32)
+        h = ctypes.c_void_p()
+        check(lib().hmd_log_open(str(path).encode(), UNITS[units], names, _ptr(dims, _u32p),
+                                 len(extra_columns), ctypes.byref(h)))
+        self.h = h
+        self.n_extra = len(extra_columns)
+
+    def write_row(self, step, time, kinetic_energy, temperature, extras=()):
+        if len(extras) != self.n_extra:
+            raise ValueError("extras length does not match the count declared at open")
+        ex = np.ascontiguousarray(list(extras) or [0.0], np.float64)
+        check(lib().hmd_log_write_row(self.h, int(step), float(time), float(kinetic_energy),
+                                      float(temperature), _ptr(ex, _dp)))
+
+    def flush(self):
+        check(lib().hmd_log_flush(self.h))
+
+    def close(self):
+        if self.h:
+            lib().hmd_log_close(self.h)
+            self.h = None

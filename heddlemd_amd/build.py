@@ -35,7 +35,7 @@ HIPCC_FLAGS = [
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
              "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{ROOT}/include"]
 
-HOST_SOURCES = ["context.cpp", "forcefield.cpp", "integrator.cpp", "nn.cpp"]
+HOST_SOURCES = ["context.cpp", "forcefield.cpp", "integrator.cpp", "nn.cpp", "io.cpp"]
 
 
 def _run(cmd):

@@ -35,7 +35,9 @@ enum {
   HMD_ERR_NBL_OVERFLOW = 3,   /* NeighborListError::PackedNeighborOverflow */
   HMD_ERR_BOX_TOO_SMALL = 4,  /* NeighborListError::BoxTooSmallForCells */
   HMD_ERR_UNSUPPORTED = 5,
-  HMD_ERR_NO_DEVICE = 6
+  HMD_ERR_NO_DEVICE = 6,
+  HMD_ERR_OUTPUT_EXISTS = 7,  /* TrajectoryWriterError/LogWriterError::OutputExists */
+  HMD_ERR_IO = 8              /* TrajectoryWriterError/LogWriterError::Io */
 };
 
 typedef struct hmd_ctx hmd_ctx;
@@ -275,6 +277,47 @@ hmd_status hmd_mlp_train(hmd_mlp* mlp, hmd_forcefield* ff, uint32_t n_steps,
  * particle forces are left as nn_ff's. */
 hmd_status hmd_mlp_judge(hmd_forcefield* nn_ff, hmd_forcefield* fine_ff,
                          double* rel_rms);
+
+/* ---- output formats ------------------------------------------------------
+ * Extended-XYZ trajectory (src/io/trajectory.rs:39-176) and CSV log
+ * (src/io/log_output.rs:36-110), byte for byte.  Values are atomic units;
+ * the writer converts them to the chosen unit system (src/units/mod.rs:
+ * 27-127) and formats Real columns with HMD_REAL_FMT_DIGITS fractional
+ * digits (src/precision.rs:72, the f32 build).  Files are created new:
+ * an existing path is HMD_ERR_OUTPUT_EXISTS. */
+#define HMD_REAL_FMT_DIGITS 9
+enum { HMD_UNITS_SI = 0, HMD_UNITS_ATOMIC = 1 };   /* UnitSystem */
+enum {                                             /* Dimension */
+  HMD_DIM_DIMENSIONLESS = 0, HMD_DIM_LENGTH, HMD_DIM_INVERSE_LENGTH, HMD_DIM_MASS,
+  HMD_DIM_CHARGE, HMD_DIM_ENERGY, HMD_DIM_TIME, HMD_DIM_INVERSE_TIME, HMD_DIM_FORCE,
+  HMD_DIM_PRESSURE, HMD_DIM_INVERSE_PRESSURE, HMD_DIM_TEMPERATURE, HMD_DIM_VELOCITY,
+  HMD_DIM_STIFFNESS
+};
+typedef struct hmd_traj hmd_traj;
+typedef struct hmd_log hmd_log;
+/* TrajectoryWriter::open (trajectory.rs:39-62) */
+hmd_status hmd_traj_open(const char* path, uint32_t units, int32_t include_velocities,
+                         int32_t include_images, const char* const* type_names,
+                         uint32_t n_type_names, hmd_traj** out);
+/* TrajectoryWriter::write_frame (trajectory.rs:64-168); lattice
+ * {lx, ly, lz, xy, xz, yz}; velocities / images may be NULL when the
+ * writer was opened without them. */
+hmd_status hmd_traj_write_frame(hmd_traj* t, uint64_t step, double dt, const float lattice[6],
+                                uint32_t n, const uint32_t* type_idx, const float* x,
+                                const float* y, const float* z, const float* vx, const float* vy,
+                                const float* vz, const int32_t* ix, const int32_t* iy,
+                                const int32_t* iz);
+hmd_status hmd_traj_flush(hmd_traj* t);         /* TrajectoryWriter::flush */
+void hmd_traj_close(hmd_traj* t);               /* flushes (Drop) */
+/* LogWriter::open (log_output.rs:39-66): extra columns with their
+ * dimensions (HMD_DIM_*) */
+hmd_status hmd_log_open(const char* path, uint32_t units, const char* const* extra_names,
+                        const uint32_t* extra_dims, uint32_t n_extra, hmd_log** out);
+/* LogWriter::write_row (log_output.rs:74-101) */
+hmd_status hmd_log_write_row(hmd_log* l, uint64_t step, double time, double kinetic_energy,
+                             double temperature, const double* extras);
+hmd_status hmd_log_flush(hmd_log* l);           /* LogWriter::flush */
+void hmd_log_close(hmd_log* l);                 /* flushes (Drop) */
 
 #ifdef __cplusplus
 }
