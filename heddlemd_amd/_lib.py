@@ -34,6 +34,8 @@ EXPORTED = [
     "hmd_particles_create", "hmd_particles_destroy", "hmd_particles_download",
     "hmd_particles_upload", "hmd_particles_download_energies", "hmd_particles_download_lo",
     "hmd_particles_kinetic_energy", "hmd_particles_energy_sums",
+    "hmd_particles_negate_velocities", "hmd_particles_upload_forces",
+    "hmd_vv_kick_drift", "hmd_vv_kick",
     "hmd_forcefield_create", "hmd_forcefield_destroy", "hmd_forcefield_compute",
     "hmd_forcefield_slots", "hmd_spme_scalars", "hmd_spme_grids", "hmd_spme_forces",
     "hmd_nbl_request_rebuild", "hmd_nbl_pair_snapshot", "hmd_nbl_count_pairs",
@@ -131,6 +133,10 @@ def _declare(L):
         "hmd_particles_download_lo": (st, [vp, _c_dp, _c_dp]),
         "hmd_particles_kinetic_energy": (st, [vp, _c_dp]),
         "hmd_particles_energy_sums": (st, [vp, _c_dp, _c_dp]),
+        "hmd_particles_negate_velocities": (st, [vp]),
+        "hmd_particles_upload_forces": (st, [vp, _c_fp]),
+        "hmd_vv_kick_drift": (st, [vp, ctypes.c_float, ctypes.c_int32]),
+        "hmd_vv_kick": (st, [vp, ctypes.c_float, ctypes.c_int32]),
         "hmd_forcefield_create": (st, [vp, vp, ctypes.POINTER(ForceFieldDesc), pvp]),
         "hmd_forcefield_destroy": (None, [vp]),
         "hmd_forcefield_compute": (st, [vp, ctypes.c_int32]),

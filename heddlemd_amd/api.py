@@ -107,6 +107,21 @@ class ParticleBuffers:
         check(lib().hmd_particles_kinetic_energy(self.h, ctypes.byref(ke)))
         return ke.value
 
+    def upload_forces(self, force):
+        f = np.ascontiguousarray(force, np.float32).reshape(-1, 3)
+        check(lib().hmd_particles_upload_forces(self.h, _ptr(f, _fp)))
+
+    def negate_velocities(self):
+        check(lib().hmd_particles_negate_velocities(self.h))
+
+    def kick_drift(self, dt, compensated=False):
+        """v += F/m dt/2, x += v dt, wrap (src/gpu/kernels.rs:34, :2292)."""
+        check(lib().hmd_vv_kick_drift(self.h, dt, 1 if compensated else 0))
+
+    def kick(self, dt, compensated=False):
+        """v += F/m dt/2 (src/gpu/kernels.rs:128, :2416)."""
+        check(lib().hmd_vv_kick(self.h, dt, 1 if compensated else 0))
+
     def energy_sums(self):
         pe = ctypes.c_double()
         w = ctypes.c_double()

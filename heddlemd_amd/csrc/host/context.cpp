@@ -138,7 +138,7 @@ hmd_status hmd_particles_create(hmd_ctx* h, uint32_t n, const float* posq,
                                 const uint32_t* type, const float lat[6],
                                 hmd_particles** out) {
   HMD_GUARD({
-    if (n == 0 || !posq || !mass || !lat) throw Error(HMD_ERR_INVALID, "particles: empty input");
+    if ((n && (!posq || !mass)) || !lat) throw Error(HMD_ERR_INVALID, "particles: null input");
     for (uint32_t i = 0; i < n; ++i)
       if (!(mass[i] > 0.0f)) throw Error(HMD_ERR_INVALID, "particles: non-positive mass");
     auto* hp = new hmd_particles();
@@ -216,6 +216,36 @@ hmd_status hmd_particles_upload(hmd_particles* hp, const float* posq, const floa
       for (uint32_t i = 0; i < p.n; ++i)
         for (int d = 0; d < 3; ++d) soa[(size_t)d * p.n + i] = vel[3 * (size_t)i + d];
       p.vel.upload(soa.data(), soa.size(), s);
+    }
+    check(hipStreamSynchronize(s), "sync");
+  });
+}
+
+hmd_status hmd_particles_upload_forces(hmd_particles* hp, const float* force) {
+  HMD_GUARD({
+    if (!hp || !force) throw Error(HMD_ERR_INVALID, "null argument");
+    Particles& p = hp->p;
+    std::vector<float> soa(3 * (size_t)p.n);
+    for (uint32_t i = 0; i < p.n; ++i)
+      for (int d = 0; d < 3; ++d) soa[(size_t)d * p.n + i] = force[3 * (size_t)i + d];
+    p.force.upload(soa.data(), soa.size(), p.ctx->stream);
+    check(hipStreamSynchronize(p.ctx->stream), "sync");
+  });
+}
+
+hmd_status hmd_particles_negate_velocities(hmd_particles* hp) {
+  HMD_GUARD({
+    Particles& p = hp->p;
+    hipStream_t s = p.ctx->stream;
+    std::vector<float> v(3 * (size_t)p.n);
+    p.vel.download(v.data(), v.size(), s);
+    for (float& x : v) x = -x;
+    p.vel.upload(v.data(), v.size(), s);
+    if (p.vel_lo.ptr) {
+      std::vector<double> lo(3 * (size_t)p.n);
+      p.vel_lo.download(lo.data(), lo.size(), s);
+      for (double& x : lo) x = -x;
+      p.vel_lo.upload(lo.data(), lo.size(), s);
     }
     check(hipStreamSynchronize(s), "sync");
   });

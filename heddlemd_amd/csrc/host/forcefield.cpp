@@ -780,6 +780,7 @@ hmd_status hmd_forcefield_create(hmd_ctx* h, hmd_particles* hp,
                                  const hmd_forcefield_desc* d, hmd_forcefield** out) {
   HMD_GUARD({
     if (!d) throw Error(HMD_ERR_INVALID, "null descriptor");
+    if (!hp || hp->p.n == 0) throw Error(HMD_ERR_INVALID, "force field: empty particle set");
     auto* f = new hmd_forcefield();
     try {
       f->ff = new ForceField(h->c, hp->p, *d);

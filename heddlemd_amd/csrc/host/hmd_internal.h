@@ -102,9 +102,11 @@ struct DevBuf {
     if (zero) check(hipMemset(ptr, 0, n * sizeof(T)), "hipMemset");
   }
   void upload(const T* host, size_t n, hipStream_t s) {
+    if (n == 0) return;
     check(hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, s), "H2D");
   }
   void download(T* host, size_t n, hipStream_t s) const {
+    if (n == 0) return;
     check(hipMemcpyAsync(host, ptr, n * sizeof(T), hipMemcpyDeviceToHost, s), "D2H");
     check(hipStreamSynchronize(s), "sync");
   }

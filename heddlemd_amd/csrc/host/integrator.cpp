@@ -93,7 +93,7 @@ struct Integrator {
     v.posq = p.posq.ptr;
     v.v = p.vel.ptr;
     v.f = p.force.ptr;
-    v.inv_m = p.inv_mass.ptr;
+    v.m = p.mass.ptr;
     v.images = p.images.ptr;
     v.pos_lo = p.pos_lo.ptr;
     v.vel_lo = p.vel_lo.ptr;
@@ -349,6 +349,35 @@ hmd_status hmd_generate_velocities(uint32_t n, uint32_t n_constraints, double te
     }
     std::copy(v.begin(), v.end(), vel);
   });
+}
+
+static hmd_status vv_half(hmd_particles* hp, float dt, int32_t compensated, bool drift) {
+  HMD_GUARD({
+    if (!hp) throw Error(HMD_ERR_INVALID, "null particles");
+    Particles& p = hp->p;
+    VvParams v{};
+    v.posq = p.posq.ptr;
+    v.v = p.vel.ptr;
+    v.f = p.force.ptr;
+    v.m = p.mass.ptr;
+    v.images = p.images.ptr;
+    v.pos_lo = p.pos_lo.ptr;
+    v.vel_lo = p.vel_lo.ptr;
+    v.L = p.lattice;
+    v.dt = dt;
+    v.n = p.n;
+    v.compensated = compensated ? 1u : 0u;
+    launch(*p.ctx, drift ? K_hmd_vv_kick_drift : K_hmd_vv_kick, ceil_div(p.n, 256), 256, 0, v);
+    check(hipStreamSynchronize(p.ctx->stream), "sync");
+  });
+}
+
+hmd_status hmd_vv_kick_drift(hmd_particles* hp, float dt, int32_t compensated) {
+  return vv_half(hp, dt, compensated, true);
+}
+
+hmd_status hmd_vv_kick(hmd_particles* hp, float dt, int32_t compensated) {
+  return vv_half(hp, dt, compensated, false);
 }
 
 hmd_status hmd_integrator_create(hmd_ctx* h, hmd_particles* hp, hmd_forcefield* f,

@@ -261,7 +261,7 @@ struct VvParams {
   float* posq;                 // canonical float4
   float* v;                    // [3][n]
   const float* f;              // [3][n]
-  const float* inv_m;          // [n]
+  const float* m;              // [n] masses
   int32_t* images;             // [3][n]
   double* pos_lo;              // [3][n] compensated mode
   double* vel_lo;              // [3][n]

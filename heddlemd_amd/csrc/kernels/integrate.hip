@@ -9,26 +9,34 @@
 #include "hmd_device.h"
 
 // -------------------------------------------------------- velocity Verlet --
+// The arithmetic is that of integrate.cu:24-170 as compiled with FMA
+// contraction on: a = F / m (IEEE divide), v + a dt/2 and x + v dt fused
+// into one rounding each; the compensated (f32 hi + f64 lo) variant adds the
+// f32 product a dt/2 to (hi + lo) in f64 and fuses (v_hi + v_lo) dt into the
+// f64 position sum.  Written with explicit fmaf / fma so the rounding does
+// not depend on the compiler's contraction choices.
+
 // v += F/m dt/2; x += v dt; wrap + image count (integrate.cu:24-170).
 extern "C" __global__ void __launch_bounds__(256)
 hmd_vv_kick_drift(const VvParams p) {
+#pragma clang fp contract(off)
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.n) return;
-  const float im = p.inv_m[i];
+  const float m = p.m[i];
   const float half_dt = p.dt * 0.5f;
   float4 pq = reinterpret_cast<float4*>(p.posq)[i];
-  float a[3] = {p.f[i] * im, p.f[p.n + i] * im, p.f[2 * p.n + i] * im};
+  float a[3] = {p.f[i] / m, p.f[p.n + i] / m, p.f[2 * p.n + i] / m};
   float x[3] = {pq.x, pq.y, pq.z};
   if (p.compensated) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const size_t k = (size_t)d * p.n + i;
-      double ext_v = (double)p.v[k] + p.vel_lo[k] + (double)(a[d] * half_dt);
+      double ext_v = ((double)p.v[k] + p.vel_lo[k]) + (double)(a[d] * half_dt);
       float vh = (float)ext_v;
       double vl = ext_v - (double)vh;
       p.v[k] = vh;
       p.vel_lo[k] = vl;
-      double ext_x = (double)x[d] + p.pos_lo[k] + ((double)vh + vl) * (double)p.dt;
+      double ext_x = fma((double)vh + vl, (double)p.dt, (double)x[d] + p.pos_lo[k]);
       float xh = (float)ext_x;
       p.pos_lo[k] = ext_x - (double)xh;
       x[d] = xh;
@@ -37,9 +45,9 @@ hmd_vv_kick_drift(const VvParams p) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const size_t k = (size_t)d * p.n + i;
-      float v = p.v[k] + a[d] * half_dt;
+      float v = fmaf(a[d], half_dt, p.v[k]);
       p.v[k] = v;
-      x[d] = x[d] + v * p.dt;
+      x[d] = fmaf(v, p.dt, x[d]);
     }
   }
   int ka, kb, kc;
@@ -53,24 +61,25 @@ hmd_vv_kick_drift(const VvParams p) {
   reinterpret_cast<float4*>(p.posq)[i] = pq;
 }
 
-// v += F/m dt/2 (integrate.cu:199-215, :296-313).
+// v += F/m dt/2 (integrate.cu:172-215).
 extern "C" __global__ void __launch_bounds__(256)
 hmd_vv_kick(const VvParams p) {
+#pragma clang fp contract(off)
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.n) return;
-  const float im = p.inv_m[i];
+  const float m = p.m[i];
   const float half_dt = p.dt * 0.5f;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const size_t k = (size_t)d * p.n + i;
-    const float a = p.f[k] * im;
+    const float a = p.f[k] / m;
     if (p.compensated) {
-      double ext_v = (double)p.v[k] + p.vel_lo[k] + (double)(a * half_dt);
+      double ext_v = ((double)p.v[k] + p.vel_lo[k]) + (double)(a * half_dt);
       float vh = (float)ext_v;
       p.v[k] = vh;
       p.vel_lo[k] = ext_v - (double)vh;
     } else {
-      p.v[k] = p.v[k] + a * half_dt;
+      p.v[k] = fmaf(a, half_dt, p.v[k]);
     }
   }
 }

@@ -76,6 +76,22 @@ hmd_status hmd_particles_download(hmd_particles* p, float* posq, float* vel,
                                   float* force, int32_t* images);
 hmd_status hmd_particles_upload(hmd_particles* p, const float* posq,
                                 const float* vel);
+/* Forces [n][3] into the force buffer (the forces_{x,y,z} part of
+ * ParticleBuffers::upload, src/gpu/buffers.rs:252-310); the next force
+ * evaluation overwrites them. */
+hmd_status hmd_particles_upload_forces(hmd_particles* p, const float* force);
+/* v <- -v, both the f32 velocities and their f64 low parts: the velocity
+ * reversal of a time-reversal check (the negate_velocities helper of
+ * tests/integration_lossless.rs). */
+hmd_status hmd_particles_negate_velocities(hmd_particles* p);
+/* One velocity-Verlet half: v += F/m dt/2, x += v dt, wrap + image count
+ * (vv_kick_drift, src/gpu/kernels.rs:34-58; compensated != 0:
+ * vv_kick_drift_lossless, :2292-2330, on the f64 low parts), with the
+ * forces currently in the buffer. */
+hmd_status hmd_vv_kick_drift(hmd_particles* p, float dt, int32_t compensated);
+/* v += F/m dt/2 (vv_kick, src/gpu/kernels.rs:128-143; vv_kick_lossless,
+ * :2416-2450). */
+hmd_status hmd_vv_kick(hmd_particles* p, float dt, int32_t compensated);
 /* Per-atom potential-energy and virial shares of the last energy-level
  * evaluation (SlotOutputView energy/virial, src/forces/mod.rs:206-212). */
 hmd_status hmd_particles_download_energies(hmd_particles* p, float* energy,

@@ -78,6 +78,9 @@ def _declare(L):
     L.orc_kinetic_energy.restype = ctypes.c_double
     L.orc_kinetic_energy.argtypes = [ctypes.c_int, _dp, _dp]
     L.orc_num_threads.restype = ctypes.c_int
+    L.orc_vv_kick_drift_f32.argtypes = [ctypes.c_int, _fp, _fp, _dp, _dp, _fp, _fp, _ip, _fp,
+                                        ctypes.c_float, ctypes.c_int]
+    L.orc_vv_kick_f32.argtypes = [ctypes.c_int, _fp, _dp, _fp, _fp, ctypes.c_float, ctypes.c_int]
 
 
 def _p(a, t):
@@ -241,6 +244,55 @@ def philox4x32_10(key, ctr):
 
 def philox_gaussian(seed_lo, seed_hi, step_lo, step_hi, pid, axis):
     return lib().orc_philox_gaussian(seed_lo, seed_hi, step_lo, step_hi, pid, axis)
+
+
+class VvState:
+    """Particle state of the f32 velocity-Verlet oracle (orc_vv_kick_drift_f32 /
+    orc_vv_kick_f32): posq float4[n], SoA [3][n] velocities, forces, f64 low
+    parts and image counters; accessors return [n][3] copies."""
+
+    def __init__(self, posq, vel, force, mass, lattice, images=None):
+        self.posq = np.ascontiguousarray(posq, np.float32).reshape(-1, 4).copy()
+        n = self.posq.shape[0]
+        self.n = n
+        soa = lambda a, t: np.ascontiguousarray(np.asarray(a, t).reshape(n, 3).T).copy()
+        self._v = soa(vel, np.float32)
+        self._f = soa(force, np.float32)
+        self._m = np.ascontiguousarray(mass, np.float32).reshape(n).copy()
+        self._L = np.ascontiguousarray(lattice, np.float32).reshape(6).copy()
+        self._im = soa(np.zeros((n, 3)) if images is None else images, np.int32)
+        self._xlo = np.zeros((3, n))
+        self._vlo = np.zeros((3, n))
+
+    def kick_drift(self, dt, compensated=False):
+        lib().orc_vv_kick_drift_f32(self.n, _p(self.posq, _fp), _p(self._v, _fp),
+                                    _p(self._xlo, _dp), _p(self._vlo, _dp), _p(self._f, _fp),
+                                    _p(self._m, _fp), _p(self._im, _ip), _p(self._L, _fp),
+                                    float(np.float32(dt)), 1 if compensated else 0)
+
+    def kick(self, dt, compensated=False):
+        lib().orc_vv_kick_f32(self.n, _p(self._v, _fp), _p(self._vlo, _dp), _p(self._f, _fp),
+                              _p(self._m, _fp), float(np.float32(dt)), 1 if compensated else 0)
+
+    def negate_velocities(self):
+        self._v = -self._v
+        self._vlo = -self._vlo
+
+    @property
+    def vel(self):
+        return self._v.T.copy()
+
+    @property
+    def images(self):
+        return self._im.T.copy()
+
+    @property
+    def pos_lo(self):
+        return self._xlo.T.copy()
+
+    @property
+    def vel_lo(self):
+        return self._vlo.T.copy()
 
 
 def kinetic_energy(v, m):

@@ -33,6 +33,11 @@
  *   orc_settle_*                          -- rigidity, COM velocity, zero
  *       bond-velocity invariants (tests/settle.rs).
  *   orc_philox4x32_10                     -- Random123 published KATs.
+ *   orc_vv_kick_drift_f32 / orc_vv_kick_f32 -- the exact-value cases of
+ *       tests/integration.rs and tests/integration_lossless.rs; constant-
+ *       force kinematics in closed form (the compensated sum to f64
+ *       rounding); wrap image counts; a perturbed variant (unfused drift)
+ *       must fail the fused-rounding case.
  */
 #include <math.h>
 #include <stdint.h>
@@ -802,6 +807,94 @@ void orc_vv_constant_force(int n, double *x, double *v, const double *F,
                 x[3 * i + d] += v[3 * i + d] * dt;
                 v[3 * i + d] += F[3 * i + d] / m[i] * 0.5 * dt;
             }
+}
+
+/* f32 velocity-Verlet halves, bit-exact with the device kernels.
+ * kernels/integrate.cu:24-215 (vv_kick_drift_body / vv_kick_body) compiled
+ * by nvcc with its default FMA contraction: a = F / m is an IEEE divide,
+ * v + a dt/2 and x + v dt round once (fmaf); the compensated ("lossless")
+ * variant forms the f32 product a dt/2, adds it to (v_hi + v_lo) in f64,
+ * re-splits, then rounds (x_hi + x_lo) + (v_hi + v_lo) dt once (fma).  The
+ * wrap is triclinic_wrap_with_image (kernels/pbc.cuh:74-91) with every f32
+ * operation rounded once.  Arrays are SoA [3][n]; posq is float4[n]. */
+/* float -> int as the GPU converts (saturating, NaN -> 0); a plain C cast
+ * of NaN or an out-of-range value is undefined. */
+static int f2i_sat(float f)
+{
+    if (f != f) return 0;
+    if (f >= 2147483648.0f) return INT32_MAX;
+    if (f <= -2147483648.0f) return INT32_MIN;
+    return (int)f;
+}
+
+static void wrap_image_f32(float *x, float *y, float *z, int *ka, int *kb, int *kc,
+                           const float *L)
+{
+    float sc = *z / L[2];
+    float sb = (*y - sc * L[5]) / L[1];
+    float sa = (*x - sb * L[3] - sc * L[4]) / L[0];
+    float fa = floorf(sa + 0.5f), fb = floorf(sb + 0.5f), fc = floorf(sc + 0.5f);
+    *x -= fa * L[0] + fb * L[3] + fc * L[4];
+    *y -= fb * L[1] + fc * L[5];
+    *z -= fc * L[2];
+    *ka = f2i_sat(fa);
+    *kb = f2i_sat(fb);
+    *kc = f2i_sat(fc);
+}
+
+void orc_vv_kick_drift_f32(int n, float *posq, float *v, double *x_lo, double *v_lo,
+                           const float *f, const float *m, int *images, const float *L,
+                           float dt, int compensated)
+{
+    const float half_dt = dt * 0.5f;
+    for (int i = 0; i < n; ++i) {
+        float x[3] = {posq[4 * i], posq[4 * i + 1], posq[4 * i + 2]};
+        for (int d = 0; d < 3; ++d) {
+            const size_t k = (size_t)d * n + i;
+            const float a = f[k] / m[i];
+            if (compensated) {
+                double ext_v = ((double)v[k] + v_lo[k]) + (double)(a * half_dt);
+                float vh = (float)ext_v;
+                double vl = ext_v - (double)vh;
+                v[k] = vh;
+                v_lo[k] = vl;
+                double ext_x = fma((double)vh + vl, (double)dt, (double)x[d] + x_lo[k]);
+                float xh = (float)ext_x;
+                x_lo[k] = ext_x - (double)xh;
+                x[d] = xh;
+            } else {
+                v[k] = fmaf(a, half_dt, v[k]);
+                x[d] = fmaf(v[k], dt, x[d]);
+            }
+        }
+        int ka, kb, kc;
+        wrap_image_f32(&x[0], &x[1], &x[2], &ka, &kb, &kc, L);
+        images[i] += ka;
+        images[n + i] += kb;
+        images[2 * n + i] += kc;
+        posq[4 * i] = x[0];
+        posq[4 * i + 1] = x[1];
+        posq[4 * i + 2] = x[2];
+    }
+}
+
+void orc_vv_kick_f32(int n, float *v, double *v_lo, const float *f, const float *m,
+                     float dt, int compensated)
+{
+    const float half_dt = dt * 0.5f;
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            const size_t k = (size_t)d * n + i;
+            const float a = f[k] / m[i];
+            if (compensated) {
+                double ext_v = ((double)v[k] + v_lo[k]) + (double)(a * half_dt);
+                float vh = (float)ext_v;
+                v[k] = vh;
+                v_lo[k] = ext_v - (double)vh;
+            } else {
+                v[k] = fmaf(a, half_dt, v[k]);
+            }
+        }
 }
 
 /* ------------------------------------------------------------------ */
