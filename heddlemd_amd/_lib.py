@@ -35,7 +35,7 @@ EXPORTED = [
     "hmd_particles_upload", "hmd_particles_download_energies", "hmd_particles_download_lo",
     "hmd_particles_kinetic_energy", "hmd_particles_energy_sums",
     "hmd_particles_negate_velocities", "hmd_particles_upload_forces",
-    "hmd_vv_kick_drift", "hmd_vv_kick",
+    "hmd_vv_kick_drift", "hmd_vv_kick", "hmd_lan_drift_half", "hmd_lan_ou_step",
     "hmd_forcefield_create", "hmd_forcefield_destroy", "hmd_forcefield_compute",
     "hmd_forcefield_slots", "hmd_spme_scalars", "hmd_spme_grids", "hmd_spme_forces",
     "hmd_nbl_request_rebuild", "hmd_nbl_pair_snapshot", "hmd_nbl_count_pairs",
@@ -137,6 +137,9 @@ def _declare(L):
         "hmd_particles_upload_forces": (st, [vp, _c_fp]),
         "hmd_vv_kick_drift": (st, [vp, ctypes.c_float, ctypes.c_int32]),
         "hmd_vv_kick": (st, [vp, ctypes.c_float, ctypes.c_int32]),
+        "hmd_lan_drift_half": (st, [vp, ctypes.c_float]),
+        "hmd_lan_ou_step": (st, [vp, ctypes.c_uint64, ctypes.c_float, ctypes.c_float,
+                                 ctypes.c_uint64]),
         "hmd_forcefield_create": (st, [vp, vp, ctypes.POINTER(ForceFieldDesc), pvp]),
         "hmd_forcefield_destroy": (None, [vp]),
         "hmd_forcefield_compute": (st, [vp, ctypes.c_int32]),

@@ -118,6 +118,14 @@ class ParticleBuffers:
         """v += F/m dt/2, x += v dt, wrap (src/gpu/kernels.rs:34, :2292)."""
         check(lib().hmd_vv_kick_drift(self.h, dt, 1 if compensated else 0))
 
+    def lan_drift_half(self, dt):
+        """BAOAB A step (src/gpu/kernels.rs:2339)."""
+        check(lib().hmd_lan_drift_half(self.h, dt))
+
+    def lan_ou_step(self, seed, alpha, kt, draw_counter):
+        """BAOAB O step (src/gpu/kernels.rs:2374)."""
+        check(lib().hmd_lan_ou_step(self.h, seed, alpha, kt, draw_counter))
+
     def kick(self, dt, compensated=False):
         """v += F/m dt/2 (src/gpu/kernels.rs:128, :2416)."""
         check(lib().hmd_vv_kick(self.h, dt, 1 if compensated else 0))

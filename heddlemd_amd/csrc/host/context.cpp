@@ -160,6 +160,7 @@ hmd_status hmd_particles_create(hmd_ctx* h, uint32_t n, const float* posq,
     p.virial.alloc(n);
     p.red_partials.alloc(std::min<uint32_t>(1024u, ceil_div(n, 256)));
     p.red_out.alloc(1);
+    p.draw_counter.alloc(1);
     p.host_mass.assign(mass, mass + n);
     std::vector<float> inv(n);
     for (uint32_t i = 0; i < n; ++i) inv[i] = 1.0f / mass[i];

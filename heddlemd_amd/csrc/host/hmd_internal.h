@@ -140,6 +140,7 @@ struct Particles {
   DevBuf<double> pos_lo, vel_lo;  // [3][n] (compensated mode)
   DevBuf<float> energy, virial;   // [n]
   DevBuf<double> red_partials, red_out;
+  DevBuf<unsigned long long> draw_counter;  // [1] for the standalone O step
   std::vector<float> host_mass;
   uint32_t max_type = 0;
 };

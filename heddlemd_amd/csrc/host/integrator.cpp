@@ -30,8 +30,8 @@ struct Integrator {
   DevBuf<float> snapshot, constraint_virial;
   // Langevin
   float lan_a = 1.0f;
+  float lan_kt = 0.0f;
   uint64_t seed = 0;
-  DevBuf<float> sqrt_kt_over_m;
   DevBuf<unsigned long long> step_counter;
   // graph
   hipGraph_t graph = nullptr;
@@ -69,13 +69,10 @@ struct Integrator {
       constraint_virial.alloc(3 * (size_t)n_groups);
     }
     if (kind == Kind::LangevinBaoab) {
-      lan_a = (float)std::exp(-d.gamma * d.dt);
+      // alpha = exp(-gamma dt) in f32 (src/integrator/langevin_baoab.rs:140)
+      lan_a = std::exp(-(float)d.gamma * (float)d.dt);
+      lan_kt = (float)d.temperature;
       seed = d.seed;
-      std::vector<float> sk(p.n);
-      for (uint32_t i = 0; i < p.n; ++i)
-        sk[i] = (float)std::sqrt(d.temperature / (double)p.host_mass[i]);
-      sqrt_kt_over_m.alloc(p.n, false);
-      sqrt_kt_over_m.upload(sk.data(), sk.size(), s);
       step_counter.alloc(1);
     }
     check(hipMemsetAsync(p.pos_lo.ptr, 0, p.pos_lo.count * sizeof(double), s), "memset");
@@ -125,8 +122,8 @@ struct Integrator {
     LangevinParams l{};
     l.posq = p.posq.ptr;
     l.v = p.vel.ptr;
-    l.inv_m = p.inv_mass.ptr;
-    l.sqrt_kt_over_m = sqrt_kt_over_m.ptr;
+    l.m = p.mass.ptr;
+    l.kt = lan_kt;
     l.images = p.images.ptr;
     l.L = p.lattice;
     l.dt = step_dt;
@@ -368,6 +365,44 @@ static hmd_status vv_half(hmd_particles* hp, float dt, int32_t compensated, bool
     v.n = p.n;
     v.compensated = compensated ? 1u : 0u;
     launch(*p.ctx, drift ? K_hmd_vv_kick_drift : K_hmd_vv_kick, ceil_div(p.n, 256), 256, 0, v);
+    check(hipStreamSynchronize(p.ctx->stream), "sync");
+  });
+}
+
+hmd_status hmd_lan_drift_half(hmd_particles* hp, float dt) {
+  HMD_GUARD({
+    if (!hp) throw Error(HMD_ERR_INVALID, "null particles");
+    Particles& p = hp->p;
+    LangevinParams l{};
+    l.posq = p.posq.ptr;
+    l.v = p.vel.ptr;
+    l.images = p.images.ptr;
+    l.L = p.lattice;
+    l.dt = dt;
+    l.n = p.n;
+    launch(*p.ctx, K_hmd_lan_drift_half, ceil_div(p.n, 256), 256, 0, l);
+    check(hipStreamSynchronize(p.ctx->stream), "sync");
+  });
+}
+
+hmd_status hmd_lan_ou_step(hmd_particles* hp, uint64_t seed, float alpha, float kt,
+                           uint64_t draw_counter) {
+  HMD_GUARD({
+    if (!hp) throw Error(HMD_ERR_INVALID, "null particles");
+    Particles& p = hp->p;
+    if (p.n == 0) return HMD_OK;
+    unsigned long long c = draw_counter;
+    p.draw_counter.upload(&c, 1, p.ctx->stream);
+    LangevinParams l{};
+    l.v = p.vel.ptr;
+    l.m = p.mass.ptr;
+    l.a = alpha;
+    l.kt = kt;
+    l.seed_lo = (uint32_t)(seed & 0xFFFFFFFFull);
+    l.seed_hi = (uint32_t)(seed >> 32);
+    l.step = p.draw_counter.ptr;
+    l.n = p.n;
+    launch(*p.ctx, K_hmd_lan_ou, ceil_div(p.n, 256), 256, 0, l);
     check(hipStreamSynchronize(p.ctx->stream), "sync");
   });
 }

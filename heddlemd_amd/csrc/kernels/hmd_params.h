@@ -289,12 +289,12 @@ struct SettleParams {
 struct LangevinParams {
   float* posq;
   float* v;
-  const float* inv_m;
-  const float* sqrt_kt_over_m; // [n]
+  const float* m;              // [n] masses
   int32_t* images;
   HmdLattice L;
   float dt;
   float a;                     // exp(-gamma dt)
+  float kt;                    // k_B T (Hartree)
   uint32_t seed_lo, seed_hi;
   const unsigned long long* step;  // device step counter
   uint32_t n;

@@ -361,14 +361,17 @@ __device__ inline float hmd_philox_gaussian(uint32_t sl, uint32_t sh, uint32_t c
   return (float)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
 }
 
-// A step: x += v dt/2, wrap (langevin.cu:13-50).
+// A step: x += v dt/2 (one rounding, as the reference's contracted
+// x + v * half_dt), wrap (langevin.cu:13-50).
 extern "C" __global__ void __launch_bounds__(256)
 hmd_lan_drift_half(const LangevinParams p) {
+#pragma clang fp contract(off)
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.n) return;
   float4 pq = reinterpret_cast<float4*>(p.posq)[i];
-  const float h = 0.5f * p.dt;
-  float x = pq.x + p.v[i] * h, y = pq.y + p.v[p.n + i] * h, z = pq.z + p.v[2 * p.n + i] * h;
+  const float h = p.dt * 0.5f;
+  float x = fmaf(p.v[i], h, pq.x), y = fmaf(p.v[p.n + i], h, pq.y),
+        z = fmaf(p.v[2 * p.n + i], h, pq.z);
   int ka, kb, kc;
   hmd_wrap(x, y, z, ka, kb, kc, p.L);
   p.images[i] += ka;
@@ -378,20 +381,23 @@ hmd_lan_drift_half(const LangevinParams p) {
   reinterpret_cast<float4*>(p.posq)[i] = pq;
 }
 
-// O step: v = a v + sqrt((1 - a^2) kT/m) xi, xi from Philox keyed by the
-// seed with counter (step_lo, step_hi, particle id, axis) (langevin.cu:52-80).
+// O step: v = a v + sigma xi with sigma = sqrt((1 - a^2) kT / m) in f32
+// (1 - a^2 and a v + (sigma xi) each rounded once, as the reference's
+// contracted expressions), xi from Philox keyed by the seed with counter
+// (draw_lo, draw_hi, particle id, axis) (langevin.cu:52-80).
 extern "C" __global__ void __launch_bounds__(256)
 hmd_lan_ou(const LangevinParams p) {
+#pragma clang fp contract(off)
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.n) return;
   const unsigned long long step = *p.step;
   const uint32_t cl = (uint32_t)(step & 0xFFFFFFFFull), ch = (uint32_t)(step >> 32);
-  const float sig = sqrtf(1.0f - p.a * p.a) * p.sqrt_kt_over_m[i];
+  const float sig = sqrtf(fmaf(-p.a, p.a, 1.0f) * p.kt / p.m[i]);
 #pragma unroll
   for (uint32_t d = 0; d < 3; ++d) {
     const float xi = hmd_philox_gaussian(p.seed_lo, p.seed_hi, cl, ch, i, d);
     const size_t k = (size_t)d * p.n + i;
-    p.v[k] = p.a * p.v[k] + sig * xi;
+    p.v[k] = fmaf(p.a, p.v[k], sig * xi);
   }
 }
 

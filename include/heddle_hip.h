@@ -92,6 +92,17 @@ hmd_status hmd_vv_kick_drift(hmd_particles* p, float dt, int32_t compensated);
 /* v += F/m dt/2 (vv_kick, src/gpu/kernels.rs:128-143; vv_kick_lossless,
  * :2416-2450). */
 hmd_status hmd_vv_kick(hmd_particles* p, float dt, int32_t compensated);
+/* Langevin BAOAB A step: x += v dt/2, wrap + image count (lan_drift_half,
+ * src/gpu/kernels.rs:2339-2372; kernels/langevin.cu:13-50). */
+hmd_status hmd_lan_drift_half(hmd_particles* p, float dt);
+/* Langevin O step v = alpha v + sqrt((1 - alpha^2) kT / m) xi, xi the
+ * Philox-4x32-10 Box-Muller normal keyed by seed with counter
+ * (draw_counter lo, hi, particle, axis) (lan_ou_step,
+ * src/gpu/kernels.rs:2374-2410; kernels/langevin.cu:52-80).  The counter is
+ * passed by value; the reference's device counter increment is the
+ * caller's. */
+hmd_status hmd_lan_ou_step(hmd_particles* p, uint64_t seed, float alpha, float kt,
+                           uint64_t draw_counter);
 /* Per-atom potential-energy and virial shares of the last energy-level
  * evaluation (SlotOutputView energy/virial, src/forces/mod.rs:206-212). */
 hmd_status hmd_particles_download_energies(hmd_particles* p, float* energy,

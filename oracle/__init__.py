@@ -80,6 +80,9 @@ def _declare(L):
     L.orc_num_threads.restype = ctypes.c_int
     L.orc_vv_kick_drift_f32.argtypes = [ctypes.c_int, _fp, _fp, _dp, _dp, _fp, _fp, _ip, _fp,
                                         ctypes.c_float, ctypes.c_int]
+    L.orc_lan_drift_half_f32.argtypes = [ctypes.c_int, _fp, _fp, _ip, _fp, ctypes.c_float]
+    L.orc_lan_ou_step_f32.argtypes = [ctypes.c_int, _fp, _fp, ctypes.c_uint64, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_uint64]
     L.orc_vv_kick_f32.argtypes = [ctypes.c_int, _fp, _dp, _fp, _fp, ctypes.c_float, ctypes.c_int]
 
 
@@ -273,6 +276,14 @@ class VvState:
     def kick(self, dt, compensated=False):
         lib().orc_vv_kick_f32(self.n, _p(self._v, _fp), _p(self._vlo, _dp), _p(self._f, _fp),
                               _p(self._m, _fp), float(np.float32(dt)), 1 if compensated else 0)
+
+    def lan_drift_half(self, dt):
+        lib().orc_lan_drift_half_f32(self.n, _p(self.posq, _fp), _p(self._v, _fp),
+                                     _p(self._im, _ip), _p(self._L, _fp), float(np.float32(dt)))
+
+    def lan_ou_step(self, seed, alpha, kt, draw_counter):
+        lib().orc_lan_ou_step_f32(self.n, _p(self._v, _fp), _p(self._m, _fp), seed,
+                                  float(np.float32(alpha)), float(np.float32(kt)), draw_counter)
 
     def negate_velocities(self):
         self._v = -self._v

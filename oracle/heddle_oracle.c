@@ -929,6 +929,45 @@ double orc_philox_gaussian(uint32_t seed_lo, uint32_t seed_hi, uint32_t step_lo,
     return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
 
+/* Langevin BAOAB A and O steps in f32 (kernels/langevin.cu:13-80) with
+ * the reference's contracted roundings: x + v dt/2 once (fmaf);
+ * sigma = sqrt(fl(1 - a^2) kT / m) with 1 - a^2 rounded once; v = a v +
+ * fl(sigma xi) rounded once; xi = (float) of the f64 Box-Muller normal.  The
+ * f64 log / cos come from the host libm: a last-bit difference from the
+ * device's can move xi by one f32 ulp. */
+void orc_lan_drift_half_f32(int n, float *posq, const float *v, int *images,
+                            const float *L, float dt)
+{
+    const float h = dt * 0.5f;
+    for (int i = 0; i < n; ++i) {
+        float x = fmaf(v[i], h, posq[4 * i]);
+        float y = fmaf(v[n + i], h, posq[4 * i + 1]);
+        float z = fmaf(v[2 * n + i], h, posq[4 * i + 2]);
+        int ka, kb, kc;
+        wrap_image_f32(&x, &y, &z, &ka, &kb, &kc, L);
+        images[i] += ka;
+        images[n + i] += kb;
+        images[2 * n + i] += kc;
+        posq[4 * i] = x;
+        posq[4 * i + 1] = y;
+        posq[4 * i + 2] = z;
+    }
+}
+
+void orc_lan_ou_step_f32(int n, float *v, const float *m, uint64_t seed, float a, float kt,
+                         uint64_t draw_counter)
+{
+    const uint32_t sl = (uint32_t)seed, sh = (uint32_t)(seed >> 32);
+    const uint32_t cl = (uint32_t)draw_counter, ch = (uint32_t)(draw_counter >> 32);
+    for (int i = 0; i < n; ++i) {
+        const float sig = sqrtf(fmaf(-a, a, 1.0f) * kt / m[i]);
+        for (int d = 0; d < 3; ++d) {
+            const float xi = (float)orc_philox_gaussian(sl, sh, cl, ch, (uint32_t)i, (uint32_t)d);
+            v[(size_t)d * n + i] = fmaf(a, v[(size_t)d * n + i], sig * xi);
+        }
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* Reductions (a22)                                                     */
 /* ------------------------------------------------------------------ */

@@ -6,7 +6,10 @@
   f32 or f64 with ties to even) of the kernel's arithmetic on seeded states,
   including a case where fusing the drift changes the result;
 - closed-form constant-force kinematics of the compensated variant;
-- time reversal of the compensated variant."""
+- time reversal of the compensated variant;
+- the Langevin A / O steps: the reference's tests/langevin.rs cases, the
+  Box-Muller normal rebuilt from the (Random123-pinned) Philox words, the
+  exact-rational model of the O-step roundings and the OU variance."""
 from fractions import Fraction
 
 import numpy as np
@@ -275,3 +278,83 @@ def test_compensated_constant_force_closed_form():
     err_p = np.max(np.abs(plain.posq[:, :3].astype(np.float64) - x_exact))
     assert err_c < 1e-11 and err_v < 1e-11
     assert err_p > 1e3 * err_c
+
+
+# ---- Langevin BAOAB A / O steps ---------------------------------------------
+
+def run_lan(name):
+    _, s, ops = next(c for c in C.langevin_cases() if c[0] == name)
+    st = oracle.VvState(s["posq"], s["vel"], s["force"], s["mass"], s["lattice"])
+    for op in ops:
+        if op[0] == "A":
+            st.lan_drift_half(op[1])
+        else:
+            st.lan_ou_step(*op[1:])
+    return s, st
+
+
+def test_lan_drift_half_reference_cases():
+    s, st = run_lan("lan_drift_half_kat")   # tests/langevin.rs:120-134
+    assert abs(st.posq[0, 0] - (1.0 + 0.5 * 0.05)) < 1e-6
+    assert abs(st.posq[0, 1] - (2.0 - 0.25 * 0.05)) < 1e-6
+    assert abs(st.posq[0, 2] - (3.0 + 0.125 * 0.05)) < 1e-6
+    assert np.array_equal(st.vel, s["vel"])
+    assert st.posq[0, 0] == r32(Q(F32(0.5)) * Q(F32(0.1) * F32(0.5)) + Q(F32(1.0)))
+    s, st = run_lan("lan_drift_half_dt_zero")
+    assert np.array_equal(st.posq, s["posq"])
+    s, st = run_lan("lan_drift_half_wrap")  # :641-664
+    assert abs(st.posq[0, 0] - (-4.95)) < 1e-5 and st.images[0].tolist() == [1, 0, 0]
+
+
+def test_lan_ou_alpha_one_kt_zero_is_identity():
+    s, st = run_lan("lan_ou_identity")      # :185-198
+    assert np.array_equal(st.vel, s["vel"])
+
+
+def test_lan_ou_normal_from_philox_words():
+    """xi = (float)(sqrt(-2 ln u1) cos(2 pi u2)), u = (word + 1/2) 2^-32 from
+    words 0 and 1 of Philox(key = seed, ctr = (draw lo, draw hi, id, axis));
+    with alpha = 0, kT = m = 1 the O step writes xi itself."""
+    n, seed, draw = 40, 0x1234_5678_9ABC_DEF0, 0x0000_0003_0000_0011
+    s = C.state(np.zeros((n, 3)), np.ones((n, 3)), np.ones(n))
+    st = oracle.VvState(s["posq"], s["vel"], s["force"], s["mass"], s["lattice"])
+    st.lan_ou_step(seed, 0.0, 1.0, draw)
+    for i in range(n):
+        for d in range(3):
+            w = oracle.philox4x32_10((seed & 0xFFFFFFFF, seed >> 32),
+                                     (draw & 0xFFFFFFFF, draw >> 32, i, d))
+            u1, u2 = (w[0] + 0.5) / 2**32, (w[1] + 0.5) / 2**32
+            xi = F32(np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2))
+            assert abs(st.vel[i, d] - xi) <= abs(xi) * 2.0**-23, (i, d)
+
+
+def test_lan_ou_roundings_match_exact_rational_model():
+    s, st = run_lan("lan_ou_n64_kt1")
+    st0 = oracle.VvState(s["posq"], np.zeros((64, 3)), s["force"], s["mass"], s["lattice"])
+    st0.lan_ou_step(1, 0.0, 1.0, 1)          # xi itself (m = 1)
+    xi = st0.vel
+    a, kt = F32(0.5), F32(1.0)
+    for i in range(64):
+        one_minus = r32(Q(1) - Q(a) * Q(a))
+        sig = F32(np.sqrt(np.float64(r32(Q(r32(Q(one_minus) * Q(kt))) / Q(s["mass"][i])))))
+        for d in range(3):
+            want = r32(Q(a) * Q(s["vel"][i, d]) + Q(r32(Q(sig) * Q(xi[i, d]))))
+            assert st.vel[i, d] == want, (i, d)
+
+
+def test_lan_ou_seed_and_counter_change_the_draws():
+    _, a = run_lan("lan_ou_n64_kt1")
+    for seed, draw in [(2, 1), (1, 2)]:           # tests/langevin.rs:238-286
+        _, s, _ = next(c for c in C.langevin_cases() if c[0] == "lan_ou_n64_kt1")
+        b = oracle.VvState(s["posq"], s["vel"], s["force"], s["mass"], s["lattice"])
+        b.lan_ou_step(seed, 0.5, 1.0, draw)
+        assert np.mean(a.vel[:, 0] != b.vel[:, 0]) >= 0.9
+
+
+def test_lan_ou_variance():
+    _, st = run_lan("lan_ou_variance")      # tests/langevin.rs:289-327
+    kt = float(F32(1.380649e-23 * 300.0))
+    want = (1.0 - 0.25) * kt / float(F32(6.6335e-26))
+    for d in range(3):
+        v = st.vel[:, d].astype(np.float64)
+        assert abs(v.var() - want) / want < 0.05

@@ -119,3 +119,30 @@ def cases():
         out.append((f"random_triclinic_{tag}", triclinic_state(2051, 12),
                     [("kd", 0.05), ("k", 0.05)] * 10, comp))
     return out
+
+
+def langevin_cases():
+    """(name, state, ops) for the BAOAB A / O steps; ops are ("A", dt) and
+    ("O", seed, alpha, kT, draw counter).  States follow tests/langevin.rs
+    (one_particle_state :40-55, n_particle_state :57-75, the OU variance case
+    :289-327, the +L/2 wrap :641-664) plus a seeded box."""
+    out = []
+    one = state([[1.0, 2.0, 3.0]], [[0.5, -0.25, 0.125]], [1.0])
+    out.append(("lan_drift_half_kat", one, [("A", 0.1)]))
+    out.append(("lan_drift_half_dt_zero", one, [("A", 0.0)]))
+    out.append(("lan_ou_identity", one, [("O", 42, 1.0, 0.0, 7)]))
+    i = np.arange(64, dtype=F32)
+    nps = state(columns(i * F32(0.1), i * F32(0.1), i * F32(0.1)),
+                columns(i * F32(0.05), i * F32(0.05), i * F32(0.05)), np.ones(64))
+    out.append(("lan_ou_n64_seed42", nps, [("O", 42, 0.5, 1.0e-21, 7)]))
+    out.append(("lan_ou_n64_kt1", nps, [("O", 1, 0.5, 1.0, 1)]))
+    heavy = state(np.zeros((10000, 3)), np.zeros((10000, 3)), np.full(10000, 6.6335e-26))
+    out.append(("lan_ou_variance", heavy, [("O", 42, 0.5, float(F32(1.380649e-23 * 300.0)), 1)]))
+    out.append(("lan_drift_half_wrap", state([[4.95, 0, 0]], [[2.0, 0, 0]], [1.0], lattice=BOX10),
+                [("A", 0.1)]))
+    rs = random_state(4099, 21)
+    steps = []
+    for k in range(5):
+        steps += [("A", 0.05), ("O", 2**40 + 7, 0.9, 0.3, 2**33 + k), ("A", 0.05)]
+    out.append(("lan_baoa_random", rs, steps))
+    return out
