@@ -298,10 +298,16 @@ ForceField::ForceField(Context& ctx, Particles& part, const hmd_forcefield_desc&
   probe_capacity();
 }
 
-// compute_cell_layout analogue (src/forces/neighbor_list.rs:1563-1583):
-// the reference needs >= 3 cells of edge r_search per axis; the fine Hilbert
-// grid here only orders atoms, but the same box requirement keeps the
-// minimum-image pair set well defined.
+// compute_cell_layout analogue (src/forces/neighbor_list.rs:1563-1583).
+// The reference's cell list needs >= 3 cells of edge r_search per axis
+// (BoxTooSmallForCells otherwise).  Nothing here depends on cells: the fine
+// Hilbert grid only orders atoms and every distance test is a minimum-image
+// one.  The minimum-image distance is a metric on the periodic cell, so the
+// skin argument (a pair within r_c now was within r_c + r_skin at the
+// rebuild when neither atom moved more than r_skin / 2) holds for any box,
+// and the in-cutoff image of a pair is unique while r_c is below half of
+// every perpendicular width.  That is the only box requirement; small
+// replicas (edges down to ~1.9 nm with r_c = 0.8 nm) rely on it.
 void ForceField::init_neighbor_list(const hmd_forcefield_desc& d) {
   r_skin = (float)d.r_skin;
   r_search = r_cut + r_skin;
@@ -321,11 +327,11 @@ void ForceField::init_neighbor_list(const hmd_forcefield_desc& d) {
   }
   const double widths[3] = {wa, wb, wc};
   for (int k = 0; k < 3; ++k)
-    if (std::floor(widths[k] / r_search) < 3.0) {
+    if (!(widths[k] > 2.0 * (double)r_cut)) {
       char msg[160];
       std::snprintf(msg, sizeof msg,
-                    "box too small for cells: width %.4f < 3 * r_search = %.4f",
-                    widths[k], 3.0 * r_search);
+                    "box too small: perpendicular width %.4f <= 2 * r_cut = %.4f",
+                    widths[k], 2.0 * (double)r_cut);
       throw Error(HMD_ERR_BOX_TOO_SMALL, msg);
     }
   orthorhombic = (L.xy == 0.0f && L.xz == 0.0f && L.yz == 0.0f);

@@ -3,6 +3,8 @@
   argon(...)       -- LJ argon on a simple-cubic lattice (10,000 atoms).
   spc_water(...)   -- rigid SPC water on a grid of molecule centres scaled
                       to a cubic box at 997 kg/m^3, random orientations.
+  spc_water_n(...) -- the same for any molecule count.
+  replica_specs    -- the 64-box replica workload (sizes, cutoffs, meshes).
   water_forcefield -- LJ O-O + real-space Ewald + SPME descriptor.
 
 Each returns plain numpy arrays and dicts accepted by heddlemd_amd.api.
@@ -46,17 +48,36 @@ def _wrap(x, L):
     return x - L * np.floor(x / L + 0.5)
 
 
+def water_edge(n_mol, density=WATER_DENSITY):
+    """Edge (bohr) of the cubic box holding n_mol SPC molecules at `density`."""
+    m_mol = (M_O + 2 * M_H) * U.DALTON_KG
+    return (n_mol * m_mol / density) ** (1.0 / 3.0) / U.LENGTH_BOHR_TO_M
+
+
 def spc_water(grid=(32, 32, 8), seed=7, density=WATER_DENSITY):
     """Molecule centres (centre of mass) on grid[0] x grid[1] x grid[2] cells of
     a cubic box at `density`; atoms ordered O, H1, H2 per molecule."""
     nx, ny, nz = grid
-    n_mol = nx * ny * nz
-    m_mol = (M_O + 2 * M_H) * U.DALTON_KG
-    L_m = (n_mol * m_mol / density) ** (1.0 / 3.0)
-    L = L_m / U.LENGTH_BOHR_TO_M
     ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
-    centres = np.stack([(ix.ravel() + 0.5) / nx, (iy.ravel() + 0.5) / ny,
-                        (iz.ravel() + 0.5) / nz], axis=1) * L - 0.5 * L
+    frac = np.stack([(ix.ravel() + 0.5) / nx, (iy.ravel() + 0.5) / ny,
+                     (iz.ravel() + 0.5) / nz], axis=1)
+    return _water_from_centres(frac, seed, density)
+
+
+def spc_water_n(n_mol, seed, density=WATER_DENSITY):
+    """n_mol molecules at `density` on a seeded random choice of the sites of
+    the smallest k^3 grid with at least n_mol sites (any molecule count)."""
+    k = int(math.ceil(n_mol ** (1.0 / 3.0) - 1e-9))
+    ix, iy, iz = np.meshgrid(np.arange(k), np.arange(k), np.arange(k), indexing="ij")
+    sites = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], axis=1)
+    pick = np.sort(np.random.default_rng(seed + 1_000_003).permutation(k ** 3)[:n_mol])
+    return _water_from_centres((sites[pick] + 0.5) / k, seed, density)
+
+
+def _water_from_centres(frac, seed, density):
+    n_mol = frac.shape[0]
+    L = water_edge(n_mol, density)
+    centres = frac * L - 0.5 * L
     half = U.deg(SPC_ANGLE) / 2.0
     local = np.array([[0.0, 0.0, 0.0],
                       [SPC_D_OH * math.sin(half), SPC_D_OH * math.cos(half), 0.0],
@@ -136,3 +157,35 @@ def argon(n_atoms=10000, n_side=22, edge_nm=7.84, seed=42, temperature_k=94.4):
     return {"n": n_atoms, "L": L, "lattice": np.array([L, L, L, 0, 0, 0]), "posq": posq,
             "mass": mass, "types": np.zeros(n_atoms, np.uint32), "ff": ff,
             "temperature": U.kelvin(temperature_k), "seed": seed}
+
+
+def _smooth_even(n):
+    """Smallest even 2-3-5-smooth integer >= n whose half is also 2-3-5
+    smooth (the FFT's supported mesh lengths)."""
+    def smooth(m):
+        for f in (2, 3, 5):
+            while m % f == 0:
+                m //= f
+        return m == 1
+    while not (n % 2 == 0 and smooth(n) and smooth(n // 2)):
+        n += 1
+    return n
+
+
+def replica_specs(n_replicas=64, seed=23, lo=216, hi=4096, spacing_nm=0.1, mesh_lo=24,
+                  mesh_hi=48, small_edge_nm=2.0):
+    """Per-replica SPC water boxes: molecule counts log-uniform in [lo, hi]
+    from `seed`; a cubic SPME mesh at about `spacing_nm` (the next supported
+    length, clamped to [mesh_lo, mesh_hi]); r_c = 1.0 nm, or 0.8 nm for
+    edges under small_edge_nm; Langevin seed = seed + replica id."""
+    rng = np.random.default_rng(seed)
+    counts = np.rint(np.exp(rng.uniform(math.log(lo), math.log(hi), n_replicas))).astype(int)
+    out = []
+    for r, n_mol in enumerate(counts.tolist()):
+        L = water_edge(n_mol)
+        edge_nm = L / U.NM
+        mesh = min(max(_smooth_even(int(math.ceil(edge_nm / spacing_nm))), mesh_lo), mesh_hi)
+        out.append({"replica": r, "n_mol": n_mol, "edge_nm": edge_nm,
+                    "r_cut_nm": 0.8 if edge_nm < small_edge_nm else 1.0,
+                    "mesh": (mesh, mesh, mesh), "seed": seed + r})
+    return out

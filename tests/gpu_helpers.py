@@ -21,6 +21,22 @@ def water_case(ctx, grid=(12, 12, 9), seed=7, mesh=(32, 32, 32), order=4, relax=
     return sysw, ff, part, field, integ
 
 
+def replica_case(ctx, spec, relax=0, kind="velocity-verlet", temperature=0.0):
+    """One replica box of systems.replica_specs on the device."""
+    sysw = systems.spc_water_n(spec["n_mol"], spec["seed"])
+    ff = systems.water_forcefield(r_cut_nm=spec["r_cut_nm"], grid=spec["mesh"],
+                                  exclusions=sysw["exclusions"])
+    part = api.ParticleBuffers(ctx, sysw["posq"], sysw["mass"], sysw["lattice"],
+                               types=sysw["types"])
+    field = api.ForceField(ctx, part, ff)
+    integ = api.Integrator(ctx, part, field, kind=kind, dt=2.0 * U.FS, settle=sysw["settle"],
+                           temperature=temperature, gamma=1.0 / (1000.0 * U.FS),
+                           seed=spec["seed"])
+    if relax:
+        integ.minimize(relax, 0.05 * U.ANGSTROM)
+    return sysw, ff, part, field, integ
+
+
 def oracle_pair(sysw, ff, posq, erfc_mode=1):
     lj = ff["lj"]
     off, par = ff["exclusions"]
