@@ -303,6 +303,32 @@ class Integrator:
             self.h = None
 
 
+class ReplicaSet:
+    """Independent systems stepped as one batch (hmd_replicas_*).  Each
+    integrator must come from its own GpuContext; fused=True captures every
+    replica's step into one graph, fused=False launches one graph per
+    replica on its own stream."""
+
+    def __init__(self, integrators, fused=True):
+        self._its = list(integrators)
+        arr = (ctypes.c_void_p * len(self._its))(*[it.h.value for it in self._its])
+        h = ctypes.c_void_p()
+        check(lib().hmd_replicas_create(arr, len(self._its), 0 if fused else 1,
+                                        ctypes.byref(h)))
+        self.h = h
+
+    def step(self, n=1):
+        check(lib().hmd_replicas_step(self.h, n))
+
+    def steps(self):
+        return int(lib().hmd_replicas_steps(self.h))
+
+    def close(self):
+        if self.h:
+            lib().hmd_replicas_destroy(self.h)
+            self.h = None
+
+
 class CorrectionNet:
     """Pair-scalar network correcting a coarse SPME mesh toward a fine one
     (the learned part of the hot path; no counterpart in the reference).

@@ -227,10 +227,118 @@ struct Integrator {
   }
 };
 
+// ------------------------------------------------------- replica batches ---
+// Independent systems, each with its own context (stream, code object),
+// particles, force field and integrator, stepped as one batch.  In the fused
+// mode every replica's step is captured into ONE graph: the master stream
+// forks to each replica's stream and joins them back, so a batched step is
+// a single graph launch whose branches the runtime may run side by side on
+// its hardware queues.  In the per-replica mode each replica keeps its own
+// step graph, launched on its own stream, all replicas before any wait.
+struct ReplicaSet {
+  std::vector<Integrator*> its;
+  bool fused = true;
+  hipStream_t master = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t steps = 0;
+
+  ReplicaSet(std::vector<Integrator*> v, bool fused_) : its(std::move(v)), fused(fused_) {
+    if (its.empty()) throw Error(HMD_ERR_INVALID, "empty replica set");
+    const int dev = its[0]->c.device;
+    for (auto* it : its) {
+      if (it->c.device != dev) throw Error(HMD_ERR_INVALID, "replicas on different devices");
+      for (auto* jt : its)
+        if (jt != it && jt->c.stream == it->c.stream)
+          throw Error(HMD_ERR_INVALID, "each replica needs its own context");
+    }
+    check(hipSetDevice(dev), "hipSetDevice");
+    check(hipStreamCreateWithFlags(&master, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "hipEventCreate");
+    joins.resize(its.size());
+    for (auto& e : joins) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+  }
+
+  ~ReplicaSet() {
+    release();
+    for (auto& e : joins) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    if (master) (void)hipStreamDestroy(master);
+  }
+
+  void release() {
+    if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+    if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+  }
+
+  void capture() {
+    for (auto* it : its) it->prime();
+    release();
+    check(hipStreamBeginCapture(master, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+      check(hipEventRecord(fork, master), "hipEventRecord");
+      for (size_t r = 0; r < its.size(); ++r) {
+        Integrator& I = *its[r];
+        check(hipStreamWaitEvent(I.c.stream, fork, 0), "hipStreamWaitEvent");
+        I.enqueue_step();
+        check(hipEventRecord(joins[r], I.c.stream), "hipEventRecord");
+        check(hipStreamWaitEvent(master, joins[r], 0), "hipStreamWaitEvent");
+      }
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(master, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    check(hipStreamEndCapture(master, &graph), "hipStreamEndCapture");
+    check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    for (auto* it : its) it->ff.graph_invalid = false;
+  }
+
+  void step(uint32_t n) {
+    if (fused) {
+      bool stale = !exec;
+      for (auto* it : its) stale = stale || it->ff.graph_invalid;
+      if (stale) capture();
+      for (uint32_t i = 0; i < n; ++i) check(hipGraphLaunch(exec, master), "hipGraphLaunch");
+      check(hipStreamSynchronize(master), "sync");
+    } else {
+      for (auto* it : its)
+        if (!it->exec || it->ff.graph_invalid) it->capture();
+      for (uint32_t i = 0; i < n; ++i)
+        for (auto* it : its) check(hipGraphLaunch(it->exec, it->c.stream), "hipGraphLaunch");
+      for (auto* it : its) check(hipStreamSynchronize(it->c.stream), "sync");
+    }
+    steps += n;
+    for (auto* it : its) it->steps += n;
+    // every replica's status is read (and grown on overflow) before the
+    // first error is reported, so one re-run of the batch suffices
+    std::string err;
+    hmd_status code = HMD_OK;
+    for (size_t r = 0; r < its.size(); ++r) {
+      try {
+        its[r]->ff.check_status();
+      } catch (const Error& e) {
+        if (code == HMD_OK) {
+          code = e.code;
+          err = "replica " + std::to_string(r) + ": " + e.what();
+        }
+      }
+    }
+    if (code != HMD_OK) throw Error(code, err);
+  }
+};
+
 }  // namespace hmd
 
 struct hmd_integrator {
   hmd::Integrator* it = nullptr;
+};
+
+struct hmd_replicas {
+  hmd::ReplicaSet* set = nullptr;
 };
 
 using namespace hmd;
@@ -435,6 +543,35 @@ void hmd_integrator_destroy(hmd_integrator* w) {
   delete w->it;
   delete w;
 }
+
+hmd_status hmd_replicas_create(hmd_integrator* const* its, uint32_t n, uint32_t flags,
+                               hmd_replicas** out) {
+  HMD_GUARD({
+    if (!its || !out || n == 0) throw Error(HMD_ERR_INVALID, "null or empty replica list");
+    std::vector<Integrator*> v;
+    for (uint32_t r = 0; r < n; ++r) {
+      if (!its[r]) throw Error(HMD_ERR_INVALID, "null integrator");
+      v.push_back(its[r]->it);
+    }
+    auto* h = new hmd_replicas();
+    try {
+      h->set = new ReplicaSet(std::move(v), (flags & HMD_REPLICAS_PER_REPLICA_GRAPHS) == 0);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+void hmd_replicas_destroy(hmd_replicas* h) {
+  if (!h) return;
+  delete h->set;
+  delete h;
+}
+
+hmd_status hmd_replicas_step(hmd_replicas* h, uint32_t n) { HMD_GUARD(h->set->step(n)); }
+uint64_t hmd_replicas_steps(hmd_replicas* h) { return h->set->steps; }
 
 hmd_status hmd_integrator_step(hmd_integrator* w, uint32_t n) { HMD_GUARD(w->it->step(n)); }
 hmd_status hmd_integrator_capture(hmd_integrator* w) { HMD_GUARD(w->it->capture()); }

@@ -230,6 +230,24 @@ hmd_status hmd_integrator_profile(hmd_integrator* it, uint32_t n_steps,
 hmd_status hmd_integrator_minimize(hmd_integrator* it, uint32_t n_iter,
                                    double max_step);
 
+/* ---- replica batches -----------------------------------------------------
+ * Independent systems stepped together (the batched-replica workload; no
+ * counterpart in the reference, which runs one system per process).  Each
+ * integrator must come from its own context (hmd_ctx) on the same device.
+ * Default: all replicas' steps are captured into one graph (one launch per
+ * batched step, a fork per replica); HMD_REPLICAS_PER_REPLICA_GRAPHS keeps
+ * one step graph per replica, each launched on its own stream.  Status is
+ * checked for every replica after each call; an overflow grows that
+ * replica's list and is reported as HMD_ERR_NBL_OVERFLOW (re-run the
+ * batch).  The integrators must outlive the set. */
+typedef struct hmd_replicas hmd_replicas;
+#define HMD_REPLICAS_PER_REPLICA_GRAPHS 1u
+hmd_status hmd_replicas_create(hmd_integrator* const* integrators, uint32_t n, uint32_t flags,
+                               hmd_replicas** out);
+void hmd_replicas_destroy(hmd_replicas* set);
+hmd_status hmd_replicas_step(hmd_replicas* set, uint32_t n_steps);
+uint64_t hmd_replicas_steps(hmd_replicas* set);
+
 /* ---- setup helpers (host only, no GPU) -----------------------------------
  * generate_velocities (src/runner.rs:3108-3200): ChaCha8 seeded with
  * seed_from_u64, Box-Muller cos branch, COM removal, rescale to
