@@ -83,6 +83,7 @@ class IntegratorDesc(ctypes.Structure):
         ("group_atoms", _c_u32p),
         ("d_oh", ctypes.c_double), ("d_hh", ctypes.c_double),
         ("m_o", ctypes.c_double), ("m_h", ctypes.c_double),
+        ("flags", ctypes.c_uint32),
     ]
 
 

@@ -252,8 +252,10 @@ class Integrator:
     """Integrator + constraint plan (src/integrator/mod.rs)."""
 
     def __init__(self, ctx, particles, forcefield, kind="velocity-verlet", dt=1.0,
-                 compensated=False, temperature=0.0, gamma=0.0, seed=0, settle=None):
+                 compensated=False, temperature=0.0, gamma=0.0, seed=0, settle=None,
+                 fused=True):
         d = IntegratorDesc()
+        d.flags = 0 if fused else 1
         self._kind = kind.encode()
         d.kind = self._kind
         d.dt = dt

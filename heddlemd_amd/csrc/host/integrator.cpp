@@ -31,6 +31,9 @@ struct Integrator {
   // Langevin
   float lan_a = 1.0f;
   float lan_kt = 0.0f;
+  bool fused = true;
+  DevBuf<uint32_t> free_atoms;   // atoms outside SETTLE groups (fused plan)
+  uint32_t n_free = 0;
   uint64_t seed = 0;
   DevBuf<unsigned long long> step_counter;
   // graph
@@ -67,6 +70,24 @@ struct Integrator {
       group_atoms.upload(d.group_atoms, 3 * (size_t)n_groups, s);
       snapshot.alloc(12 * (size_t)n_groups);
       constraint_virial.alloc(3 * (size_t)n_groups);
+    }
+    fused = (d.flags & HMD_INTEGRATOR_UNFUSED) == 0;
+    if (n_groups) {
+      std::vector<uint8_t> in_group(p.n, 0);
+      for (uint32_t k2 = 0; k2 < 3 * n_groups; ++k2) {
+        if (in_group[d.group_atoms[k2]]) throw Error(HMD_ERR_INVALID, "atom in two SETTLE groups");
+        in_group[d.group_atoms[k2]] = 1;
+      }
+      std::vector<uint32_t> fr;
+      for (uint32_t i = 0; i < p.n; ++i)
+        if (!in_group[i]) fr.push_back(i);
+      n_free = (uint32_t)fr.size();
+      if (n_free) {
+        free_atoms.alloc(n_free, false);
+        free_atoms.upload(fr.data(), n_free, s);
+      }
+    } else {
+      n_free = p.n;   // identity
     }
     if (kind == Kind::LangevinBaoab) {
       // alpha = exp(-gamma dt) in f32 (src/integrator/langevin_baoab.rs:140)
@@ -148,9 +169,32 @@ struct Integrator {
     launch(c, K_hmd_settle_velocities, ceil_div(n_groups, 256), 256, 0, settle(dt));
   }
 
+  StepParams fused_params() const {
+    StepParams sp{};
+    sp.vv = vv();
+    sp.st = settle(dt);
+    sp.lan = lan(dt);
+    sp.free_atoms = n_groups ? free_atoms.ptr : nullptr;
+    sp.n_free = n_free;
+    sp.dt = dt;
+    sp.counter = kind == Kind::LangevinBaoab ? step_counter.ptr : nullptr;
+    return sp;
+  }
+
   // One step, enqueued only (capturable).
   void enqueue_step() {
     const uint32_t nb = ceil_div(p.n, 256);
+    if (fused) {
+      const StepParams sp = fused_params();
+      const uint32_t nu = ceil_div((uint64_t)n_groups + n_free, 256);
+      launch(c, kind == Kind::VelocityVerlet ? K_hmd_step_vv_first : K_hmd_step_baoa, nu, 256,
+             0, sp);
+      c.mark(kind == Kind::VelocityVerlet ? "kick_drift_settle" : "baoa");
+      ff.compute(0);
+      launch(c, K_hmd_step_kick_last, nu, 256, 0, sp);
+      c.mark("kick_settle");
+      return;
+    }
     if (kind == Kind::VelocityVerlet) {
       settle_snapshot();                                        // BeforeDrift
       launch(c, K_hmd_vv_kick_drift, nb, 256, 0, vv());         // KickDrift

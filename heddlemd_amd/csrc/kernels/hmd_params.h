@@ -300,6 +300,18 @@ struct LangevinParams {
   uint32_t n;
 };
 
+// Fused integrator sub-steps (hmd_step_*): the per-op parameter blocks,
+// the free (unconstrained) atoms and the Langevin draw counter.
+struct StepParams {
+  VvParams vv;
+  SettleParams st;
+  LangevinParams lan;
+  const uint32_t* free_atoms;  // [n_free]; null: atoms 0..n_free-1
+  uint32_t n_free;
+  float dt;
+  unsigned long long* counter; // Langevin draw counter (null for VV)
+};
+
 struct CounterParams {
   unsigned long long* counter;
 };

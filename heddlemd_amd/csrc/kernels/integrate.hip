@@ -17,11 +17,8 @@
 // not depend on the compiler's contraction choices.
 
 // v += F/m dt/2; x += v dt; wrap + image count (integrate.cu:24-170).
-extern "C" __global__ void __launch_bounds__(256)
-hmd_vv_kick_drift(const VvParams p) {
+__device__ inline void hmd_vv_kick_drift_atom(const VvParams& p, const uint32_t i) {
 #pragma clang fp contract(off)
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= p.n) return;
   const float m = p.m[i];
   const float half_dt = p.dt * 0.5f;
   float4 pq = reinterpret_cast<float4*>(p.posq)[i];
@@ -62,11 +59,8 @@ hmd_vv_kick_drift(const VvParams p) {
 }
 
 // v += F/m dt/2 (integrate.cu:172-215).
-extern "C" __global__ void __launch_bounds__(256)
-hmd_vv_kick(const VvParams p) {
+__device__ inline void hmd_vv_kick_atom(const VvParams& p, const uint32_t i) {
 #pragma clang fp contract(off)
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= p.n) return;
   const float m = p.m[i];
   const float half_dt = p.dt * 0.5f;
 #pragma unroll
@@ -85,13 +79,14 @@ hmd_vv_kick(const VvParams p) {
 }
 
 // ------------------------------------------------------------------ SETTLE --
+// Every f32 operation rounds once (no contraction), so that the result does
+// not depend on how the compiler fuses multiplies and adds in the context a
+// group function is inlined into (one kernel per sub-step or the fused step
+// kernels).
 __device__ inline float hmd_csqrt(float x) { return sqrtf(x > 0.0f ? x : 0.0f); }
 
 // Snapshot of the pre-drift positions (settle.cu:47-67).
-extern "C" __global__ void __launch_bounds__(256)
-hmd_settle_snapshot(const SettleParams p) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= p.n_groups) return;
+__device__ inline void hmd_settle_snapshot_group(const SettleParams& p, const uint32_t g) {
   const float4* ps = reinterpret_cast<const float4*>(p.posq);
   float4* sn = reinterpret_cast<float4*>(p.snapshot);
 #pragma unroll
@@ -103,6 +98,7 @@ hmd_settle_snapshot(const SettleParams p) {
 __device__ inline void hmd_apply_pos(const SettleParams& p, uint32_t idx, float4& pq,
                                      const float cur[3], const float corr[3],
                                      float inv_dt) {
+#pragma clang fp contract(off)
   float out[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -129,10 +125,9 @@ __device__ inline void hmd_apply_pos(const SettleParams& p, uint32_t idx, float4
 // Analytic rigid-water position reset (Miyamoto & Kollman, J. Comput.
 // Chem. 13(8), 952-962, 1992) with the half-step velocity fix and the
 // position-level constraint virial, as settle.cu:145-335.
-extern "C" __global__ void __launch_bounds__(256)
-hmd_settle_positions(const SettleParams p) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= p.n_groups) return;
+__device__ inline void hmd_settle_positions_group(const SettleParams& p, const uint32_t g,
+                                                  const float dt) {
+#pragma clang fp contract(off)
   const HmdLattice L = p.L;
   float4* ps = reinterpret_cast<float4*>(p.posq);
   const float4* sn = reinterpret_cast<const float4*>(p.snapshot);
@@ -218,7 +213,7 @@ hmd_settle_positions(const SettleParams p) {
   const float pr[3][3] = {{-a2y * sinth, a2y * costh, a1z},
                           {b2x * costh - b2y * sinth, b2x * sinth + b2y * costh, b1z},
                           {-b2x * costh - c2y * sinth, -b2x * sinth + c2y * costh, c1z}};
-  const float inv_dt = p.dt != 0.0f ? 1.0f / p.dt : 0.0f;
+  const float inv_dt = dt != 0.0f ? 1.0f / dt : 0.0f;
   const float inv_dt2 = inv_dt * inv_dt;
   const float mass[3] = {m0, m1, m2};
 #pragma unroll
@@ -239,10 +234,9 @@ hmd_settle_positions(const SettleParams p) {
 
 // Analytic velocity projection: 3x3 Cramer solve for the bond impulses
 // (settle.cu:376-490).
-extern "C" __global__ void __launch_bounds__(256)
-hmd_settle_velocities(const SettleParams p) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= p.n_groups) return;
+__device__ inline void hmd_settle_velocities_group(const SettleParams& p, const uint32_t g,
+                                                   const float dt) {
+#pragma clang fp contract(off)
   const HmdLattice L = p.L;
   const float4* ps = reinterpret_cast<const float4*>(p.posq);
   const uint32_t idx[3] = {p.group_atoms[3 * g], p.group_atoms[3 * g + 1],
@@ -321,8 +315,8 @@ hmd_settle_velocities(const SettleParams p) {
         p.v[k] = v[a][d] + dv[a][d];
       }
     }
-  if (p.constraint_virial && p.dt > 0.0f) {
-    const float inv_dt = 1.0f / p.dt;
+  if (p.constraint_virial && dt > 0.0f) {
+    const float inv_dt = 1.0f / dt;
     const float tm = p.m_o + 2.0f * p.m_h;
     const float cx = (p.m_o * px[0] + p.m_h * px[1] + p.m_h * px[2]) / tm;
     const float cy = (p.m_o * py[0] + p.m_h * py[1] + p.m_h * py[2]) / tm;
@@ -363,11 +357,8 @@ __device__ inline float hmd_philox_gaussian(uint32_t sl, uint32_t sh, uint32_t c
 
 // A step: x += v dt/2 (one rounding, as the reference's contracted
 // x + v * half_dt), wrap (langevin.cu:13-50).
-extern "C" __global__ void __launch_bounds__(256)
-hmd_lan_drift_half(const LangevinParams p) {
+__device__ inline void hmd_lan_drift_half_atom(const LangevinParams& p, const uint32_t i) {
 #pragma clang fp contract(off)
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= p.n) return;
   float4 pq = reinterpret_cast<float4*>(p.posq)[i];
   const float h = p.dt * 0.5f;
   float x = fmaf(p.v[i], h, pq.x), y = fmaf(p.v[p.n + i], h, pq.y),
@@ -385,11 +376,8 @@ hmd_lan_drift_half(const LangevinParams p) {
 // (1 - a^2 and a v + (sigma xi) each rounded once, as the reference's
 // contracted expressions), xi from Philox keyed by the seed with counter
 // (draw_lo, draw_hi, particle id, axis) (langevin.cu:52-80).
-extern "C" __global__ void __launch_bounds__(256)
-hmd_lan_ou(const LangevinParams p) {
+__device__ inline void hmd_lan_ou_atom(const LangevinParams& p, const uint32_t i) {
 #pragma clang fp contract(off)
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= p.n) return;
   const unsigned long long step = *p.step;
   const uint32_t cl = (uint32_t)(step & 0xFFFFFFFFull), ch = (uint32_t)(step >> 32);
   const float sig = sqrtf(fmaf(-p.a, p.a, 1.0f) * p.kt / p.m[i]);
@@ -398,6 +386,116 @@ hmd_lan_ou(const LangevinParams p) {
     const float xi = hmd_philox_gaussian(p.seed_lo, p.seed_hi, cl, ch, i, d);
     const size_t k = (size_t)d * p.n + i;
     p.v[k] = fmaf(p.a, p.v[k], sig * xi);
+  }
+}
+
+// ---------------------------------------------------- one kernel per op --
+// The reference's sub-steps one launch each (the standalone operator entry
+// points and the unfused integrator plan).
+extern "C" __global__ void __launch_bounds__(256) hmd_vv_kick_drift(const VvParams p) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < p.n) hmd_vv_kick_drift_atom(p, i);
+}
+extern "C" __global__ void __launch_bounds__(256) hmd_vv_kick(const VvParams p) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < p.n) hmd_vv_kick_atom(p, i);
+}
+extern "C" __global__ void __launch_bounds__(256) hmd_settle_snapshot(const SettleParams p) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < p.n_groups) hmd_settle_snapshot_group(p, g);
+}
+extern "C" __global__ void __launch_bounds__(256) hmd_settle_positions(const SettleParams p) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < p.n_groups) hmd_settle_positions_group(p, g, p.dt);
+}
+extern "C" __global__ void __launch_bounds__(256) hmd_settle_velocities(const SettleParams p) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < p.n_groups) hmd_settle_velocities_group(p, g, p.dt);
+}
+extern "C" __global__ void __launch_bounds__(256) hmd_lan_drift_half(const LangevinParams p) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < p.n) hmd_lan_drift_half_atom(p, i);
+}
+extern "C" __global__ void __launch_bounds__(256) hmd_lan_ou(const LangevinParams p) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < p.n) hmd_lan_ou_atom(p, i);
+}
+
+// ------------------------------------------------------ fused step kernels --
+// The integrator's sub-steps between two force evaluations in one launch.
+// Thread u < n_groups owns SETTLE group u and runs every sub-step for its
+// three atoms in plan order (the constraint couples only the atoms of one
+// group, so no other thread's data is read); thread n_groups + k owns free
+// atom free_atoms[k] (atom k when free_atoms is null).  Each atom sees the
+// same operations, in the same order and with the same rounding, as in the
+// one-kernel-per-sub-step plan.
+__device__ inline bool hmd_unit(const StepParams& p, uint32_t u, uint32_t& g, uint32_t& atom) {
+  if (u < p.st.n_groups) { g = u; return true; }
+  const uint32_t k = u - p.st.n_groups;
+  g = HMD_SENTINEL;
+  atom = p.free_atoms ? p.free_atoms[k] : k;
+  return k < p.n_free;
+}
+
+// velocity Verlet: [snapshot] kick-drift [SETTLE positions]
+extern "C" __global__ void __launch_bounds__(256) hmd_step_vv_first(const StepParams p) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t g, atom;
+  if (!hmd_unit(p, u, g, atom)) return;
+  if (g != HMD_SENTINEL) {
+    hmd_settle_snapshot_group(p.st, g);
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) hmd_vv_kick_drift_atom(p.vv, p.st.group_atoms[3 * g + a]);
+    hmd_settle_positions_group(p.st, g, p.dt);
+  } else {
+    hmd_vv_kick_drift_atom(p.vv, atom);
+  }
+}
+
+// Langevin BAOAB up to the force evaluation: B [SETTLE v] [snapshot] A
+// [SETTLE x, dt/2] O [SETTLE v] [snapshot] A [SETTLE x, dt/2]
+extern "C" __global__ void __launch_bounds__(256) hmd_step_baoa(const StepParams p) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t g, atom;
+  if (!hmd_unit(p, u, g, atom)) return;
+  const float h = 0.5f * p.dt;
+  if (g != HMD_SENTINEL) {
+    const uint32_t* ga = p.st.group_atoms + 3 * g;
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) hmd_vv_kick_atom(p.vv, ga[a]);
+    hmd_settle_velocities_group(p.st, g, p.dt);
+    hmd_settle_snapshot_group(p.st, g);
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) hmd_lan_drift_half_atom(p.lan, ga[a]);
+    hmd_settle_positions_group(p.st, g, h);
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) hmd_lan_ou_atom(p.lan, ga[a]);
+    hmd_settle_velocities_group(p.st, g, p.dt);
+    hmd_settle_snapshot_group(p.st, g);
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) hmd_lan_drift_half_atom(p.lan, ga[a]);
+    hmd_settle_positions_group(p.st, g, h);
+  } else {
+    hmd_vv_kick_atom(p.vv, atom);
+    hmd_lan_drift_half_atom(p.lan, atom);
+    hmd_lan_ou_atom(p.lan, atom);
+    hmd_lan_drift_half_atom(p.lan, atom);
+  }
+}
+
+// after the force evaluation: kick [SETTLE velocities]; the Langevin draw
+// counter (read by this step's O sub-step) advances here
+extern "C" __global__ void __launch_bounds__(256) hmd_step_kick_last(const StepParams p) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u == 0 && p.counter) p.counter[0] += 1ull;
+  uint32_t g, atom;
+  if (!hmd_unit(p, u, g, atom)) return;
+  if (g != HMD_SENTINEL) {
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) hmd_vv_kick_atom(p.vv, p.st.group_atoms[3 * g + a]);
+    hmd_settle_velocities_group(p.st, g, p.dt);
+  } else {
+    hmd_vv_kick_atom(p.vv, atom);
   }
 }
 

@@ -205,7 +205,13 @@ typedef struct {
   uint32_t n_groups;
   const uint32_t* group_atoms;
   double d_oh, d_hh, m_o, m_h;
+  /* 0: the sub-steps between two force evaluations run as one fused
+   * launch per group of sub-steps (same per-atom arithmetic and order);
+   * HMD_INTEGRATOR_UNFUSED: one launch per sub-step, as the reference's
+   * plan executes them. */
+  uint32_t flags;
 } hmd_integrator_desc;
+#define HMD_INTEGRATOR_UNFUSED 1u
 
 hmd_status hmd_integrator_create(hmd_ctx* ctx, hmd_particles* particles,
                                  hmd_forcefield* ff,
