@@ -299,6 +299,20 @@ class Integrator:
         labels = [s for s in names.value.decode().split("\n") if s]
         return dict(zip(labels, us[:k.value].tolist()))
 
+    def timings(self, n_steps):
+        """Per-stage kernel timings of n eager steps: [{name, count,
+        total_ns, min_ns, max_ns}] in plan order (src/timings.rs)."""
+        m = 64
+        arrs = [np.zeros(m, np.uint64) for _ in range(4)]
+        k = ctypes.c_int32()
+        names = ctypes.create_string_buffer(4096)
+        u64p = ctypes.POINTER(ctypes.c_uint64)
+        check(lib().hmd_integrator_timings(self.h, n_steps, m, ctypes.byref(k), names, 4096,
+                                           *[_ptr(a, u64p) for a in arrs]))
+        labels = [s for s in names.value.decode().split("\n") if s]
+        return [{"name": labels[i], "count": int(arrs[0][i]), "total_ns": int(arrs[1][i]),
+                 "min_ns": int(arrs[2][i]), "max_ns": int(arrs[3][i])} for i in range(k.value)]
+
     def close(self):
         if self.h:
             lib().hmd_integrator_destroy(self.h)
@@ -483,6 +497,17 @@ class TrajectoryWriter:
         if self.h:
             lib().hmd_traj_close(self.h)
             self.h = None
+
+
+def write_timings(path, stages):
+    """The .timings report (src/timings.rs:513-553) of [{name, count,
+    total_ns, min_ns, max_ns}]; stages with a zero count are left out."""
+    n = len(stages)
+    names = (ctypes.c_char_p * max(1, n))(*[st["name"].encode() for st in stages])
+    cols = [np.ascontiguousarray([st[k] for st in stages] or [0], np.uint64)
+            for k in ("count", "total_ns", "min_ns", "max_ns")]
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    check(lib().hmd_timings_write(str(path).encode(), n, names, *[_ptr(c, u64p) for c in cols]))
 
 
 class LogWriter:

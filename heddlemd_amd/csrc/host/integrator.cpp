@@ -665,6 +665,66 @@ hmd_status hmd_integrator_profile(hmd_integrator* w, uint32_t n, double* us, int
   });
 }
 
+hmd_status hmd_integrator_timings(hmd_integrator* w, uint32_t n, int32_t max_stages,
+                                  int32_t* n_stages, char* names, int32_t names_len,
+                                  uint64_t* counts, uint64_t* total_ns, uint64_t* min_ns,
+                                  uint64_t* max_ns) {
+  HMD_GUARD({
+    if (!w || !n_stages || max_stages < 0) throw Error(HMD_ERR_INVALID, "null argument");
+    Integrator& I = *w->it;
+    Context& c = I.c;
+    I.prime();
+    struct Acc { uint64_t count = 0, total = 0, lo = UINT64_MAX, hi = 0; };
+    std::vector<std::string> order;
+    std::map<std::string, Acc> acc;
+    for (uint32_t s = 0; s < n; ++s) {
+      for (auto& m : c.marks) (void)hipEventDestroy(m.second);
+      c.marks.clear();
+      c.profiling = true;
+      try {
+        c.mark("begin");
+        I.enqueue_step();
+      } catch (...) {
+        c.profiling = false;
+        throw;
+      }
+      c.profiling = false;
+      check(hipStreamSynchronize(c.stream), "sync");
+      for (size_t k = 1; k < c.marks.size(); ++k) {
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, c.marks[k - 1].second, c.marks[k].second), "elapsed");
+        const std::string& nm = c.marks[k].first;
+        if (!acc.count(nm)) order.push_back(nm);
+        Acc& a = acc[nm];
+        const uint64_t ns = (uint64_t)std::llround((double)ms * 1.0e6);
+        a.count += 1;
+        a.total += ns;
+        a.lo = std::min(a.lo, ns);
+        a.hi = std::max(a.hi, ns);
+      }
+    }
+    for (auto& m : c.marks) (void)hipEventDestroy(m.second);
+    c.marks.clear();
+    I.steps += n;
+    I.ff.check_status();
+    std::string all;
+    int32_t k = 0;
+    for (auto& nm : order) {
+      if (k < max_stages) {
+        const Acc& a = acc[nm];
+        if (counts) counts[k] = a.count;
+        if (total_ns) total_ns[k] = a.total;
+        if (min_ns) min_ns[k] = a.lo;
+        if (max_ns) max_ns[k] = a.hi;
+        all += nm + "\n";
+      }
+      ++k;
+    }
+    *n_stages = std::min(k, max_stages);
+    if (names && names_len > 0) std::snprintf(names, (size_t)names_len, "%s", all.c_str());
+  });
+}
+
 hmd_status hmd_integrator_minimize(hmd_integrator* w, uint32_t n_iter, double max_step) {
   HMD_GUARD({
     // Steepest descent with an adaptive step: move every atom by h F (the

@@ -261,4 +261,33 @@ hmd_status hmd_log_flush(hmd_log* l) {
 
 void hmd_log_close(hmd_log* l) { delete l; }
 
+// src/timings.rs:513-553 (write_timings_file): a fixed-width table, header
+// "{:<28} {:>10} {:>14} {:>13} {:>11} {:>11}", one row per stage with a
+// nonzero count as "{:<28} {:>10} {:>14.3} {:>13.1} {:>11.1} {:>11.1}" of
+// name, count, total ms, mean us, min us, max us; create-new semantics.
+hmd_status hmd_timings_write(const char* path, uint32_t n_stages, const char* const* names,
+                             const uint64_t* counts, const uint64_t* total_ns,
+                             const uint64_t* min_ns, const uint64_t* max_ns) {
+  HMD_GUARD({
+    if (!path || (n_stages && (!names || !counts || !total_ns || !min_ns || !max_ns)))
+      throw Error(HMD_ERR_INVALID, "null argument");
+    OutFile out;
+    out.open_new(path);
+    char line[512];
+    std::snprintf(line, sizeof line, "%-28s %10s %14s %13s %11s %11s\n", "stage", "count",
+                  "total_ms", "mean_us", "min_us", "max_us");
+    out.buf += line;
+    for (uint32_t k = 0; k < n_stages; ++k) {
+      if (counts[k] == 0) continue;
+      const double total_ms = (double)total_ns[k] / 1000000.0;
+      const double mean_us = (double)total_ns[k] / 1000.0 / (double)counts[k];
+      std::snprintf(line, sizeof line, "%-28s %10" PRIu64 " %14.3f %13.1f %11.1f %11.1f\n",
+                    names[k], counts[k], total_ms, mean_us, (double)min_ns[k] / 1000.0,
+                    (double)max_ns[k] / 1000.0);
+      out.buf += line;
+    }
+    out.flush();
+  });
+}
+
 }  // extern "C"

@@ -233,8 +233,23 @@ hmd_status hmd_integrator_profile(hmd_integrator* it, uint32_t n_steps,
                                   int32_t names_len);
 /* Steepest-descent relaxation with SETTLE position projection (setup only;
  * the reference's minimizer, src/minimizer/). */
+/* Per-stage kernel timings of n eager steps (HIP events between the
+ * stage marks on the context stream): the Timings collector of
+ * src/timings.rs:228-495 (count, total, min, max per stage, in plan
+ * order).  Arrays hold max_stages entries; names are '\n'-separated. */
+hmd_status hmd_integrator_timings(hmd_integrator* it, uint32_t n_steps, int32_t max_stages,
+                                  int32_t* n_stages, char* names, int32_t names_len,
+                                  uint64_t* counts, uint64_t* total_ns, uint64_t* min_ns,
+                                  uint64_t* max_ns);
 hmd_status hmd_integrator_minimize(hmd_integrator* it, uint32_t n_iter,
                                    double max_step);
+
+/* The .timings report (write_timings_file, src/timings.rs:513-553): a
+ * fixed-width table of the stages with a nonzero count; an existing path
+ * is HMD_ERR_OUTPUT_EXISTS. */
+hmd_status hmd_timings_write(const char* path, uint32_t n_stages, const char* const* names,
+                             const uint64_t* counts, const uint64_t* total_ns,
+                             const uint64_t* min_ns, const uint64_t* max_ns);
 
 /* ---- replica batches -----------------------------------------------------
  * Independent systems stepped together (the batched-replica workload; no

@@ -48,3 +48,28 @@ def test_trajectory_and_log_are_byte_identical_run_to_run(gpu_ctx, tmp_path):
     assert len(a[0].splitlines()) == 5 * (2 + 3888)
     assert a[0] == b[0]
     assert a[1] == b[1]
+
+
+def test_timings_report_of_eager_steps(gpu_ctx, tmp_path):
+    """hmd_integrator_timings over 5 eager steps: every stage of the plan is
+    counted once per step with min <= mean <= max; the .timings report has
+    the reference's header and one row per stage (tests/timings.rs:136-151,
+    :583-600)."""
+    sysw, ff, part, field, integ = water_case(gpu_ctx, relax=5)
+    try:
+        stages = integ.timings(5)
+    finally:
+        for h in (integ, field, part):
+            h.close()
+    names = [s["name"] for s in stages]
+    print("stages:", ", ".join(f"{s['name']} {s['total_ns'] / s['count'] / 1e3:.1f} us"
+                               for s in stages))
+    assert "pair_force" in names and "kick_settle" in names
+    for s in stages:
+        assert s["count"] == 5
+        assert s["min_ns"] * 5 <= s["total_ns"] <= s["max_ns"] * 5
+    p = tmp_path / "run.out.timings"
+    api.write_timings(p, stages)
+    lines = p.read_text().splitlines()
+    assert lines[0].split() == ["stage", "count", "total_ms", "mean_us", "min_us", "max_us"]
+    assert [ln.split()[0] for ln in lines[1:]] == names

@@ -163,3 +163,54 @@ def test_format_rule_on_random_f32(tmp_path):
     got = [l.split()[1] for l in read(p).splitlines()[2:]]
     assert got[1] == "-0.000000000e0"
     assert got == [rust_exp(float(v)) for v in x]
+
+
+# ---- .timings report (src/timings.rs:513-553) -------------------------------
+
+def test_timings_header_only_when_empty(tmp_path):
+    """tests/timings.rs:851-862: an empty report is the header line alone."""
+    p = tmp_path / "run.timings"
+    api.write_timings(p, [])
+    lines = read(p).splitlines()
+    assert len(lines) == 1
+    assert lines[0].split() == ["stage", "count", "total_ms", "mean_us", "min_us", "max_us"]
+    assert lines[0] == ("stage" + " " * 23 + " " + "count".rjust(10) + " " + "total_ms".rjust(14)
+                        + " " + "mean_us".rjust(13) + " " + "min_us".rjust(11) + " "
+                        + "max_us".rjust(11))
+
+
+def test_timings_rows_widths_and_precision(tmp_path):
+    """Row layout {:<28} {:>10} {:>14.3} {:>13.1} {:>11.1} {:>11.1}; the
+    reference's accumulator example (tests/timings.rs:925-939: 3 calls of
+    50, 100 and 200 us) and a zero-count stage, which is left out."""
+    p = tmp_path / "run.timings"
+    api.write_timings(p, [
+        {"name": "jit_composed_pair_force", "count": 3, "total_ns": 350_000,
+         "min_ns": 50_000, "max_ns": 200_000},
+        {"name": "log_write", "count": 0, "total_ns": 0, "min_ns": 0, "max_ns": 0},
+        {"name": "vv_kick", "count": 10, "total_ns": 1_234_567, "min_ns": 1_250,
+         "max_ns": 987_654_321},
+    ])
+    lines = read(p).splitlines()
+    assert len(lines) == 3
+    assert lines[1] == ("jit_composed_pair_force".ljust(28) + " " + "3".rjust(10) + " "
+                        + "0.350".rjust(14) + " " + "116.7".rjust(13) + " " + "50.0".rjust(11)
+                        + " " + "200.0".rjust(11))
+    # 1.25 us is a tie at one decimal: round half to even gives 1.2
+    assert lines[2].split() == ["vv_kick", "10", "1.235", "123.5", "1.2", "987654.3"]
+    for line in lines[1:]:
+        cols = line.split()
+        assert [len(c.split(".")[1]) for c in cols[2:]] == [3, 1, 1, 1]
+
+
+def test_timings_create_new_and_missing_parent(tmp_path):
+    """tests/timings.rs:824-849: an existing file is kept and reported; a
+    missing parent directory is an I/O error."""
+    p = tmp_path / "run.timings"
+    p.write_text("existing")
+    with pytest.raises(HmdError) as e:
+        api.write_timings(p, [])
+    assert e.value.code == HMD_ERR_OUTPUT_EXISTS
+    assert p.read_text() == "existing"
+    with pytest.raises(HmdError):
+        api.write_timings(tmp_path / "missing" / "run.timings", [])
