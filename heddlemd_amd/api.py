@@ -107,6 +107,10 @@ class ParticleBuffers:
         check(lib().hmd_particles_kinetic_energy(self.h, ctypes.byref(ke)))
         return ke.value
 
+    def upload_images(self, images):
+        im = np.ascontiguousarray(images, np.int32).reshape(-1, 3)
+        check(lib().hmd_particles_upload_images(self.h, _ptr(im, _i32p)))
+
     def upload_forces(self, force):
         f = np.ascontiguousarray(force, np.float32).reshape(-1, 3)
         check(lib().hmd_particles_upload_forces(self.h, _ptr(f, _fp)))

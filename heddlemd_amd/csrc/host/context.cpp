@@ -222,6 +222,18 @@ hmd_status hmd_particles_upload(hmd_particles* hp, const float* posq, const floa
   });
 }
 
+hmd_status hmd_particles_upload_images(hmd_particles* hp, const int32_t* images) {
+  HMD_GUARD({
+    if (!hp || !images) throw Error(HMD_ERR_INVALID, "null argument");
+    Particles& p = hp->p;
+    std::vector<int32_t> soa(3 * (size_t)p.n);
+    for (uint32_t i = 0; i < p.n; ++i)
+      for (int d = 0; d < 3; ++d) soa[(size_t)d * p.n + i] = images[3 * (size_t)i + d];
+    p.images.upload(soa.data(), soa.size(), p.ctx->stream);
+    check(hipStreamSynchronize(p.ctx->stream), "sync");
+  });
+}
+
 hmd_status hmd_particles_upload_forces(hmd_particles* hp, const float* force) {
   HMD_GUARD({
     if (!hp || !force) throw Error(HMD_ERR_INVALID, "null argument");

@@ -76,6 +76,9 @@ hmd_status hmd_particles_download(hmd_particles* p, float* posq, float* vel,
                                   float* force, int32_t* images);
 hmd_status hmd_particles_upload(hmd_particles* p, const float* posq,
                                 const float* vel);
+/* Image counters [n][3] (the images of an init state,
+ * src/io/init_state.rs InitImages). */
+hmd_status hmd_particles_upload_images(hmd_particles* p, const int32_t* images);
 /* Forces [n][3] into the force buffer (the forces_{x,y,z} part of
  * ParticleBuffers::upload, src/gpu/buffers.rs:252-310); the next force
  * evaluation overwrites them. */

@@ -1,0 +1,149 @@
+"""Small `.in.toml` runs written into a directory: an argon-like LJ box and
+a rigid SPC water box with SPME and SETTLE (the structure of the
+reference's examples/lj-10000-argon and examples/spc-water-* inputs, at
+test sizes)."""
+import numpy as np
+
+from heddlemd_amd import systems
+from heddlemd_amd import units as U
+
+
+def _fmt(x):
+    return f"{x:.9e}"
+
+
+def write_xyz(path, lattice_m, species, pos_m, vel=None):
+    lx, ly, lz = lattice_m
+    props = "species:S:1:pos:R:3" + (":velo:R:3" if vel is not None else "")
+    lines = [str(len(species)),
+             f'Lattice="{_fmt(lx)} 0 0 0 {_fmt(ly)} 0 0 0 {_fmt(lz)}" Properties={props}']
+    for k, (s, p) in enumerate(zip(species, pos_m)):
+        row = f"{s} {_fmt(p[0])} {_fmt(p[1])} {_fmt(p[2])}"
+        if vel is not None:
+            row += f" {_fmt(vel[k][0])} {_fmt(vel[k][1])} {_fmt(vel[k][2])}"
+        lines.append(row)
+    path.write_text("\n".join(lines) + "\n")
+
+
+def argon_run(d, n_side=8, n_steps=20, lossless=False, extra=""):
+    L = 2.7 * n_side * 1e-10 * 1.3
+    a = L / n_side
+    idx = np.stack(np.meshgrid(*[np.arange(n_side)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    pos = (idx + 0.5) * a - 0.5 * L
+    write_xyz(d / "argon.in.xyz", (L, L, L), ["Ar"] * len(pos), pos)
+    (d / "argon.in.toml").write_text(f"""schema_version = 1
+init = "argon.in.xyz"
+
+[simulation]
+seed = 1
+temperature = 100.0
+
+[[phase]]
+name = "run"
+n_steps = {n_steps}
+dt = 2.0e-15
+
+[phase.integrator]
+kind = "velocity-verlet"
+lossless = {str(lossless).lower()}
+
+[phase.output]
+trajectory_every = 10
+include_velocities = true
+log_every = 5
+
+[[particle_types]]
+name = "Ar"
+mass = 6.6335e-26
+
+[[pair_interactions]]
+between = ["Ar", "Ar"]
+kind = "lennard-jones"
+sigma = 3.40e-10
+epsilon = 1.65e-21
+cutoff = 8.5e-10
+r_switch = 8.5e-10
+{extra}""")
+    return d / "argon.in.toml"
+
+
+def water_run(d, grid=(8, 8, 8), n_steps=20, kind="langevin-baoab"):
+    w = systems.spc_water(grid=grid, seed=3)
+    L_m = w["L"] * U.LENGTH_BOHR_TO_M
+    pos = w["posq"][:, :3].astype(np.float64) * U.LENGTH_BOHR_TO_M
+    write_xyz(d / "water.in.xyz", (L_m,) * 3, ["O", "H", "H"] * w["n_mol"], pos)
+    rows = "\n".join(f"{3 * m} {3 * m + 1} {3 * m + 2} SPC" for m in range(w["n_mol"]))
+    (d / "water.in.topology").write_text("# rigid SPC\n[constraints]\n" + rows + "\n")
+    integ = ('kind = "langevin-baoab"\nfriction = 1.0e12\ntemperature = 300.0\nseed = 7'
+             if kind == "langevin-baoab" else 'kind = "velocity-verlet"\nlossless = true')
+    (d / "water.in.toml").write_text(f"""schema_version = 1
+init = "water.in.xyz"
+topology = "water.in.topology"
+
+[simulation]
+seed = 5
+temperature = 300.0
+
+[[minimization]]
+name = "min"
+[minimization.algorithm]
+kind = "steepest-descent"
+max_iterations = 30
+
+[[phase]]
+name = "md"
+n_steps = {n_steps}
+dt = 2.0e-15
+[phase.integrator]
+{integ}
+[phase.output]
+trajectory_every = 10
+log_every = 10
+
+[[particle_types]]
+name = "O"
+mass = {15.9994 * U.DALTON_KG!r}
+charge = {-0.82 * U.CHARGE_E_TO_C!r}
+
+[[particle_types]]
+name = "H"
+mass = {1.008 * U.DALTON_KG!r}
+charge = {0.41 * U.CHARGE_E_TO_C!r}
+
+[[pair_interactions]]
+between = ["O", "O"]
+kind = "lennard-jones"
+sigma = 3.166e-10
+epsilon = {0.650e3 / 6.02214076e23!r}
+cutoff = 9.0e-10
+
+[[pair_interactions]]
+between = ["H", "H"]
+kind = "lennard-jones"
+sigma = 1.0e-10
+epsilon = 0.0
+cutoff = 9.0e-10
+
+[[pair_interactions]]
+between = ["H", "O"]
+kind = "lennard-jones"
+sigma = 1.0e-10
+epsilon = 0.0
+cutoff = 9.0e-10
+
+[spme]
+alpha = 3.47e9
+r_cut_real = 9.0e-10
+grid = [24, 24, 24]
+
+[[constraint_types]]
+name = "SPC"
+kind = "settle"
+d_OH = 1.0e-10
+d_HH = 1.633e-10
+
+[neighbor_list]
+mode = "cell-list"
+r_skin = 1.0e-10
+""")
+    return d / "water.in.toml"
