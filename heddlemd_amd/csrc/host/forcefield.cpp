@@ -371,6 +371,7 @@ void ForceField::init_neighbor_list(const hmd_forcefield_desc& d) {
   bbox.alloc(8 * (size_t)n_clusters);
   sub_bbox.alloc(64 * (size_t)n_clusters);
   entry_count.alloc(n_clusters);
+  excl_entries.alloc(n_clusters);
   entry_offset.alloc(n_clusters + 1);
   n_work.alloc(1);
   acc.alloc(5 * (size_t)n_pad);
@@ -380,6 +381,12 @@ void ForceField::init_neighbor_list(const hmd_forcefield_desc& d) {
 }
 
 void ForceField::alloc_entries(uint32_t cap) {
+  // the list build stages a cluster's list in LDS (cap * 64 u32 beside its
+  // ~29 KiB of static LDS, within the 64 KiB a workgroup may use)
+  if ((size_t)cap * HMD_ENTRY * 4 > kNblListLdsBytes)
+    throw Error(HMD_ERR_UNSUPPORTED,
+                "neighbour list needs " + std::to_string(cap) + " entries of 64 j-atoms per "
+                "cluster (at most " + std::to_string(kNblListLdsBytes / (HMD_ENTRY * 4)) + ")");
   cap_entries = cap;
   const size_t slots_total = (size_t)n_clusters * cap * HMD_ENTRY;
   entry_j.alloc(slots_total, false);
@@ -547,13 +554,15 @@ void ForceField::rebuild_chain() {
   bd.entry_j = entry_j.ptr;
   bd.entry_excl = entry_excl.ptr;
   bd.entry_count = entry_count.ptr;
+  bd.excl_entries = excl_entries.ptr;
   bd.excl_offsets = excl_off.ptr;
   bd.excl_partners = excl_par.ptr;
   bd.status = status.ptr;
 #ifndef HMD_NBL_WAVES
 #define HMD_NBL_WAVES 4u
 #endif
-  launch(c, K_hmd_nbl_build, n_clusters, 64 * HMD_NBL_WAVES, 0, bd);
+  launch(c, K_hmd_nbl_build, n_clusters, 64 * HMD_NBL_WAVES,
+         (size_t)cap_entries * HMD_ENTRY * 4, bd);
   entry_scan.run(c, entry_count.ptr, entry_offset.ptr, gate, HMD_STATUS_DISPLACED);
   NblWorkParams wp{entry_count.ptr, entry_offset.ptr, n_clusters, cap_entries,
                    work.ptr, n_work.ptr, status.ptr};
@@ -578,6 +587,7 @@ void ForceField::pair_forces(bool fev) {
   pp.n_work = n_work.ptr;
   pp.entry_j = entry_j.ptr;
   pp.entry_excl = entry_excl.ptr;
+  pp.excl_entries = excl_entries.ptr;
   pp.bbox = bbox.ptr;
   pp.L = p.lattice;
   pp.n = n;

@@ -6,6 +6,10 @@ namespace hmd {
 
 struct Mlp;
 
+// LDS the list build may use for a cluster's staged j list (64 KiB per
+// workgroup minus the kernel's static arrays).
+constexpr size_t kNblListLdsBytes = 65536 - 29696;
+
 struct ForceField {
   Context& c;
   Particles& p;
@@ -40,6 +44,7 @@ struct ForceField {
   DevBuf<int32_t> image_sorted;   // int4[n_pad]
   DevBuf<uint32_t> entry_j, entry_count, entry_offset, work, n_work;
   DevBuf<uint64_t> entry_excl;
+  DevBuf<uint32_t> excl_entries;   // [n_clusters]
   DevBuf<long long> acc;            // [5][n_pad] fixed-point accumulators
   Scanner cell_scan, entry_scan;
   bool graph_invalid = false;

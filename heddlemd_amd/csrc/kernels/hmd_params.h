@@ -96,6 +96,7 @@ struct NblBuildParams {
   uint32_t* entry_j;           // [n_clusters*cap_entries*64]
   uint64_t* entry_excl;        // [n_clusters*cap_entries*64]
   uint32_t* entry_count;       // [n_clusters] (entries written / needed)
+  uint32_t* excl_entries;      // [n_clusters] leading entries with masks
   const uint32_t* excl_offsets;  // canonical CSR [n+1]
   const uint32_t* excl_partners;
   uint32_t* status;
@@ -143,6 +144,7 @@ struct PairParams {
   const uint32_t* n_work;
   const uint32_t* entry_j;
   const uint64_t* entry_excl;
+  const uint32_t* excl_entries;  // [n_clusters]: masks exist for entries below
   const float* bbox;           // cluster centres + half extents
   HmdLattice L;
   uint32_t n, cap_entries;

@@ -316,6 +316,7 @@ hmd_nbl_build(const NblBuildParams p) {
   __shared__ uint64_t s_hit[HMD_NBL_CAND_CAP];
   __shared__ uint32_t s_off[HMD_NBL_CAND_CAP];
   __shared__ uint32_t s_fill;
+  extern __shared__ uint32_t s_list[];   // [cap_entries * 64]: the cluster's j list
   const uint32_t c = blockIdx.x;
   const uint32_t wave = threadIdx.x / HMD_WAVE;
   const uint32_t lane = hmd_lane();
@@ -524,15 +525,13 @@ hmd_nbl_build(const NblBuildParams p) {
     }
     __syncthreads();
 
-    // placement, all waves: lane k of candidate q writes its j-atom at list
-    // position s_off[q] + (hits of q below lane k)
+    // placement, all waves: lane k of candidate q puts its j-atom at list
+    // position s_off[q] + (hits of q below lane k) of the LDS list
     for (uint32_t k = wave; k < cnt; k += HMD_NBL_WAVES) {
       const uint64_t H = s_hit[k];
       if ((H >> lane) & 1ull) {
         const uint32_t pos = s_off[k] + (uint32_t)__builtin_popcountll(H & lt_mask);
-        if (pos < cap_slots)
-          p.entry_j[base_entry * HMD_ENTRY + pos] =
-              (s_cand[k] & 0x7FFFFFFFu) * HMD_CLUSTER + lane;
+        if (pos < cap_slots) s_list[pos] = (s_cand[k] & 0x7FFFFFFFu) * HMD_CLUSTER + lane;
       }
     }
     __syncthreads();
@@ -540,13 +539,25 @@ hmd_nbl_build(const NblBuildParams p) {
   const uint32_t fill = s_fill;
   const uint32_t n_entries = (fill + HMD_ENTRY - 1) / HMD_ENTRY;
   const uint32_t n_written = min(n_entries, p.cap_entries);
-  // empty slots of the last entry, and the exclusion masks of every entry
-  if (wave == 1 && n_entries > 0 && n_entries <= p.cap_entries) {
-    const uint32_t last = (n_entries - 1) * HMD_ENTRY + lane;
-    if (last >= fill) p.entry_j[base_entry * HMD_ENTRY + last] = HMD_SENTINEL;
+  // whole entries out of LDS, one coalesced 256-byte row per wave and entry
+  // (scattered partial-row stores of each candidate's hits straight to
+  // memory took half of the build's time); empty slots get the sentinel
+  for (uint32_t e = wave; e < n_written; e += HMD_NBL_WAVES) {
+    const uint32_t pos = e * HMD_ENTRY + lane;
+    p.entry_j[base_entry * HMD_ENTRY + pos] = pos < fill ? s_list[pos] : HMD_SENTINEL;
   }
   if (wave == 0) {
-    for (uint32_t e = 0; e < n_written; ++e) {
+    // Exclusion masks only for the leading entries that hold a partner (the
+    // own cluster and its neighbours come first in the list); the force
+    // kernels read no mask past excl_entries[c].
+    uint32_t e_hi = 0;
+#pragma unroll
+    for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m)
+      if (ex_pos[m] != HMD_SENTINEL) e_hi = max(e_hi, ex_pos[m] / HMD_ENTRY + 1u);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) e_hi = max(e_hi, (uint32_t)__shfl_xor((int)e_hi, o, HMD_WAVE));
+    const uint32_t n_ex = min(e_hi, n_written);
+    for (uint32_t e = 0; e < n_ex; ++e) {
       uint64_t mask = 0;
 #pragma unroll
       for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m)
@@ -556,6 +567,7 @@ hmd_nbl_build(const NblBuildParams p) {
     }
     if (lane == 0) {
       p.entry_count[c] = n_written;
+      p.excl_entries[c] = n_ex;
       atomicMax(&p.status[2], n_entries);   // high-water mark: the host resizes
       if (n_entries > p.cap_entries) atomicOr(p.status, HMD_STATUS_ENTRY_OVERFLOW);
     }

@@ -168,12 +168,13 @@ template <bool FEV, bool LJ, bool COUL, bool SHIFT>
 __device__ inline void hmd_pair_entry(const PairParams& p, const float4* s_lj,
                                       const float4* s_ct, float4* s_jp, uint2* s_jt, HmdPairState& st,
                                       uint32_t slot, float cx, float cy, float cz,
-                                      uint32_t c_end, uint32_t lane) {
+                                      uint32_t c_end, bool has_ex, uint32_t lane) {
   const float4* ps = reinterpret_cast<const float4*>(p.posq_sorted);
   const HmdLattice L = p.L;
   const size_t ebase = (size_t)slot * HMD_ENTRY;
   const uint32_t my_j = p.entry_j[ebase + lane];
-  const uint64_t my_ex = p.entry_excl[ebase + lane];
+  // masks exist only for the cluster's leading entries that hold a partner
+  const uint64_t my_ex = has_ex ? p.entry_excl[ebase + lane] : 0ull;
   const bool valid_j = my_j != HMD_SENTINEL;
   const uint32_t jj_l = valid_j ? my_j : 0u;
   float4 pj = ps[jj_l];
@@ -250,6 +251,7 @@ __device__ inline void hmd_pair_body(const PairParams& p) {
 
   HmdPairState st{};
   uint32_t cur_c = HMD_SENTINEL;
+  uint32_t cur_nex = 0;
   bool shift = false;
   float cx = 0, cy = 0, cz = 0;
   for (uint32_t w = w0; w < w1; ++w) {
@@ -258,6 +260,7 @@ __device__ inline void hmd_pair_body(const PairParams& p) {
     if (c != cur_c) {
       if (cur_c != HMD_SENTINEL) hmd_flush_i<FEV>(p, st);
       cur_c = c;
+      cur_nex = p.excl_entries[c];
       st.si = c * HMD_CLUSTER + lane;
       st.vi = st.si < p.n;
       const float4 a = ps[st.si];
@@ -282,12 +285,13 @@ __device__ inline void hmd_pair_body(const PairParams& p) {
       st.fix = st.fiy = st.fiz = st.eiv = st.wiv = 0.0f;
     }
     const uint32_t wv = threadIdx.x / HMD_WAVE;
+    const bool has_ex = slot - c * p.cap_entries < cur_nex;
     if (shift)
       hmd_pair_entry<FEV, LJ, COUL, true>(p, s_lj, s_ct, s_jp[wv], s_jt[wv], st, slot, cx, cy, cz,
-                                          (c + 1u) * HMD_CLUSTER, lane);
+                                          (c + 1u) * HMD_CLUSTER, has_ex, lane);
     else
       hmd_pair_entry<FEV, LJ, COUL, false>(p, s_lj, s_ct, s_jp[wv], s_jt[wv], st, slot, cx, cy, cz,
-                                           (c + 1u) * HMD_CLUSTER, lane);
+                                           (c + 1u) * HMD_CLUSTER, has_ex, lane);
   }
   hmd_flush_i<FEV>(p, st);
 }
