@@ -72,6 +72,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def overflow_safe(run):
+    """run(n) that re-runs a batch which reported a neighbour-list overflow
+    (HMD_ERR_NBL_OVERFLOW: the library has already grown the capacity and
+    the batch is to be re-run, as the C ABI specifies).  Returns the number
+    of re-runs through `run.reruns`."""
+    from heddlemd_amd._lib import HMD_ERR_NBL_OVERFLOW, HmdError
+
+    def wrapped(n):
+        for attempt in range(3):
+            try:
+                return run(n)
+            except HmdError as e:
+                if e.code != HMD_ERR_NBL_OVERFLOW or attempt == 2:
+                    raise
+                wrapped.reruns += 1
+                log(f"neighbour list overflow, batch re-run: {e}")
+    wrapped.reruns = 0
+    return wrapped
+
+
 def build_system(api, systems, U, mesh=128, seed=11, minimize_iter=200):
     sysw = systems.spc_water(grid=(64, 64, 16), seed=seed)
     ff = systems.water_forcefield(grid=(mesh,) * 3, exclusions=sysw["exclusions"])
@@ -148,7 +168,7 @@ def main():
     t_setup = time.perf_counter()
     ctx, sysw, ff, part, field, integ = build_system(api, systems, U)
     log(f"setup {time.perf_counter() - t_setup:.1f} s")
-    run = integ.step if args.eager else integ.replay
+    run = overflow_safe(integ.step if args.eager else integ.replay)
     if not args.eager:
         integ.capture()
     run(args.warmup)
@@ -216,6 +236,7 @@ def main():
                    "launch": "eager" if args.eager else "hipGraph replay",
                    "atoms": int(sysw["n"]),
                    "atom_steps_per_s": round(sysw["n"] * 1e3 / ms_val, 1),
+                   "overflow_reruns": run.reruns,
                    "plain_128": plain, "nn_64": nn},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -320,7 +341,7 @@ def nn_leg(api, systems, U, args, ctx, sysw, part, fine, integ_fine, run_fine):
     net.attach(coarse)
     integ = api.Integrator(ctx, part, coarse, dt=DT_FS * U.FS, compensated=True,
                            settle=sysw["settle"])
-    run = integ.step if args.eager else integ.replay
+    run = overflow_safe(integ.step if args.eager else integ.replay)
     if not args.eager:
         integ.capture()
     run(args.warmup)

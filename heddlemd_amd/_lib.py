@@ -20,6 +20,7 @@ HMD_OK = 0
 STATUS_NAMES = {1: "HMD_ERR_GPU", 2: "HMD_ERR_INVALID", 3: "HMD_ERR_NBL_OVERFLOW",
                 4: "HMD_ERR_BOX_TOO_SMALL", 5: "HMD_ERR_UNSUPPORTED", 6: "HMD_ERR_NO_DEVICE",
                 7: "HMD_ERR_OUTPUT_EXISTS", 8: "HMD_ERR_IO"}
+HMD_ERR_NBL_OVERFLOW = 3
 HMD_ERR_OUTPUT_EXISTS = 7
 UNITS = {"si": 0, "atomic": 1}
 # Dimension (src/units/mod.rs), in the order of include/heddle_hip.h

@@ -133,7 +133,7 @@ static const KernelId kP2C2r[5] = {K_hmd_fft_c2r_16, K_hmd_fft_c2r_32, K_hmd_fft
 // Workgroups per CU the FFT passes aim for (several, so that one group's
 // loads overlap another's transform).
 #ifndef HMD_FFT_BLOCKS_PER_CU
-#define HMD_FFT_BLOCKS_PER_CU 2
+#define HMD_FFT_BLOCKS_PER_CU 3
 #endif
 
 // `tiles` LDS tiles of n x W complex values per workgroup (the a-axis pass

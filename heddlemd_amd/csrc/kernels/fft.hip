@@ -444,7 +444,9 @@ __device__ inline void hmd_fft_cols(float2* tile, uint32_t W, const float2* tw) 
 // any of them is waited for: a plain strided loop waits for each load before
 // its LDS store, which serialises the memory latency (~10 round trips per
 // tile).  src(t) returns element t, dst(t) its LDS index.
+#ifndef HMD_FFT_BATCH
 #define HMD_FFT_BATCH 8
+#endif
 template <typename Src, typename Dst>
 __device__ inline void hmd_fill_tile(float2* tile, uint32_t total, Src src, Dst dst) {
   for (uint32_t t0 = threadIdx.x; t0 < total; t0 += HMD_FFT_BATCH * blockDim.x) {
