@@ -28,7 +28,7 @@ struct ForceField {
   void build_coulomb_table(double alpha, double r_cut, hipStream_t s);
   DevBuf<uint32_t> excl_off, excl_par;
 #ifndef HMD_PAIR_BLOCKS_PER_CU
-#define HMD_PAIR_BLOCKS_PER_CU 24
+#define HMD_PAIR_BLOCKS_PER_CU 32
 #endif
   uint32_t pair_blocks_per_cu = HMD_PAIR_BLOCKS_PER_CU;
 
