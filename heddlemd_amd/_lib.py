@@ -40,6 +40,8 @@ EXPORTED = [
     "hmd_vv_kick_drift", "hmd_vv_kick", "hmd_lan_drift_half", "hmd_lan_ou_step",
     "hmd_replicas_create", "hmd_replicas_destroy", "hmd_replicas_step", "hmd_replicas_steps",
     "hmd_integrator_timings", "hmd_timings_write",
+    "hmd_settle_create", "hmd_settle_destroy", "hmd_settle_before_drift",
+    "hmd_settle_after_drift", "hmd_settle_after_kick",
     "hmd_forcefield_create", "hmd_forcefield_destroy", "hmd_forcefield_compute",
     "hmd_forcefield_slots", "hmd_spme_scalars", "hmd_spme_grids", "hmd_spme_forces",
     "hmd_nbl_request_rebuild", "hmd_nbl_pair_snapshot", "hmd_nbl_count_pairs",
@@ -145,6 +147,12 @@ def _declare(L):
         "hmd_vv_kick": (st, [vp, ctypes.c_float, ctypes.c_int32]),
         "hmd_lan_drift_half": (st, [vp, ctypes.c_float]),
         "hmd_replicas_create": (st, [ctypes.POINTER(vp), ctypes.c_uint32, ctypes.c_uint32, pvp]),
+        "hmd_settle_create": (st, [vp, ctypes.c_uint32, _c_u32p] + [ctypes.c_double] * 4
+                              + [pvp]),
+        "hmd_settle_destroy": (None, [vp]),
+        "hmd_settle_before_drift": (st, [vp]),
+        "hmd_settle_after_drift": (st, [vp, ctypes.c_float]),
+        "hmd_settle_after_kick": (st, [vp, ctypes.c_float]),
         "hmd_integrator_timings": (st, [vp, ctypes.c_uint32, ctypes.c_int32,
                                         ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p,
                                         ctypes.c_int32] + [ctypes.POINTER(ctypes.c_uint64)] * 4),

@@ -323,6 +323,31 @@ class Integrator:
             self.h = None
 
 
+class Settle:
+    """SETTLE constraint hooks as operators (src/integrator/settle.rs:310-404)."""
+
+    def __init__(self, particles, groups, d_oh, d_hh, m_o, m_h):
+        self._g = np.ascontiguousarray(groups, np.uint32).reshape(-1, 3)
+        h = ctypes.c_void_p()
+        check(lib().hmd_settle_create(particles.h, self._g.shape[0], _ptr(self._g, _u32p),
+                                      d_oh, d_hh, m_o, m_h, ctypes.byref(h)))
+        self.h = h
+
+    def before_drift(self):
+        check(lib().hmd_settle_before_drift(self.h))
+
+    def after_drift(self, dt):
+        check(lib().hmd_settle_after_drift(self.h, dt))
+
+    def after_kick(self, dt):
+        check(lib().hmd_settle_after_kick(self.h, dt))
+
+    def close(self):
+        if self.h:
+            lib().hmd_settle_destroy(self.h)
+            self.h = None
+
+
 class ReplicaSet:
     """Independent systems stepped as one batch (hmd_replicas_*).  Each
     integrator must come from its own GpuContext; fused=True captures every

@@ -385,6 +385,30 @@ struct hmd_replicas {
   hmd::ReplicaSet* set = nullptr;
 };
 
+// A SETTLE constraint set on its own: the three hooks of the reference's
+// Constraint interface (src/integrator/settle.rs:310-404) as operators.
+struct hmd_settle {
+  hmd::Particles* p = nullptr;
+  hmd::DevBuf<uint32_t> group_atoms;
+  hmd::DevBuf<float> snapshot, constraint_virial;
+  uint32_t n_groups = 0;
+  float ra = 0, rb = 0, rc = 0, m_o = 0, m_h = 0;
+  SettleParams params(float dt) const {
+    SettleParams s{};
+    s.posq = p->posq.ptr;
+    s.v = p->vel.ptr;
+    s.snapshot = snapshot.ptr;
+    s.group_atoms = group_atoms.ptr;
+    s.n_groups = n_groups;
+    s.n = p->n;
+    s.ra = ra; s.rb = rb; s.rc = rc; s.m_o = m_o; s.m_h = m_h;
+    s.L = p->lattice;
+    s.dt = dt;
+    s.constraint_virial = constraint_virial.ptr;
+    return s;
+  }
+};
+
 using namespace hmd;
 
 // ------------------------------------------------------ velocity set-up ---
@@ -519,6 +543,59 @@ static hmd_status vv_half(hmd_particles* hp, float dt, int32_t compensated, bool
     launch(*p.ctx, drift ? K_hmd_vv_kick_drift : K_hmd_vv_kick, ceil_div(p.n, 256), 256, 0, v);
     check(hipStreamSynchronize(p.ctx->stream), "sync");
   });
+}
+
+hmd_status hmd_settle_create(hmd_particles* hp, uint32_t n_groups, const uint32_t* group_atoms,
+                             double d_oh, double d_hh, double m_o, double m_h, hmd_settle** out) {
+  HMD_GUARD({
+    if (!hp || !out || (n_groups && !group_atoms)) throw Error(HMD_ERR_INVALID, "null argument");
+    if (!(d_oh > 0.0 && d_hh > 0.0 && d_hh < 2.0 * d_oh && m_o > 0.0 && m_h > 0.0))
+      throw Error(HMD_ERR_INVALID, "SETTLE geometry: need 0 < d_HH < 2 d_OH and positive masses");
+    for (uint64_t k = 0; k < 3ull * n_groups; ++k)
+      if (group_atoms[k] >= hp->p.n) throw Error(HMD_ERR_INVALID, "SETTLE atom out of range");
+    auto* h = new hmd_settle();
+    h->p = &hp->p;
+    h->n_groups = n_groups;
+    // canonical geometry (src/integrator/settle.rs:83-93)
+    const double total = m_o + 2.0 * m_h;
+    const double ht = std::sqrt(d_oh * d_oh - 0.25 * d_hh * d_hh);
+    h->ra = (float)(2.0 * m_h / total * ht);
+    h->rb = (float)(m_o / total * ht);
+    h->rc = (float)(0.5 * d_hh);
+    h->m_o = (float)m_o;
+    h->m_h = (float)m_h;
+    if (n_groups) {
+      hipStream_t s = hp->p.ctx->stream;
+      h->group_atoms.alloc(3 * (size_t)n_groups, false);
+      h->group_atoms.upload(group_atoms, 3 * (size_t)n_groups, s);
+      h->snapshot.alloc(12 * (size_t)n_groups);
+      h->constraint_virial.alloc(3 * (size_t)n_groups);
+      check(hipStreamSynchronize(s), "sync");
+    }
+    *out = h;
+  });
+}
+
+void hmd_settle_destroy(hmd_settle* h) { delete h; }
+
+static hmd_status settle_op(hmd_settle* h, float dt, KernelId k) {
+  HMD_GUARD({
+    if (!h) throw Error(HMD_ERR_INVALID, "null settle");
+    if (h->n_groups == 0) return HMD_OK;
+    Context& c = *h->p->ctx;
+    launch(c, k, ceil_div(h->n_groups, 256), 256, 0, h->params(dt));
+    check(hipStreamSynchronize(c.stream), "sync");
+  });
+}
+
+hmd_status hmd_settle_before_drift(hmd_settle* h) {
+  return settle_op(h, 0.0f, K_hmd_settle_snapshot);
+}
+hmd_status hmd_settle_after_drift(hmd_settle* h, float dt) {
+  return settle_op(h, dt, K_hmd_settle_positions);
+}
+hmd_status hmd_settle_after_kick(hmd_settle* h, float dt) {
+  return settle_op(h, dt, K_hmd_settle_velocities);
 }
 
 hmd_status hmd_lan_drift_half(hmd_particles* hp, float dt) {

@@ -254,6 +254,21 @@ hmd_status hmd_timings_write(const char* path, uint32_t n_stages, const char* co
                              const uint64_t* counts, const uint64_t* total_ns,
                              const uint64_t* min_ns, const uint64_t* max_ns);
 
+/* ---- SETTLE as operators ---------------------------------------------------
+ * The Constraint hooks of the reference's SETTLE (src/integrator/settle.rs:
+ * 310-404): before_drift snapshots the group positions, after_drift resets
+ * each group to the rigid geometry (Miyamoto & Kollman 1992) and adds the
+ * correction / dt to the velocities, after_kick projects out the bond
+ * velocities; f32 state (the integrators use the same kernels, and the
+ * compensated forms, internally).  Groups are (O, H1, H2) atom ids. */
+typedef struct hmd_settle hmd_settle;
+hmd_status hmd_settle_create(hmd_particles* p, uint32_t n_groups, const uint32_t* group_atoms,
+                             double d_oh, double d_hh, double m_o, double m_h, hmd_settle** out);
+void hmd_settle_destroy(hmd_settle* s);
+hmd_status hmd_settle_before_drift(hmd_settle* s);
+hmd_status hmd_settle_after_drift(hmd_settle* s, float dt);
+hmd_status hmd_settle_after_kick(hmd_settle* s, float dt);
+
 /* ---- replica batches -----------------------------------------------------
  * Independent systems stepped together (the batched-replica workload; no
  * counterpart in the reference, which runs one system per process).  Each

@@ -338,6 +338,80 @@ def test_settle_velocity_projection_zeroes_bond_velocities():
     assert np.allclose((m * v2).sum(0), (m * v).sum(0), rtol=1e-12, atol=1e-18)
 
 
+def _shake(snap, cur, m, d, tol=1e-13, max_iter=100000):
+    """Textbook SHAKE (Ryckaert, Ciccotti & Berendsen 1977): displace along
+    the reference bond vectors until every |r_a - r_b| = d_ab."""
+    r = cur.copy()
+    bonds = [(0, 1, d[0]), (0, 2, d[1]), (1, 2, d[2])]
+    for _ in range(max_iter):
+        worst = 0.0
+        for a, b, dd in bonds:
+            s = r[a] - r[b]
+            diff = dd * dd - s @ s
+            worst = max(worst, abs(diff) / (dd * dd))
+            r0 = snap[a] - snap[b]
+            g = diff / (2.0 * (s @ r0) * (1.0 / m[a] + 1.0 / m[b]))
+            r[a] += g * r0 / m[a]
+            r[b] -= g * r0 / m[b]
+        if worst < tol:
+            return r
+    raise AssertionError("SHAKE did not converge")
+
+
+def test_settle_positions_equal_converged_shake():
+    """SETTLE solves the SHAKE equations in closed form: the reset positions
+    equal the converged iterative solution (tests/settle.rs:930), and the
+    velocity correction is (reset - unconstrained) / dt."""
+    rng = np.random.default_rng(12)
+    m_o, m_h = 15.9994 * U.AMU, 1.008 * U.AMU
+    m = np.array([m_o, m_h, m_h])
+    lat = [60, 60, 60, 0, 0, 0]
+    dt = 80.0
+    for _ in range(20):
+        snap, d_oh, d_hh = _water(rng)
+        geom = oracle.settle_geometry(d_oh, d_hh, m_o, m_h)
+        cur = snap + rng.standard_normal((3, 3)) * 0.03
+        v = rng.standard_normal((3, 3)) * 1e-3
+        new, v2, _ = oracle.settle_positions_one(snap, cur, v, m_o, m_h, geom, dt, lat)
+        ref = _shake(snap, cur, m, (d_oh, d_oh, d_hh))
+        assert np.max(np.abs(new - ref)) < 1e-9 * d_oh
+        assert np.allclose(v2, v + (new - cur) / dt, rtol=0, atol=1e-12)
+
+
+def test_settle_velocities_equal_linear_constraint_solve():
+    """The velocity projection is the RATTLE step: impulses along the three
+    bonds from the 3x3 linear system that zeroes every relative velocity
+    along its bond (solved here with numpy)."""
+    rng = np.random.default_rng(13)
+    m_o, m_h = 15.9994 * U.AMU, 1.008 * U.AMU
+    m = np.array([m_o, m_h, m_h])
+    bonds = [(0, 1), (0, 2), (1, 2)]
+    for _ in range(20):
+        pos, _, _ = _water(rng)
+        v = rng.standard_normal((3, 3)) * 1e-3
+        got = oracle.settle_velocities_one(pos, v, m_o, m_h, [60, 60, 60, 0, 0, 0])
+        r = [pos[a] - pos[b] for a, b in bonds]
+        A = np.zeros((3, 3))
+        rhs = np.zeros(3)
+        for k, (a, b) in enumerate(bonds):
+            rhs[k] = -(v[a] - v[b]) @ r[k]
+            for l, (c, d) in enumerate(bonds):
+                # d/dg_l of (v_a - v_b) . r_k with v_c += g_l r_l / m_c, v_d -= g_l r_l / m_d
+                coef = 0.0
+                for atom, sgn in ((c, 1.0), (d, -1.0)):
+                    if atom == a:
+                        coef += sgn / m[atom]
+                    if atom == b:
+                        coef -= sgn / m[atom]
+                A[k, l] = coef * (r[l] @ r[k])
+        g = np.linalg.solve(A, rhs)
+        want = v.copy()
+        for l, (c, d) in enumerate(bonds):
+            want[c] += g[l] * r[l] / m[c]
+            want[d] -= g[l] * r[l] / m[d]
+        assert np.max(np.abs(got - want)) < 1e-12 * np.max(np.abs(v)) + 1e-18
+
+
 def test_kinetic_energy_closed_form():
     v = np.array([[1.0, 2.0, 3.0], [0.5, 0, 0]])
     m = np.array([2.0, 4.0])
