@@ -40,6 +40,7 @@ EXPORTED = [
     "hmd_vv_kick_drift", "hmd_vv_kick", "hmd_lan_drift_half", "hmd_lan_ou_step",
     "hmd_replicas_create", "hmd_replicas_destroy", "hmd_replicas_step", "hmd_replicas_steps",
     "hmd_integrator_timings", "hmd_timings_write",
+    "hmd_forcefield_set_pair_fragment", "hmd_pair_fragment_compile",
     "hmd_settle_create", "hmd_settle_destroy", "hmd_settle_before_drift",
     "hmd_settle_after_drift", "hmd_settle_after_kick",
     "hmd_forcefield_create", "hmd_forcefield_destroy", "hmd_forcefield_compute",
@@ -150,6 +151,10 @@ def _declare(L):
         "hmd_settle_create": (st, [vp, ctypes.c_uint32, _c_u32p] + [ctypes.c_double] * 4
                               + [pvp]),
         "hmd_settle_destroy": (None, [vp]),
+        "hmd_forcefield_set_pair_fragment": (st, [vp, ctypes.c_char_p, ctypes.c_char_p, _c_fp,
+                                                  ctypes.c_uint32]),
+        "hmd_pair_fragment_compile": (st, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                           ctypes.c_int32]),
         "hmd_settle_before_drift": (st, [vp]),
         "hmd_settle_after_drift": (st, [vp, ctypes.c_float]),
         "hmd_settle_after_kick": (st, [vp, ctypes.c_float]),

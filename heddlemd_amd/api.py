@@ -218,6 +218,13 @@ class ForceField:
         check(lib().hmd_spme_forces(self.h, _ptr(f, _fp)))
         return f
 
+    def set_pair_fragment(self, name, source, params=()):
+        """Compose a user pair term into this force field's kernels
+        (hmd_forcefield_set_pair_fragment)."""
+        prm = np.ascontiguousarray(list(params) or [0.0], np.float32)
+        check(lib().hmd_forcefield_set_pair_fragment(self.h, name.encode(), source.encode(),
+                                                     _ptr(prm, _fp), len(params)))
+
     def request_rebuild(self):
         check(lib().hmd_nbl_request_rebuild(self.h))
 
@@ -526,6 +533,13 @@ class TrajectoryWriter:
         if self.h:
             lib().hmd_traj_close(self.h)
             self.h = None
+
+
+def compile_pair_fragment(name, source):
+    """Compile a pair fragment without a device: (ok, compiler log)."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    st = lib().hmd_pair_fragment_compile(name.encode(), source.encode(), buf, len(buf))
+    return st == 0, buf.value.decode(errors="replace")
 
 
 def write_timings(path, stages):

@@ -609,7 +609,15 @@ void ForceField::pair_forces(bool fev) {
   KernelId k = lj_on && coul_on ? (fev ? K_hmd_pair_force_fev : K_hmd_pair_force_f)
                : lj_on ? (fev ? K_hmd_pair_force_lj_fev : K_hmd_pair_force_lj_f)
                        : (fev ? K_hmd_pair_force_coul_fev : K_hmd_pair_force_coul_f);
-  launch(c, k, blocks, 256, 0, pp);
+  pp.user_params = user_params.ptr;
+  if (jit) {
+    void* args[] = {&pp};
+    check(hipModuleLaunchKernel(pair_function(k), blocks, 1, 1, 256, 1, 1, 0, c.stream, args,
+                                nullptr),
+          "composed pair kernel");
+  } else {
+    launch(c, k, blocks, 256, 0, pp);
+  }
   c.mark("pair_force");
 }
 

@@ -1,10 +1,23 @@
 // ForceField host object: potential slots, neighbour list, SPME mesh.
 #pragma once
+#include <memory>
+
 #include "hmd_internal.h"
 
 namespace hmd {
 
 struct Mlp;
+
+// Pair kernels recompiled at run time with a user pair term (compose.cpp).
+struct JitPairModule {
+  hipModule_t module = nullptr;
+  KernelId kernel[6] = {};
+  hipFunction_t fn[6] = {};
+  JitPairModule() = default;
+  JitPairModule(const JitPairModule&) = delete;
+  JitPairModule& operator=(const JitPairModule&) = delete;
+  ~JitPairModule();
+};
 
 // LDS the list build may use for a cluster's staged j list (64 KiB per
 // workgroup minus the kernel's static arrays).
@@ -31,6 +44,12 @@ struct ForceField {
 #define HMD_PAIR_BLOCKS_PER_CU 32
 #endif
   uint32_t pair_blocks_per_cu = HMD_PAIR_BLOCKS_PER_CU;
+  // run-time composed pair term (null: the built-in kernels)
+  std::unique_ptr<JitPairModule> jit;
+  DevBuf<float> user_params;
+  void set_pair_fragment(const std::string& name, const std::string& source,
+                         const float* params, uint32_t n_params);
+  hipFunction_t pair_function(KernelId k) const;
 
   // ---- neighbour list
   float r_skin = 0, r_search = 0, r_search_sq = 0;

@@ -1,7 +1,9 @@
 // Device helpers shared by every gfx950 kernel: periodic boundaries,
 // fixed-point conversion, wave64 primitives.
 #pragma once
+#ifndef __HIPCC_RTC__   // hiprtc provides the runtime declarations itself
 #include <hip/hip_runtime.h>
+#endif
 #include "hmd_params.h"
 
 #define HMD_WAVE 64

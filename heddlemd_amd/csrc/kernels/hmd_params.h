@@ -7,7 +7,15 @@
 //
 // Plain C layout only: pointers, 32/64-bit integers, float, double.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <stdint.h>
+#else   // hiprtc: its runtime header keeps the fixed-width types in a namespace
+using __hip_internal::int32_t;
+using __hip_internal::uint16_t;
+using __hip_internal::int64_t;
+using __hip_internal::uint32_t;
+using __hip_internal::uint64_t;
+#endif
 
 #define HMD_CLUSTER 64u             // atoms per cluster = lanes per wave
 #define HMD_ENTRY 64u               // j-atom slots per neighbour entry
@@ -145,6 +153,7 @@ struct PairParams {
   const uint32_t* entry_j;
   const uint64_t* entry_excl;
   const uint32_t* excl_entries;  // [n_clusters]: masks exist for entries below
+  const float* user_params;    // parameters of a run-time composed pair term
   const float* bbox;           // cluster centres + half extents
   HmdLattice L;
   uint32_t n, cap_entries;

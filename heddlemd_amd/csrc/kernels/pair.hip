@@ -38,7 +38,7 @@
 
 struct HmdPairState {
   float xi, yi, zi, kqi;
-  uint32_t si, ti_row;      // sorted index; type * n_types (LJ table row)
+  uint32_t si, ti, ti_row;  // sorted index; type; type * n_types (LJ table row)
   bool vi;
   float fix, fiy, fiz, eiv, wiv;
 };
@@ -118,6 +118,16 @@ __device__ inline void hmd_pair_steps(const PairParams& p, const float4* s_lj,
           if (FEV) energy += sw * e_lj;
         }
       }
+#ifdef HMD_USER_FUNCTOR
+      {
+        // a user pair term composed in at run time: force / r and energy
+        // from (types, r^2, 1/r); masked with the built-in terms below
+        float f_u = 0.0f, e_u = 0.0f;
+        HMD_USER_FUNCTOR{}(p.user_params, st.ti, s_jt[k].y, r2s, inv_r, f_u, e_u);
+        factor += f_u;
+        if (FEV) energy += e_u;
+      }
+#endif
       factor = ok ? factor : 0.0f;
       // the i-side sums use their own FMAs, so that the j-side values are
       // dead once the lane swaps consume them (no copies)
@@ -266,7 +276,8 @@ __device__ inline void hmd_pair_body(const PairParams& p) {
       const float4 a = ps[st.si];
       st.xi = a.x; st.yi = a.y; st.zi = a.z;
       st.kqi = p.k_coulomb * a.w;
-      st.ti_row = p.type_sorted[st.si] * p.n_types;
+      st.ti = p.type_sorted[st.si];
+      st.ti_row = st.ti * p.n_types;
       const float4 cb = bb[2 * c], hb = bb[2 * c + 1];
       // Single-image fast path (compare jit_composed.rs:1396-1430): valid
       // when every atom that can interact with the cluster lies within half

@@ -254,6 +254,30 @@ hmd_status hmd_timings_write(const char* path, uint32_t n_stages, const char* co
                              const uint64_t* counts, const uint64_t* total_ns,
                              const uint64_t* min_ns, const uint64_t* max_ns);
 
+/* ---- run-time composed pair terms -------------------------------------------
+ * A user pair potential compiled into the force kernels with hiprtc (the
+ * PairForceFragment composer of src/forces/jit_composed.rs:112-151,
+ * :697-720, :1162-1643).  `functor_source` defines
+ *   struct <functor_name> {
+ *     __device__ void operator()(const float* params, uint32_t type_i,
+ *         uint32_t type_j, float r2, float inv_r,
+ *         float& force_over_r, float& energy) const;
+ *   };
+ * called for every listed pair (before the cutoff and exclusion masks,
+ * which then apply to it as to the built-in terms); force_over_r * (x_i -
+ * x_j) is added to atom i and subtracted from atom j, energy is split half
+ * and half.  `params` (n_params floats) is copied to the device.  The
+ * force field's pair kernels are replaced by the composed ones (captured
+ * graphs are re-captured).  A source that does not compile returns
+ * HMD_ERR_INVALID with the compiler log in hmd_last_error(). */
+hmd_status hmd_forcefield_set_pair_fragment(hmd_forcefield* ff, const char* functor_name,
+                                            const char* functor_source, const float* params,
+                                            uint32_t n_params);
+/* Compile a fragment without a device (validation); the compiler log goes
+ * to `log` (log_len bytes, may be 0). */
+hmd_status hmd_pair_fragment_compile(const char* functor_name, const char* functor_source,
+                                     char* log, int32_t log_len);
+
 /* ---- SETTLE as operators ---------------------------------------------------
  * The Constraint hooks of the reference's SETTLE (src/integrator/settle.rs:
  * 310-404): before_drift snapshots the group positions, after_drift resets
