@@ -36,7 +36,7 @@ EXPORTED = [
     "hmd_particles_upload", "hmd_particles_download_energies", "hmd_particles_download_lo",
     "hmd_particles_kinetic_energy", "hmd_particles_energy_sums",
     "hmd_particles_negate_velocities", "hmd_particles_upload_forces",
-    "hmd_particles_upload_images",
+    "hmd_particles_upload_images", "hmd_particles_set_box",
     "hmd_vv_kick_drift", "hmd_vv_kick", "hmd_lan_drift_half", "hmd_lan_ou_step",
     "hmd_replicas_create", "hmd_replicas_destroy", "hmd_replicas_step", "hmd_replicas_steps",
     "hmd_integrator_timings", "hmd_timings_write",
@@ -144,6 +144,7 @@ def _declare(L):
         "hmd_particles_negate_velocities": (st, [vp]),
         "hmd_particles_upload_forces": (st, [vp, _c_fp]),
         "hmd_particles_upload_images": (st, [vp, _c_i32p]),
+        "hmd_particles_set_box": (st, [vp, _c_fp]),
         "hmd_vv_kick_drift": (st, [vp, ctypes.c_float, ctypes.c_int32]),
         "hmd_vv_kick": (st, [vp, ctypes.c_float, ctypes.c_int32]),
         "hmd_lan_drift_half": (st, [vp, ctypes.c_float]),

@@ -107,6 +107,12 @@ class ParticleBuffers:
         check(lib().hmd_particles_kinetic_energy(self.h, ctypes.byref(ke)))
         return ke.value
 
+    def set_box(self, lattice):
+        """New box {lx, ly, lz, xy, xz, yz} (hmd_particles_set_box)."""
+        lat = np.ascontiguousarray(lattice, np.float32)
+        check(lib().hmd_particles_set_box(self.h, _ptr(lat, _fp)))
+        self.lattice = lat
+
     def upload_images(self, images):
         im = np.ascontiguousarray(images, np.int32).reshape(-1, 3)
         check(lib().hmd_particles_upload_images(self.h, _ptr(im, _i32p)))

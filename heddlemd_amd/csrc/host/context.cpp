@@ -3,6 +3,9 @@
 #include <algorithm>
 #include <cstdio>
 
+#include <cmath>
+
+#include "forcefield.h"
 #include "hmd_internal.h"
 
 // The gfx950 code object, embedded at link time (kernels_blob.S).
@@ -219,6 +222,28 @@ hmd_status hmd_particles_upload(hmd_particles* hp, const float* posq, const floa
       p.vel.upload(soa.data(), soa.size(), s);
     }
     check(hipStreamSynchronize(s), "sync");
+  });
+}
+
+hmd_status hmd_particles_set_box(hmd_particles* hp, const float lat[6]) {
+  HMD_GUARD({
+    if (!hp || !lat) throw Error(HMD_ERR_INVALID, "null argument");
+    for (int k = 0; k < 6; ++k)
+      if (!std::isfinite(lat[k])) throw Error(HMD_ERR_INVALID, "lattice is not finite");
+    if (!(lat[0] > 0.0f && lat[1] > 0.0f && lat[2] > 0.0f))
+      throw Error(HMD_ERR_INVALID, "lattice lengths must be positive");
+    Particles& p = hp->p;
+    const HmdLattice old = p.lattice;
+    p.lattice = HmdLattice{lat[0], lat[1], lat[2], lat[3], lat[4], lat[5]};
+    try {
+      for (ForceField* f : p.observers) f->on_box_change();
+    } catch (...) {
+      p.lattice = old;   // a box the force fields refuse leaves the old one in place
+      for (ForceField* f : p.observers) {
+        try { f->on_box_change(); } catch (...) {}
+      }
+      throw;
+    }
   });
 }
 

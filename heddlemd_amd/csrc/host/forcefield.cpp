@@ -296,22 +296,12 @@ ForceField::ForceField(Context& ctx, Particles& part, const hmd_forcefield_desc&
   if (spme_on) init_spme(d);
   check(hipStreamSynchronize(s), "sync");
   probe_capacity();
+  p.observers.push_back(this);   // box changes reach this force field
 }
 
-// compute_cell_layout analogue (src/forces/neighbor_list.rs:1563-1583).
-// The reference's cell list needs >= 3 cells of edge r_search per axis
-// (BoxTooSmallForCells otherwise).  Nothing here depends on cells: the fine
-// Hilbert grid only orders atoms and every distance test is a minimum-image
-// one.  The minimum-image distance is a metric on the periodic cell, so the
-// skin argument (a pair within r_c now was within r_c + r_skin at the
-// rebuild when neither atom moved more than r_skin / 2) holds for any box,
-// and the in-cutoff image of a pair is unique while r_c is below half of
-// every perpendicular width.  That is the only box requirement; small
-// replicas (edges down to ~1.9 nm with r_c = 0.8 nm) rely on it.
-void ForceField::init_neighbor_list(const hmd_forcefield_desc& d) {
-  r_skin = (float)d.r_skin;
-  r_search = r_cut + r_skin;
-  r_search_sq = r_search * r_search;
+// The box requirement (r_c below half of every perpendicular width) and
+// the orthorhombic flags, for the current lattice.
+void ForceField::check_box() {
   const HmdLattice& L = p.lattice;
   // perpendicular widths of the lower-triangular cell
   double wa = L.lx, wb = L.ly, wc = L.lz;  // exact for orthorhombic
@@ -337,6 +327,25 @@ void ForceField::init_neighbor_list(const hmd_forcefield_desc& d) {
   orthorhombic = (L.xy == 0.0f && L.xz == 0.0f && L.yz == 0.0f);
   ortho_fast = orthorhombic && 0.5 * L.lx > 1.001 * r_search &&
                0.5 * L.ly > 1.001 * r_search && 0.5 * L.lz > 1.001 * r_search;
+
+}
+
+// compute_cell_layout analogue (src/forces/neighbor_list.rs:1563-1583).
+// The reference's cell list needs >= 3 cells of edge r_search per axis
+// (BoxTooSmallForCells otherwise).  Nothing here depends on cells: the fine
+// Hilbert grid only orders atoms and every distance test is a minimum-image
+// one.  The minimum-image distance is a metric on the periodic cell, so the
+// skin argument (a pair within r_c now was within r_c + r_skin at the
+// rebuild when neither atom moved more than r_skin / 2) holds for any box,
+// and the in-cutoff image of a pair is unique while r_c is below half of
+// every perpendicular width.  That is the only box requirement; small
+// replicas (edges down to ~1.9 nm with r_c = 0.8 nm) rely on it.
+void ForceField::init_neighbor_list(const hmd_forcefield_desc& d) {
+  r_skin = (float)d.r_skin;
+  r_search = r_cut + r_skin;
+  r_search_sq = r_search * r_search;
+  check_box();
+  const HmdLattice& L = p.lattice;
 
   // Fine ordering grid of 2^bits cells per axis (the Hilbert curve needs a
   // power of two), at least as fine as ~1.6 atoms per cell.
@@ -478,10 +487,16 @@ void ForceField::init_spme(const hmd_forcefield_desc& d) {
     partial_blocks = ceil_div((size_t)n1 * nch, La);
   }
   partials.alloc(2 * (size_t)partial_blocks);
-  // influence table for the power-of-two a-axis pass (the box is fixed)
+  if (pass_a.p2) gv.alloc(2 * (size_t)n0 * n1 * nch, false);
+  compute_gv_table();
+}
+
+// (G, vf) influence table of the power-of-two a-axis pass for the current
+// box (rebuilt when the box changes).
+void ForceField::compute_gv_table() {
   if (pass_a.p2) {
+    const uint32_t n0 = grid[0], n1 = grid[1], n2 = grid[2], nch = n2 / 2 + 1;
     const size_t m_c = (size_t)n0 * n1 * nch;
-    gv.alloc(2 * m_c, false);
     FftParams g{};
     g.L = p.lattice;
     g.alpha = alpha;
@@ -511,6 +526,20 @@ void ForceField::probe_capacity() {
     status.upload(clear, 4, c.stream);
   }
   check(hipStreamSynchronize(c.stream), "sync");
+}
+
+// A new box (hmd_particles_set_box): re-check it, refresh what depends on
+// it, and have the next evaluation rebuild the list and re-capture graphs.
+void ForceField::on_box_change() {
+  check_box();
+  if (spme_on) compute_gv_table();
+  request_rebuild();
+  graph_invalid = true;
+}
+
+ForceField::~ForceField() {
+  auto& obs = p.observers;
+  obs.erase(std::remove(obs.begin(), obs.end(), this), obs.end());
 }
 
 void ForceField::request_rebuild() {

@@ -101,6 +101,10 @@ struct ForceField {
   void alloc_entries(uint32_t cap);
   void probe_capacity();
   void request_rebuild();
+  void check_box();
+  void compute_gv_table();
+  void on_box_change();
+  ~ForceField();
   void rebuild_chain();
   void neighbor_pre_step();
   void pair_forces(bool fev);

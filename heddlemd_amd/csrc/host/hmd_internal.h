@@ -128,8 +128,11 @@ inline void launch(Context& c, KernelId k, uint32_t grid, uint32_t block,
 
 inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+struct ForceField;
+
 struct Particles {
   Context* ctx = nullptr;
+  std::vector<ForceField*> observers;   // force fields built on these particles
   uint32_t n = 0;
   HmdLattice lattice{};
   DevBuf<float> posq;      // float4[n]

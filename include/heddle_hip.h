@@ -76,6 +76,13 @@ hmd_status hmd_particles_download(hmd_particles* p, float* posq, float* vel,
                                   float* force, int32_t* images);
 hmd_status hmd_particles_upload(hmd_particles* p, const float* posq,
                                 const float* vel);
+/* New box {lx, ly, lz, xy, xz, yz} (SimulationBox update with its generation
+ * counter, src/pbc.rs; a barostat's box change).  Positions are not
+ * touched (scale them with hmd_particles_upload).  Every force field built
+ * on these particles re-checks the box (HMD_ERR_BOX_TOO_SMALL leaves the
+ * old box in place), refreshes its SPME influence table, rebuilds its list
+ * at the next evaluation and re-captures its step graphs. */
+hmd_status hmd_particles_set_box(hmd_particles* p, const float lattice[6]);
 /* Image counters [n][3] (the images of an init state,
  * src/io/init_state.rs InitImages). */
 hmd_status hmd_particles_upload_images(hmd_particles* p, const int32_t* images);
