@@ -39,6 +39,7 @@ EXPORTED = [
     "hmd_particles_upload_images", "hmd_particles_set_box",
     "hmd_vv_kick_drift", "hmd_vv_kick", "hmd_lan_drift_half", "hmd_lan_ou_step",
     "hmd_replicas_create", "hmd_replicas_destroy", "hmd_replicas_step", "hmd_replicas_steps",
+    "hmd_replicas_plan",
     "hmd_integrator_timings", "hmd_timings_write",
     "hmd_forcefield_set_pair_fragment", "hmd_pair_fragment_compile",
     "hmd_settle_create", "hmd_settle_destroy", "hmd_settle_before_drift",
@@ -168,6 +169,8 @@ def _declare(L):
         "hmd_replicas_destroy": (None, [vp]),
         "hmd_replicas_step": (st, [vp, ctypes.c_uint32]),
         "hmd_replicas_steps": (ctypes.c_uint64, [vp]),
+        "hmd_replicas_plan": (st, [vp, ctypes.POINTER(ctypes.c_uint32),
+                                   ctypes.POINTER(ctypes.c_uint32)]),
         "hmd_lan_ou_step": (st, [vp, ctypes.c_uint64, ctypes.c_float, ctypes.c_float,
                                  ctypes.c_uint64]),
         "hmd_forcefield_create": (st, [vp, vp, ctypes.POINTER(ForceFieldDesc), pvp]),

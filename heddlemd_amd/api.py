@@ -363,17 +363,33 @@ class Settle:
 
 class ReplicaSet:
     """Independent systems stepped as one batch (hmd_replicas_*).  Each
-    integrator must come from its own GpuContext; fused=True captures every
-    replica's step into one graph, fused=False launches one graph per
-    replica on its own stream."""
+    integrator must come from its own GpuContext.  mode "fused" captures every
+    replica's step into one graph, "per_replica" launches one graph per
+    replica on its own stream, "batched" merges the replicas' steps into one
+    launch per stage (each kernel's batched twin over all replicas' blocks).
+    fused=False is mode "per_replica"."""
 
-    def __init__(self, integrators, fused=True):
+    MODES = {"fused": 0, "per_replica": 1, "batched": 2}
+
+    def __init__(self, integrators, fused=True, mode=None):
+        if mode is None:
+            mode = "fused" if fused else "per_replica"
+        if mode not in self.MODES:
+            raise ValueError(f"unknown replica mode {mode!r}")
         self._its = list(integrators)
+        self.mode = mode
         arr = (ctypes.c_void_p * len(self._its))(*[it.h.value for it in self._its])
         h = ctypes.c_void_p()
-        check(lib().hmd_replicas_create(arr, len(self._its), 0 if fused else 1,
+        check(lib().hmd_replicas_create(arr, len(self._its), self.MODES[mode],
                                         ctypes.byref(h)))
         self.h = h
+
+    def plan(self):
+        """(batched launches, single launches) per step of the merged plan
+        (batched mode, after the first step; zeros otherwise)."""
+        nb, ns = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().hmd_replicas_plan(self.h, ctypes.byref(nb), ctypes.byref(ns)))
+        return nb.value, ns.value
 
     def step(self, n=1):
         check(lib().hmd_replicas_step(self.h, n))

@@ -106,8 +106,14 @@ hmd_status hmd_ctx_create(int device, hmd_ctx** out) {
                     "code object is gfx950-only; device is " + c.device_name);
       check(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate");
       check(hipModuleLoadData(&c.module, hmd_code_object), "hipModuleLoadData");
-      for (int k = 0; k < K_COUNT; ++k)
+      for (int k = 0; k < K_COUNT; ++k) {
         check(hipModuleGetFunction(&c.fn[k], c.module, kKernelNames[k]), kKernelNames[k]);
+        const std::string bn = std::string(kKernelNames[k]) + "_batch";
+        if (hipModuleGetFunction(&c.fn_batch[k], c.module, bn.c_str()) != hipSuccess) {
+          c.fn_batch[k] = nullptr;
+          (void)hipGetLastError();
+        }
+      }
     } catch (...) {
       hmd_ctx_destroy(h);
       throw;

@@ -634,12 +634,20 @@ void ForceField::pair_forces(bool fev) {
   pp.shift_margin = r_cut + 0.5f * r_skin;
   pp.acc = acc.ptr;
   pp.acc_stride = n_pad;
-  const uint32_t blocks = std::max(1u, (uint32_t)c.num_cus * pair_blocks_per_cu);
+  // four waves per block, and never more waves than the list has entries
+  // (small systems would otherwise launch mostly idle blocks)
+  const uint32_t entry_blocks = ceil_div((uint64_t)n_clusters * cap_entries, 4);
+  const uint32_t blocks =
+      std::max(1u, std::min((uint32_t)c.num_cus * pair_blocks_per_cu, entry_blocks));
   KernelId k = lj_on && coul_on ? (fev ? K_hmd_pair_force_fev : K_hmd_pair_force_f)
                : lj_on ? (fev ? K_hmd_pair_force_lj_fev : K_hmd_pair_force_lj_f)
                        : (fev ? K_hmd_pair_force_coul_fev : K_hmd_pair_force_coul_f);
   pp.user_params = user_params.ptr;
-  if (jit) {
+  if (jit && c.rec) {
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(&pp);
+    c.rec->push_back(LaunchRec{k, pair_function(k), nullptr, blocks, 256, 0,
+                               std::vector<unsigned char>(b, b + sizeof pp)});
+  } else if (jit) {
     void* args[] = {&pp};
     check(hipModuleLaunchKernel(pair_function(k), blocks, 1, 1, 256, 1, 1, 0, c.stream, args,
                                 nullptr),

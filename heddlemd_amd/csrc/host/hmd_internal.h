@@ -67,6 +67,17 @@ inline void check(hipError_t e, const char* what) {
     throw Error(HMD_ERR_GPU, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// One kernel launch as recorded instead of issued: a replica batch records
+// every replica's step and merges the records into batched launches.
+struct LaunchRec {
+  KernelId k;
+  hipFunction_t fn;        // the kernel (a composed pair kernel where one is set)
+  hipFunction_t fn_batch;  // its batched twin (null: launched per replica)
+  uint32_t grid, block;
+  size_t shmem;
+  std::vector<unsigned char> params;
+};
+
 struct Context {
   int device = 0;
   int num_cus = 0;
@@ -74,6 +85,9 @@ struct Context {
   hipStream_t stream = nullptr;
   hipModule_t module = nullptr;
   hipFunction_t fn[K_COUNT] = {};
+  hipFunction_t fn_batch[K_COUNT] = {};   // NAME_batch twins (null where absent)
+  // When set, launches are appended here instead of being issued.
+  std::vector<LaunchRec>* rec = nullptr;
   // Optional per-stage event recording (profile mode, eager only).
   bool profiling = false;
   std::vector<std::pair<std::string, hipEvent_t>> marks;
@@ -120,6 +134,12 @@ template <class P>
 inline void launch(Context& c, KernelId k, uint32_t grid, uint32_t block,
                    size_t shmem, const P& params) {
   if (grid == 0) return;
+  if (c.rec) {
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(&params);
+    c.rec->push_back(LaunchRec{k, c.fn[k], c.fn_batch[k], grid, block, shmem,
+                               std::vector<unsigned char>(b, b + sizeof(P))});
+    return;
+  }
   void* args[] = {const_cast<P*>(&params)};
   check(hipModuleLaunchKernel(c.fn[k], grid, 1, 1, block, 1, 1,
                               (unsigned)shmem, c.stream, args, nullptr),

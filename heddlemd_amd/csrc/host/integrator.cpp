@@ -278,10 +278,20 @@ struct Integrator {
 // forks to each replica's stream and joins them back, so a batched step is
 // a single graph launch whose branches the runtime may run side by side on
 // its hardware queues.  In the per-replica mode each replica keeps its own
-// step graph, launched on its own stream, all replicas before any wait.
+// step graph, launched on its own stream, all replicas before any wait.  In
+// the batched mode every replica's step is recorded (not launched), and the
+// records are merged into one launch per stage: the kernel's _batch twin over
+// the blocks of all replicas that reach that stage together, each block
+// reading its replica's parameter block from device memory.  The merged
+// sequence is captured into one graph on the master stream.
 struct ReplicaSet {
+  enum class Mode { Fused, PerReplica, Batched };
   std::vector<Integrator*> its;
-  bool fused = true;
+  Mode mode = Mode::Fused;
+  // batched mode: parameter blocks and block tables of the merged launches
+  DevBuf<unsigned char> batch_params;
+  DevBuf<uint32_t> batch_starts;
+  uint32_t n_batched = 0, n_single = 0;   // launches per step of the merged plan
   hipStream_t master = nullptr;
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> joins;
@@ -289,7 +299,7 @@ struct ReplicaSet {
   hipGraphExec_t exec = nullptr;
   uint64_t steps = 0;
 
-  ReplicaSet(std::vector<Integrator*> v, bool fused_) : its(std::move(v)), fused(fused_) {
+  ReplicaSet(std::vector<Integrator*> v, Mode m) : its(std::move(v)), mode(m) {
     if (its.empty()) throw Error(HMD_ERR_INVALID, "empty replica set");
     const int dev = its[0]->c.device;
     for (auto* it : its) {
@@ -341,11 +351,143 @@ struct ReplicaSet {
     for (auto* it : its) it->ff.graph_invalid = false;
   }
 
+  // Two recorded launches can share a batched launch: the same kernel of
+  // the shared code object (each replica's context loads its own copy, so
+  // the function handles differ) at the same block size.  Kernels without a
+  // batched twin (composed pair kernels) pair up only with themselves.
+  static bool same_stage(const LaunchRec& x, const LaunchRec& y) {
+    if (x.block != y.block) return false;
+    if (x.fn_batch && y.fn_batch) return x.k == y.k;
+    return x.fn == y.fn;
+  }
+
+  // Batched mode: record each replica's step, merge, capture one graph.
+  // Merging is greedy over the replicas' next unissued launches: the
+  // (kernel, block size) that heads the most replica sequences is issued
+  // next, as one batched launch over those replicas (ties go to the
+  // replica with the most launches left).  Each replica's launches keep
+  // their order, so every replica sees exactly its own step.
+  void capture_batched() {
+    for (auto* it : its) it->prime();
+    release();
+    const size_t R = its.size();
+    std::vector<std::vector<LaunchRec>> recs(R);
+    for (size_t r = 0; r < R; ++r) {
+      Context& c = its[r]->c;
+      c.rec = &recs[r];
+      try {
+        its[r]->enqueue_step();
+      } catch (...) {
+        c.rec = nullptr;
+        throw;
+      }
+      c.rec = nullptr;
+    }
+    struct Item {
+      const LaunchRec* first;                // kernel, block size
+      std::vector<const LaunchRec*> members;
+      size_t param_off = 0, start_off = 0;   // into the device tables
+      uint32_t blocks = 0;
+      size_t shmem = 0;
+    };
+    std::vector<Item> plan;
+    std::vector<size_t> head(R, 0);
+    for (;;) {
+      int best = -1;
+      size_t best_count = 0, best_left = 0;
+      for (size_t r = 0; r < R; ++r) {
+        if (head[r] >= recs[r].size()) continue;
+        const LaunchRec& a = recs[r][head[r]];
+        size_t count = 0;
+        for (size_t q = 0; q < R; ++q)
+          if (head[q] < recs[q].size() && same_stage(recs[q][head[q]], a)) ++count;
+        const size_t left = recs[r].size() - head[r];
+        if (count > best_count || (count == best_count && left > best_left)) {
+          best = (int)r;
+          best_count = count;
+          best_left = left;
+        }
+      }
+      if (best < 0) break;
+      const LaunchRec& a = recs[best][head[best]];
+      Item item;
+      item.first = &a;
+      for (size_t q = 0; q < R; ++q)
+        if (head[q] < recs[q].size() && same_stage(recs[q][head[q]], a))
+          item.members.push_back(&recs[q][head[q]++]);
+      plan.push_back(std::move(item));
+    }
+    // device tables for the batched items (parameter blocks 256-B aligned)
+    std::vector<unsigned char> hp;
+    std::vector<uint32_t> hs;
+    n_batched = n_single = 0;
+    for (Item& it : plan) {
+      if (it.members.size() < 2 || !it.first->fn_batch) {
+        n_single += (uint32_t)it.members.size();
+        continue;
+      }
+      ++n_batched;
+      const size_t stride = it.first->params.size();
+      it.param_off = (hp.size() + 255) / 256 * 256;
+      hp.resize(it.param_off + stride * it.members.size());
+      it.start_off = hs.size();
+      uint64_t total = 0;
+      for (size_t m = 0; m < it.members.size(); ++m) {
+        const LaunchRec* L = it.members[m];
+        if (L->params.size() != stride) throw Error(HMD_ERR_INVALID, "batched parameter size");
+        std::memcpy(&hp[it.param_off + m * stride], L->params.data(), stride);
+        hs.push_back((uint32_t)total);
+        total += L->grid;
+        it.shmem = std::max(it.shmem, L->shmem);
+      }
+      if (total > 0x7FFFFFFFull) throw Error(HMD_ERR_INVALID, "batched launch too large");
+      hs.push_back((uint32_t)total);
+      it.blocks = (uint32_t)total;
+    }
+    batch_params.alloc(std::max<size_t>(hp.size(), 1), false);
+    batch_starts.alloc(std::max<size_t>(hs.size(), 1), false);
+    if (!hp.empty())
+      check(hipMemcpy(batch_params.ptr, hp.data(), hp.size(), hipMemcpyHostToDevice), "H2D");
+    if (!hs.empty())
+      check(hipMemcpy(batch_starts.ptr, hs.data(), hs.size() * 4, hipMemcpyHostToDevice), "H2D");
+    check(hipStreamBeginCapture(master, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+      for (const Item& it : plan) {
+        if (it.blocks) {
+          HmdBatch hb{batch_params.ptr + it.param_off, batch_starts.ptr + it.start_off,
+                      (uint32_t)it.members.size(), (uint32_t)it.first->params.size()};
+          void* args[] = {&hb};
+          check(hipModuleLaunchKernel(it.first->fn_batch, it.blocks, 1, 1, it.first->block, 1, 1,
+                                      (unsigned)it.shmem, master, args, nullptr),
+                (std::string(kKernelNames[it.first->k]) + "_batch").c_str());
+        } else {
+          for (const LaunchRec* L : it.members) {
+            void* args[] = {const_cast<unsigned char*>(L->params.data())};
+            check(hipModuleLaunchKernel(L->fn, L->grid, 1, 1, L->block, 1, 1,
+                                        (unsigned)L->shmem, master, args, nullptr),
+                  kKernelNames[L->k]);
+          }
+        }
+      }
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(master, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    check(hipStreamEndCapture(master, &graph), "hipStreamEndCapture");
+    check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    for (auto* it : its) it->ff.graph_invalid = false;
+  }
+
   void step(uint32_t n) {
-    if (fused) {
+    if (mode != Mode::PerReplica) {
       bool stale = !exec;
       for (auto* it : its) stale = stale || it->ff.graph_invalid;
-      if (stale) capture();
+      if (stale) {
+        if (mode == Mode::Batched) capture_batched();
+        else capture();
+      }
       for (uint32_t i = 0; i < n; ++i) check(hipGraphLaunch(exec, master), "hipGraphLaunch");
       check(hipStreamSynchronize(master), "sync");
     } else {
@@ -676,7 +818,13 @@ hmd_status hmd_replicas_create(hmd_integrator* const* its, uint32_t n, uint32_t 
     }
     auto* h = new hmd_replicas();
     try {
-      h->set = new ReplicaSet(std::move(v), (flags & HMD_REPLICAS_PER_REPLICA_GRAPHS) == 0);
+      if ((flags & HMD_REPLICAS_PER_REPLICA_GRAPHS) && (flags & HMD_REPLICAS_BATCHED))
+        throw Error(HMD_ERR_INVALID, "replica modes are exclusive");
+      const ReplicaSet::Mode m = (flags & HMD_REPLICAS_BATCHED) ? ReplicaSet::Mode::Batched
+                                 : (flags & HMD_REPLICAS_PER_REPLICA_GRAPHS)
+                                     ? ReplicaSet::Mode::PerReplica
+                                     : ReplicaSet::Mode::Fused;
+      h->set = new ReplicaSet(std::move(v), m);
     } catch (...) {
       delete h;
       throw;
@@ -693,6 +841,14 @@ void hmd_replicas_destroy(hmd_replicas* h) {
 
 hmd_status hmd_replicas_step(hmd_replicas* h, uint32_t n) { HMD_GUARD(h->set->step(n)); }
 uint64_t hmd_replicas_steps(hmd_replicas* h) { return h->set->steps; }
+
+hmd_status hmd_replicas_plan(hmd_replicas* h, uint32_t* n_batched, uint32_t* n_single) {
+  HMD_GUARD({
+    if (!h || !n_batched || !n_single) throw Error(HMD_ERR_INVALID, "null argument");
+    *n_batched = h->set->n_batched;
+    *n_single = h->set->n_single;
+  });
+}
 
 hmd_status hmd_integrator_step(hmd_integrator* w, uint32_t n) { HMD_GUARD(w->it->step(n)); }
 hmd_status hmd_integrator_capture(hmd_integrator* w) { HMD_GUARD(w->it->capture()); }

@@ -118,13 +118,13 @@ __device__ inline size_t hmd_line_base(const FftParams& p, uint32_t l) {
 
 // Pass 1: real -> complex along c.  data = real mesh (n0*n1 lines of n2
 // reals), out = complex mesh.  n = n2/2.
-extern "C" __global__ void __launch_bounds__(256)
-hmd_fft_r2c_c(const FftParams p) {
+__device__ __forceinline__ void hmd_fft_r2c_c_body(const FftParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   extern __shared__ float2 s_buf[];
   const uint32_t n = p.n, L = p.lines_per_block, ls = n + 1;
   float2* b0 = s_buf;
   float2* b1 = s_buf + L * ls;
-  const uint32_t l0 = blockIdx.x * L;
+  const uint32_t l0 = hmd_bx * L;
   const uint32_t lines = min(L, p.n_lines - l0);
   const float2* in = reinterpret_cast<const float2*>(p.data);
   for (uint32_t k = threadIdx.x; k < lines * n; k += blockDim.x) {
@@ -146,16 +146,17 @@ hmd_fft_r2c_c(const FftParams p) {
     out[(size_t)(l0 + l) * nch + m] = cadd(e, cmul(wr[m], o));
   }
 }
+HMD_KERNEL(hmd_fft_r2c_c, FftParams, 256)
 
 // Pass 5: complex -> real along c (unnormalised).  data = complex mesh,
 // out = real mesh.  n = n2/2.
-extern "C" __global__ void __launch_bounds__(256)
-hmd_fft_c2r_c(const FftParams p) {
+__device__ __forceinline__ void hmd_fft_c2r_c_body(const FftParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   extern __shared__ float2 s_buf[];
   const uint32_t n = p.n, L = p.lines_per_block, ls = n + 1;
   float2* b0 = s_buf;
   float2* b1 = s_buf + L * ls;
-  const uint32_t l0 = blockIdx.x * L;
+  const uint32_t l0 = hmd_bx * L;
   const uint32_t lines = min(L, p.n_lines - l0);
   const uint32_t nch = n + 1;
   const float2* in = reinterpret_cast<const float2*>(p.data);
@@ -177,6 +178,7 @@ hmd_fft_c2r_c(const FftParams p) {
     out[(size_t)l0 * n + k] = z[l * ls + m];
   }
 }
+HMD_KERNEL(hmd_fft_c2r_c, FftParams, 256)
 
 // Influence function G(k) and virial factor at mesh index (ka, kb, kc)
 // (kernels/spme_recip.cu:34-114).  Evaluated in FP64: exp(-k^2/4a^2) turns
@@ -213,9 +215,9 @@ __device__ inline void hmd_influence(const FftParams& p, uint32_t ka, uint32_t k
 // rounded once to f32, in the layout of the complex mesh.  The box is fixed
 // for the life of a force field, so the a-axis pass reads this table instead
 // of evaluating exp per point every step.
-extern "C" __global__ void __launch_bounds__(256)
-hmd_spme_gv_table(const FftParams p) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void hmd_spme_gv_table_body(const FftParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const size_t i = (size_t)hmd_bx * blockDim.x + threadIdx.x;
   const size_t total = (size_t)p.n0 * p.n1 * p.nch;
   if (i >= total) return;
   const uint32_t kc = (uint32_t)(i % p.nch);
@@ -225,19 +227,20 @@ hmd_spme_gv_table(const FftParams p) {
   hmd_influence(p, ka, kb, kc, p.bfac0[ka] * p.bfac1[kb] * p.bfac2[kc], G, vf);
   reinterpret_cast<float2*>(const_cast<float*>(p.gv))[i] = make_float2((float)G, (float)vf);
 }
+HMD_KERNEL(hmd_spme_gv_table, FftParams, 256)
 
 // Passes 2-4: complex FFT along a strided axis (b or a).  With
 // apply_influence the a-axis line is transformed forward, multiplied by
 // G(k), and transformed back in place; per-block virial / energy partial
 // sums go to partials[block] in a fixed order (deterministic).
-extern "C" __global__ void __launch_bounds__(256)
-hmd_fft_axis(const FftParams p) {
+__device__ __forceinline__ void hmd_fft_axis_body(const FftParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   extern __shared__ float2 s_buf[];
   __shared__ double s_red[2][HMD_FFT_THREADS];
   const uint32_t n = p.n, L = p.lines_per_block, ls = n + 1;
   float2* b0 = s_buf;
   float2* b1 = s_buf + L * ls;
-  const uint32_t l0 = blockIdx.x * L;
+  const uint32_t l0 = hmd_bx * L;
   const uint32_t lines = min(L, p.n_lines - l0);
   float2* data = reinterpret_cast<float2*>(p.data);
   // element k of line l sits at base(l) + k * elem_stride; thread mapping
@@ -277,8 +280,8 @@ hmd_fft_axis(const FftParams p) {
       __syncthreads();
     }
     if (threadIdx.x == 0) {
-      p.partials[2 * blockIdx.x] = s_red[0][0];
-      p.partials[2 * blockIdx.x + 1] = s_red[1][0];
+      p.partials[2 * hmd_bx] = s_red[0][0];
+      p.partials[2 * hmd_bx + 1] = s_red[1][0];
     }
     float2* other = (res == b0) ? b1 : b0;
     __syncthreads();
@@ -289,12 +292,13 @@ hmd_fft_axis(const FftParams p) {
     data[hmd_line_base(p, l0 + l) + (size_t)k * p.elem_stride] = res[l * ls + k];
   }
 }
+HMD_KERNEL(hmd_fft_axis, FftParams, 256)
 
 // Fixed-order reduction of the per-block partials: out[0] = W_recip =
 // 0.5 * sum hw vf |rho_hat|^2, out[1] = U_recip = 0.5 * sum hw G |rho_hat|^2
 // (kernels/spme_recip.cu:376-402 with the 0.5 half-sum factor).
-extern "C" __global__ void __launch_bounds__(256)
-hmd_spme_reduce(const SpmeReduceParams p) {
+__device__ __forceinline__ void hmd_spme_reduce_body(const SpmeReduceParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   __shared__ double s[2][256];
   double a = 0.0, b = 0.0;
   for (uint32_t i = threadIdx.x; i < p.n_partials; i += blockDim.x) {
@@ -316,6 +320,7 @@ hmd_spme_reduce(const SpmeReduceParams p) {
     p.out[1] = 0.5 * s[1][0];
   }
 }
+HMD_KERNEL(hmd_spme_reduce, SpmeReduceParams, 256)
 
 // ---------------------------------------------------------------------------
 // Power-of-two line lengths n = N1 N2 (16 ... 256): compile-time two-stage
@@ -471,14 +476,15 @@ __device__ inline void hmd_fill_tile(float2* tile, uint32_t total, Src src, Dst 
 // b, column = c) the line is multiplied by G(k) between the forward and the
 // inverse transform, with fixed-order virial / energy block partials.
 template <int N1, int N2>
-__device__ inline void hmd_fft_axis_slab(const FftParams& p) {
+__device__ inline void hmd_fft_axis_slab(const FftParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   extern __shared__ float2 s_tile[];
   __shared__ double s_red[2][HMD_FFT_THREADS];
   constexpr uint32_t N = (uint32_t)(N1 * N2);
   const uint32_t W = p.lines_per_block;
   const uint32_t n_tiles = (p.line_minor + W - 1) / W;
-  const uint32_t outer = blockIdx.x / n_tiles;
-  const uint32_t c0 = (blockIdx.x - outer * n_tiles) * W;
+  const uint32_t outer = hmd_bx / n_tiles;
+  const uint32_t c0 = (hmd_bx - outer * n_tiles) * W;
   const uint32_t Wc = min(W, p.line_minor - c0);
   float2* data = reinterpret_cast<float2*>(p.data);
   const size_t base = (size_t)outer * p.major_stride + c0;
@@ -532,8 +538,8 @@ __device__ inline void hmd_fft_axis_slab(const FftParams& p) {
       __syncthreads();
     }
     if (threadIdx.x == 0) {
-      p.partials[2 * blockIdx.x] = s_red[0][0];
-      p.partials[2 * blockIdx.x + 1] = s_red[1][0];
+      p.partials[2 * hmd_bx] = s_red[0][0];
+      p.partials[2 * hmd_bx + 1] = s_red[1][0];
     }
     hmd_fft_cols<N1, N2, +1>(s_tile, W, tw);
   }
@@ -546,11 +552,12 @@ __device__ inline void hmd_fft_axis_slab(const FftParams& p) {
 // Pass 1 for n2 = 2N: W contiguous real lines per block viewed as N complex
 // values each, transformed, then split into the n2/2 + 1 half spectrum.
 template <int N1, int N2>
-__device__ inline void hmd_fft_r2c_slab(const FftParams& p) {
+__device__ inline void hmd_fft_r2c_slab(const FftParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   extern __shared__ float2 s_tile[];
   constexpr uint32_t N = (uint32_t)(N1 * N2);
   const uint32_t W = p.lines_per_block;
-  const uint32_t l0 = blockIdx.x * W;
+  const uint32_t l0 = hmd_bx * W;
   const uint32_t Wc = min(W, p.n_lines - l0);
   const float2* in = reinterpret_cast<const float2*>(p.data);
   hmd_fill_tile(
@@ -580,11 +587,12 @@ __device__ inline void hmd_fft_r2c_slab(const FftParams& p) {
 
 // Pass 5 for n2 = 2N (unnormalised complex -> real).
 template <int N1, int N2>
-__device__ inline void hmd_fft_c2r_slab(const FftParams& p) {
+__device__ inline void hmd_fft_c2r_slab(const FftParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   extern __shared__ float2 s_tile[];
   constexpr uint32_t N = (uint32_t)(N1 * N2);
   const uint32_t W = p.lines_per_block;
-  const uint32_t l0 = blockIdx.x * W;
+  const uint32_t l0 = hmd_bx * W;
   const uint32_t Wc = min(W, p.n_lines - l0);
   const uint32_t nch = N + 1;
   const float2* in = reinterpret_cast<const float2*>(p.data);
@@ -617,15 +625,21 @@ __device__ inline void hmd_fft_c2r_slab(const FftParams& p) {
 }
 
 #define HMD_FFT_P2(N, N1, N2)                                                       \
-  extern "C" __global__ void __launch_bounds__(256) hmd_fft_axis_##N(const FftParams p) { \
-    hmd_fft_axis_slab<N1, N2>(p);                                                   \
+  __device__ __forceinline__ void hmd_fft_axis_##N##_body(                         \
+      const FftParams& p, const uint32_t hmd_bx, const uint32_t hmd_gx) {           \
+    hmd_fft_axis_slab<N1, N2>(p, hmd_bx, hmd_gx);                                   \
   }                                                                                 \
-  extern "C" __global__ void __launch_bounds__(256) hmd_fft_r2c_##N(const FftParams p) { \
-    hmd_fft_r2c_slab<N1, N2>(p);                                                    \
+  __device__ __forceinline__ void hmd_fft_r2c_##N##_body(                          \
+      const FftParams& p, const uint32_t hmd_bx, const uint32_t hmd_gx) {           \
+    hmd_fft_r2c_slab<N1, N2>(p, hmd_bx, hmd_gx);                                    \
   }                                                                                 \
-  extern "C" __global__ void __launch_bounds__(256) hmd_fft_c2r_##N(const FftParams p) { \
-    hmd_fft_c2r_slab<N1, N2>(p);                                                    \
-  }
+  __device__ __forceinline__ void hmd_fft_c2r_##N##_body(                          \
+      const FftParams& p, const uint32_t hmd_bx, const uint32_t hmd_gx) {           \
+    hmd_fft_c2r_slab<N1, N2>(p, hmd_bx, hmd_gx);                                    \
+  }                                                                                 \
+  HMD_KERNEL(hmd_fft_axis_##N, FftParams, 256)                                      \
+  HMD_KERNEL(hmd_fft_r2c_##N, FftParams, 256)                                       \
+  HMD_KERNEL(hmd_fft_c2r_##N, FftParams, 256)
 HMD_FFT_P2(16, 4, 4)
 HMD_FFT_P2(32, 8, 4)
 HMD_FFT_P2(64, 8, 8)

@@ -196,3 +196,33 @@ __device__ inline float hmd_readlane_f(float v, uint32_t lane) {
 __device__ inline bool hmd_gate(const uint32_t* status, uint32_t mask) {
   return status == nullptr || (*(volatile const uint32_t*)status & mask) != 0u;
 }
+
+// Batched launches: one launch runs the blocks of several independent
+// instances of a kernel (the replicas of a batch), each with its own
+// parameter block.  Instance r owns blocks block_start[r] .. block_start[r+1]
+// of the launch; every block finds its instance by a wave-uniform binary
+// search and runs the kernel body with its block index inside the instance
+// and the instance's own block count.
+__device__ inline uint32_t hmd_batch_member(const HmdBatch& b) {
+  uint32_t lo = 0, hi = b.n;
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (b.block_start[mid] <= blockIdx.x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// The kernel NAME (one instance, the parameter struct by value) and its
+// batched twin NAME_batch, both running NAME_body(p, block, blocks).
+#define HMD_KERNEL(NAME, PT, ...)                                              \
+  extern "C" __global__ void __launch_bounds__(__VA_ARGS__) NAME(const PT p) { \
+    NAME##_body(p, blockIdx.x, gridDim.x);                                     \
+  }                                                                            \
+  extern "C" __global__ void __launch_bounds__(__VA_ARGS__)                    \
+      NAME##_batch(const HmdBatch b) {                                         \
+    const uint32_t r = hmd_batch_member(b);                                    \
+    const uint32_t b0 = b.block_start[r];                                      \
+    NAME##_body(*reinterpret_cast<const PT*>(b.params + (size_t)r * b.stride), \
+                blockIdx.x - b0, b.block_start[r + 1] - b0);                   \
+  }

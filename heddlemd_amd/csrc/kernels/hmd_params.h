@@ -30,6 +30,16 @@ using __hip_internal::uint64_t;
 #define HMD_STATUS_CELL_OVERFLOW 4u
 #define HMD_STATUS_REBUILT 8u       // a rebuild ran since the host cleared it
 
+// A batched launch (one launch over the blocks of several instances of a
+// kernel): instance r's parameter struct sits at params + r * stride and its
+// blocks are block_start[r] .. block_start[r + 1] - 1 (n + 1 entries).
+struct HmdBatch {
+  const unsigned char* params;
+  const uint32_t* block_start;
+  uint32_t n;
+  uint32_t stride;
+};
+
 struct HmdLattice {
   float lx, ly, lz, xy, xz, yz;
 };

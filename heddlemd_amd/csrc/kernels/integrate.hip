@@ -392,34 +392,48 @@ __device__ inline void hmd_lan_ou_atom(const LangevinParams& p, const uint32_t i
 // ---------------------------------------------------- one kernel per op --
 // The reference's sub-steps one launch each (the standalone operator entry
 // points and the unfused integrator plan).
-extern "C" __global__ void __launch_bounds__(256) hmd_vv_kick_drift(const VvParams p) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void hmd_vv_kick_drift_body(const VvParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const uint32_t i = hmd_bx * blockDim.x + threadIdx.x;
   if (i < p.n) hmd_vv_kick_drift_atom(p, i);
 }
-extern "C" __global__ void __launch_bounds__(256) hmd_vv_kick(const VvParams p) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+HMD_KERNEL(hmd_vv_kick_drift, VvParams, 256)
+__device__ __forceinline__ void hmd_vv_kick_body(const VvParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const uint32_t i = hmd_bx * blockDim.x + threadIdx.x;
   if (i < p.n) hmd_vv_kick_atom(p, i);
 }
-extern "C" __global__ void __launch_bounds__(256) hmd_settle_snapshot(const SettleParams p) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+HMD_KERNEL(hmd_vv_kick, VvParams, 256)
+__device__ __forceinline__ void hmd_settle_snapshot_body(const SettleParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const uint32_t g = hmd_bx * blockDim.x + threadIdx.x;
   if (g < p.n_groups) hmd_settle_snapshot_group(p, g);
 }
-extern "C" __global__ void __launch_bounds__(256) hmd_settle_positions(const SettleParams p) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+HMD_KERNEL(hmd_settle_snapshot, SettleParams, 256)
+__device__ __forceinline__ void hmd_settle_positions_body(const SettleParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const uint32_t g = hmd_bx * blockDim.x + threadIdx.x;
   if (g < p.n_groups) hmd_settle_positions_group(p, g, p.dt);
 }
-extern "C" __global__ void __launch_bounds__(256) hmd_settle_velocities(const SettleParams p) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+HMD_KERNEL(hmd_settle_positions, SettleParams, 256)
+__device__ __forceinline__ void hmd_settle_velocities_body(const SettleParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const uint32_t g = hmd_bx * blockDim.x + threadIdx.x;
   if (g < p.n_groups) hmd_settle_velocities_group(p, g, p.dt);
 }
-extern "C" __global__ void __launch_bounds__(256) hmd_lan_drift_half(const LangevinParams p) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+HMD_KERNEL(hmd_settle_velocities, SettleParams, 256)
+__device__ __forceinline__ void hmd_lan_drift_half_body(const LangevinParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const uint32_t i = hmd_bx * blockDim.x + threadIdx.x;
   if (i < p.n) hmd_lan_drift_half_atom(p, i);
 }
-extern "C" __global__ void __launch_bounds__(256) hmd_lan_ou(const LangevinParams p) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+HMD_KERNEL(hmd_lan_drift_half, LangevinParams, 256)
+__device__ __forceinline__ void hmd_lan_ou_body(const LangevinParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const uint32_t i = hmd_bx * blockDim.x + threadIdx.x;
   if (i < p.n) hmd_lan_ou_atom(p, i);
 }
+HMD_KERNEL(hmd_lan_ou, LangevinParams, 256)
 
 // ------------------------------------------------------ fused step kernels --
 // The integrator's sub-steps between two force evaluations in one launch.
@@ -438,8 +452,9 @@ __device__ inline bool hmd_unit(const StepParams& p, uint32_t u, uint32_t& g, ui
 }
 
 // velocity Verlet: [snapshot] kick-drift [SETTLE positions]
-extern "C" __global__ void __launch_bounds__(256) hmd_step_vv_first(const StepParams p) {
-  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void hmd_step_vv_first_body(const StepParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const uint32_t u = hmd_bx * blockDim.x + threadIdx.x;
   uint32_t g, atom;
   if (!hmd_unit(p, u, g, atom)) return;
   if (g != HMD_SENTINEL) {
@@ -451,11 +466,13 @@ extern "C" __global__ void __launch_bounds__(256) hmd_step_vv_first(const StepPa
     hmd_vv_kick_drift_atom(p.vv, atom);
   }
 }
+HMD_KERNEL(hmd_step_vv_first, StepParams, 256)
 
 // Langevin BAOAB up to the force evaluation: B [SETTLE v] [snapshot] A
 // [SETTLE x, dt/2] O [SETTLE v] [snapshot] A [SETTLE x, dt/2]
-extern "C" __global__ void __launch_bounds__(256) hmd_step_baoa(const StepParams p) {
-  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void hmd_step_baoa_body(const StepParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const uint32_t u = hmd_bx * blockDim.x + threadIdx.x;
   uint32_t g, atom;
   if (!hmd_unit(p, u, g, atom)) return;
   const float h = 0.5f * p.dt;
@@ -482,11 +499,13 @@ extern "C" __global__ void __launch_bounds__(256) hmd_step_baoa(const StepParams
     hmd_lan_drift_half_atom(p.lan, atom);
   }
 }
+HMD_KERNEL(hmd_step_baoa, StepParams, 256)
 
 // after the force evaluation: kick [SETTLE velocities]; the Langevin draw
 // counter (read by this step's O sub-step) advances here
-extern "C" __global__ void __launch_bounds__(256) hmd_step_kick_last(const StepParams p) {
-  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void hmd_step_kick_last_body(const StepParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const uint32_t u = hmd_bx * blockDim.x + threadIdx.x;
   if (u == 0 && p.counter) p.counter[0] += 1ull;
   uint32_t g, atom;
   if (!hmd_unit(p, u, g, atom)) return;
@@ -498,10 +517,13 @@ extern "C" __global__ void __launch_bounds__(256) hmd_step_kick_last(const StepP
     hmd_vv_kick_atom(p.vv, atom);
   }
 }
+HMD_KERNEL(hmd_step_kick_last, StepParams, 256)
 
-extern "C" __global__ void hmd_counter_increment(const CounterParams p) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) p.counter[0] += 1ull;
+__device__ __forceinline__ void hmd_counter_increment_body(const CounterParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  if (threadIdx.x == 0 && hmd_bx == 0) p.counter[0] += 1ull;
 }
+HMD_KERNEL(hmd_counter_increment, CounterParams, 64)
 
 // -------------------------------------------------------------- reductions --
 // Fixed-shape FP64 reduction: block b sums a fixed contiguous chunk in a
@@ -509,11 +531,11 @@ extern "C" __global__ void hmd_counter_increment(const CounterParams p) {
 // block partials the same way.  Mode 0: sum x[0..n); mode 1: kinetic energy
 // 0.5 m |v|^2 (nose_hoover.cu:22-104 and barostat.cu:11-85 restated in
 // FP64).
-extern "C" __global__ void __launch_bounds__(256)
-hmd_reduce_partial(const ReduceParams p) {
+__device__ __forceinline__ void hmd_reduce_partial_body(const ReduceParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   __shared__ double s[256];
-  const uint32_t chunk = (p.n + gridDim.x - 1) / gridDim.x;
-  const uint32_t beg = blockIdx.x * chunk, end = min(beg + chunk, p.n);
+  const uint32_t chunk = (p.n + hmd_gx - 1) / hmd_gx;
+  const uint32_t beg = hmd_bx * chunk, end = min(beg + chunk, p.n);
   double acc = 0.0;
   for (uint32_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
     if (p.mode == 0) {
@@ -529,11 +551,12 @@ hmd_reduce_partial(const ReduceParams p) {
     if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) p.partials[blockIdx.x] = s[0];
+  if (threadIdx.x == 0) p.partials[hmd_bx] = s[0];
 }
+HMD_KERNEL(hmd_reduce_partial, ReduceParams, 256)
 
-extern "C" __global__ void __launch_bounds__(256)
-hmd_reduce_final(const ReduceParams p) {
+__device__ __forceinline__ void hmd_reduce_final_body(const ReduceParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   __shared__ double s[256];
   double acc = 0.0;
   for (uint32_t i = threadIdx.x; i < p.n_blocks; i += blockDim.x) acc += p.partials[i];
@@ -545,14 +568,15 @@ hmd_reduce_final(const ReduceParams p) {
   }
   if (threadIdx.x == 0) p.out[0] = s[0];
 }
+HMD_KERNEL(hmd_reduce_final, ReduceParams, 256)
 
 // ------------------------------------------------------- steepest descent --
 // x += h F, with |h F| capped at max_step per atom, then wrap (setup-time
 // relaxation of synthetic lattices; the reference relaxes its examples with
 // its steepest-descent minimizer, src/minimizer/).
-extern "C" __global__ void __launch_bounds__(256)
-hmd_sd_step(const SdParams p) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void hmd_sd_step_body(const SdParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const uint32_t i = hmd_bx * blockDim.x + threadIdx.x;
   if (i >= p.n) return;
   float4 pq = reinterpret_cast<float4*>(p.posq)[i];
   float d[3] = {p.f[i] * p.h, p.f[p.n + i] * p.h, p.f[2 * p.n + i] * p.h};
@@ -564,3 +588,4 @@ hmd_sd_step(const SdParams p) {
   pq.x = x; pq.y = y; pq.z = z;
   reinterpret_cast<float4*>(p.posq)[i] = pq;
 }
+HMD_KERNEL(hmd_sd_step, SdParams, 256)

@@ -91,9 +91,9 @@ __device__ inline void hmd_fine_cell(float x, float y, float z,
 
 // Skin trigger (kernels/neighbor.cu:50-76): any atom whose min-image
 // displacement from its rebuild-time position exceeds r_skin/2 sets bit 0.
-extern "C" __global__ void __launch_bounds__(256)
-hmd_nbl_displacement(const NblDispParams p) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void hmd_nbl_displacement_body(const NblDispParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  uint32_t i = hmd_bx * blockDim.x + threadIdx.x;
   if (i >= p.n) return;
   float4 a = reinterpret_cast<const float4*>(p.posq)[i];
   float4 r = reinterpret_cast<const float4*>(p.ref_pos)[i];
@@ -102,12 +102,13 @@ hmd_nbl_displacement(const NblDispParams p) {
   float d2 = dx * dx + dy * dy + dz * dz;
   if (d2 > p.threshold_sq) atomicOr(p.status, HMD_STATUS_DISPLACED);
 }
+HMD_KERNEL(hmd_nbl_displacement, NblDispParams, 256)
 
 // Fine-cell Hilbert key + histogram.
-extern "C" __global__ void __launch_bounds__(256)
-hmd_nbl_keys(const NblKeyParams p) {
+__device__ __forceinline__ void hmd_nbl_keys_body(const NblKeyParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = hmd_bx * blockDim.x + threadIdx.x;
   if (i >= p.n) return;
   float4 a = reinterpret_cast<const float4*>(p.posq)[i];
   uint32_t c0, c1, c2;
@@ -116,26 +117,28 @@ hmd_nbl_keys(const NblKeyParams p) {
   p.atom_key[i] = key;
   atomicAdd(&p.cell_count[key], 1u);
 }
+HMD_KERNEL(hmd_nbl_keys, NblKeyParams, 256)
 
 // Scatter into cells.  The count array doubles as the cursor and is driven
 // back to zero, ready for the next rebuild.  Order inside a cell is fixed
 // afterwards by hmd_nbl_sort_cells.
-extern "C" __global__ void __launch_bounds__(256)
-hmd_nbl_scatter(const NblKeyParams p) {
+__device__ __forceinline__ void hmd_nbl_scatter_body(const NblKeyParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = hmd_bx * blockDim.x + threadIdx.x;
   if (i >= p.n) return;
   uint32_t key = p.atom_key[i];
   uint32_t k = atomicSub(&p.cell_count[key], 1u) - 1u;
   p.sorted_id[p.cell_start[key] + k] = i;
 }
+HMD_KERNEL(hmd_nbl_scatter, NblKeyParams, 256)
 
 // Canonical (cell, atom id) order: insertion sort inside each fine cell
 // (a cell holds a couple of atoms; compare kernels/neighbor.cu:218-238).
-extern "C" __global__ void __launch_bounds__(256)
-hmd_nbl_sort_cells(const NblSortParams p) {
+__device__ __forceinline__ void hmd_nbl_sort_cells_body(const NblSortParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
-  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t c = hmd_bx * blockDim.x + threadIdx.x;
   if (c >= p.n_keys) return;
   uint32_t s = p.cell_start[c], e = p.cell_start[c + 1];
   for (uint32_t k = s + 1; k < e; ++k) {
@@ -148,6 +151,7 @@ hmd_nbl_sort_cells(const NblSortParams p) {
     p.sorted_id[pos] = key;
   }
 }
+HMD_KERNEL(hmd_nbl_sort_cells, NblSortParams, 256)
 
 // Sorted views.  Every step it refreshes the sorted positions
 // (kernels/neighbor.cu:251-285 does the same refresh every step in
@@ -159,11 +163,11 @@ hmd_nbl_sort_cells(const NblSortParams p) {
 // the wrapped positions the cells were taken from, so every cluster's box is
 // compact even for atoms that a constraint step left just outside the primary
 // cell, and between rebuilds they move continuously.
-extern "C" __global__ void __launch_bounds__(256)
-hmd_nbl_gather(const NblGatherParams p) {
+__device__ __forceinline__ void hmd_nbl_gather_body(const NblGatherParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
 #pragma clang fp contract(off)
   const bool rebuild = hmd_gate(p.status, HMD_STATUS_DISPLACED);
-  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t s = hmd_bx * blockDim.x + threadIdx.x;
   if (s >= p.n_pad) return;
   float4* ps = reinterpret_cast<float4*>(p.posq_sorted);
   if (s >= p.n) {
@@ -197,16 +201,17 @@ hmd_nbl_gather(const NblGatherParams p) {
   w.z -= kc * p.L.lz;
   ps[s] = w;
 }
+HMD_KERNEL(hmd_nbl_gather, NblGatherParams, 256)
 
 // Per-cluster axis-aligned bounding box (centre and half extents) of the
 // valid atoms, one wave per cluster (compare compute_block_bbox,
 // kernels/neighbor.cu:291-361), plus the boxes of the cluster's eight
 // 8-atom sub-clusters (consecutive sorted atoms) against which the list build
 // tests each j-atom.
-extern "C" __global__ void __launch_bounds__(256)
-hmd_nbl_bbox(const NblBboxParams p) {
+__device__ __forceinline__ void hmd_nbl_bbox_body(const NblBboxParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
-  uint32_t c = blockIdx.x * (blockDim.x / HMD_WAVE) + threadIdx.x / HMD_WAVE;
+  uint32_t c = hmd_bx * (blockDim.x / HMD_WAVE) + threadIdx.x / HMD_WAVE;
   if (c >= p.n_clusters) return;
   uint32_t lane = hmd_lane();
   uint32_t s = c * HMD_CLUSTER + lane;
@@ -238,6 +243,7 @@ hmd_nbl_bbox(const NblBboxParams p) {
                                0.5f * (hi[2] - lo[2]), 0.0f);
   }
 }
+HMD_KERNEL(hmd_nbl_bbox, NblBboxParams, 256)
 
 // Conservative min-image gap between two boxes (compare
 // packed_block_bbox_dist_sq, kernels/neighbor.cu:365-388).  Used only as a
@@ -306,8 +312,8 @@ __device__ inline uint32_t hmd_wave_scan_incl(uint32_t v, uint32_t lane) {
 #ifndef HMD_NBL_BOX_BATCH
 #define HMD_NBL_BOX_BATCH 4u          // candidate chunks loaded together
 #endif
-extern "C" __global__ void __launch_bounds__(64 * HMD_NBL_WAVES, HMD_NBL_BUILD_WAVES_PER_EU)
-hmd_nbl_build(const NblBuildParams p) {
+__device__ __forceinline__ void hmd_nbl_build_body(const NblBuildParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
   __shared__ uint64_t s_mask[HMD_NBL_MAX_CHUNKS];
   __shared__ uint64_t s_nowrap[HMD_NBL_MAX_CHUNKS];
@@ -317,7 +323,7 @@ hmd_nbl_build(const NblBuildParams p) {
   __shared__ uint32_t s_off[HMD_NBL_CAND_CAP];
   __shared__ uint32_t s_fill;
   extern __shared__ uint32_t s_list[];   // [cap_entries * 64]: the cluster's j list
-  const uint32_t c = blockIdx.x;
+  const uint32_t c = hmd_bx;
   const uint32_t wave = threadIdx.x / HMD_WAVE;
   const uint32_t lane = hmd_lane();
   const float4* ps = reinterpret_cast<const float4*>(p.posq_sorted);
@@ -573,24 +579,26 @@ hmd_nbl_build(const NblBuildParams p) {
     }
   }
 }
+HMD_KERNEL(hmd_nbl_build, NblBuildParams, 64 * HMD_NBL_WAVES, HMD_NBL_BUILD_WAVES_PER_EU)
 
 // Flatten (cluster, entry) into the work list the force kernel walks, one
 // wave per cluster (entry_offset is the exclusive scan of entry_count with
 // the total at [n_clusters]).
-extern "C" __global__ void __launch_bounds__(256)
-hmd_nbl_worklist(const NblWorkParams p) {
+__device__ __forceinline__ void hmd_nbl_worklist_body(const NblWorkParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
-  const uint32_t c = blockIdx.x * (blockDim.x / HMD_WAVE) + threadIdx.x / HMD_WAVE;
+  const uint32_t c = hmd_bx * (blockDim.x / HMD_WAVE) + threadIdx.x / HMD_WAVE;
   if (c >= p.n_clusters) return;
   const uint32_t cnt = p.entry_count[c];
   const uint32_t off = p.entry_offset[c];
   for (uint32_t k = hmd_lane(); k < cnt; k += HMD_WAVE) p.work[off + k] = c * p.cap_entries + k;
 }
+HMD_KERNEL(hmd_nbl_worklist, NblWorkParams, 256)
 
 // Close the rebuild: work-list length, status flags and generation (a kernel
 // of its own, after every block of the work-list kernel has read the gate).
-extern "C" __global__ void __launch_bounds__(64)
-hmd_nbl_close(const NblWorkParams p) {
+__device__ __forceinline__ void hmd_nbl_close_body(const NblWorkParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
   if (threadIdx.x == 0) {
     p.n_work[0] = p.entry_offset[p.n_clusters];
@@ -600,14 +608,15 @@ hmd_nbl_close(const NblWorkParams p) {
     st[1] += 1u;   // rebuild generation
   }
 }
+HMD_KERNEL(hmd_nbl_close, NblWorkParams, 64)
 
 // Decode the list into the exact pair set (canonical (min, max) ids of every
 // listed pair whose f32 min-image distance is <= r_search), one wave per
 // work item.  This is the PairSnapshot of src/forces/neighbor_list.rs:
 // 1358-1445 restricted to pairs inside the search radius.
-extern "C" __global__ void __launch_bounds__(256)
-hmd_nbl_pairs(const NblPairsParams p) {
-  uint32_t w = blockIdx.x * (blockDim.x / HMD_WAVE) + threadIdx.x / HMD_WAVE;
+__device__ __forceinline__ void hmd_nbl_pairs_body(const NblPairsParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  uint32_t w = hmd_bx * (blockDim.x / HMD_WAVE) + threadIdx.x / HMD_WAVE;
   if (w >= *p.n_work) return;
   uint32_t lane = hmd_lane();
   uint32_t slot = p.work[w];
@@ -633,3 +642,4 @@ hmd_nbl_pairs(const NblPairsParams p) {
     }
   }
 }
+HMD_KERNEL(hmd_nbl_pairs, NblPairsParams, 256)

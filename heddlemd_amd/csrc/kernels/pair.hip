@@ -226,7 +226,8 @@ __device__ inline void hmd_flush_i(const PairParams& p, const HmdPairState& st) 
 }
 
 template <bool FEV, bool LJ, bool COUL>
-__device__ inline void hmd_pair_body(const PairParams& p) {
+__device__ inline void hmd_pair_body(const PairParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   __shared__ float4 s_lj[HMD_MAX_TYPES * HMD_MAX_TYPES];   // (c6, c12, 6 c6, 12 c12)
   // erfc table: force coefficients, then (energy variants) energy coefficients
   __shared__ float4 s_ct[COUL ? (FEV ? 2 * HMD_COUL_BINS : HMD_COUL_BINS) : 1];
@@ -246,8 +247,8 @@ __device__ inline void hmd_pair_body(const PairParams& p) {
 
   const uint32_t lane = hmd_lane();
   const uint32_t waves_per_block = blockDim.x / HMD_WAVE;
-  const uint32_t n_waves = gridDim.x * waves_per_block;
-  const uint32_t gw = blockIdx.x * waves_per_block + threadIdx.x / HMD_WAVE;
+  const uint32_t n_waves = hmd_gx * waves_per_block;
+  const uint32_t gw = hmd_bx * waves_per_block + threadIdx.x / HMD_WAVE;
   const uint32_t nw = *p.n_work;
   const uint32_t per = (nw + n_waves - 1) / n_waves;
   const uint32_t w0 = gw * per;
@@ -312,10 +313,12 @@ __device__ inline void hmd_pair_body(const PairParams& p) {
 #define HMD_PAIR_WAVES_PER_EU 1
 #endif
 #define HMD_PAIR_VARIANT(NAME, FEV, LJ, COUL)                                  \
-  extern "C" __global__ void __launch_bounds__(256, HMD_PAIR_WAVES_PER_EU)     \
-  NAME(const PairParams p) {                                                   \
-    hmd_pair_body<FEV, LJ, COUL>(p);                                          \
-  }
+  __device__ __forceinline__ void NAME##_body(const PairParams& p,             \
+                                              const uint32_t hmd_bx,           \
+                                              const uint32_t hmd_gx) {         \
+    hmd_pair_body<FEV, LJ, COUL>(p, hmd_bx, hmd_gx);                           \
+  }                                                                            \
+  HMD_KERNEL(NAME, PairParams, 256, HMD_PAIR_WAVES_PER_EU)
 HMD_PAIR_VARIANT(hmd_pair_force_f, false, true, true)
 HMD_PAIR_VARIANT(hmd_pair_force_fev, true, true, true)
 HMD_PAIR_VARIANT(hmd_pair_force_lj_f, false, true, false)
@@ -327,9 +330,9 @@ HMD_PAIR_VARIANT(hmd_pair_force_coul_fev, true, false, true)
 // forces: the reference's finalize_packed_forces + combine_class_totals
 // (kernels/neighbor.cu:728-760, kernels/forces.cu:13-41).  The pair
 // accumulators are cleared for the next evaluation as they are read.
-extern "C" __global__ void __launch_bounds__(256)
-hmd_pair_finalize(const PairFinalizeParams p) {
-  const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void hmd_pair_finalize_body(const PairFinalizeParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const uint32_t a = hmd_bx * blockDim.x + threadIdx.x;
   if (a >= p.n) return;
   const uint32_t s = p.slot_of[a];
   const uint32_t st = p.acc_stride;
@@ -353,3 +356,4 @@ hmd_pair_finalize(const PairFinalizeParams p) {
     p.w[a] = w;
   }
 }
+HMD_KERNEL(hmd_pair_finalize, PairFinalizeParams, 256)

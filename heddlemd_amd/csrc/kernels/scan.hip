@@ -20,12 +20,12 @@ __device__ inline uint32_t hmd_wave_incl_scan(uint32_t x, uint32_t lane) {
 
 // Scan one 4096-element tile per block.  block_sums[b] receives the tile
 // total.  With total_slot != 0 (single-block use) out[len] = total.
-extern "C" __global__ void __launch_bounds__(1024)
-hmd_scan_tiles(const ScanParams p) {
+__device__ __forceinline__ void hmd_scan_tiles_body(const ScanParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   if (p.gate && !hmd_gate(p.gate, p.gate_mask)) return;
   __shared__ uint32_t s_wave[HMD_SCAN_THREADS / HMD_WAVE];
   const uint32_t t = threadIdx.x, lane = hmd_lane(), wave = t / HMD_WAVE;
-  const uint32_t base = blockIdx.x * HMD_SCAN_TILE + t * HMD_SCAN_PER_THREAD;
+  const uint32_t base = hmd_bx * HMD_SCAN_TILE + t * HMD_SCAN_PER_THREAD;
   uint32_t v[HMD_SCAN_PER_THREAD];
   uint32_t sum = 0;
 #pragma unroll
@@ -49,22 +49,24 @@ hmd_scan_tiles(const ScanParams p) {
     run += v[k];
   }
   if (t == HMD_SCAN_THREADS - 1) {
-    if (p.block_sums) p.block_sums[blockIdx.x] = run;
+    if (p.block_sums) p.block_sums[hmd_bx] = run;
     if (p.total_slot) p.out[p.len] = run;
   }
 }
+HMD_KERNEL(hmd_scan_tiles, ScanParams, 1024)
 
 // Second level: add the scanned tile offsets (in) to every element of out;
 // the last block also writes out[len] = total (block_sums holds the
 // unscanned tile totals).
-extern "C" __global__ void __launch_bounds__(1024)
-hmd_scan_add(const ScanParams p) {
+__device__ __forceinline__ void hmd_scan_add_body(const ScanParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   if (p.gate && !hmd_gate(p.gate, p.gate_mask)) return;
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = hmd_bx;
   const uint32_t off = p.in[tile];
   const uint32_t base = tile * HMD_SCAN_TILE;
   for (uint32_t k = threadIdx.x; k < HMD_SCAN_TILE; k += blockDim.x)
     if (base + k < p.len) p.out[base + k] += off;
-  if (p.total_slot && tile == gridDim.x - 1 && threadIdx.x == 0)
+  if (p.total_slot && tile == hmd_gx - 1 && threadIdx.x == 0)
     p.out[p.len] = off + p.block_sums[tile];
 }
+HMD_KERNEL(hmd_scan_add, ScanParams, 1024)

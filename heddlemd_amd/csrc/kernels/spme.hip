@@ -129,9 +129,9 @@ __device__ inline uint32_t hmd_spme_bin_of(const SpmeBinParams& p, float4 pq) {
          (uint32_t)g[2] / p.b2;
 }
 
-extern "C" __global__ void __launch_bounds__(256)
-hmd_spme_bin(const SpmeBinParams p) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void hmd_spme_bin_body(const SpmeBinParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const uint32_t s = hmd_bx * blockDim.x + threadIdx.x;
   const bool valid = s < p.n;
   uint32_t bin = HMD_SENTINEL;
   if (valid) {
@@ -147,12 +147,13 @@ hmd_spme_bin(const SpmeBinParams p) {
     pending &= ~same;
   }
 }
+HMD_KERNEL(hmd_spme_bin, SpmeBinParams, 256)
 
 // Scatter canonical atom ids into their bins.  The count array is driven
 // back to zero (one atomicSub per (wave, bin) claims a block of positions).
-extern "C" __global__ void __launch_bounds__(256)
-hmd_spme_bin_scatter(const SpmeBinParams p) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void hmd_spme_bin_scatter_body(const SpmeBinParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const uint32_t s = hmd_bx * blockDim.x + threadIdx.x;
   const bool valid = s < p.n;
   const uint32_t lane = hmd_lane();
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -173,6 +174,7 @@ hmd_spme_bin_scatter(const SpmeBinParams p) {
     pending &= ~same;
   }
 }
+HMD_KERNEL(hmd_spme_bin_scatter, SpmeBinParams, 256)
 
 // A contribution v (in units of 2^-24 e, |v| < 2^31) as the i64 count of
 // 2^-32 e rounded to nearest, from two i32 roundings: r1 = rint(v), the
@@ -187,10 +189,11 @@ __device__ inline long long hmd_fixed32_of24(float v) {
 }
 
 template <int P>
-__device__ inline void hmd_spread_body(const SpmeSpreadParams& p) {
+__device__ inline void hmd_spread_body(const SpmeSpreadParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   extern __shared__ long long s_acc[];
   const uint32_t bsz = p.b0 * p.b1 * p.b2;
-  const uint32_t B = blockIdx.x;
+  const uint32_t B = hmd_bx;
   const uint32_t B2 = B % p.nb2, B1 = (B / p.nb2) % p.nb1, B0 = B / (p.nb2 * p.nb1);
   const int o0 = (int)(B0 * p.b0), o1 = (int)(B1 * p.b1), o2 = (int)(B2 * p.b2);
   for (uint32_t k = threadIdx.x; k < bsz; k += blockDim.x) s_acc[k] = 0;
@@ -272,10 +275,11 @@ __device__ inline void hmd_spread_body(const SpmeSpreadParams& p) {
 }
 
 template <int P>
-__device__ inline void hmd_gather_body(const SpmeGatherParams& p) {
+__device__ inline void hmd_gather_body(const SpmeGatherParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
   extern __shared__ float s_v[];
   const uint32_t h0 = p.b0 + P - 1, h1 = p.b1 + P - 1, h2 = p.b2 + P - 1;
-  const uint32_t B = blockIdx.x;
+  const uint32_t B = hmd_bx;
   const uint32_t B2 = B % p.nb2, B1 = (B / p.nb2) % p.nb1, B0 = B / (p.nb2 * p.nb1);
   const int o0 = (int)(B0 * p.b0) - (P - 1);
   const int o1 = (int)(B1 * p.b1) - (P - 1);
@@ -344,14 +348,16 @@ __device__ inline void hmd_gather_body(const SpmeGatherParams& p) {
 }
 
 #define HMD_SPME_ORDER(P)                                                    \
-  extern "C" __global__ void __launch_bounds__(256)                         \
-      hmd_spme_spread_p##P(const SpmeSpreadParams p) {                      \
-    hmd_spread_body<P>(p);                                                   \
+  __device__ __forceinline__ void hmd_spme_spread_p##P##_body(              \
+      const SpmeSpreadParams& p, const uint32_t hmd_bx, const uint32_t hmd_gx) { \
+    hmd_spread_body<P>(p, hmd_bx, hmd_gx);                                   \
   }                                                                          \
-  extern "C" __global__ void __launch_bounds__(256)                         \
-      hmd_spme_gather_p##P(const SpmeGatherParams p) {                      \
-    hmd_gather_body<P>(p);                                                   \
-  }
+  __device__ __forceinline__ void hmd_spme_gather_p##P##_body(              \
+      const SpmeGatherParams& p, const uint32_t hmd_bx, const uint32_t hmd_gx) { \
+    hmd_gather_body<P>(p, hmd_bx, hmd_gx);                                   \
+  }                                                                          \
+  HMD_KERNEL(hmd_spme_spread_p##P, SpmeSpreadParams, 256)                    \
+  HMD_KERNEL(hmd_spme_gather_p##P, SpmeGatherParams, 256)
 
 HMD_SPME_ORDER(4)
 HMD_SPME_ORDER(5)

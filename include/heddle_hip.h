@@ -306,17 +306,27 @@ hmd_status hmd_settle_after_kick(hmd_settle* s, float dt);
  * integrator must come from its own context (hmd_ctx) on the same device.
  * Default: all replicas' steps are captured into one graph (one launch per
  * batched step, a fork per replica); HMD_REPLICAS_PER_REPLICA_GRAPHS keeps
- * one step graph per replica, each launched on its own stream.  Status is
+ * one step graph per replica, each launched on its own stream;
+ * HMD_REPLICAS_BATCHED merges the replicas' steps into one launch per stage
+ * (each kernel's batched twin over the blocks of every replica, with the
+ * replicas' parameter blocks in device memory), captured into one graph --
+ * the same arithmetic per replica, so the results are those of stepping
+ * each replica alone.  Status is
  * checked for every replica after each call; an overflow grows that
  * replica's list and is reported as HMD_ERR_NBL_OVERFLOW (re-run the
  * batch).  The integrators must outlive the set. */
 typedef struct hmd_replicas hmd_replicas;
 #define HMD_REPLICAS_PER_REPLICA_GRAPHS 1u
+#define HMD_REPLICAS_BATCHED 2u
 hmd_status hmd_replicas_create(hmd_integrator* const* integrators, uint32_t n, uint32_t flags,
                                hmd_replicas** out);
 void hmd_replicas_destroy(hmd_replicas* set);
 hmd_status hmd_replicas_step(hmd_replicas* set, uint32_t n_steps);
 uint64_t hmd_replicas_steps(hmd_replicas* set);
+/* The merged plan of a HMD_REPLICAS_BATCHED set (after its first step):
+ * batched launches and single launches per batched step (zeros in the other
+ * modes or before the first step). */
+hmd_status hmd_replicas_plan(hmd_replicas* set, uint32_t* n_batched, uint32_t* n_single);
 
 /* ---- setup helpers (host only, no GPU) -----------------------------------
  * generate_velocities (src/runner.rs:3108-3200): ChaCha8 seeded with

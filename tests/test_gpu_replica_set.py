@@ -1,8 +1,9 @@
 """Batched replicas (hmd_replicas_*): stepping independent boxes as one batch
 changes nothing in any of them.  Three small boxes of the replica workload
 (Langevin BAOAB + SETTLE at 300 K, each its own seed and context) are run
-for 30 steps three ways: each on its own through its step graph, all as one
-fused graph, and all with per-replica graphs on their own streams.  Bar:
+for 30 steps four ways: each on its own through its step graph, all as one
+fused graph, all with per-replica graphs on their own streams, and all
+merged into one batched launch per stage.  Bar:
 positions and velocities byte-identical; per-replica CSV logs written from
 two fused runs byte-identical."""
 import numpy as np
@@ -49,10 +50,14 @@ def run(mode, log_dir=None):
             for r in reps:
                 r[3].replay(STEPS)
         else:
-            rs = api.ReplicaSet([r[3] for r in reps], fused=(mode == "fused"))
+            rs = api.ReplicaSet([r[3] for r in reps], mode=mode)
             try:
                 if log_dir is None:
                     rs.step(STEPS)
+                    if mode == "batched":
+                        nb, ns = rs.plan()
+                        print(f"batched plan: {nb} batched launches, {ns} single", flush=True)
+                        assert nb > 0
                 else:
                     logs = [api.LogWriter(log_dir / f"replica{rid}.out.log", units="si",
                                           extra_columns=[("potential_energy", "energy")])
@@ -78,7 +83,7 @@ def run(mode, log_dir=None):
 
 def test_batched_steps_equal_solo_steps():
     alone = run("alone")
-    for mode in ("fused", "per-replica"):
+    for mode in ("fused", "per_replica", "batched"):
         got = run(mode)
         for rid, (a, b) in zip(RIDS, zip(alone, got)):
             print(f"{mode}: replica {rid} positions/velocities identical: "
