@@ -42,6 +42,7 @@ EXPORTED = [
     "hmd_replicas_plan",
     "hmd_integrator_timings", "hmd_timings_write",
     "hmd_forcefield_set_pair_fragment", "hmd_pair_fragment_compile",
+    "hmd_forcefield_autotune", "hmd_forcefield_set_tuning", "hmd_forcefield_get_tuning",
     "hmd_settle_create", "hmd_settle_destroy", "hmd_settle_before_drift",
     "hmd_settle_after_drift", "hmd_settle_after_kick",
     "hmd_forcefield_create", "hmd_forcefield_destroy", "hmd_forcefield_compute",
@@ -75,6 +76,17 @@ class ForceFieldDesc(ctypes.Structure):
         ("r_skin", ctypes.c_double),
         ("excl_offsets", _c_u32p), ("excl_partners", _c_u32p),
         ("excl_lj_scale", _c_fp), ("excl_coul_scale", _c_fp),
+    ]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("pair_blocks_per_cu", ctypes.c_uint32),
+        ("fft_blocks_per_cu", ctypes.c_uint32),
+        ("brick", ctypes.c_uint32 * 3),
+        ("n_candidates", ctypes.c_uint32),
+        ("us_initial", ctypes.c_double),
+        ("us_tuned", ctypes.c_double),
     ]
 
 
@@ -153,6 +165,9 @@ def _declare(L):
         "hmd_settle_create": (st, [vp, ctypes.c_uint32, _c_u32p] + [ctypes.c_double] * 4
                               + [pvp]),
         "hmd_settle_destroy": (None, [vp]),
+        "hmd_forcefield_autotune": (st, [vp, ctypes.c_uint32, ctypes.POINTER(Tuning)]),
+        "hmd_forcefield_set_tuning": (st, [vp, ctypes.POINTER(Tuning)]),
+        "hmd_forcefield_get_tuning": (st, [vp, ctypes.POINTER(Tuning)]),
         "hmd_forcefield_set_pair_fragment": (st, [vp, ctypes.c_char_p, ctypes.c_char_p, _c_fp,
                                                   ctypes.c_uint32]),
         "hmd_pair_fragment_compile": (st, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,

@@ -5,7 +5,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import ForceFieldDesc, IntegratorDesc, MlpDesc, check, lib
+from ._lib import ForceFieldDesc, IntegratorDesc, MlpDesc, Tuning, check, lib
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -230,6 +230,35 @@ class ForceField:
         prm = np.ascontiguousarray(list(params) or [0.0], np.float32)
         check(lib().hmd_forcefield_set_pair_fragment(self.h, name.encode(), source.encode(),
                                                      _ptr(prm, _fp), len(params)))
+
+    @staticmethod
+    def _tuning_dict(t):
+        return {"pair_blocks_per_cu": t.pair_blocks_per_cu,
+                "fft_blocks_per_cu": t.fft_blocks_per_cu,
+                "brick": tuple(t.brick), "n_candidates": t.n_candidates,
+                "us_initial": t.us_initial, "us_tuned": t.us_tuned}
+
+    def autotune(self, n_evals=20):
+        """Time candidate launch shapes (pair grid, FFT grids, SPME brick) on
+        the current positions and keep the fastest (hmd_forcefield_autotune)."""
+        t = Tuning()
+        check(lib().hmd_forcefield_autotune(self.h, n_evals, ctypes.byref(t)))
+        return self._tuning_dict(t)
+
+    def tuning(self):
+        t = Tuning()
+        check(lib().hmd_forcefield_get_tuning(self.h, ctypes.byref(t)))
+        return self._tuning_dict(t)
+
+    def set_tuning(self, tuning):
+        """Apply recorded launch shapes (a dict as autotune / tuning return)."""
+        t = Tuning()
+        t.pair_blocks_per_cu = int(tuning["pair_blocks_per_cu"])
+        t.fft_blocks_per_cu = int(tuning.get("fft_blocks_per_cu", 0))
+        br = tuning.get("brick", (0, 0, 0))
+        for k in range(3):
+            t.brick[k] = int(br[k])
+        check(lib().hmd_forcefield_set_tuning(self.h, ctypes.byref(t)))
 
     def request_rebuild(self):
         check(lib().hmd_nbl_request_rebuild(self.h))

@@ -129,22 +129,16 @@ static const KernelId kP2C2r[5] = {K_hmd_fft_c2r_16, K_hmd_fft_c2r_32, K_hmd_fft
 
 // Strided axis of length n over n_outer slabs of `cols` adjacent lines:
 // column tiles of width W (stage-1 capacity 2 x 256 / N2, 64 KiB of LDS),
-// at least about one block per compute unit.
-// Workgroups per CU the FFT passes aim for (several, so that one group's
-// loads overlap another's transform).
-#ifndef HMD_FFT_BLOCKS_PER_CU
-#define HMD_FFT_BLOCKS_PER_CU 3
-#endif
-
+// at least `target` blocks in all (a few per compute unit).
 // `tiles` LDS tiles of n x W complex values per workgroup (the a-axis pass
 // stages the influence table beside the data), at most 64 KiB in all.
-static ForceField::FftPass plan_axis(uint32_t n, uint32_t n_outer, uint32_t cols, uint32_t cus,
-                                     uint32_t tiles_lds = 1) {
+static ForceField::FftPass plan_axis(uint32_t n, uint32_t n_outer, uint32_t cols,
+                                     uint32_t target, uint32_t tiles_lds = 1) {
   ForceField::FftPass ps;
   const int ix = fft_p2_index(n);
   if (ix < 0) return ps;
   const uint32_t wmax = std::min(512u / kP2N2[ix], 8192u / (n * tiles_lds));
-  uint32_t tiles = std::max(ceil_div(cols, wmax), ceil_div(cus * HMD_FFT_BLOCKS_PER_CU, n_outer));
+  uint32_t tiles = std::max(ceil_div(cols, wmax), ceil_div(target, n_outer));
   tiles = std::min(tiles, cols);
   ps.p2 = true;
   ps.k = kP2Axis[ix];
@@ -155,14 +149,14 @@ static ForceField::FftPass plan_axis(uint32_t n, uint32_t n_outer, uint32_t cols
 }
 
 // Contiguous lines (the c axis, N = n2 / 2 complex points each).
-static ForceField::FftPass plan_lines(uint32_t N, uint32_t n_lines, uint32_t cus, bool r2c) {
+static ForceField::FftPass plan_lines(uint32_t N, uint32_t n_lines, uint32_t target, bool r2c) {
   ForceField::FftPass ps;
   const int ix = fft_p2_index(N);
   if (ix < 0) return ps;
   const uint32_t wmax = std::min(512u / kP2N2[ix], 8192u / N);
   ps.p2 = true;
   ps.k = r2c ? kP2R2c[ix] : kP2C2r[ix];
-  ps.W = std::max(1u, std::min(wmax, ceil_div(n_lines, cus * HMD_FFT_BLOCKS_PER_CU)));
+  ps.W = std::max(1u, std::min(wmax, ceil_div(n_lines, target)));
   ps.grid = ceil_div(n_lines, ps.W);
   ps.shmem = (size_t)N * ps.W * 8;
   return ps;
@@ -420,25 +414,13 @@ void ForceField::init_spme(const hmd_forcefield_desc& d) {
   if (!d.brick[0] && !d.brick[1] && !d.brick[2] && brick[0] == 16 && brick[1] == 16 &&
       brick[2] == 16 && 8 >= order)
     brick[2] = 8;
-  for (int k = 0; k < 3; ++k) {
-    if (grid[k] % brick[k] != 0 || brick[k] < order)
-      throw Error(HMD_ERR_INVALID, "SPME brick must divide the grid and be >= order");
-    nb[k] = grid[k] / brick[k];
-  }
+  check_brick(brick);
   if (grid[2] % 2 != 0) throw Error(HMD_ERR_UNSUPPORTED, "SPME grid[2] must be even");
-  if ((size_t)brick[0] * brick[1] * brick[2] * 8 > 65536 ||
-      (size_t)(brick[0] + order - 1) * (brick[1] + order - 1) * (brick[2] + order - 1) * 4 > 65536)
-    throw Error(HMD_ERR_INVALID, "SPME brick exceeds 64 KiB of LDS");
   hipStream_t s = c.stream;
   const uint32_t n0 = grid[0], n1 = grid[1], n2 = grid[2];
   nch = n2 / 2 + 1;
   M = (size_t)n0 * n1 * n2;
-  n_bins = nb[0] * nb[1] * nb[2];
-  atom_bin.alloc(n);
-  bin_count.alloc(n_bins);
-  bin_start.alloc(n_bins + 1);
-  bin_atoms.alloc(n);
-  bin_scan.init(n_bins);
+  alloc_bins();
   rho.alloc(M);
   V.alloc(M);
   mesh_c.alloc(2 * (size_t)n0 * n1 * nch);
@@ -475,11 +457,38 @@ void ForceField::init_spme(const hmd_forcefield_desc& d) {
     u_self_total += u;
   }
   up(u_self, us);
-  const uint32_t cus = (uint32_t)c.num_cus;
-  pass_r2c = plan_lines(n2 / 2, n0 * n1, cus, true);
-  pass_c2r = plan_lines(n2 / 2, n0 * n1, cus, false);
-  pass_b = plan_axis(n1, n0, nch, cus);
-  pass_a = plan_axis(n0, n1, nch, cus, 2);
+  plan_fft();
+}
+
+void ForceField::check_brick(const uint32_t b[3]) const {
+  for (int k = 0; k < 3; ++k)
+    if (b[k] == 0 || grid[k] % b[k] != 0 || b[k] < order)
+      throw Error(HMD_ERR_INVALID, "SPME brick must divide the grid and be >= order");
+  if ((size_t)b[0] * b[1] * b[2] * 8 > 65536 ||
+      (size_t)(b[0] + order - 1) * (b[1] + order - 1) * (b[2] + order - 1) * 4 > 65536)
+    throw Error(HMD_ERR_INVALID, "SPME brick exceeds 64 KiB of LDS");
+}
+
+// Atom bins of the brick spread / gather for the current brick shape.
+void ForceField::alloc_bins() {
+  for (int k = 0; k < 3; ++k) nb[k] = grid[k] / brick[k];
+  n_bins = nb[0] * nb[1] * nb[2];
+  atom_bin.alloc(n);
+  bin_count.alloc(n_bins);
+  bin_start.alloc(n_bins + 1);
+  bin_atoms.alloc(n);
+  bin_scan.init(n_bins);
+}
+
+// Launch shapes of the five FFT passes (fft_blocks_per_cu workgroups per
+// CU as the target) and the a-axis pass's energy partials.
+void ForceField::plan_fft() {
+  const uint32_t n0 = grid[0], n1 = grid[1], n2 = grid[2];
+  const uint32_t target = (uint32_t)c.num_cus * fft_blocks_per_cu;
+  pass_r2c = plan_lines(n2 / 2, n0 * n1, target, true);
+  pass_c2r = plan_lines(n2 / 2, n0 * n1, target, false);
+  pass_b = plan_axis(n1, n0, nch, target);
+  pass_a = plan_axis(n0, n1, nch, target, 2);
   if (pass_a.p2) {
     partial_blocks = pass_a.grid;
   } else {

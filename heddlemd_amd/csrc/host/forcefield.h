@@ -44,6 +44,22 @@ struct ForceField {
 #define HMD_PAIR_BLOCKS_PER_CU 32
 #endif
   uint32_t pair_blocks_per_cu = HMD_PAIR_BLOCKS_PER_CU;
+// Workgroups per CU the FFT passes aim for (several, so that one group's
+// loads overlap another's transform).
+#ifndef HMD_FFT_BLOCKS_PER_CU
+#define HMD_FFT_BLOCKS_PER_CU 3
+#endif
+  uint32_t fft_blocks_per_cu = HMD_FFT_BLOCKS_PER_CU;
+  // Launch-shape autotuner: times candidate pair grids, FFT grids and SPME
+  // bricks on this system and keeps the fastest; set_tuning applies a
+  // recorded choice (so that a run can repeat another's bytes).
+  void autotune(uint32_t n_evals, hmd_tuning* out);
+  void set_tuning(const hmd_tuning& t);
+  void get_tuning(hmd_tuning* t) const;
+  double time_compute(uint32_t n_evals);
+  void check_brick(const uint32_t b[3]) const;
+  void alloc_bins();
+  void plan_fft();
   // run-time composed pair term (null: the built-in kernels)
   std::unique_ptr<JitPairModule> jit;
   DevBuf<float> user_params;

@@ -300,6 +300,33 @@ hmd_status hmd_settle_before_drift(hmd_settle* s);
 hmd_status hmd_settle_after_drift(hmd_settle* s, float dt);
 hmd_status hmd_settle_after_kick(hmd_settle* s, float dt);
 
+/* ---- launch-shape autotuner ----------------------------------------------
+ * No counterpart in the reference (its launch shapes are fixed).  The
+ * shapes a force field launches with:
+ *   pair_blocks_per_cu  pair-kernel workgroups per CU (capped by the list's
+ *                       entries).  The f32 per-wave force sums follow the
+ *                       work split, so each value gives reproducible results
+ *                       that differ from another value's in the last bits;
+ *   fft_blocks_per_cu   FFT-pass workgroups per CU (forces unchanged);
+ *   brick[3]            SPME spread / gather brick (forces unchanged: the
+ *                       mesh sums are exact integers).
+ * hmd_forcefield_autotune times candidates of each (coordinate descent,
+ * n_evals eager force evaluations per candidate on the current positions)
+ * and keeps the fastest; hmd_forcefield_set_tuning applies a recorded
+ * choice, so that a later run repeats a tuned run's bytes.  Both invalidate
+ * captured step graphs. */
+typedef struct hmd_tuning {
+  uint32_t pair_blocks_per_cu;
+  uint32_t fft_blocks_per_cu;  /* 0 without SPME */
+  uint32_t brick[3];           /* 0 without SPME */
+  uint32_t n_candidates;       /* autotune: configurations timed */
+  double us_initial;           /* autotune: us per evaluation with the shapes at entry */
+  double us_tuned;             /* autotune: us per evaluation with the chosen shapes */
+} hmd_tuning;
+hmd_status hmd_forcefield_autotune(hmd_forcefield* f, uint32_t n_evals, hmd_tuning* out);
+hmd_status hmd_forcefield_set_tuning(hmd_forcefield* f, const hmd_tuning* t);
+hmd_status hmd_forcefield_get_tuning(const hmd_forcefield* f, hmd_tuning* out);
+
 /* ---- replica batches -----------------------------------------------------
  * Independent systems stepped together (the batched-replica workload; no
  * counterpart in the reference, which runs one system per process).  Each
