@@ -153,6 +153,8 @@ def main():
     ap.add_argument("--workload", choices=["water196k", "replicas"], default="water196k",
                     help="replicas: the 64-box batched-replica workload instead of the "
                          "196,608-atom box")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the launch-shape autotuner on the box before timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -168,6 +170,10 @@ def main():
     t_setup = time.perf_counter()
     ctx, sysw, ff, part, field, integ = build_system(api, systems, U)
     log(f"setup {time.perf_counter() - t_setup:.1f} s")
+    tuned = None
+    if args.autotune:
+        tuned = field.autotune(20)
+        log(f"autotune: {tuned}")
     run = overflow_safe(integ.step if args.eager else integ.replay)
     if not args.eager:
         integ.capture()
@@ -237,6 +243,9 @@ def main():
                    "atoms": int(sysw["n"]),
                    "atom_steps_per_s": round(sysw["n"] * 1e3 / ms_val, 1),
                    "overflow_reruns": run.reruns,
+                   "launch_shapes": {k: v for k, v in dict(field.tuning(),
+                                                            autotuned=tuned is not None).items()
+                                     if not k.startswith(("us_", "n_cand"))},
                    "plain_128": plain, "nn_64": nn},
         "roofline": roofline,
         "cpu_baseline": cpu,

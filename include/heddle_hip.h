@@ -310,8 +310,9 @@ hmd_status hmd_settle_after_kick(hmd_settle* s, float dt);
  *   fft_blocks_per_cu   FFT-pass workgroups per CU (forces unchanged);
  *   brick[3]            SPME spread / gather brick (forces unchanged: the
  *                       mesh sums are exact integers).
- * hmd_forcefield_autotune times candidates of each (coordinate descent,
- * n_evals eager force evaluations per candidate on the current positions)
+ * hmd_forcefield_autotune times candidates of each (coordinate descent;
+ * per candidate the best of three runs of n_evals eager force evaluations on
+ * the current positions)
  * and keeps the fastest; hmd_forcefield_set_tuning applies a recorded
  * choice, so that a later run repeats a tuned run's bytes.  Both invalidate
  * captured step graphs. */
