@@ -235,7 +235,8 @@ class ForceField:
     def _tuning_dict(t):
         return {"pair_blocks_per_cu": t.pair_blocks_per_cu,
                 "fft_blocks_per_cu": t.fft_blocks_per_cu,
-                "brick": tuple(t.brick), "n_candidates": t.n_candidates,
+                "brick": tuple(t.brick), "spme_stream": t.spme_stream,
+                "n_candidates": t.n_candidates,
                 "us_initial": t.us_initial, "us_tuned": t.us_tuned}
 
     def autotune(self, n_evals=20):
@@ -258,6 +259,7 @@ class ForceField:
         br = tuning.get("brick", (0, 0, 0))
         for k in range(3):
             t.brick[k] = int(br[k])
+        t.spme_stream = int(tuning.get("spme_stream", 1))
         check(lib().hmd_forcefield_set_tuning(self.h, ctypes.byref(t)))
 
     def request_rebuild(self):

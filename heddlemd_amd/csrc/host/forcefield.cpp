@@ -290,6 +290,11 @@ ForceField::ForceField(Context& ctx, Particles& part, const hmd_forcefield_desc&
   if (spme_on) init_spme(d);
   check(hipStreamSynchronize(s), "sync");
   probe_capacity();
+  if (spme_on) {   // the SPME stream, made here rather than during a capture
+    check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "hipEventCreate");
+  }
   p.observers.push_back(this);   // box changes reach this force field
 }
 
@@ -549,6 +554,10 @@ void ForceField::on_box_change() {
 ForceField::~ForceField() {
   auto& obs = p.observers;
   obs.erase(std::remove(obs.begin(), obs.end(), this), obs.end());
+  if (side) (void)hipStreamSynchronize(side);
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
+  if (side) (void)hipStreamDestroy(side);
 }
 
 void ForceField::request_rebuild() {
@@ -812,8 +821,29 @@ void ForceField::compute(int level) {
   const bool fev = level != 0;
   neighbor_pre_step();
   PairFinalizeParams fp{};
+  // SPME on a second stream beside the pair kernel: the pair kernel is
+  // VALU-bound, the spread LDS-bound and the FFT passes latency-bound, so
+  // they share the CUs.  The two write disjoint buffers (fixed-point pair
+  // accumulators; slow-class forces, energies and mesh), which the finalize
+  // combines after the join: the result is the same bytes either way.  Not
+  // while recording a batched replica step or marking per-stage events.
+  const bool fork = spme_on && spme_stream && side && !c.rec && !c.profiling;
+  if (fork) {
+    check(hipEventRecord(ev_fork, c.stream), "hipEventRecord");
+    check(hipStreamWaitEvent(side, ev_fork, 0), "hipStreamWaitEvent");
+    std::swap(c.stream, side);
+    try {
+      spme_forces(fev);
+    } catch (...) {
+      std::swap(c.stream, side);
+      throw;
+    }
+    std::swap(c.stream, side);
+    check(hipEventRecord(ev_join, side), "hipEventRecord");
+  }
   pair_forces(fev);
-  if (spme_on) spme_forces(fev);
+  if (spme_on && !fork) spme_forces(fev);
+  if (fork) check(hipStreamWaitEvent(c.stream, ev_join, 0), "hipStreamWaitEvent");
   if (mlp) mlp_correct(*mlp, *this);
   fp.acc = acc.ptr;
   fp.acc_stride = n_pad;

@@ -50,6 +50,10 @@ struct ForceField {
 #define HMD_FFT_BLOCKS_PER_CU 3
 #endif
   uint32_t fft_blocks_per_cu = HMD_FFT_BLOCKS_PER_CU;
+  // SPME enqueued on a second stream beside the pair kernel (fork / join)
+  uint32_t spme_stream = 1;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // Launch-shape autotuner: times candidate pair grids, FFT grids and SPME
   // bricks on this system and keeps the fastest; set_tuning applies a
   // recorded choice (so that a run can repeat another's bytes).

@@ -1,5 +1,5 @@
-// Launch-shape autotuner: the pair grid, the FFT pass grids and the SPME
-// brick are timed on the force field's own system and the fastest kept.
+// Launch-shape autotuner: the pair grid, the FFT pass grids, the SPME
+// brick and whether SPME runs on its own stream are timed on the force field's own system and the fastest kept.
 // None of these shapes is visible in the reference, whose launches are
 // fixed; the candidates are the ones the shapes' costs trade between
 // (occupancy against tail waves, LDS footprint against halo overhead).
@@ -51,6 +51,7 @@ void ForceField::get_tuning(hmd_tuning* t) const {
   if (spme_on) {
     t->fft_blocks_per_cu = fft_blocks_per_cu;
     for (int k = 0; k < 3; ++k) t->brick[k] = brick[k];
+    t->spme_stream = spme_stream;
   }
 }
 
@@ -61,10 +62,12 @@ void ForceField::set_tuning(const hmd_tuning& t) {
     if (t.fft_blocks_per_cu == 0 || t.fft_blocks_per_cu > 64)
       throw Error(HMD_ERR_INVALID, "fft_blocks_per_cu must be 1..64");
     check_brick(t.brick);
+    if (t.spme_stream > 1) throw Error(HMD_ERR_INVALID, "spme_stream must be 0 or 1");
   }
   check(hipStreamSynchronize(c.stream), "sync");   // nothing in flight uses the old buffers
   pair_blocks_per_cu = t.pair_blocks_per_cu;
   if (spme_on) {
+    spme_stream = t.spme_stream;
     if (t.fft_blocks_per_cu != fft_blocks_per_cu) {
       fft_blocks_per_cu = t.fft_blocks_per_cu;
       plan_fft();
@@ -136,6 +139,11 @@ void ForceField::autotune(uint32_t n_evals, hmd_tuning* out) {
       if (v == best.fft_blocks_per_cu) continue;
       hmd_tuning t = best;
       t.fft_blocks_per_cu = v;
+      trial(t);
+    }
+    {
+      hmd_tuning t = best;
+      t.spme_stream = best.spme_stream ? 0u : 1u;
       trial(t);
     }
     for (const auto& b : brick_candidates(*this)) {

@@ -309,7 +309,10 @@ hmd_status hmd_settle_after_kick(hmd_settle* s, float dt);
  *                       that differ from another value's in the last bits;
  *   fft_blocks_per_cu   FFT-pass workgroups per CU (forces unchanged);
  *   brick[3]            SPME spread / gather brick (forces unchanged: the
- *                       mesh sums are exact integers).
+ *                       mesh sums are exact integers);
+ *   spme_stream         1: SPME runs on a second stream beside the pair
+ *                       kernel (fork / join in the step graph), 0: after it
+ *                       on one stream (results unchanged).
  * hmd_forcefield_autotune times candidates of each (coordinate descent;
  * per candidate the best of three runs of n_evals eager force evaluations on
  * the current positions)
@@ -320,6 +323,7 @@ typedef struct hmd_tuning {
   uint32_t pair_blocks_per_cu;
   uint32_t fft_blocks_per_cu;  /* 0 without SPME */
   uint32_t brick[3];           /* 0 without SPME */
+  uint32_t spme_stream;        /* 0 without SPME */
   uint32_t n_candidates;       /* autotune: configurations timed */
   double us_initial;           /* autotune: us per evaluation with the shapes at entry */
   double us_tuned;             /* autotune: us per evaluation with the chosen shapes */
