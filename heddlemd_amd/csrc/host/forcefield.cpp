@@ -570,6 +570,13 @@ void ForceField::request_rebuild() {
 
 // ----------------------------------------------------------------- step ---
 void ForceField::rebuild_chain() {
+  rebuild_sort();
+  rebuild_list();
+}
+
+// Cell keys and the cell sort (gated on a rebuild), then the sorted
+// positions (every step).
+void ForceField::rebuild_sort() {
   const uint32_t* gate = status.ptr;
   const HmdLattice L = p.lattice;
   NblKeyParams kp{p.posq.ptr, n, L, g[0], g[1], g[2], grid_bits, atom_key.ptr, cell_count.ptr,
@@ -583,6 +590,13 @@ void ForceField::rebuild_chain() {
                      posq_sorted.ptr, type_sorted.ptr, ref_pos.ptr, gate, p.lattice,
                      image_sorted.ptr};
   launch(c, K_hmd_nbl_gather, ceil_div(n_pad, 256), 256, 0, gp);
+}
+
+// Cluster boxes, the cluster-pair list and the work list (gated on a
+// rebuild).
+void ForceField::rebuild_list() {
+  const uint32_t* gate = status.ptr;
+  const HmdLattice L = p.lattice;
   NblBboxParams bp{posq_sorted.ptr, n, n_clusters, bbox.ptr, sub_bbox.ptr, gate};
   launch(c, K_hmd_nbl_bbox, ceil_div(n_clusters, 4), 256, 0, bp);
   NblBuildParams bd{};
@@ -617,12 +631,14 @@ void ForceField::rebuild_chain() {
   launch(c, K_hmd_nbl_close, 1, 64, 0, wp);
 }
 
-void ForceField::neighbor_pre_step() {
+void ForceField::neighbor_pre_step(const std::function<void()>& after_sort) {
   NblDispParams dp{p.posq.ptr, ref_pos.ptr, n, p.lattice,
                    0.25f * r_skin * r_skin, status.ptr};
   launch(c, K_hmd_nbl_displacement, ceil_div(n, 256), 256, 0, dp);
   c.mark("nbl_displacement");
-  rebuild_chain();
+  rebuild_sort();
+  if (after_sort) after_sort();
+  rebuild_list();
   c.mark("nbl_rebuild");
 }
 
@@ -819,16 +835,18 @@ void ForceField::spme_forces(bool fev) {
 
 void ForceField::compute(int level) {
   const bool fev = level != 0;
-  neighbor_pre_step();
   PairFinalizeParams fp{};
-  // SPME on a second stream beside the pair kernel: the pair kernel is
-  // VALU-bound, the spread LDS-bound and the FFT passes latency-bound, so
-  // they share the CUs.  The two write disjoint buffers (fixed-point pair
+  // SPME on a second stream: forked once the sorted positions are written
+  // (it reads nothing else of the list), so that on a rebuild step it runs
+  // beside the list build, and on every step beside the pair kernel.  The
+  // pair kernel is VALU-bound, the list build latency-bound, the spread
+  // LDS-bound and the FFT passes latency-bound, so they share the CUs.  The
+  // two sides write disjoint buffers (list and fixed-point pair
   // accumulators; slow-class forces, energies and mesh), which the finalize
   // combines after the join: the result is the same bytes either way.  Not
   // while recording a batched replica step or marking per-stage events.
   const bool fork = spme_on && spme_stream && side && !c.rec && !c.profiling;
-  if (fork) {
+  auto fork_spme = [&]() {
     check(hipEventRecord(ev_fork, c.stream), "hipEventRecord");
     check(hipStreamWaitEvent(side, ev_fork, 0), "hipStreamWaitEvent");
     std::swap(c.stream, side);
@@ -840,7 +858,9 @@ void ForceField::compute(int level) {
     }
     std::swap(c.stream, side);
     check(hipEventRecord(ev_join, side), "hipEventRecord");
-  }
+  };
+  if (fork) neighbor_pre_step(fork_spme);
+  else neighbor_pre_step();
   pair_forces(fev);
   if (spme_on && !fork) spme_forces(fev);
   if (fork) check(hipStreamWaitEvent(c.stream, ev_join, 0), "hipStreamWaitEvent");

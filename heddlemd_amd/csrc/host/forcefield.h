@@ -1,5 +1,6 @@
 // ForceField host object: potential slots, neighbour list, SPME mesh.
 #pragma once
+#include <functional>
 #include <memory>
 
 #include "hmd_internal.h"
@@ -126,7 +127,9 @@ struct ForceField {
   void on_box_change();
   ~ForceField();
   void rebuild_chain();
-  void neighbor_pre_step();
+  void rebuild_sort();
+  void rebuild_list();
+  void neighbor_pre_step(const std::function<void()>& after_sort = {});
   void pair_forces(bool fev);
   void spme_forces(bool fev);
   // One force evaluation, enqueued only (no host synchronisation): safe to
