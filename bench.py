@@ -148,8 +148,9 @@ def main():
     ap.add_argument("--train-steps", type=int, default=50, help="Adam steps (batches of 8)")
     ap.add_argument("--no-nn", action="store_true", help="time the plain leg only")
     ap.add_argument("--eager", action="store_true",
-                    help="time eager launches instead of graph replays (for kernel "
-                         "tracers that do not follow graph launches)")
+                    help="time eager launches instead of graph replays, with SPME on "
+                         "the main stream (for kernel tracers that do not follow graph "
+                         "launches: each kernel's duration without a concurrent one)")
     ap.add_argument("--workload", choices=["water196k", "replicas"], default="water196k",
                     help="replicas: the 64-box batched-replica workload instead of the "
                          "196,608-atom box")
@@ -170,6 +171,8 @@ def main():
     t_setup = time.perf_counter()
     ctx, sysw, ff, part, field, integ = build_system(api, systems, U)
     log(f"setup {time.perf_counter() - t_setup:.1f} s")
+    if args.eager:
+        field.set_tuning(dict(field.tuning(), spme_stream=0))
     tuned = None
     if args.autotune:
         tuned = field.autotune(20)
@@ -239,7 +242,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{leg} leg: 196,608-atom rigid SPC water, compensated VV NVE "
                                "300 K, dt 2 fs, r_c 1.0 nm, skin 0.1 nm",
-                   "launch": "eager" if args.eager else "hipGraph replay",
+                   "launch": ("eager, SPME on the main stream" if args.eager
+                              else "hipGraph replay, SPME forked beside list build and pair"),
                    "atoms": int(sysw["n"]),
                    "atom_steps_per_s": round(sysw["n"] * 1e3 / ms_val, 1),
                    "overflow_reruns": run.reruns,
@@ -328,6 +332,8 @@ def nn_leg(api, systems, U, args, ctx, sysw, part, fine, integ_fine, run_fine):
     64^3 leg with the fine-mesh judge every 50 steps."""
     ff64 = systems.water_forcefield(grid=(64, 64, 64), exclusions=sysw["exclusions"])
     coarse = api.ForceField(ctx, part, ff64)
+    if args.eager:
+        coarse.set_tuning(dict(coarse.tuning(), spme_stream=0))
     net = api.CorrectionNet(ctx, 1.0 * U.NM, 1.0, [0, 1], seed=31, lr=1e-3)
     t0 = time.perf_counter()
     net.reserve(part.n, args.train_frames)

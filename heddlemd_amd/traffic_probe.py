@@ -34,6 +34,8 @@ def main():
     part = api.ParticleBuffers(ctx, sysw["posq"], sysw["mass"], sysw["lattice"],
                                types=sysw["types"])
     field = api.ForceField(ctx, part, ff)
+    # SPME on the main stream: every kernel's counters without a concurrent one
+    field.set_tuning(dict(field.tuning(), spme_stream=0))
     integ = api.Integrator(ctx, part, field, dt=2.0 * U.FS, compensated=True,
                            settle=sysw["settle"])
     integ.minimize(20, 0.1 * U.ANGSTROM)
