@@ -191,9 +191,14 @@ class ForceField:
             d.spline_order = int(sp.get("order", 4))
         d.r_skin = ff["r_skin"]
         if ff.get("exclusions") is not None:
-            off, par = ff["exclusions"]
+            # (offsets, partners) or (offsets, partners, lj_scale, coul_scale)
+            ex = ff["exclusions"]
+            off, par = ex[0], ex[1]
             d.excl_offsets = _ptr(arr(off, np.uint32), _u32p)
             d.excl_partners = _ptr(arr(par, np.uint32), _u32p)
+            if len(ex) == 4:
+                d.excl_lj_scale = _ptr(arr(ex[2], np.float32), _fp)
+                d.excl_coul_scale = _ptr(arr(ex[3], np.float32), _fp)
         self._keep = keep
         h = ctypes.c_void_p()
         check(lib().hmd_forcefield_create(ctx.h, particles.h, ctypes.byref(d), ctypes.byref(h)))

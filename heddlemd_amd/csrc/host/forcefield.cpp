@@ -273,11 +273,35 @@ ForceField::ForceField(Context& ctx, Particles& part, const hmd_forcefield_desc&
         throw Error(HMD_ERR_UNSUPPORTED, "more than 8 exclusion partners for one atom");
     }
     epar.assign(d.excl_partners, d.excl_partners + eoff[n]);
-    for (uint32_t m = 0; m < eoff[n]; ++m) {
-      if ((d.excl_lj_scale && d.excl_lj_scale[m] != 0.0f) ||
-          (d.excl_coul_scale && d.excl_coul_scale[m] != 0.0f))
-        throw Error(HMD_ERR_UNSUPPORTED, "exclusion scales other than 0");
+    for (uint32_t m = 0; m < eoff[n]; ++m)
       if (epar[m] >= n) throw Error(HMD_ERR_INVALID, "exclusion partner out of range");
+    // pairs with a non-zero scale (taken from i's list for i < j), added
+    // back after the fused kernel by hmd_pair_scaled
+    std::vector<uint32_t> si, sj;
+    std::vector<float> slj, sco;
+    for (uint32_t i = 0; i < n; ++i)
+      for (uint32_t m = eoff[i]; m < eoff[i + 1]; ++m) {
+        const float a = d.excl_lj_scale ? d.excl_lj_scale[m] : 0.0f;
+        const float b = d.excl_coul_scale ? d.excl_coul_scale[m] : 0.0f;
+        if (epar[m] <= i || (a == 0.0f && b == 0.0f)) continue;
+        if (!std::isfinite(a) || !std::isfinite(b))
+          throw Error(HMD_ERR_INVALID, "exclusion scale not finite");
+        si.push_back(i);
+        sj.push_back(epar[m]);
+        slj.push_back(lj_on ? a : 0.0f);
+        sco.push_back(coul_on ? b : 0.0f);
+      }
+    n_scaled = (uint32_t)si.size();
+    if (n_scaled) {
+      sc_i.alloc(n_scaled, false);
+      sc_j.alloc(n_scaled, false);
+      sc_lj.alloc(n_scaled, false);
+      sc_coul.alloc(n_scaled, false);
+      sc_i.upload(si.data(), n_scaled, s);
+      sc_j.upload(sj.data(), n_scaled, s);
+      sc_lj.upload(slj.data(), n_scaled, s);
+      sc_coul.upload(sco.data(), n_scaled, s);
+      check(hipStreamSynchronize(s), "sync");
     }
   }
   if (epar.empty()) epar.push_back(0u);
@@ -688,6 +712,32 @@ void ForceField::pair_forces(bool fev) {
           "composed pair kernel");
   } else {
     launch(c, k, blocks, 256, 0, pp);
+  }
+  if (n_scaled) {
+    PairScaledParams q{};
+    q.posq = p.posq.ptr;
+    q.type = p.type.ptr;
+    q.slot_of = slot_of.ptr;
+    q.pair_i = sc_i.ptr;
+    q.pair_j = sc_j.ptr;
+    q.s_lj = sc_lj.ptr;
+    q.s_coul = sc_coul.ptr;
+    q.n_pairs = n_scaled;
+    q.n_types = n_types;
+    q.L = p.lattice;
+    q.lj_c6c12 = lj_table.ptr;
+    q.rc2 = rc2;
+    q.lj_rs2 = lj_rs2;
+    q.lj_inv_delta = lj_inv_delta;
+    q.k_coulomb = k_coulomb;
+    q.coul_table = coul_table.ptr;
+    q.coul_inv_h = coul_inv_h;
+    q.lj_on = lj_on ? 1u : 0u;
+    q.coul_on = coul_on ? 1u : 0u;
+    q.fev = fev ? 1u : 0u;
+    q.acc = acc.ptr;
+    q.acc_stride = n_pad;
+    launch(c, K_hmd_pair_scaled, ceil_div(n_scaled, 256), 256, 0, q);
   }
   c.mark("pair_force");
 }

@@ -41,6 +41,10 @@ struct ForceField {
   float coul_inv_h = 0;
   void build_coulomb_table(double alpha, double r_cut, hipStream_t s);
   DevBuf<uint32_t> excl_off, excl_par;
+  // listed pairs with non-zero exclusion scales (i < j)
+  uint32_t n_scaled = 0;
+  DevBuf<uint32_t> sc_i, sc_j;
+  DevBuf<float> sc_lj, sc_coul;
 #ifndef HMD_PAIR_BLOCKS_PER_CU
 #define HMD_PAIR_BLOCKS_PER_CU 32
 #endif

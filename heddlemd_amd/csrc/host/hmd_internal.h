@@ -22,7 +22,7 @@
   X(hmd_nbl_worklist) X(hmd_nbl_close) X(hmd_nbl_pairs)                     \
   X(hmd_pair_force_f) X(hmd_pair_force_fev) X(hmd_pair_force_lj_f)           \
   X(hmd_pair_force_lj_fev) X(hmd_pair_force_coul_f) X(hmd_pair_force_coul_fev)\
-  X(hmd_pair_finalize)                                                       \
+  X(hmd_pair_finalize) X(hmd_pair_scaled)                                   \
   X(hmd_spme_bin) X(hmd_spme_bin_scatter)                                    \
   X(hmd_spme_spread_p4) X(hmd_spme_gather_p4)                                \
   X(hmd_spme_spread_p5) X(hmd_spme_gather_p5)                                \

@@ -184,6 +184,31 @@ struct PairParams {
   uint32_t acc_stride;         // n_pad
 };
 
+// Listed partner pairs with a non-zero exclusion scale (1-4 pairs and the
+// like): the fused kernel skips every listed partner, and this pass adds
+// s_lj x LJ + s_coul x real-space Coulomb for each scaled pair back into the
+// same fixed-point accumulators.
+struct PairScaledParams {
+  const float* posq;            // float4[n], canonical order
+  const uint32_t* type;         // [n]
+  const uint32_t* slot_of;      // canonical -> sorted slot
+  const uint32_t* pair_i;       // [n_pairs], i < j
+  const uint32_t* pair_j;
+  const float* s_lj;            // [n_pairs]
+  const float* s_coul;
+  uint32_t n_pairs;
+  uint32_t n_types;
+  HmdLattice L;
+  const float* lj_c6c12;        // float2 per type pair
+  float rc2, lj_rs2, lj_inv_delta;
+  float k_coulomb;
+  const float* coul_table;      // float4 cubic coefficients, force then energy
+  float coul_inv_h;
+  uint32_t lj_on, coul_on, fev;
+  long long* acc;
+  uint32_t acc_stride;
+};
+
 struct PairFinalizeParams {
   const long long* acc;
   uint32_t acc_stride;

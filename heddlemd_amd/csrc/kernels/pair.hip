@@ -357,3 +357,68 @@ __device__ __forceinline__ void hmd_pair_finalize_body(const PairFinalizeParams&
   }
 }
 HMD_KERNEL(hmd_pair_finalize, PairFinalizeParams, 256)
+
+// Scaled exclusions: one thread per listed pair with a non-zero scale, the
+// same per-pair arithmetic as the fused kernel (full minimum image, the
+// cutoff mask, LJ 12-6 with the C1 switch, the tabulated real-space Ewald
+// term) times the pair's scales, the exclusion-scaled sum of the reference
+// (src/forces/jit_composed.rs:939-972 with heddle_jit_exclusion_scale
+// :1260-1272).  Fixed-point atomics make the totals independent of order.
+__device__ __forceinline__ void hmd_pair_scaled_body(const PairScaledParams& p,
+                                                     const uint32_t hmd_bx,
+                                                     const uint32_t hmd_gx) {
+  const uint32_t k = hmd_bx * blockDim.x + threadIdx.x;
+  if (k >= p.n_pairs) return;
+  const uint32_t i = p.pair_i[k], j = p.pair_j[k];
+  const float4 a = reinterpret_cast<const float4*>(p.posq)[i];
+  const float4 b = reinterpret_cast<const float4*>(p.posq)[j];
+  float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  hmd_min_image(dx, dy, dz, p.L);
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  if (!(r2 <= p.rc2) || r2 == 0.0f) return;
+  const float inv_r = __builtin_amdgcn_rsqf(r2);
+  const float inv_r2 = inv_r * inv_r;
+  float factor = 0.0f, energy = 0.0f;
+  if (p.coul_on && p.s_coul[k] != 0.0f) {
+    const float qq = p.k_coulomb * a.w * b.w;
+    const float x = fminf(r2 * inv_r * p.coul_inv_h, (float)HMD_COUL_BINS - 0.01f);
+    const float t = __builtin_amdgcn_fractf(x);
+    const uint32_t bin = (uint32_t)x;
+    const float4 g = reinterpret_cast<const float4*>(p.coul_table)[bin];
+    factor += p.s_coul[k] * (qq * (((g.w * t + g.z) * t + g.y) * t + g.x));
+    if (p.fev) {
+      const float4 e = reinterpret_cast<const float4*>(p.coul_table)[HMD_COUL_BINS + bin];
+      energy += p.s_coul[k] * (qq * (((e.w * t + e.z) * t + e.y) * t + e.x));
+    }
+  }
+  if (p.lj_on && p.s_lj[k] != 0.0f) {
+    const float2 cc =
+        reinterpret_cast<const float2*>(p.lj_c6c12)[p.type[i] * p.n_types + p.type[j]];
+    const float ir6 = inv_r2 * inv_r2 * inv_r2;
+    const float e_lj = (cc.y * ir6 - cc.x) * ir6;
+    const float f_lj = (12.0f * cc.y * ir6 - 6.0f * cc.x) * ir6 * inv_r2;
+    const float tau = fmaxf((r2 - p.lj_rs2) * p.lj_inv_delta, 0.0f);
+    const float omt = 1.0f - tau;
+    const float sw = omt * omt * (1.0f + 2.0f * tau);
+    const float chain = (12.0f * p.lj_inv_delta * tau) * omt;
+    factor += p.s_lj[k] * (sw * f_lj + chain * e_lj);
+    if (p.fev) energy += p.s_lj[k] * (sw * e_lj);
+  }
+  const uint32_t si = p.slot_of[i], sj = p.slot_of[j], st = p.acc_stride;
+  const float f3[3] = {factor * dx, factor * dy, factor * dz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const long long v = hmd_to_fixed(f3[d], HMD_FIXED_SHIFT);
+    hmd_atomic_add_fixed(&p.acc[d * st + si], v);
+    hmd_atomic_add_fixed(&p.acc[d * st + sj], -v);
+  }
+  if (p.fev) {
+    const long long he = hmd_to_fixed(0.5f * energy, HMD_FIXED_SHIFT);
+    const long long hw = hmd_to_fixed(0.5f * factor * r2, HMD_FIXED_SHIFT);
+    hmd_atomic_add_fixed(&p.acc[3 * st + si], he);
+    hmd_atomic_add_fixed(&p.acc[3 * st + sj], he);
+    hmd_atomic_add_fixed(&p.acc[4 * st + si], hw);
+    hmd_atomic_add_fixed(&p.acc[4 * st + sj], hw);
+  }
+}
+HMD_KERNEL(hmd_pair_scaled, PairScaledParams, 256)

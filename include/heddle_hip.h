@@ -155,9 +155,11 @@ typedef struct {
   /* [neighbor_list] (src/io/config.rs): r_skin */
   double r_skin;
   /* exclusion CSR over canonical atom ids, symmetric, with per-slot scales
-   * (src/forces/topology.rs:312-430).  Listed partners are fully excluded
-   * from the fused pair kernel; scales other than 0 are added back by an
-   * exact correction (not yet supported: HMD_ERR_UNSUPPORTED). */
+   * (src/forces/topology.rs:312-430; NULL scales = 0).  Listed partners are
+   * fully excluded from the fused pair kernel; a pair with a non-zero scale
+   * (read from the lower atom's list) gets scale x its LJ / real-space
+   * Coulomb term added back by a correction pass over the scaled pairs,
+   * under the same cutoff (jit_composed.rs:939-972). */
   const uint32_t* excl_offsets;  /* [n+1] or NULL */
   const uint32_t* excl_partners;
   const float* excl_lj_scale;

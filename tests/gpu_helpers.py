@@ -39,8 +39,11 @@ def replica_case(ctx, spec, relax=0, kind="velocity-verlet", temperature=0.0):
 
 def oracle_pair(sysw, ff, posq, erfc_mode=1):
     lj = ff["lj"]
-    off, par = ff["exclusions"]
-    ex = (off.astype(np.int32), par.astype(np.int32), np.zeros(par.size), np.zeros(par.size))
+    exl = ff["exclusions"]
+    off, par = exl[0], exl[1]
+    slj = np.asarray(exl[2], np.float64) if len(exl) == 4 else np.zeros(par.size)
+    sco = np.asarray(exl[3], np.float64) if len(exl) == 4 else np.zeros(par.size)
+    ex = (off.astype(np.int32), par.astype(np.int32), slj, sco)
     ps = oracle.PairSystem(posq[:, :3].astype(np.float64), posq[:, 3].astype(np.float64),
                            sysw["types"].astype(np.int32), sysw["lattice"],
                            lj=lj, coulomb=ff["coulomb"], exclusions=ex, erfc_mode=erfc_mode)
