@@ -154,6 +154,9 @@ def main():
     ap.add_argument("--workload", choices=["water196k", "replicas"], default="water196k",
                     help="replicas: the 64-box batched-replica workload instead of the "
                          "196,608-atom box")
+    ap.add_argument("--replica-modes", default="fused,per_replica,batched",
+                    help="replica batch modes to time (comma-separated; batched_eager "
+                         "issues the batched launches without a graph, for kernel tracers)")
     ap.add_argument("--autotune", action="store_true",
                     help="run the launch-shape autotuner on the box before timing")
     args = ap.parse_args()
@@ -291,10 +294,10 @@ def replica_workload(api, systems, U, args, minimize_iter=50):
     n_atoms = sum(r[4]["n"] for r in reps)
     log(f"replicas: {len(reps)} boxes, {n_atoms} atoms, setup {time.perf_counter() - t0:.1f} s")
     legs = {}
-    for mode in ("fused", "per_replica", "batched"):
+    for mode in args.replica_modes.split(","):
         rs = api.ReplicaSet([r[3] for r in reps], mode=mode)
         rs.step(args.warmup)
-        if mode == "batched":
+        if mode.startswith("batched"):
             nb, ns = rs.plan()
             log(f"replicas batched plan: {nb} batched + {ns} single launches per step")
         t0 = time.perf_counter()

@@ -402,10 +402,11 @@ class ReplicaSet:
     integrator must come from its own GpuContext.  mode "fused" captures every
     replica's step into one graph, "per_replica" launches one graph per
     replica on its own stream, "batched" merges the replicas' steps into one
-    launch per stage (each kernel's batched twin over all replicas' blocks).
-    fused=False is mode "per_replica"."""
+    launch per stage (each kernel's batched twin over all replicas' blocks),
+    "batched_eager" issues those launches directly every step (no graph; for
+    kernel tracers).  fused=False is mode "per_replica"."""
 
-    MODES = {"fused": 0, "per_replica": 1, "batched": 2}
+    MODES = {"fused": 0, "per_replica": 1, "batched": 2, "batched_eager": 2 | 4}
 
     def __init__(self, integrators, fused=True, mode=None):
         if mode is None:

@@ -292,6 +292,17 @@ struct ReplicaSet {
   DevBuf<unsigned char> batch_params;
   DevBuf<uint32_t> batch_starts;
   uint32_t n_batched = 0, n_single = 0;   // launches per step of the merged plan
+  // the merged plan as launches (kernel, shape, argument bytes); eager:
+  // issued directly every step instead of through a graph (kernel tracers)
+  struct Planned {
+    hipFunction_t fn;
+    uint32_t grid, block;
+    size_t shmem;
+    std::vector<unsigned char> args;
+    std::string name;
+  };
+  std::vector<Planned> planned;
+  bool eager = false;
   hipStream_t master = nullptr;
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> joins;
@@ -367,7 +378,7 @@ struct ReplicaSet {
   // next, as one batched launch over those replicas (ties go to the
   // replica with the most launches left).  Each replica's launches keep
   // their order, so every replica sees exactly its own step.
-  void capture_batched() {
+  void capture_batched(bool capture_graph = true) {
     for (auto* it : its) it->prime();
     release();
     const size_t R = its.size();
@@ -450,25 +461,26 @@ struct ReplicaSet {
       check(hipMemcpy(batch_params.ptr, hp.data(), hp.size(), hipMemcpyHostToDevice), "H2D");
     if (!hs.empty())
       check(hipMemcpy(batch_starts.ptr, hs.data(), hs.size() * 4, hipMemcpyHostToDevice), "H2D");
+    planned.clear();
+    for (const Item& it : plan) {
+      if (it.blocks) {
+        HmdBatch hb{batch_params.ptr + it.param_off, batch_starts.ptr + it.start_off,
+                    (uint32_t)it.members.size(), (uint32_t)it.first->params.size()};
+        const unsigned char* b = reinterpret_cast<const unsigned char*>(&hb);
+        planned.push_back(Planned{it.first->fn_batch, it.blocks, it.first->block, it.shmem,
+                                  std::vector<unsigned char>(b, b + sizeof hb),
+                                  std::string(kKernelNames[it.first->k]) + "_batch"});
+      } else {
+        for (const LaunchRec* L : it.members)
+          planned.push_back(Planned{L->fn, L->grid, L->block, L->shmem, L->params,
+                                    kKernelNames[L->k]});
+      }
+    }
+    for (auto* it : its) it->ff.graph_invalid = false;
+    if (!capture_graph) return;
     check(hipStreamBeginCapture(master, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
-      for (const Item& it : plan) {
-        if (it.blocks) {
-          HmdBatch hb{batch_params.ptr + it.param_off, batch_starts.ptr + it.start_off,
-                      (uint32_t)it.members.size(), (uint32_t)it.first->params.size()};
-          void* args[] = {&hb};
-          check(hipModuleLaunchKernel(it.first->fn_batch, it.blocks, 1, 1, it.first->block, 1, 1,
-                                      (unsigned)it.shmem, master, args, nullptr),
-                (std::string(kKernelNames[it.first->k]) + "_batch").c_str());
-        } else {
-          for (const LaunchRec* L : it.members) {
-            void* args[] = {const_cast<unsigned char*>(L->params.data())};
-            check(hipModuleLaunchKernel(L->fn, L->grid, 1, 1, L->block, 1, 1,
-                                        (unsigned)L->shmem, master, args, nullptr),
-                  kKernelNames[L->k]);
-          }
-        }
-      }
+      issue_planned();
     } catch (...) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(master, &g);
@@ -477,11 +489,26 @@ struct ReplicaSet {
     }
     check(hipStreamEndCapture(master, &graph), "hipStreamEndCapture");
     check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-    for (auto* it : its) it->ff.graph_invalid = false;
+  }
+
+  // The merged plan's launches, in order, on the master stream.
+  void issue_planned() {
+    for (Planned& L : planned) {
+      void* args[] = {L.args.data()};
+      check(hipModuleLaunchKernel(L.fn, L.grid, 1, 1, L.block, 1, 1, (unsigned)L.shmem, master,
+                                  args, nullptr),
+            L.name.c_str());
+    }
   }
 
   void step(uint32_t n) {
-    if (mode != Mode::PerReplica) {
+    if (mode == Mode::Batched && eager) {
+      bool stale = planned.empty();
+      for (auto* it : its) stale = stale || it->ff.graph_invalid;
+      if (stale) capture_batched(false);
+      for (uint32_t i = 0; i < n; ++i) issue_planned();
+      check(hipStreamSynchronize(master), "sync");
+    } else if (mode != Mode::PerReplica) {
       bool stale = !exec;
       for (auto* it : its) stale = stale || it->ff.graph_invalid;
       if (stale) {
@@ -824,7 +851,10 @@ hmd_status hmd_replicas_create(hmd_integrator* const* its, uint32_t n, uint32_t 
                                  : (flags & HMD_REPLICAS_PER_REPLICA_GRAPHS)
                                      ? ReplicaSet::Mode::PerReplica
                                      : ReplicaSet::Mode::Fused;
+      if ((flags & HMD_REPLICAS_EAGER) && m != ReplicaSet::Mode::Batched)
+        throw Error(HMD_ERR_INVALID, "HMD_REPLICAS_EAGER needs HMD_REPLICAS_BATCHED");
       h->set = new ReplicaSet(std::move(v), m);
+      h->set->eager = (flags & HMD_REPLICAS_EAGER) != 0;
     } catch (...) {
       delete h;
       throw;

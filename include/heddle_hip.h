@@ -352,6 +352,9 @@ hmd_status hmd_forcefield_get_tuning(const hmd_forcefield* f, hmd_tuning* out);
 typedef struct hmd_replicas hmd_replicas;
 #define HMD_REPLICAS_PER_REPLICA_GRAPHS 1u
 #define HMD_REPLICAS_BATCHED 2u
+/* with HMD_REPLICAS_BATCHED: issue the merged launches directly every step
+ * instead of through a graph (for kernel tracers; same results) */
+#define HMD_REPLICAS_EAGER 4u
 hmd_status hmd_replicas_create(hmd_integrator* const* integrators, uint32_t n, uint32_t flags,
                                hmd_replicas** out);
 void hmd_replicas_destroy(hmd_replicas* set);

@@ -54,7 +54,7 @@ def run(mode, log_dir=None):
             try:
                 if log_dir is None:
                     rs.step(STEPS)
-                    if mode == "batched":
+                    if mode.startswith("batched"):
                         nb, ns = rs.plan()
                         print(f"batched plan: {nb} batched launches, {ns} single", flush=True)
                         assert nb > 0
@@ -83,7 +83,7 @@ def run(mode, log_dir=None):
 
 def test_batched_steps_equal_solo_steps():
     alone = run("alone")
-    for mode in ("fused", "per_replica", "batched"):
+    for mode in ("fused", "per_replica", "batched", "batched_eager"):
         got = run(mode)
         for rid, (a, b) in zip(RIDS, zip(alone, got)):
             print(f"{mode}: replica {rid} positions/velocities identical: "
