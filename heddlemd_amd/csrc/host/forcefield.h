@@ -59,6 +59,7 @@ struct ForceField {
   uint32_t spme_stream = 1;
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  void make_side_stream();
   // Launch-shape autotuner: times candidate pair grids, FFT grids and SPME
   // bricks on this system and keeps the fastest; set_tuning applies a
   // recorded choice (so that a run can repeat another's bytes).

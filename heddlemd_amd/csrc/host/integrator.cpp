@@ -374,10 +374,13 @@ struct ReplicaSet {
 
   // Batched mode: record each replica's step, merge, capture one graph.
   // Merging is greedy over the replicas' next unissued launches: the
-  // (kernel, block size) that heads the most replica sequences is issued
-  // next, as one batched launch over those replicas (ties go to the
-  // replica with the most launches left).  Each replica's launches keep
-  // their order, so every replica sees exactly its own step.
+  // (kernel, block size) at the head of the replica with the most launches
+  // left is issued next, as one batched launch over every replica whose
+  // next launch it is (ties go to the head shared by more replicas).
+  // Replicas whose step has extra launches (a multi-tile scan) thus catch
+  // up before the others move on, and each stage stays one launch.  Each
+  // replica's launches keep their order, so every replica sees exactly its
+  // own step.
   void capture_batched(bool capture_graph = true) {
     for (auto* it : its) it->prime();
     release();
@@ -413,7 +416,7 @@ struct ReplicaSet {
         for (size_t q = 0; q < R; ++q)
           if (head[q] < recs[q].size() && same_stage(recs[q][head[q]], a)) ++count;
         const size_t left = recs[r].size() - head[r];
-        if (count > best_count || (count == best_count && left > best_left)) {
+        if (left > best_left || (left == best_left && count > best_count)) {
           best = (int)r;
           best_count = count;
           best_left = left;

@@ -68,6 +68,7 @@ void ForceField::set_tuning(const hmd_tuning& t) {
   pair_blocks_per_cu = t.pair_blocks_per_cu;
   if (spme_on) {
     spme_stream = t.spme_stream;
+    make_side_stream();
     if (t.fft_blocks_per_cu != fft_blocks_per_cu) {
       fft_blocks_per_cu = t.fft_blocks_per_cu;
       plan_fft();
