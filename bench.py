@@ -72,14 +72,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def overflow_safe(run):
+def overflow_safe(run, part, field):
     """run(n) that re-runs a batch which reported a neighbour-list overflow
     (HMD_ERR_NBL_OVERFLOW: the library has already grown the capacity and
-    the batch is to be re-run, as the C ABI specifies).  Returns the number
-    of re-runs through `run.reruns`."""
+    the batch is to be re-run, as the C ABI specifies).  The steps of such a
+    batch after the overflow ran on an incomplete list, so the particle state
+    from before the batch (positions, velocities, images, forces) is put back
+    and the list rebuilt before the re-run.  The state is saved before each
+    batch, or once by prepare() ahead of a timed batch run with
+    snapshot=False.  Returns the number of re-runs through `run.reruns`."""
     from heddlemd_amd._lib import HMD_ERR_NBL_OVERFLOW, HmdError
 
-    def wrapped(n):
+    def prepare():
+        wrapped.snap = part.download()
+
+    def wrapped(n, snapshot=True):
+        if snapshot or wrapped.snap is None:
+            prepare()
         for attempt in range(3):
             try:
                 return run(n)
@@ -87,8 +96,15 @@ def overflow_safe(run):
                 if e.code != HMD_ERR_NBL_OVERFLOW or attempt == 2:
                     raise
                 wrapped.reruns += 1
-                log(f"neighbour list overflow, batch re-run: {e}")
+                log(f"neighbour list overflow, batch re-run from its start: {e}")
+                st = wrapped.snap
+                part.upload(posq=st["posq"], vel=st["vel"])
+                part.upload_images(st["images"])
+                part.upload_forces(st["force"])
+                field.request_rebuild()
     wrapped.reruns = 0
+    wrapped.snap = None
+    wrapped.prepare = prepare
     return wrapped
 
 
@@ -180,14 +196,15 @@ def main():
     if args.autotune:
         tuned = field.autotune(20)
         log(f"autotune: {tuned}")
-    run = overflow_safe(integ.step if args.eager else integ.replay)
+    run = overflow_safe(integ.step if args.eager else integ.replay, part, field)
     if not args.eager:
         integ.capture()
     run(args.warmup)
     ctx.synchronize()
     gen0 = field.stats()["generation"]
+    run.prepare()
     t0 = time.perf_counter()
-    run(args.steps)
+    run(args.steps, snapshot=False)
     ctx.synchronize()
     el = time.perf_counter() - t0
     gen1 = field.stats()["generation"]
@@ -363,7 +380,7 @@ def nn_leg(api, systems, U, args, ctx, sysw, part, fine, integ_fine, run_fine):
     net.attach(coarse)
     integ = api.Integrator(ctx, part, coarse, dt=DT_FS * U.FS, compensated=True,
                            settle=sysw["settle"])
-    run = overflow_safe(integ.step if args.eager else integ.replay)
+    run = overflow_safe(integ.step if args.eager else integ.replay, part, coarse)
     if not args.eager:
         integ.capture()
     run(args.warmup)

@@ -220,12 +220,20 @@ hmd_status hmd_particles_upload(hmd_particles* hp, const float* posq, const floa
   HMD_GUARD({
     Particles& p = hp->p;
     hipStream_t s = p.ctx->stream;
-    if (posq) p.posq.upload(posq, 4 * (size_t)p.n, s);
+    // the uploaded f32 values are the whole new state: the compensated low
+    // parts of what is replaced start again at zero
+    if (posq) {
+      p.posq.upload(posq, 4 * (size_t)p.n, s);
+      if (p.pos_lo.ptr && p.pos_lo.count)
+        check(hipMemsetAsync(p.pos_lo.ptr, 0, p.pos_lo.count * sizeof(double), s), "memset");
+    }
     if (vel) {
       std::vector<float> soa(3 * (size_t)p.n);
       for (uint32_t i = 0; i < p.n; ++i)
         for (int d = 0; d < 3; ++d) soa[(size_t)d * p.n + i] = vel[3 * (size_t)i + d];
       p.vel.upload(soa.data(), soa.size(), s);
+      if (p.vel_lo.ptr && p.vel_lo.count)
+        check(hipMemsetAsync(p.vel_lo.ptr, 0, p.vel_lo.count * sizeof(double), s), "memset");
     }
     check(hipStreamSynchronize(s), "sync");
   });

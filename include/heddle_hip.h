@@ -74,6 +74,8 @@ void hmd_particles_destroy(hmd_particles* p);
 /* Any output pointer may be NULL. */
 hmd_status hmd_particles_download(hmd_particles* p, float* posq, float* vel,
                                   float* force, int32_t* images);
+/* Uploading positions (velocities) also clears their compensated low
+ * parts: the uploaded values are the whole new state. */
 hmd_status hmd_particles_upload(hmd_particles* p, const float* posq,
                                 const float* vel);
 /* New box {lx, ly, lz, xy, xz, yz} (SimulationBox update with its generation
