@@ -48,7 +48,7 @@ EXPORTED = [
     "hmd_forcefield_create", "hmd_forcefield_destroy", "hmd_forcefield_compute",
     "hmd_forcefield_slots", "hmd_spme_scalars", "hmd_spme_grids", "hmd_spme_forces",
     "hmd_nbl_request_rebuild", "hmd_nbl_pair_snapshot", "hmd_nbl_count_pairs",
-    "hmd_nbl_stats",
+    "hmd_nbl_stats", "hmd_nbl_set_capacity",
     "hmd_integrator_create", "hmd_integrator_destroy", "hmd_integrator_step",
     "hmd_integrator_capture", "hmd_integrator_replay", "hmd_integrator_steps",
     "hmd_integrator_profile", "hmd_integrator_minimize",
@@ -200,6 +200,7 @@ def _declare(L):
         "hmd_nbl_pair_snapshot": (st, [vp, _c_u32p, ctypes.c_uint64, _c_u64p]),
         "hmd_nbl_count_pairs": (st, [vp, ctypes.c_double, _c_u64p]),
         "hmd_nbl_stats": (st, [vp, _c_u32p]),
+        "hmd_nbl_set_capacity": (st, [vp, ctypes.c_uint32, ctypes.c_uint32]),
         "hmd_integrator_create": (st, [vp, vp, vp, ctypes.POINTER(IntegratorDesc), pvp]),
         "hmd_integrator_destroy": (None, [vp]),
         "hmd_integrator_step": (st, [vp, ctypes.c_uint32]),

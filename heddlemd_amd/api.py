@@ -279,14 +279,22 @@ class ForceField:
             check(lib().hmd_nbl_pair_snapshot(self.h, _ptr(out, _u32p), cap, ctypes.byref(k)))
             if k.value <= cap:
                 pairs = out[:k.value]
-                order = np.lexsort((pairs[:, 1], pairs[:, 0]))
-                return pairs[order]
+                key = np.sort((pairs[:, 0].astype(np.uint64) << np.uint64(32)) |
+                              pairs[:, 1].astype(np.uint64))
+                return np.stack([(key >> np.uint64(32)).astype(np.uint32),
+                                 (key & np.uint64(0xFFFFFFFF)).astype(np.uint32)], axis=1)
             cap = int(k.value)
 
     def count_pairs(self, radius):
         k = ctypes.c_uint64()
         check(lib().hmd_nbl_count_pairs(self.h, radius, ctypes.byref(k)))
         return int(k.value)
+
+    def set_capacity(self, entries, lds_entries=0):
+        """Seed the list capacity (entries per cluster) and, for tests of the
+        build's direct-write path, the entries staged in LDS
+        (hmd_nbl_set_capacity); the list is rebuilt at the next evaluation."""
+        check(lib().hmd_nbl_set_capacity(self.h, int(entries), int(lds_entries)))
 
     def stats(self):
         out = np.zeros(8, np.uint32)
@@ -306,9 +314,10 @@ class Integrator:
 
     def __init__(self, ctx, particles, forcefield, kind="velocity-verlet", dt=1.0,
                  compensated=False, temperature=0.0, gamma=0.0, seed=0, settle=None,
-                 fused=True):
+                 fused=True, scalars=False):
         d = IntegratorDesc()
-        d.flags = 0 if fused else 1
+        # HMD_INTEGRATOR_UNFUSED = 1, HMD_INTEGRATOR_SCALARS = 2
+        d.flags = (0 if fused else 1) | (2 if scalars else 0)
         self._kind = kind.encode()
         d.kind = self._kind
         d.dt = dt

@@ -424,13 +424,15 @@ void ForceField::init_neighbor_list(const hmd_forcefield_desc& d) {
   alloc_entries(8);
 }
 
+// Entries per cluster: at most n_clusters (a cluster lists every atom above
+// it; all_pairs_tile_capacity, src/forces/neighbor_list.rs:276-287), and the
+// work-list indices c * cap + k stay below 2^32.
 void ForceField::alloc_entries(uint32_t cap) {
-  // the list build stages a cluster's list in LDS (cap * 64 u32 beside its
-  // ~29 KiB of static LDS, within the 64 KiB a workgroup may use)
-  if ((size_t)cap * HMD_ENTRY * 4 > kNblListLdsBytes)
+  cap = std::max(1u, std::min(cap, n_clusters));
+  if ((uint64_t)n_clusters * cap >= (1ull << 32))
     throw Error(HMD_ERR_UNSUPPORTED,
-                "neighbour list needs " + std::to_string(cap) + " entries of 64 j-atoms per "
-                "cluster (at most " + std::to_string(kNblListLdsBytes / (HMD_ENTRY * 4)) + ")");
+                "neighbour list needs " + std::to_string(cap) + " entries per cluster for " +
+                    std::to_string(n_clusters) + " clusters (work indices exceed 32 bits)");
   cap_entries = cap;
   const size_t slots_total = (size_t)n_clusters * cap * HMD_ENTRY;
   entry_j.alloc(slots_total, false);
@@ -558,6 +560,13 @@ void ForceField::compute_gv_table() {
   }
 }
 
+// Capacity for a build that needed `need` entries in its fullest cluster:
+// 1.5x headroom, clamped to the all-pairs ceiling (alloc_entries).
+uint32_t ForceField::grown_capacity(uint32_t need) const {
+  const uint64_t want = (uint64_t)std::ceil((double)need * kNblGrowth) + 1;
+  return (uint32_t)std::max<uint64_t>(4, std::min<uint64_t>(want, n_clusters));
+}
+
 // Probe rebuild: size the per-cluster entry capacity from the measured high
 // water (the reference's probe sizing + x1.5 growth,
 // src/forces/neighbor_list.rs:1151-1221, 421-445).
@@ -567,11 +576,10 @@ void ForceField::probe_capacity() {
     rebuild_chain();
     uint32_t st[4];
     status.download(st, 4, c.stream);
-    uint32_t need = st[2];
-    if (!(st[0] & HMD_STATUS_ENTRY_OVERFLOW) && need <= cap_entries) {
-      if (cap_entries >= need + need / 2 + 1) break;
-    }
-    alloc_entries(std::max(4u, need + need / 2 + 2));
+    const uint32_t need = st[2];
+    const bool fits = !(st[0] & HMD_STATUS_ENTRY_OVERFLOW) && need <= cap_entries;
+    if (fits && (cap_entries >= grown_capacity(need) || cap_entries == n_clusters)) break;
+    alloc_entries(grown_capacity(need));
     uint32_t clear[4] = {HMD_STATUS_DISPLACED, st[1], 0u, 0u};
     status.upload(clear, 4, c.stream);
   }
@@ -648,6 +656,9 @@ void ForceField::rebuild_list() {
   bd.r_search = r_search;
   bd.orthorhombic_fast = ortho_fast ? 1u : 0u;
   bd.cap_entries = cap_entries;
+  uint32_t lds_max = (uint32_t)(kNblListLdsBytes / (HMD_ENTRY * 4));
+  if (lds_entries) lds_max = std::min(lds_max, lds_entries);
+  bd.lds_slots = std::min(cap_entries, lds_max) * HMD_ENTRY;
   bd.entry_j = entry_j.ptr;
   bd.entry_excl = entry_excl.ptr;
   bd.entry_count = entry_count.ptr;
@@ -658,8 +669,7 @@ void ForceField::rebuild_list() {
 #ifndef HMD_NBL_WAVES
 #define HMD_NBL_WAVES 4u
 #endif
-  launch(c, K_hmd_nbl_build, n_clusters, 64 * HMD_NBL_WAVES,
-         (size_t)cap_entries * HMD_ENTRY * 4, bd);
+  launch(c, K_hmd_nbl_build, n_clusters, 64 * HMD_NBL_WAVES, (size_t)bd.lds_slots * 4, bd);
   entry_scan.run(c, entry_count.ptr, entry_offset.ptr, gate, HMD_STATUS_DISPLACED);
   NblWorkParams wp{entry_count.ptr, entry_offset.ptr, n_clusters, cap_entries,
                    work.ptr, n_work.ptr, status.ptr};
@@ -944,21 +954,34 @@ void ForceField::compute(int level) {
 }
 
 // Host-side status check at a batch boundary: the analogue of the 4-byte
-// neighbour status read in pre_step (neighbor_list.rs:1308-1320).
+// neighbour status read in pre_step (neighbor_list.rs:1308-1320).  The
+// high-water word holds the most entries any cluster needed since the
+// capacity was last set.  An overflow (entries dropped) grows the capacity and is reported:
+// the batch is to be re-run.  A build past the fill threshold grows the
+// capacity ahead of any dropped entry (neighbor_list.rs:401-413): nothing
+// was lost, so the batch stands.  Either way the list is rebuilt at once
+// from the current positions into the new buffers, so that it is whole for
+// any reader before the next step, and step graphs are re-captured.
 void ForceField::check_status() {
   uint32_t st[4];
   status.download(st, 4, c.stream);
-  if (st[0] & HMD_STATUS_ENTRY_OVERFLOW) {
-    uint32_t need = st[2];
-    alloc_entries(std::max(4u, need + need / 2 + 2));
-    uint32_t clear[4] = {HMD_STATUS_DISPLACED, st[1], 0u, 0u};
-    status.upload(clear, 4, c.stream);
-    check(hipStreamSynchronize(c.stream), "sync");
-    graph_invalid = true;
+  const uint32_t need = st[2];
+  const bool overflow = (st[0] & HMD_STATUS_ENTRY_OVERFLOW) != 0;
+  const bool near_full = need > (uint32_t)((double)cap_entries * kNblFillThreshold) &&
+                         cap_entries < n_clusters;
+  if (!overflow && !near_full) return;
+  const uint32_t old_cap = cap_entries;
+  alloc_entries(std::max(grown_capacity(need), cap_entries));
+  uint32_t reset[4] = {HMD_STATUS_DISPLACED, st[1], 0u, 0u};
+  status.upload(reset, 4, c.stream);
+  rebuild_chain();
+  check(hipStreamSynchronize(c.stream), "sync");
+  graph_invalid = true;
+  if (overflow)
     throw Error(HMD_ERR_NBL_OVERFLOW,
-                "packed neighbour list overflowed; capacity grown to " +
-                    std::to_string(cap_entries) + " entries per cluster");
-  }
+                "packed neighbour list overflowed: a cluster needed " + std::to_string(need) +
+                    " entries of 64 j-atoms (capacity " + std::to_string(old_cap) +
+                    "); capacity grown to " + std::to_string(cap_entries) + ", re-run the batch");
 }
 
 }  // namespace hmd
@@ -1081,6 +1104,21 @@ hmd_status hmd_nbl_count_pairs(hmd_forcefield* f, double radius, uint64_t* n_pai
     unsigned long long total = 0;
     cnt.download(&total, 1, c.stream);
     *n_pairs = total;
+  });
+}
+
+hmd_status hmd_nbl_set_capacity(hmd_forcefield* f, uint32_t entries, uint32_t lds_entries) {
+  HMD_GUARD({
+    ForceField& F = *f->ff;
+    if (entries == 0) throw Error(HMD_ERR_INVALID, "capacity must be at least one entry");
+    F.lds_entries = lds_entries;
+    F.alloc_entries(entries);
+    uint32_t st[4];
+    F.status.download(st, 4, F.c.stream);
+    uint32_t reset[4] = {st[0] | HMD_STATUS_DISPLACED, st[1], 0u, 0u};
+    F.status.upload(reset, 4, F.c.stream);
+    check(hipStreamSynchronize(F.c.stream), "sync");
+    F.graph_invalid = true;
   });
 }
 

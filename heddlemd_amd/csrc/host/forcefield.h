@@ -20,9 +20,15 @@ struct JitPairModule {
   ~JitPairModule();
 };
 
-// LDS the list build may use for a cluster's staged j list (64 KiB per
-// workgroup minus the kernel's static arrays).
+// LDS the list build may use for the head of a cluster's staged j list
+// (64 KiB per workgroup minus the kernel's static arrays); list positions
+// past it are written to the entry array directly.
 constexpr size_t kNblListLdsBytes = 65536 - 29696;
+// Capacity policy of the packed list (src/forces/neighbor_list.rs:235-244,
+// 415-445): grow by 1.5x, and grow ahead of any dropped entry once a build
+// fills more than 0.8 of the capacity.
+constexpr double kNblGrowth = 1.5;
+constexpr double kNblFillThreshold = 0.8;
 
 struct ForceField {
   Context& c;
@@ -83,6 +89,7 @@ struct ForceField {
   uint32_t g[3] = {1, 1, 1};
   uint32_t grid_bits = 1;
   uint32_t n_keys = 1, n_clusters = 0, n_pad = 0, cap_entries = 0;
+  uint32_t lds_entries = 0;         // list entries staged in LDS (0: as many as fit)
   DevBuf<uint32_t> status;          // [0] flags [1] generation [2] high water
   DevBuf<uint32_t> atom_key, cell_count, cell_start, sorted_id, slot_of, type_sorted;
   DevBuf<float> posq_sorted, ref_pos, bbox, sub_bbox;
@@ -125,6 +132,7 @@ struct ForceField {
   void init_neighbor_list(const hmd_forcefield_desc& d);
   void init_spme(const hmd_forcefield_desc& d);
   void alloc_entries(uint32_t cap);
+  uint32_t grown_capacity(uint32_t need) const;
   void probe_capacity();
   void request_rebuild();
   void check_box();

@@ -37,6 +37,7 @@
   X(hmd_spme_gv_table)                                                        \
   X(hmd_vv_kick_drift) X(hmd_vv_kick)                                        \
   X(hmd_settle_snapshot) X(hmd_settle_positions) X(hmd_settle_velocities)    \
+  X(hmd_settle_virial)                                                       \
   X(hmd_lan_drift_half) X(hmd_lan_ou) X(hmd_counter_increment)               \
   X(hmd_step_vv_first) X(hmd_step_baoa) X(hmd_step_kick_last)                \
   X(hmd_sd_step) X(hmd_reduce_partial) X(hmd_reduce_final)                 \

@@ -32,6 +32,8 @@ struct Integrator {
   float lan_a = 1.0f;
   float lan_kt = 0.0f;
   bool fused = true;
+  // every step's evaluation at the energy level, SETTLE virial folded in
+  bool scalars = false;
   DevBuf<uint32_t> free_atoms;   // atoms outside SETTLE groups (fused plan)
   uint32_t n_free = 0;
   uint64_t seed = 0;
@@ -72,6 +74,7 @@ struct Integrator {
       constraint_virial.alloc(3 * (size_t)n_groups);
     }
     fused = (d.flags & HMD_INTEGRATOR_UNFUSED) == 0;
+    scalars = (d.flags & HMD_INTEGRATOR_SCALARS) != 0;
     if (n_groups) {
       std::vector<uint8_t> in_group(p.n, 0);
       for (uint32_t k2 = 0; k2 < 3 * n_groups; ++k2) {
@@ -136,6 +139,7 @@ struct Integrator {
     s.L = p.lattice;
     s.dt = step_dt;
     s.constraint_virial = constraint_virial.ptr;
+    s.particle_virial = scalars ? p.virial.ptr : nullptr;
     return s;
   }
 
@@ -168,6 +172,12 @@ struct Integrator {
     if (!n_groups) return;
     launch(c, K_hmd_settle_velocities, ceil_div(n_groups, 256), 256, 0, settle(dt));
   }
+  // AfterKick of the step's last kick: the constraint virial into the
+  // particle virials (energy-level steps only)
+  void settle_virial() {
+    if (!n_groups || !scalars) return;
+    launch(c, K_hmd_settle_virial, ceil_div(n_groups, 256), 256, 0, settle(dt));
+  }
 
   StepParams fused_params() const {
     StepParams sp{};
@@ -181,6 +191,8 @@ struct Integrator {
     return sp;
   }
 
+  int level() const { return scalars ? 1 : 0; }
+
   // One step, enqueued only (capturable).
   void enqueue_step() {
     const uint32_t nb = ceil_div(p.n, 256);
@@ -190,7 +202,7 @@ struct Integrator {
       launch(c, kind == Kind::VelocityVerlet ? K_hmd_step_vv_first : K_hmd_step_baoa, nu, 256,
              0, sp);
       c.mark(kind == Kind::VelocityVerlet ? "kick_drift_settle" : "baoa");
-      ff.compute(0);
+      ff.compute(level());
       launch(c, K_hmd_step_kick_last, nu, 256, 0, sp);
       c.mark("kick_settle");
       return;
@@ -200,9 +212,10 @@ struct Integrator {
       launch(c, K_hmd_vv_kick_drift, nb, 256, 0, vv());         // KickDrift
       settle_positions(dt);                                     // AfterDrift
       c.mark("kick_drift_settle");
-      ff.compute(0);                                            // ForceEval
+      ff.compute(level());                                      // ForceEval
       launch(c, K_hmd_vv_kick, nb, 256, 0, vv());               // KickHalf
       settle_velocities();                                      // AfterKick
+      settle_virial();
       c.mark("kick_settle");
     } else {
       // B A O A (force) B, with SETTLE after each position and velocity
@@ -219,9 +232,10 @@ struct Integrator {
       launch(c, K_hmd_lan_drift_half, nb, 256, 0, lan(dt));
       settle_positions(0.5f * dt);
       c.mark("baoa");
-      ff.compute(0);
+      ff.compute(level());
       launch(c, K_hmd_vv_kick, nb, 256, 0, k);
       settle_velocities();
+      settle_virial();
       CounterParams cp{step_counter.ptr};
       launch(c, K_hmd_counter_increment, 1, 64, 0, cp);
       c.mark("kick_settle");
@@ -230,7 +244,7 @@ struct Integrator {
 
   void prime() {
     if (primed) return;
-    ff.compute(0);
+    ff.compute(level());
     check(hipStreamSynchronize(c.stream), "sync");
     ff.check_status();
     primed = true;

@@ -111,6 +111,7 @@ struct NblBuildParams {
   float r_search;
   uint32_t orthorhombic_fast;  // 1: exact fast min-image valid (see kernel)
   uint32_t cap_entries;        // entries per cluster
+  uint32_t lds_slots;          // list slots staged in LDS (later ones go to entry_j directly)
   uint32_t* entry_j;           // [n_clusters*cap_entries*64]
   uint64_t* entry_excl;        // [n_clusters*cap_entries*64]
   uint32_t* entry_count;       // [n_clusters] (entries written / needed)
@@ -330,6 +331,7 @@ struct SettleParams {
   HmdLattice L;
   float dt;
   float* constraint_virial;    // [3*n_groups]
+  float* particle_virial;      // [n]: the constraint virial is added here (may be null)
 };
 
 struct LangevinParams {

@@ -330,6 +330,18 @@ __device__ inline void hmd_settle_velocities_group(const SettleParams& p, const 
   }
 }
 
+// The group's constraint virial (position level from the last reset plus
+// velocity level from the last projection) added to its atoms' virials:
+// settle_virial_scatter (kernels/settle.cu:498-510).  Groups are disjoint,
+// so the adds need no atomics.
+__device__ inline void hmd_settle_virial_group(const SettleParams& p, const uint32_t g) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const uint32_t i = p.group_atoms[3 * g + a];
+    p.particle_virial[i] = p.particle_virial[i] + p.constraint_virial[3 * g + a];
+  }
+}
+
 // --------------------------------------------------------------- Langevin --
 // Philox-4x32-10 (Salmon et al., SC'11), as kernels/philox.cuh:25-70.
 __device__ inline void hmd_philox(uint32_t k0, uint32_t k1, uint32_t c[4]) {
@@ -422,6 +434,12 @@ __device__ __forceinline__ void hmd_settle_velocities_body(const SettleParams& p
   if (g < p.n_groups) hmd_settle_velocities_group(p, g, p.dt);
 }
 HMD_KERNEL(hmd_settle_velocities, SettleParams, 256)
+__device__ __forceinline__ void hmd_settle_virial_body(const SettleParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const uint32_t g = hmd_bx * blockDim.x + threadIdx.x;
+  if (g < p.n_groups) hmd_settle_virial_group(p, g);
+}
+HMD_KERNEL(hmd_settle_virial, SettleParams, 256)
 __device__ __forceinline__ void hmd_lan_drift_half_body(const LangevinParams& p, const uint32_t hmd_bx,
     const uint32_t hmd_gx) {
   const uint32_t i = hmd_bx * blockDim.x + threadIdx.x;
@@ -513,6 +531,7 @@ __device__ __forceinline__ void hmd_step_kick_last_body(const StepParams& p, con
 #pragma unroll 1
     for (int a = 0; a < 3; ++a) hmd_vv_kick_atom(p.vv, p.st.group_atoms[3 * g + a]);
     hmd_settle_velocities_group(p.st, g, p.dt);
+    if (p.st.particle_virial) hmd_settle_virial_group(p.st, g);
   } else {
     hmd_vv_kick_atom(p.vv, atom);
   }

@@ -91,8 +91,11 @@ __device__ inline void hmd_fine_cell(float x, float y, float z,
 
 // Skin trigger (kernels/neighbor.cu:50-76): any atom whose min-image
 // displacement from its rebuild-time position exceeds r_skin/2 sets bit 0.
+// The squared displacement is the exact f32 min-image distance (no
+// contraction), so the trigger is one specified function of the positions.
 __device__ __forceinline__ void hmd_nbl_displacement_body(const NblDispParams& p, const uint32_t hmd_bx,
     const uint32_t hmd_gx) {
+#pragma clang fp contract(off)
   uint32_t i = hmd_bx * blockDim.x + threadIdx.x;
   if (i >= p.n) return;
   float4 a = reinterpret_cast<const float4*>(p.posq)[i];
@@ -276,11 +279,12 @@ __device__ inline float hmd_box_gap2(float4 ca, float4 ha, float4 cb, float4 hb,
 // sub-boxes and keeps it when a sub-box holding an i-atom with a lower sorted
 // index lies within r_search of it; the next candidate's j-atom is loaded
 // while the current one is tested.  The 64-bit
-// hit mask of each candidate goes to LDS, and wave 0 then places the kept
-// atoms, candidate by candidate, at consecutive entry positions in
-// increasing sorted order (ballot prefix count), written straight to the
-// entry array.  Each lane of wave 0 keeps the sorted slots of its atom's
-// exclusion partners (at most HMD_MAX_INLINE_EXCL) in registers and, per
+// hit mask of each candidate goes to LDS; wave 0 turns the masks into list
+// positions (prefix of the hit counts in candidate order) and all waves place
+// the kept atoms there, in increasing sorted order, in an LDS copy of the
+// cluster's list (positions past its lds_slots go straight to the entry
+// array), written out as whole entries.  Each lane of wave 0 keeps the
+// sorted slots of its atom's exclusion partners (at most HMD_MAX_INLINE_EXCL) in registers and, per
 // candidate, sets the bit of the entry position that holds a partner.  The
 // output depends only on the positions (deterministic).  More candidates than
 // fit in LDS are handled in rounds.
@@ -322,7 +326,7 @@ __device__ __forceinline__ void hmd_nbl_build_body(const NblBuildParams& p, cons
   __shared__ uint64_t s_hit[HMD_NBL_CAND_CAP];
   __shared__ uint32_t s_off[HMD_NBL_CAND_CAP];
   __shared__ uint32_t s_fill;
-  extern __shared__ uint32_t s_list[];   // [cap_entries * 64]: the cluster's j list
+  extern __shared__ uint32_t s_list[];   // [lds_slots]: the head of the cluster's j list
   const uint32_t c = hmd_bx;
   const uint32_t wave = threadIdx.x / HMD_WAVE;
   const uint32_t lane = hmd_lane();
@@ -532,12 +536,15 @@ __device__ __forceinline__ void hmd_nbl_build_body(const NblBuildParams& p, cons
     __syncthreads();
 
     // placement, all waves: lane k of candidate q puts its j-atom at list
-    // position s_off[q] + (hits of q below lane k) of the LDS list
+    // position s_off[q] + (hits of q below lane k): in the LDS list, or,
+    // past the slots staged there, straight into the entry array
     for (uint32_t k = wave; k < cnt; k += HMD_NBL_WAVES) {
       const uint64_t H = s_hit[k];
       if ((H >> lane) & 1ull) {
         const uint32_t pos = s_off[k] + (uint32_t)__builtin_popcountll(H & lt_mask);
-        if (pos < cap_slots) s_list[pos] = (s_cand[k] & 0x7FFFFFFFu) * HMD_CLUSTER + lane;
+        const uint32_t sj = (s_cand[k] & 0x7FFFFFFFu) * HMD_CLUSTER + lane;
+        if (pos < p.lds_slots) s_list[pos] = sj;
+        else if (pos < cap_slots) p.entry_j[base_entry * HMD_ENTRY + pos] = sj;
       }
     }
     __syncthreads();
@@ -547,10 +554,15 @@ __device__ __forceinline__ void hmd_nbl_build_body(const NblBuildParams& p, cons
   const uint32_t n_written = min(n_entries, p.cap_entries);
   // whole entries out of LDS, one coalesced 256-byte row per wave and entry
   // (scattered partial-row stores of each candidate's hits straight to
-  // memory took half of the build's time); empty slots get the sentinel
+  // memory took half of the build's time); empty slots get the sentinel.
+  // Slots past the LDS staging were written by the placement; only their
+  // empty tail is left.
   for (uint32_t e = wave; e < n_written; e += HMD_NBL_WAVES) {
     const uint32_t pos = e * HMD_ENTRY + lane;
-    p.entry_j[base_entry * HMD_ENTRY + pos] = pos < fill ? s_list[pos] : HMD_SENTINEL;
+    if (pos < p.lds_slots)
+      p.entry_j[base_entry * HMD_ENTRY + pos] = pos < fill ? s_list[pos] : HMD_SENTINEL;
+    else if (pos >= fill)
+      p.entry_j[base_entry * HMD_ENTRY + pos] = HMD_SENTINEL;
   }
   if (wave == 0) {
     // Exclusion masks only for the leading entries that hold a partner (the

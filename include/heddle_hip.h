@@ -199,6 +199,15 @@ hmd_status hmd_nbl_pair_snapshot(hmd_forcefield* ff, uint32_t* pairs,
  * the pair kernel evaluates inside the cutoff. */
 hmd_status hmd_nbl_count_pairs(hmd_forcefield* ff, double radius,
                                uint64_t* n_pairs);
+/* Set the list capacity (entries of 64 j-atoms per cluster; clamped to the
+ * number of clusters, the all-pairs ceiling) and rebuild at the next
+ * evaluation: the capacity field of PackedNeighborData and its growth
+ * (neighbor_list.rs:415-445).  The library sizes the capacity itself (probe
+ * at creation, 1.5x growth on overflow and past 0.8 fill); a smaller seed
+ * exercises that growth.  lds_entries caps the entries the build stages in
+ * LDS (0: as many as fit, 140); list positions past them are written to
+ * the entry array directly, with the same result. */
+hmd_status hmd_nbl_set_capacity(hmd_forcefield* ff, uint32_t entries, uint32_t lds_entries);
 /* out[0] entries in the work list, [1] rebuild generation, [2] status bits,
  * [3] high-water entries per cluster, [4] capacity per cluster,
  * [5] clusters, [6] fine cells per axis, [7] pad. */
@@ -222,10 +231,17 @@ typedef struct {
   /* 0: the sub-steps between two force evaluations run as one fused
    * launch per group of sub-steps (same per-atom arithmetic and order);
    * HMD_INTEGRATOR_UNFUSED: one launch per sub-step, as the reference's
-   * plan executes them. */
+   * plan executes them.  HMD_INTEGRATOR_SCALARS: every step's force
+   * evaluation is at the energy level (per-atom energies and virials,
+   * AggregateLevel) and, with SETTLE, the constraint virial of the step
+   * (position-level after the drift plus velocity-level after the kick) is
+   * added to each atom's virial after the last kick, as
+   * settle_virial_scatter does (kernels/settle.cu:498-510, called from
+   * apply_after_kick, src/integrator/settle.rs:371-401). */
   uint32_t flags;
 } hmd_integrator_desc;
 #define HMD_INTEGRATOR_UNFUSED 1u
+#define HMD_INTEGRATOR_SCALARS 2u
 
 hmd_status hmd_integrator_create(hmd_ctx* ctx, hmd_particles* particles,
                                  hmd_forcefield* ff,

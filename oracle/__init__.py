@@ -54,8 +54,12 @@ class PairParams(ctypes.Structure):
 def _declare(L):
     L.orc_r2_f32.restype = ctypes.c_float
     L.orc_r2_f32.argtypes = [_fp, _fp, _fp]
+    L.orc_r2_f32_many.argtypes = [ctypes.c_int, _fp, _fp, _fp, _fp]
     L.orc_pair_set_f32.restype = ctypes.c_int64
     L.orc_pair_set_f32.argtypes = [ctypes.c_int, _fp, _fp, ctypes.c_float, _up, ctypes.c_int64]
+    L.orc_pair_set_f32_cells.restype = ctypes.c_int64
+    L.orc_pair_set_f32_cells.argtypes = [ctypes.c_int, _fp, _fp, ctypes.c_float, _up,
+                                         ctypes.c_int64]
     L.orc_pair_forces.argtypes = [ctypes.POINTER(PairParams), _dp, _dp, _dp]
     L.orc_pair_forces_cells.argtypes = [ctypes.POINTER(PairParams), _dp, _dp, _dp]
     L.orc_ewald_recip_direct.restype = ctypes.c_double
@@ -71,7 +75,8 @@ def _declare(L):
                                  ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp, _dp]
     L.orc_settle_geometry.argtypes = [ctypes.c_double] * 4 + [_dp]
     L.orc_settle_positions_one.argtypes = [_dp, _dp, _dp] + [ctypes.c_double] * 6 + [_dp, _dp]
-    L.orc_settle_velocities_one.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double, _dp]
+    L.orc_settle_velocities_one.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double, _dp,
+                                            ctypes.c_double, _dp]
     L.orc_philox4x32_10.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _up, _up]
     L.orc_philox_gaussian.restype = ctypes.c_double
     L.orc_philox_gaussian.argtypes = [ctypes.c_uint32] * 6
@@ -101,16 +106,27 @@ def r2_f32(pi, pj, lattice):
     return lib().orc_r2_f32(_p(pi, _fp), _p(pj, _fp), _p(la, _fp))
 
 
-def pair_set_f32(posq, lattice, r_search_sq):
-    """Canonical (i<j) pairs within the exact f32 search radius: uint32[k,2]."""
+def r2_f32_many(a, b, lattice):
+    """Exact f32 min-image squared distance of each row pair (posq rows)."""
+    a = np.ascontiguousarray(a, np.float32).reshape(-1, 4)
+    b = np.ascontiguousarray(b, np.float32).reshape(-1, 4)
+    la = np.ascontiguousarray(lattice, np.float32)
+    out = np.empty(a.shape[0], np.float32)
+    lib().orc_r2_f32_many(a.shape[0], _p(a, _fp), _p(b, _fp), _p(la, _fp), _p(out, _fp))
+    return out
+
+
+def pair_set_f32(posq, lattice, r_search_sq, cells=False):
+    """Canonical (i<j) pairs within the exact f32 search radius: uint32[k,2]
+    in ascending order (cells=True: the cell-list form, same output)."""
     posq = np.ascontiguousarray(posq, np.float32).reshape(-1, 4)
     la = np.ascontiguousarray(lattice, np.float32)
     n = posq.shape[0]
+    fn = lib().orc_pair_set_f32_cells if cells else lib().orc_pair_set_f32
     cap = 1 << 16
     while True:
         out = np.empty((cap, 2), np.uint32)
-        k = lib().orc_pair_set_f32(n, _p(posq, _fp), _p(la, _fp),
-                                   ctypes.c_float(r_search_sq), _p(out, _up), cap)
+        k = fn(n, _p(posq, _fp), _p(la, _fp), ctypes.c_float(r_search_sq), _p(out, _up), cap)
         if k <= cap:
             return out[:k].copy()
         cap = int(k)
@@ -233,10 +249,16 @@ def settle_positions_one(snap, cur, v, m_o, m_h, geom, dt, lattice):
     return cur.reshape(3, 3), v.reshape(3, 3), vir
 
 
-def settle_velocities_one(pos, v, m_o, m_h, lattice):
+def settle_velocities_one(pos, v, m_o, m_h, lattice, dt=None):
+    """Projected velocities (3, 3); with dt, also the velocity-level
+    constraint virial per atom (3,)."""
     pos = _f64(pos).reshape(9); v = _f64(v).reshape(9).copy(); la = _f64(lattice)
-    lib().orc_settle_velocities_one(_p(pos, _dp), _p(v, _dp), m_o, m_h, _p(la, _dp))
-    return v.reshape(3, 3)
+    vir = np.zeros(3)
+    lib().orc_settle_velocities_one(_p(pos, _dp), _p(v, _dp), m_o, m_h, _p(la, _dp),
+                                    0.0 if dt is None else float(dt), _p(vir, _dp))
+    if dt is None:
+        return v.reshape(3, 3)
+    return v.reshape(3, 3), vir
 
 
 def philox4x32_10(key, ctr):

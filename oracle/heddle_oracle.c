@@ -92,6 +92,13 @@ float orc_r2_f32(const float *pi, const float *pj, const float *L)
     return dx * dx + dy * dy + dz * dz;
 }
 
+/* orc_r2_f32 of n pairs (a_i, b_i), posq layout (f32[4n] each): the skin
+ * trigger's displacement test (kernels/neighbor.cu:50-76) per atom. */
+void orc_r2_f32_many(int n, const float *a, const float *b, const float *L, float *out)
+{
+    for (int i = 0; i < n; ++i) out[i] = orc_r2_f32(a + 4 * i, b + 4 * i, L);
+}
+
 /* Wrap positions into the primary image (kernels/pbc.cuh:74-92), f64. */
 void orc_wrap_f64(int n, double *pos, const double *L)
 {
@@ -130,6 +137,95 @@ int64_t orc_pair_set_f32(int n, const float *posq, const float *L,
         }
     }
     return count;
+}
+
+/* Cell-list form of orc_pair_set_f32 for large orthorhombic boxes: cells
+ * at least 1.001 r_search wide (>= 3 per axis, else the brute force), every
+ * candidate pair from the 27 neighbouring cells given the same exact f32
+ * test, output in the same ascending (i, j) order.  Pinned against the
+ * brute force in the tests. */
+static int cmp_int(const void *a, const void *b)
+{
+    int x = *(const int *)a, y = *(const int *)b;
+    return (x > y) - (x < y);
+}
+
+int64_t orc_pair_set_f32_cells(int n, const float *posq, const float *L,
+                               float r_search_sq, uint32_t *pairs, int64_t cap)
+{
+    const double rs = sqrt((double)r_search_sq) * 1.001;
+    int nc[3];
+    for (int d = 0; d < 3; ++d) {
+        nc[d] = (int)floor((double)L[d] / rs);
+        if (nc[d] < 3 || L[3] != 0.0f || L[4] != 0.0f || L[5] != 0.0f)
+            return orc_pair_set_f32(n, posq, L, r_search_sq, pairs, cap);
+    }
+    const size_t ncell = (size_t)nc[0] * nc[1] * nc[2];
+    int *cell = malloc(sizeof(int) * n);
+    int *start = calloc(ncell + 1, sizeof(int));
+    int *members = malloc(sizeof(int) * n);
+    for (int i = 0; i < n; ++i) {
+        int ci[3];
+        for (int d = 0; d < 3; ++d) {
+            double s = (double)posq[4 * i + d] / (double)L[d] + 0.5;
+            s -= floor(s);
+            int k = (int)(s * nc[d]);
+            ci[d] = k >= nc[d] ? nc[d] - 1 : k;
+        }
+        cell[i] = (ci[0] * nc[1] + ci[1]) * nc[2] + ci[2];
+        start[cell[i] + 1]++;
+    }
+    for (size_t c = 0; c < ncell; ++c) start[c + 1] += start[c];
+    int *fill = calloc(ncell, sizeof(int));
+    for (int i = 0; i < n; ++i) members[start[cell[i]] + fill[cell[i]]++] = i;
+    free(fill);
+    /* pass 1: count per i; pass 2: write at the prefix offsets, j ascending */
+    int64_t *cnt = calloc((size_t)n + 1, sizeof(int64_t));
+    for (int pass = 0; pass < 2; ++pass) {
+        #pragma omp parallel for schedule(dynamic, 64)
+        for (int i = 0; i < n; ++i) {
+            const float *pi = posq + 4 * i;
+            int c = cell[i];
+            int ca = c / (nc[1] * nc[2]), cb = (c / nc[2]) % nc[1], cc = c % nc[2];
+            int nbr[27], nn = 0;
+            for (int da = -1; da <= 1; ++da)
+            for (int db = -1; db <= 1; ++db)
+            for (int dc = -1; dc <= 1; ++dc) {
+                int a = (ca + da + nc[0]) % nc[0], b = (cb + db + nc[1]) % nc[1];
+                int k = (cc + dc + nc[2]) % nc[2];
+                int id = (a * nc[1] + b) * nc[2] + k, dup = 0;
+                for (int m = 0; m < nn; ++m) dup |= nbr[m] == id;
+                if (!dup) nbr[nn++] = id;
+            }
+            /* candidates j > i, ascending */
+            int cap_j = 0;
+            for (int m = 0; m < nn; ++m) cap_j += start[nbr[m] + 1] - start[nbr[m]];
+            int *js = malloc(sizeof(int) * (cap_j + 1));
+            int k = 0;
+            for (int m = 0; m < nn; ++m)
+                for (int t = start[nbr[m]]; t < start[nbr[m] + 1]; ++t)
+                    if (members[t] > i) js[k++] = members[t];
+            qsort(js, (size_t)k, sizeof(int), cmp_int);
+            int64_t o = pass ? cnt[i] : 0;
+            for (int t = 0; t < k; ++t) {
+                float r2 = orc_r2_f32(pi, posq + 4 * js[t], L);
+                if (r2 <= r_search_sq) {
+                    if (pass && o < cap) {
+                        pairs[2 * o] = (uint32_t)i;
+                        pairs[2 * o + 1] = (uint32_t)js[t];
+                    }
+                    ++o;
+                }
+            }
+            if (!pass) cnt[i + 1] = o;
+            free(js);
+        }
+        if (!pass)
+            for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    }
+    int64_t total = cnt[n];
+    free(cnt); free(cell); free(start); free(members);
+    return total;
 }
 
 /* ------------------------------------------------------------------ */
@@ -748,7 +844,8 @@ void orc_settle_positions_one(const double *snp, double *cur, double *v,
  * solve the 3x3 system for bond impulses so that (v_i - v_j).r_ij = 0 for
  * the three bonds. pos/v: 3 atoms x 3. */
 void orc_settle_velocities_one(const double *pos_in, double *v, double m_o,
-                               double m_h, const double *L)
+                               double m_h, const double *L, double dt,
+                               double *vir)
 {
     double p[3][3];
     for (int a = 0; a < 3; ++a) for (int d = 0; d < 3; ++d) p[a][d] = pos_in[3 * a + d];
@@ -783,10 +880,25 @@ void orc_settle_velocities_one(const double *pos_in, double *v, double m_o,
               - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0])
               + A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0])) / det;
     }
+    double dv[3][3];
     for (int d = 0; d < 3; ++d) {
-        v[d] += (g[0] * r1[d] + g[1] * r2[d]) * io;
-        v[3 + d] += (-g[0] * r1[d] + g[2] * r3[d]) * ih;
-        v[6 + d] += (-g[1] * r2[d] - g[2] * r3[d]) * ih;
+        dv[0][d] = (g[0] * r1[d] + g[1] * r2[d]) * io;
+        dv[1][d] = (-g[0] * r1[d] + g[2] * r3[d]) * ih;
+        dv[2][d] = (-g[1] * r2[d] - g[2] * r3[d]) * ih;
+        for (int a = 0; a < 3; ++a) v[3 * a + d] += dv[a][d];
+    }
+    /* Velocity-level constraint virial per atom (dt > 0):
+     * (m_a dv_a / dt) . (r_a - r_com), the impulse of the projection as a
+     * force over the step (kernels/settle.cu, settle_velocities). */
+    if (vir && dt > 0.0) {
+        double m[3] = {m_o, m_h, m_h}, com[3];
+        for (int d = 0; d < 3; ++d)
+            com[d] = (m_o * p[0][d] + m_h * p[1][d] + m_h * p[2][d]) / (m_o + 2.0 * m_h);
+        for (int a = 0; a < 3; ++a) {
+            double w = 0.0;
+            for (int d = 0; d < 3; ++d) w += dv[a][d] * (p[a][d] - com[d]);
+            vir[a] = m[a] / dt * w;
+        }
     }
 }
 

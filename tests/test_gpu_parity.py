@@ -60,13 +60,23 @@ def test_pair_plus_spme_forces_match_oracle(water):
     sc = field.spme_scalars()
     err_u = abs(sc["u_recip"] - u_rec) / abs(u_rec)
     err_w = abs(sc["w_recip"] - w_rec) / abs(w_rec)
+    # per-atom virial: the pair slots' half shares plus W_recip / N
+    _, _, w_pair = oracle_pair(sysw, ff, posq, erfc_mode=1)
+    w_ref = w_pair + w_rec / len(w_pair)
+    err_wi = rel_rms(w_gpu, w_ref)
+    err_wsum = abs(vir_gpu - w_ref.sum()) / np.abs(w_ref).sum()
     print(f"rel RMS force: total {err_f:.3e} pair {err_pair:.3e} recip {err_rec:.3e}; "
-          f"rel PE {err_e:.3e}; U_recip {err_u:.3e}; W_recip {err_w:.3e}")
+          f"rel PE {err_e:.3e}; U_recip {err_u:.3e}; W_recip {err_w:.3e}; per-atom virial "
+          f"{err_wi:.3e}; virial sum {err_wsum:.3e} of sum |W_i| (W = {vir_gpu:.6e}, "
+          f"oracle {w_ref.sum():.6e})")
     assert err_pair <= FORCE_TOL
     assert err_rec <= FORCE_TOL
     assert err_f <= FORCE_TOL
     assert err_u <= ENERGY_TOL
     assert err_e <= ENERGY_TOL
+    assert err_w <= ENERGY_TOL
+    assert err_wi <= FORCE_TOL
+    assert err_wsum <= ENERGY_TOL
 
 
 def test_pair_forces_vs_exact_erfc(water):

@@ -228,6 +228,21 @@ def test_pair_forces_are_energy_gradient_with_switch_and_erfc():
         assert F[i, d] == pytest.approx(-(ep - em) / (2 * h), rel=1e-5, abs=1e-9)
 
 
+def test_pair_set_cells_equals_brute_force():
+    """The cell-list pair set (used for the large GPU parity cases) is the
+    brute-force f32 set, pair for pair, in the same order."""
+    rng = np.random.default_rng(3)
+    for n, L, rs in [(3000, 40.0, 12.0), (1500, 30.0, 9.5)]:
+        posq = np.zeros((n, 4), np.float32)
+        posq[:, :3] = (rng.random((n, 3)) - 0.5) * L
+        lat = np.array([L, L, L, 0, 0, 0], np.float32)
+        rs2 = np.float32(rs * rs)
+        a = oracle.pair_set_f32(posq, lat, rs2)
+        b = oracle.pair_set_f32(posq, lat, rs2, cells=True)
+        assert len(a) > 1000
+        assert np.array_equal(a, b)
+
+
 def test_pair_forces_newton3_and_cells_match_brute_force():
     pos, q, types, lat, lj = _random_fluid(n=343, L=32.0)
     co = {"alpha": 0.35, "r_cut": 9.0}
@@ -237,6 +252,39 @@ def test_pair_forces_newton3_and_cells_match_brute_force():
     assert np.abs(F.sum(axis=0)).max() < 1e-12 * np.abs(F).max() * len(F)
     assert np.allclose(F, Fc, rtol=1e-12, atol=1e-15)
     assert np.allclose(E, Ec, rtol=1e-12, atol=1e-15)
+
+
+def _scaled_energy(pos, q, types, lat, lj, co, eps, mesh, alpha):
+    """Pair + reciprocal energy with the box and every position scaled by
+    (1 + eps)."""
+    f = 1.0 + eps
+    lat_s = [x * f for x in lat]
+    e_pair = oracle.PairSystem(pos * f, q, types, lat_s, lj=lj, coulomb=co).forces()[1].sum()
+    u_rec = oracle.spme_recip(pos * f, q, lat_s, mesh, 4, alpha)[0]
+    return e_pair, u_rec
+
+
+def test_virials_are_minus_energy_derivative_under_uniform_scaling():
+    """The scalar virial W = sum over pairs of F . r (pair slots, with the
+    LJ switch and erfc) and W_recip = 1/2 sum vf |rho_hat|^2 both equal
+    -dU/d eps when the box and all positions scale by (1 + eps): central
+    differences of the oracle's energies, no pair within 1e-4 of a cutoff."""
+    pos, q, types, lat, lj = _random_fluid(n=64, L=30.0, seed=5)
+    q = q - q.mean()
+    co = {"alpha": 0.35, "r_cut": 9.0}
+    # no pair close to the cutoffs, where the truncated energy jumps
+    d = pos[:, None, :] - pos[None, :, :]
+    d -= 30.0 * np.round(d / 30.0)
+    r = np.sqrt((d ** 2).sum(-1))[np.triu_indices(64, 1)]
+    assert np.min(np.abs(r / 9.0 - 1.0)) > 1e-4
+    mesh = (24, 24, 24)
+    _, _, W = oracle.PairSystem(pos, q, types, lat, lj=lj, coulomb=co).forces()
+    _, _, _, w_rec = oracle.spme_recip(pos, q, lat, mesh, 4, co["alpha"])
+    h = 1e-6
+    ep, up = _scaled_energy(pos, q, types, lat, lj, co, h, mesh, co["alpha"])
+    em, um = _scaled_energy(pos, q, types, lat, lj, co, -h, mesh, co["alpha"])
+    assert W.sum() == pytest.approx(-(ep - em) / (2 * h), rel=1e-6)
+    assert w_rec == pytest.approx(-(up - um) / (2 * h), rel=1e-6)
 
 
 def test_erfc_approximation_error_bound():
@@ -410,6 +458,54 @@ def test_settle_velocities_equal_linear_constraint_solve():
             want[c] += g[l] * r[l] / m[c]
             want[d] -= g[l] * r[l] / m[d]
         assert np.max(np.abs(got - want)) < 1e-12 * np.max(np.abs(v)) + 1e-18
+
+
+def _bond_form(G, ref_pos, x, bonds):
+    """Write per-atom constraint forces G as sum_b g_b (+/-) r_b(ref) (one
+    multiplier per bond, least squares: exact when the forces are bond
+    forces) and return (residual, sum_b g_b r_b(ref) . r_b(x)), the virial
+    sum_a G_a . x_a in bond form, independent of the origin."""
+    B = np.zeros((9, len(bonds)))
+    for k, (a, b) in enumerate(bonds):
+        rb = ref_pos[a] - ref_pos[b]
+        B[3 * a:3 * a + 3, k] += rb
+        B[3 * b:3 * b + 3, k] -= rb
+    g, *_ = np.linalg.lstsq(B, G.reshape(9), rcond=None)
+    resid = np.abs(B @ g - G.reshape(9)).max() / np.abs(G).max()
+    w = sum(g[k] * ((ref_pos[a] - ref_pos[b]) @ (x[a] - x[b])) for k, (a, b) in enumerate(bonds))
+    return resid, w
+
+
+def test_settle_constraint_virials_equal_bond_form():
+    """The SETTLE virials (position level: forces m corr / dt^2; velocity
+    level: m dv / dt, each dotted with the atom's position relative to the
+    COM, kernels/settle.cu) equal the SHAKE / RATTLE bond form
+    sum_b g_b r_b(ref) . r_b: the forces lie along the reference bonds and
+    the per-atom virials sum to an origin-free total."""
+    rng = np.random.default_rng(21)
+    m_o, m_h = 15.9994 * U.AMU, 1.008 * U.AMU
+    m = np.array([m_o, m_h, m_h])
+    bonds = [(0, 1), (0, 2), (1, 2)]
+    lat = [60, 60, 60, 0, 0, 0]
+    dt = 80.0
+    for _ in range(10):
+        snap, d_oh, d_hh = _water(rng)
+        geom = oracle.settle_geometry(d_oh, d_hh, m_o, m_h)
+        cur = snap + rng.standard_normal((3, 3)) * 0.03
+        new, _, vir = oracle.settle_positions_one(snap, cur, np.zeros((3, 3)), m_o, m_h, geom,
+                                                  dt, lat)
+        G = m[:, None] * (new - cur) / dt ** 2
+        resid, w = _bond_form(G, snap, new, bonds)
+        assert resid < 1e-9
+        assert vir.sum() == pytest.approx(w, rel=1e-9)
+        # velocity level at the reset positions
+        v = rng.standard_normal((3, 3)) * 1e-3
+        v2, vir_v = oracle.settle_velocities_one(new, v, m_o, m_h, lat, dt=dt)
+        G = m[:, None] * (v2 - v) / dt
+        resid, w = _bond_form(G, new, new, bonds)
+        assert resid < 1e-9
+        assert vir_v.sum() == pytest.approx(w, rel=1e-9)
+        assert oracle.settle_velocities_one(new, v, m_o, m_h, lat) == pytest.approx(v2)
 
 
 def test_kinetic_energy_closed_form():
