@@ -30,7 +30,8 @@ DIMENSIONS = ["dimensionless", "length", "inverse_length", "mass", "charge", "en
 
 # Every symbol include/heddle_hip.h declares (checked by the CPU tests).
 EXPORTED = [
-    "hmd_last_error", "hmd_version", "hmd_ctx_create", "hmd_ctx_destroy",
+    "hmd_last_error", "hmd_version", "hmd_ctx_create", "hmd_ctx_create_on_stream",
+    "hmd_ctx_destroy",
     "hmd_ctx_synchronize", "hmd_ctx_device_info", "hmd_kernel_count", "hmd_kernel_name",
     "hmd_particles_create", "hmd_particles_destroy", "hmd_particles_download",
     "hmd_particles_upload", "hmd_particles_download_energies", "hmd_particles_download_lo",
@@ -49,9 +50,11 @@ EXPORTED = [
     "hmd_forcefield_slots", "hmd_spme_scalars", "hmd_spme_grids", "hmd_spme_forces",
     "hmd_nbl_request_rebuild", "hmd_nbl_pair_snapshot", "hmd_nbl_count_pairs",
     "hmd_nbl_stats", "hmd_nbl_set_capacity",
+    "hmd_nbl_pre_step", "hmd_pair_force", "hmd_spme_compute", "hmd_forcefield_check",
     "hmd_integrator_create", "hmd_integrator_destroy", "hmd_integrator_step",
     "hmd_integrator_capture", "hmd_integrator_replay", "hmd_integrator_steps",
     "hmd_integrator_profile", "hmd_integrator_minimize",
+    "hmd_integrator_set_forcefield", "hmd_integrator_run_judged",
     "hmd_generate_velocities", "hmd_chacha_block",
     "hmd_mlp_param_count", "hmd_mlp_create", "hmd_mlp_destroy", "hmd_mlp_set_params",
     "hmd_mlp_get_params", "hmd_mlp_set_force_scale", "hmd_mlp_descriptors",
@@ -105,6 +108,13 @@ class IntegratorDesc(ctypes.Structure):
         ("m_o", ctypes.c_double), ("m_h", ctypes.c_double),
         ("flags", ctypes.c_uint32),
     ]
+
+
+class SlotOutput(ctypes.Structure):
+    """hmd_slot_output: device pointers of a caller-owned SlotOutputView."""
+    _fields_ = [("force_x", ctypes.c_void_p), ("force_y", ctypes.c_void_p),
+                ("force_z", ctypes.c_void_p), ("energy", ctypes.c_void_p),
+                ("virial", ctypes.c_void_p)]
 
 
 class MlpDesc(ctypes.Structure):
@@ -201,12 +211,21 @@ def _declare(L):
         "hmd_nbl_count_pairs": (st, [vp, ctypes.c_double, _c_u64p]),
         "hmd_nbl_stats": (st, [vp, _c_u32p]),
         "hmd_nbl_set_capacity": (st, [vp, ctypes.c_uint32, ctypes.c_uint32]),
+        "hmd_nbl_pre_step": (st, [vp]),
+        "hmd_pair_force": (st, [vp, ctypes.c_int32, ctypes.POINTER(SlotOutput)]),
+        "hmd_spme_compute": (st, [vp, ctypes.c_int32, ctypes.POINTER(SlotOutput)]),
+        "hmd_forcefield_check": (st, [vp]),
+        "hmd_ctx_create_on_stream": (st, [ctypes.c_int, vp, pvp]),
         "hmd_integrator_create": (st, [vp, vp, vp, ctypes.POINTER(IntegratorDesc), pvp]),
         "hmd_integrator_destroy": (None, [vp]),
         "hmd_integrator_step": (st, [vp, ctypes.c_uint32]),
         "hmd_integrator_capture": (st, [vp]),
         "hmd_integrator_replay": (st, [vp, ctypes.c_uint32]),
         "hmd_integrator_steps": (ctypes.c_uint64, [vp]),
+        "hmd_integrator_set_forcefield": (st, [vp, vp]),
+        "hmd_integrator_run_judged": (st, [vp, vp, ctypes.c_uint32, ctypes.c_uint32,
+                                           ctypes.c_double, _c_dp, ctypes.c_uint32,
+                                           _c_u32p, _c_u64p]),
         "hmd_integrator_profile": (st, [vp, ctypes.c_uint32, _c_dp, ctypes.c_int32,
                                         _c_i32p, ctypes.c_char_p, ctypes.c_int32]),
         "hmd_integrator_minimize": (st, [vp, ctypes.c_uint32, ctypes.c_double]),

@@ -21,9 +21,16 @@ class GpuContext:
     """init_device (src/gpu/device.rs:67): one device, one stream, the
     embedded gfx950 code object loaded."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, stream=None):
+        """stream: a caller's HIP stream handle (an int, e.g. a torch
+        stream's cuda_stream) to enqueue on instead of a stream of the
+        context's own (hmd_ctx_create_on_stream)."""
         h = ctypes.c_void_p()
-        check(lib().hmd_ctx_create(device, ctypes.byref(h)))
+        if stream is None:
+            check(lib().hmd_ctx_create(device, ctypes.byref(h)))
+        else:
+            check(lib().hmd_ctx_create_on_stream(device, ctypes.c_void_p(int(stream)),
+                                                 ctypes.byref(h)))
         self.h = h
 
     def synchronize(self):
@@ -296,6 +303,22 @@ class ForceField:
         (hmd_nbl_set_capacity); the list is rebuilt at the next evaluation."""
         check(lib().hmd_nbl_set_capacity(self.h, int(entries), int(lds_entries)))
 
+    def pre_step(self):
+        """Skin check and device-gated rebuild, enqueued (hmd_nbl_pre_step)."""
+        check(lib().hmd_nbl_pre_step(self.h))
+
+    def pair_force(self, level, out):
+        """The pair slots added into a SlotOutput's device arrays, enqueued."""
+        check(lib().hmd_pair_force(self.h, level, ctypes.byref(out)))
+
+    def spme_compute(self, level, out):
+        """The reciprocal slot added into a SlotOutput's device arrays, enqueued."""
+        check(lib().hmd_spme_compute(self.h, level, ctypes.byref(out)))
+
+    def check(self):
+        """Synchronize and read the neighbour status (hmd_forcefield_check)."""
+        check(lib().hmd_forcefield_check(self.h))
+
     def stats(self):
         out = np.zeros(8, np.uint32)
         check(lib().hmd_nbl_stats(self.h, _ptr(out, _u32p)))
@@ -348,6 +371,25 @@ class Integrator:
 
     def steps(self):
         return int(lib().hmd_integrator_steps(self.h))
+
+    def set_forcefield(self, forcefield):
+        """Step with another force field on the same particles from here on
+        (hmd_integrator_set_forcefield)."""
+        check(lib().hmd_integrator_set_forcefield(self.h, forcefield.h))
+
+    def run_judged(self, fine, n_steps, judge_every=50, threshold=1e-3):
+        """Graph-replayed steps with the fallback judge every judge_every
+        steps; the first judge above threshold switches to `fine` for good
+        (hmd_integrator_run_judged).  Returns (judge values, step of the
+        switch or None)."""
+        m = n_steps // judge_every + 1
+        js = np.zeros(m)
+        k = ctypes.c_uint32()
+        at = ctypes.c_uint64()
+        check(lib().hmd_integrator_run_judged(self.h, fine.h, n_steps, judge_every, threshold,
+                                              _ptr(js, _dp), m, ctypes.byref(k),
+                                              ctypes.byref(at)))
+        return js[:k.value].tolist(), (None if at.value == 2 ** 64 - 1 else int(at.value))
 
     def minimize(self, n_iter, max_step):
         check(lib().hmd_integrator_minimize(self.h, n_iter, max_step))

@@ -87,7 +87,7 @@ const char* hmd_kernel_name(int i) {
   return (i >= 0 && i < K_COUNT) ? kKernelNames[i] : nullptr;
 }
 
-hmd_status hmd_ctx_create(int device, hmd_ctx** out) {
+static hmd_status ctx_create(int device, hipStream_t caller_stream, hmd_ctx** out) {
   HMD_GUARD({
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= device)
@@ -104,7 +104,12 @@ hmd_status hmd_ctx_create(int device, hmd_ctx** out) {
       if (c.device_name.rfind("gfx950", 0) != 0)
         throw Error(HMD_ERR_UNSUPPORTED,
                     "code object is gfx950-only; device is " + c.device_name);
-      check(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate");
+      if (caller_stream) {
+        c.stream = caller_stream;
+        c.owns_stream = false;
+      } else {
+        check(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate");
+      }
       check(hipModuleLoadData(&c.module, hmd_code_object), "hipModuleLoadData");
       for (int k = 0; k < K_COUNT; ++k) {
         check(hipModuleGetFunction(&c.fn[k], c.module, kKernelNames[k]), kKernelNames[k]);
@@ -122,12 +127,22 @@ hmd_status hmd_ctx_create(int device, hmd_ctx** out) {
   });
 }
 
+hmd_status hmd_ctx_create(int device, hmd_ctx** out) { return ctx_create(device, nullptr, out); }
+
+hmd_status hmd_ctx_create_on_stream(int device, void* stream, hmd_ctx** out) {
+  if (!stream) {
+    hmd_set_error("null stream");
+    return HMD_ERR_INVALID;
+  }
+  return ctx_create(device, static_cast<hipStream_t>(stream), out);
+}
+
 void hmd_ctx_destroy(hmd_ctx* h) {
   if (!h) return;
   Context& c = h->c;
   for (auto& m : c.marks) (void)hipEventDestroy(m.second);
   if (c.module) (void)hipModuleUnload(c.module);
-  if (c.stream) (void)hipStreamDestroy(c.stream);
+  if (c.stream && c.owns_stream) (void)hipStreamDestroy(c.stream);
   delete h;
 }
 

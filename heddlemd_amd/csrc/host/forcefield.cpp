@@ -318,7 +318,7 @@ ForceField::ForceField(Context& ctx, Particles& part, const hmd_forcefield_desc&
   // (196,608 atoms: 0.543 -> 0.469 ms/step); for small ones the extra graph
   // branches cost more than they overlap (64 boxes of 648-12,288 atoms in one
   // graph: 4.9 -> 7.5 ms per batched step), so it starts on at 16,384 atoms.
-  spme_stream = n >= 16384 ? 1u : 0u;
+  spme_stream = (n >= 16384 && c.owns_stream) ? 1u : 0u;
   make_side_stream();
   p.observers.push_back(this);   // box changes reach this force field
 }
@@ -400,7 +400,7 @@ void ForceField::init_neighbor_list(const hmd_forcefield_desc& d) {
 
   n_clusters = ceil_div(n, HMD_CLUSTER);
   n_pad = n_clusters * HMD_CLUSTER;
-  if (n_clusters > 65536u)   // the list build ballots 64 x 1024 j-clusters per i-cluster
+  if (n_clusters > 65536u)   // the build's candidate masks (20 B per 64 clusters) share its LDS
     throw Error(HMD_ERR_UNSUPPORTED, "more than 65,536 clusters (4,194,304 atoms)");
   status.alloc(4);
   atom_key.alloc(n);
@@ -436,6 +436,7 @@ void ForceField::alloc_entries(uint32_t cap) {
   cap_entries = cap;
   const size_t slots_total = (size_t)n_clusters * cap * HMD_ENTRY;
   entry_j.alloc(slots_total, false);
+  entry_mask.alloc((size_t)n_clusters * cap, false);
   entry_excl.alloc(slots_total, false);
   work.alloc((size_t)n_clusters * cap);
 }
@@ -656,10 +657,13 @@ void ForceField::rebuild_list() {
   bd.r_search = r_search;
   bd.orthorhombic_fast = ortho_fast ? 1u : 0u;
   bd.cap_entries = cap_entries;
-  uint32_t lds_max = (uint32_t)(kNblListLdsBytes / (HMD_ENTRY * 4));
+  bd.max_chunks = ceil_div(n_clusters, 64);   // one ballot per 64 j-clusters
+  const size_t chunk_lds = 16 * (size_t)bd.max_chunks + 4 * (size_t)((bd.max_chunks + 2) & ~1u);
+  uint32_t lds_max = (uint32_t)((kNblBuildLds - kNblBuildStaticLds - chunk_lds) / (HMD_ENTRY * 4));
   if (lds_entries) lds_max = std::min(lds_max, lds_entries);
   bd.lds_slots = std::min(cap_entries, lds_max) * HMD_ENTRY;
   bd.entry_j = entry_j.ptr;
+  bd.entry_mask = entry_mask.ptr;
   bd.entry_excl = entry_excl.ptr;
   bd.entry_count = entry_count.ptr;
   bd.excl_entries = excl_entries.ptr;
@@ -669,7 +673,8 @@ void ForceField::rebuild_list() {
 #ifndef HMD_NBL_WAVES
 #define HMD_NBL_WAVES 4u
 #endif
-  launch(c, K_hmd_nbl_build, n_clusters, 64 * HMD_NBL_WAVES, (size_t)bd.lds_slots * 4, bd);
+  launch(c, K_hmd_nbl_build, n_clusters, 64 * HMD_NBL_WAVES, chunk_lds + (size_t)bd.lds_slots * 4,
+         bd);
   entry_scan.run(c, entry_count.ptr, entry_offset.ptr, gate, HMD_STATUS_DISPLACED);
   NblWorkParams wp{entry_count.ptr, entry_offset.ptr, n_clusters, cap_entries,
                    work.ptr, n_work.ptr, status.ptr};
@@ -695,6 +700,7 @@ void ForceField::pair_forces(bool fev) {
   pp.work = work.ptr;
   pp.n_work = n_work.ptr;
   pp.entry_j = entry_j.ptr;
+  pp.entry_mask = entry_mask.ptr;
   pp.entry_excl = entry_excl.ptr;
   pp.excl_entries = excl_entries.ptr;
   pp.bbox = bbox.ptr;
@@ -1015,6 +1021,69 @@ void hmd_forcefield_destroy(hmd_forcefield* f) {
 hmd_status hmd_forcefield_compute(hmd_forcefield* f, int32_t level) {
   HMD_GUARD({
     f->ff->compute(level);
+    check(hipStreamSynchronize(f->ff->c.stream), "sync");
+    f->ff->check_status();
+  });
+}
+
+// ---- slot level: the reference's Potential::compute per slot, writing +=
+// into caller-owned per-atom outputs (SlotOutputView), enqueue only.
+static void check_slot_output(const ForceField& F, int32_t level, const hmd_slot_output* o) {
+  if (!o || !o->force_x || !o->force_y || !o->force_z)
+    throw Error(HMD_ERR_INVALID, "slot output: null force arrays");
+  (void)F;
+  (void)level;
+}
+
+hmd_status hmd_nbl_pre_step(hmd_forcefield* f) {
+  HMD_GUARD({
+    if (!f) throw Error(HMD_ERR_INVALID, "null force field");
+    f->ff->neighbor_pre_step();
+  });
+}
+
+hmd_status hmd_pair_force(hmd_forcefield* f, int32_t level, const hmd_slot_output* out) {
+  HMD_GUARD({
+    if (!f) throw Error(HMD_ERR_INVALID, "null force field");
+    ForceField& F = *f->ff;
+    check_slot_output(F, level, out);
+    const bool fev = level != 0;
+    F.pair_forces(fev);
+    SlotAddParams q{};
+    q.acc = F.acc.ptr;
+    q.acc_stride = F.n_pad;
+    q.slot_of = F.slot_of.ptr;
+    q.acc_clear = F.acc.ptr;
+    q.out_x = out->force_x; q.out_y = out->force_y; q.out_z = out->force_z;
+    q.out_e = out->energy; q.out_w = out->virial;
+    q.n = F.n;
+    q.fev = fev ? 1u : 0u;
+    launch(F.c, K_hmd_slot_add, ceil_div(F.n, 256), 256, 0, q);
+  });
+}
+
+hmd_status hmd_spme_compute(hmd_forcefield* f, int32_t level, const hmd_slot_output* out) {
+  HMD_GUARD({
+    if (!f) throw Error(HMD_ERR_INVALID, "null force field");
+    ForceField& F = *f->ff;
+    if (!F.spme_on) throw Error(HMD_ERR_INVALID, "no spme-reciprocal slot");
+    check_slot_output(F, level, out);
+    const bool fev = level != 0;
+    F.spme_forces(fev);
+    if (F.mlp) mlp_correct(*F.mlp, F);
+    SlotAddParams q{};
+    q.src_f = F.slow_f.ptr; q.src_e = F.slow_e.ptr; q.src_w = F.slow_w.ptr;
+    q.out_x = out->force_x; q.out_y = out->force_y; q.out_z = out->force_z;
+    q.out_e = out->energy; q.out_w = out->virial;
+    q.n = F.n;
+    q.fev = fev ? 1u : 0u;
+    launch(F.c, K_hmd_slot_add, ceil_div(F.n, 256), 256, 0, q);
+  });
+}
+
+hmd_status hmd_forcefield_check(hmd_forcefield* f) {
+  HMD_GUARD({
+    if (!f) throw Error(HMD_ERR_INVALID, "null force field");
     check(hipStreamSynchronize(f->ff->c.stream), "sync");
     f->ff->check_status();
   });

@@ -20,10 +20,12 @@ struct JitPairModule {
   ~JitPairModule();
 };
 
-// LDS the list build may use for the head of a cluster's staged j list
-// (64 KiB per workgroup minus the kernel's static arrays); list positions
-// past it are written to the entry array directly.
-constexpr size_t kNblListLdsBytes = 65536 - 29696;
+// LDS of the list build: 64 KiB per workgroup, less its static arrays (at
+// most 24 KiB) and its candidate-mask arrays (20 bytes per 64 clusters); the
+// rest stages the head of a cluster's j list (list positions past it are
+// written to the entry array directly).
+constexpr size_t kNblBuildLds = 65536;
+constexpr size_t kNblBuildStaticLds = 24576;
 // Capacity policy of the packed list (src/forces/neighbor_list.rs:235-244,
 // 415-445): grow by 1.5x, and grow ahead of any dropped entry once a build
 // fills more than 0.8 of the capacity.
@@ -94,7 +96,7 @@ struct ForceField {
   DevBuf<uint32_t> atom_key, cell_count, cell_start, sorted_id, slot_of, type_sorted;
   DevBuf<float> posq_sorted, ref_pos, bbox, sub_bbox;
   DevBuf<int32_t> image_sorted;   // int4[n_pad]
-  DevBuf<uint32_t> entry_j, entry_count, entry_offset, work, n_work;
+  DevBuf<uint32_t> entry_j, entry_mask, entry_count, entry_offset, work, n_work;
   DevBuf<uint64_t> entry_excl;
   DevBuf<uint32_t> excl_entries;   // [n_clusters]
   DevBuf<long long> acc;            // [5][n_pad] fixed-point accumulators

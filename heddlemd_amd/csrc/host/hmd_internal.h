@@ -22,7 +22,7 @@
   X(hmd_nbl_worklist) X(hmd_nbl_close) X(hmd_nbl_pairs)                     \
   X(hmd_pair_force_f) X(hmd_pair_force_fev) X(hmd_pair_force_lj_f)           \
   X(hmd_pair_force_lj_fev) X(hmd_pair_force_coul_f) X(hmd_pair_force_coul_fev)\
-  X(hmd_pair_finalize) X(hmd_pair_scaled)                                   \
+  X(hmd_pair_finalize) X(hmd_pair_scaled) X(hmd_slot_add)                   \
   X(hmd_spme_bin) X(hmd_spme_bin_scatter)                                    \
   X(hmd_spme_spread_p4) X(hmd_spme_gather_p4)                                \
   X(hmd_spme_spread_p5) X(hmd_spme_gather_p5)                                \
@@ -84,6 +84,7 @@ struct Context {
   int num_cus = 0;
   std::string device_name;
   hipStream_t stream = nullptr;
+  bool owns_stream = true;   // false: the caller's stream (hmd_ctx_create_on_stream)
   hipModule_t module = nullptr;
   hipFunction_t fn[K_COUNT] = {};
   hipFunction_t fn_batch[K_COUNT] = {};   // NAME_batch twins (null where absent)

@@ -19,7 +19,7 @@ enum class Kind { VelocityVerlet, LangevinBaoab };
 struct Integrator {
   Context& c;
   Particles& p;
-  ForceField& ff;
+  ForceField* ff;   // the force field the steps evaluate (a judged run may switch it)
   Kind kind = Kind::VelocityVerlet;
   float dt = 0;
   bool compensated = false;
@@ -45,7 +45,7 @@ struct Integrator {
   bool primed = false;
 
   Integrator(Context& cc, Particles& pp, ForceField& f, const hmd_integrator_desc& d)
-      : c(cc), p(pp), ff(f) {
+      : c(cc), p(pp), ff(&f) {
     std::string k = d.kind ? d.kind : "velocity-verlet";
     if (k == "velocity-verlet") kind = Kind::VelocityVerlet;
     else if (k == "langevin-baoab") kind = Kind::LangevinBaoab;
@@ -202,7 +202,7 @@ struct Integrator {
       launch(c, kind == Kind::VelocityVerlet ? K_hmd_step_vv_first : K_hmd_step_baoa, nu, 256,
              0, sp);
       c.mark(kind == Kind::VelocityVerlet ? "kick_drift_settle" : "baoa");
-      ff.compute(level());
+      ff->compute(level());
       launch(c, K_hmd_step_kick_last, nu, 256, 0, sp);
       c.mark("kick_settle");
       return;
@@ -212,7 +212,7 @@ struct Integrator {
       launch(c, K_hmd_vv_kick_drift, nb, 256, 0, vv());         // KickDrift
       settle_positions(dt);                                     // AfterDrift
       c.mark("kick_drift_settle");
-      ff.compute(level());                                      // ForceEval
+      ff->compute(level());                                      // ForceEval
       launch(c, K_hmd_vv_kick, nb, 256, 0, vv());               // KickHalf
       settle_velocities();                                      // AfterKick
       settle_virial();
@@ -232,7 +232,7 @@ struct Integrator {
       launch(c, K_hmd_lan_drift_half, nb, 256, 0, lan(dt));
       settle_positions(0.5f * dt);
       c.mark("baoa");
-      ff.compute(level());
+      ff->compute(level());
       launch(c, K_hmd_vv_kick, nb, 256, 0, k);
       settle_velocities();
       settle_virial();
@@ -244,9 +244,9 @@ struct Integrator {
 
   void prime() {
     if (primed) return;
-    ff.compute(level());
+    ff->compute(level());
     check(hipStreamSynchronize(c.stream), "sync");
-    ff.check_status();
+    ff->check_status();
     primed = true;
   }
 
@@ -255,7 +255,7 @@ struct Integrator {
     for (uint32_t i = 0; i < n; ++i) enqueue_step();
     check(hipStreamSynchronize(c.stream), "sync");
     steps += n;
-    ff.check_status();
+    ff->check_status();
   }
 
   void capture() {
@@ -273,15 +273,29 @@ struct Integrator {
     }
     check(hipStreamEndCapture(c.stream, &graph), "hipStreamEndCapture");
     check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-    ff.graph_invalid = false;
+    ff->graph_invalid = false;
+  }
+
+  // Steps from here on evaluate forces with `f` (same particles): the
+  // forces in the buffer are re-evaluated with it at the current positions
+  // (the next kick uses them) and the step graph is captured anew.
+  void set_forcefield(ForceField& f) {
+    if (&f.p != &p) throw Error(HMD_ERR_INVALID, "force field is on other particles");
+    ff = &f;
+    ff->compute(level());
+    check(hipStreamSynchronize(c.stream), "sync");
+    ff->check_status();
+    primed = true;
+    if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+    if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
   }
 
   void replay(uint32_t n) {
-    if (!exec || ff.graph_invalid) capture();
+    if (!exec || ff->graph_invalid) capture();
     for (uint32_t i = 0; i < n; ++i) check(hipGraphLaunch(exec, c.stream), "hipGraphLaunch");
     check(hipStreamSynchronize(c.stream), "sync");
     steps += n;
-    ff.check_status();
+    ff->check_status();
   }
 };
 
@@ -373,7 +387,7 @@ struct ReplicaSet {
     }
     check(hipStreamEndCapture(master, &graph), "hipStreamEndCapture");
     check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-    for (auto* it : its) it->ff.graph_invalid = false;
+    for (auto* it : its) it->ff->graph_invalid = false;
   }
 
   // Two recorded launches can share a batched launch: the same kernel of
@@ -493,7 +507,7 @@ struct ReplicaSet {
                                     kKernelNames[L->k]});
       }
     }
-    for (auto* it : its) it->ff.graph_invalid = false;
+    for (auto* it : its) it->ff->graph_invalid = false;
     if (!capture_graph) return;
     check(hipStreamBeginCapture(master, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
@@ -521,13 +535,13 @@ struct ReplicaSet {
   void step(uint32_t n) {
     if (mode == Mode::Batched && eager) {
       bool stale = planned.empty();
-      for (auto* it : its) stale = stale || it->ff.graph_invalid;
+      for (auto* it : its) stale = stale || it->ff->graph_invalid;
       if (stale) capture_batched(false);
       for (uint32_t i = 0; i < n; ++i) issue_planned();
       check(hipStreamSynchronize(master), "sync");
     } else if (mode != Mode::PerReplica) {
       bool stale = !exec;
-      for (auto* it : its) stale = stale || it->ff.graph_invalid;
+      for (auto* it : its) stale = stale || it->ff->graph_invalid;
       if (stale) {
         if (mode == Mode::Batched) capture_batched();
         else capture();
@@ -536,7 +550,7 @@ struct ReplicaSet {
       check(hipStreamSynchronize(master), "sync");
     } else {
       for (auto* it : its)
-        if (!it->exec || it->ff.graph_invalid) it->capture();
+        if (!it->exec || it->ff->graph_invalid) it->capture();
       for (uint32_t i = 0; i < n; ++i)
         for (auto* it : its) check(hipGraphLaunch(it->exec, it->c.stream), "hipGraphLaunch");
       for (auto* it : its) check(hipStreamSynchronize(it->c.stream), "sync");
@@ -549,7 +563,7 @@ struct ReplicaSet {
     hmd_status code = HMD_OK;
     for (size_t r = 0; r < its.size(); ++r) {
       try {
-        its[r]->ff.check_status();
+        its[r]->ff->check_status();
       } catch (const Error& e) {
         if (code == HMD_OK) {
           code = e.code;
@@ -900,6 +914,50 @@ hmd_status hmd_replicas_plan(hmd_replicas* h, uint32_t* n_batched, uint32_t* n_s
 hmd_status hmd_integrator_step(hmd_integrator* w, uint32_t n) { HMD_GUARD(w->it->step(n)); }
 hmd_status hmd_integrator_capture(hmd_integrator* w) { HMD_GUARD(w->it->capture()); }
 hmd_status hmd_integrator_replay(hmd_integrator* w, uint32_t n) { HMD_GUARD(w->it->replay(n)); }
+
+hmd_status hmd_integrator_set_forcefield(hmd_integrator* w, hmd_forcefield* f) {
+  HMD_GUARD({
+    if (!w || !f) throw Error(HMD_ERR_INVALID, "null argument");
+    w->it->set_forcefield(*f->ff);
+  });
+}
+
+// The learned leg's guarded run: batches of judge_every graph-replayed steps,
+// the judge after each (hmd_mlp_judge: the integrator's corrected forces
+// against fine_ff's at the same positions), and on the first judge above
+// the threshold a switch of the integrator to fine_ff for the rest of the
+// run.  The reference's nearest policy is the runner's fallback from graph
+// replay to eager steps when a capture fails (src/runner.rs:1650-1656).
+hmd_status hmd_integrator_run_judged(hmd_integrator* w, hmd_forcefield* fine, uint32_t n_steps,
+                                     uint32_t judge_every, double threshold, double* judges,
+                                     uint32_t max_judges, uint32_t* n_judges,
+                                     uint64_t* reverted_at) {
+  HMD_GUARD({
+    if (!w || !fine || !reverted_at || judge_every == 0)
+      throw Error(HMD_ERR_INVALID, "null argument or zero judge interval");
+    Integrator& I = *w->it;
+    if (&fine->ff->p != &I.p) throw Error(HMD_ERR_INVALID, "fine force field is on other particles");
+    *reverted_at = UINT64_MAX;
+    uint32_t k = 0;
+    for (uint32_t done = 0; done < n_steps;) {
+      const uint32_t b = std::min(judge_every, n_steps - done);
+      I.replay(b);
+      done += b;
+      if (I.ff == fine->ff) continue;   // reverted: plain fine-mesh steps
+      hmd_forcefield nn{I.ff};
+      double e = 0.0;
+      const hmd_status st = hmd_mlp_judge(&nn, fine, &e);
+      if (st != HMD_OK) throw Error(st, hmd_last_error());
+      if (judges && k < max_judges) judges[k] = e;
+      ++k;
+      if (!(e <= threshold)) {
+        *reverted_at = I.steps;
+        I.set_forcefield(*fine->ff);
+      }
+    }
+    if (n_judges) *n_judges = k;
+  });
+}
 uint64_t hmd_integrator_steps(hmd_integrator* w) { return w->it->steps; }
 
 hmd_status hmd_integrator_profile(hmd_integrator* w, uint32_t n, double* us, int32_t max_stages,
@@ -932,7 +990,7 @@ hmd_status hmd_integrator_profile(hmd_integrator* w, uint32_t n, double* us, int
       }
     }
     I.steps += n;
-    I.ff.check_status();
+    I.ff->check_status();
     std::string all;
     int32_t k = 0;
     for (auto& nm : order) {
@@ -986,7 +1044,7 @@ hmd_status hmd_integrator_timings(hmd_integrator* w, uint32_t n, int32_t max_sta
     for (auto& m : c.marks) (void)hipEventDestroy(m.second);
     c.marks.clear();
     I.steps += n;
-    I.ff.check_status();
+    I.ff->check_status();
     std::string all;
     int32_t k = 0;
     for (auto& nm : order) {
@@ -1021,10 +1079,10 @@ hmd_status hmd_integrator_minimize(hmd_integrator* w, uint32_t n_iter, double ma
     // evaluation is then repeated at the same positions.
     auto energy = [&]() {
       for (int attempt = 0;; ++attempt) {
-        I.ff.compute(1);
+        I.ff->compute(1);
         check(hipStreamSynchronize(c.stream), "sync");
         try {
-          I.ff.check_status();
+          I.ff->check_status();
         } catch (const Error& e) {
           if (e.code != HMD_ERR_NBL_OVERFLOW || attempt >= 3) throw;
           continue;

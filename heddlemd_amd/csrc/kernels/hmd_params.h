@@ -112,7 +112,9 @@ struct NblBuildParams {
   uint32_t orthorhombic_fast;  // 1: exact fast min-image valid (see kernel)
   uint32_t cap_entries;        // entries per cluster
   uint32_t lds_slots;          // list slots staged in LDS (later ones go to entry_j directly)
+  uint32_t max_chunks;         // ceil(n_clusters / 64): LDS candidate-mask arrays
   uint32_t* entry_j;           // [n_clusters*cap_entries*64]
+  uint32_t* entry_mask;        // [n_clusters*cap_entries] i-sub-clusters any j reaches
   uint64_t* entry_excl;        // [n_clusters*cap_entries*64]
   uint32_t* entry_count;       // [n_clusters] (entries written / needed)
   uint32_t* excl_entries;      // [n_clusters] leading entries with masks
@@ -162,6 +164,7 @@ struct PairParams {
   const uint32_t* work;
   const uint32_t* n_work;
   const uint32_t* entry_j;
+  const uint32_t* entry_mask;  // per entry: 8-atom i-sub-clusters any of its j-atoms reaches
   const uint64_t* entry_excl;
   const uint32_t* excl_entries;  // [n_clusters]: masks exist for entries below
   const float* user_params;    // parameters of a run-time composed pair term
@@ -223,6 +226,27 @@ struct PairFinalizeParams {
   uint32_t n;
   uint32_t fev;
   long long* acc_clear;        // accumulators zeroed after reading
+};
+
+// One slot's per-atom results added (+=) into caller-owned outputs, the
+// SlotOutputView of the reference's Potential::compute: either the pair
+// slots' fixed-point accumulators (sorted order, cleared as read) or the
+// reciprocal slot's f32 arrays (canonical order).
+struct SlotAddParams {
+  const long long* acc;        // [5][acc_stride] fixed point, or null
+  uint32_t acc_stride;
+  const uint32_t* slot_of;     // canonical -> sorted (with acc)
+  long long* acc_clear;
+  const float* src_f;          // [3][n] canonical, or null (with acc null)
+  const float* src_e;          // [n]
+  const float* src_w;          // [n]
+  float* out_x;                // [n] each, canonical; +=
+  float* out_y;
+  float* out_z;
+  float* out_e;                // energy level only (may be null)
+  float* out_w;
+  uint32_t n;
+  uint32_t fev;
 };
 
 // ------------------------------------------------------------- spme ----

@@ -276,21 +276,33 @@ __device__ inline float hmd_box_gap2(float4 ca, float4 ha, float4 cb, float4 hb,
 // After a prefix over the per-chunk masks the candidates are listed in
 // increasing index order in LDS, and the four waves share them round robin:
 // lane k tests j-atom k of the candidate against the i-cluster's eight 8-atom
-// sub-boxes and keeps it when a sub-box holding an i-atom with a lower sorted
-// index lies within r_search of it; the next candidate's j-atom is loaded
-// while the current one is tested.  The 64-bit
-// hit mask of each candidate goes to LDS; wave 0 turns the masks into list
-// positions (prefix of the hit counts in candidate order) and all waves place
-// the kept atoms there, in increasing sorted order, in an LDS copy of the
-// cluster's list (positions past its lds_slots go straight to the entry
-// array), written out as whole entries.  Each lane of wave 0 keeps the
-// sorted slots of its atom's exclusion partners (at most HMD_MAX_INLINE_EXCL) in registers and, per
-// candidate, sets the bit of the entry position that holds a partner.  The
-// output depends only on the positions (deterministic).  More candidates than
-// fit in LDS are handled in rounds.
+// sub-boxes (consecutive sorted atoms, so sub-box s holds i-atoms 8s..8s+7)
+// and keeps it when a sub-box holding an i-atom with a lower sorted index lies
+// within r_search of it; the next candidate's j-atoms are loaded while the
+// current one is tested.  Each kept j-atom carries its 8-bit mask of reachable
+// sub-boxes.
+//
+// Placement.  The force kernel walks an entry's 64 j-atoms (one per lane)
+// over the i-atoms of the sub-boxes that any of them reaches, so entries whose
+// j-atoms share their masks cost least.  A cluster's list is therefore ordered
+// by a key of the mask: 0 for j-atoms of the cluster itself (first, with most
+// of the exclusion partners), else 1 + lowest + highest reachable sub-box (a
+// mask is close to a contiguous run along the Hilbert curve, and the key
+// groups similar runs), and by candidate then lane within a key: a stable
+// counting sort from per-(candidate, key) counts and ranks taken with ballots
+// in the hit test.  A cluster with more candidates than one round holds
+// keeps plain candidate order.  Either way the list depends only on the
+// positions (deterministic).  All waves place the kept atoms in an LDS copy of
+// the list (positions past its lds_slots go straight to the entry array),
+// written out as whole entries, together with each entry's mask (the union of
+// its j-atoms' masks).  Each lane of wave 0 keeps the sorted slots of its
+// atom's exclusion partners (at most HMD_MAX_INLINE_EXCL) in registers and
+// finds their list positions, from which the per-(entry, i-lane) exclusion
+// bit masks are written.
 #define HMD_MAX_INLINE_EXCL 8
-#define HMD_NBL_MAX_CHUNKS 1024u    // 64 j-clusters per chunk: 65,536 clusters
-#define HMD_NBL_CAND_CAP 512u
+#define HMD_NBL_CAND_CAP 128u       // candidates per round (one round: key-sorted list)
+#define HMD_NBL_KEYS 16u            // placement keys: own cluster, 1 + lo + hi
+#define HMD_NBL_EMASK_CAP 256u      // entries whose masks are kept (later ones: all 8)
 #ifndef HMD_NBL_PREFETCH
 #define HMD_NBL_PREFETCH 3       // candidates' j-atoms loaded ahead per wave
 #endif
@@ -307,6 +319,11 @@ __device__ inline uint32_t hmd_wave_scan_incl(uint32_t v, uint32_t lane) {
   return v;
 }
 
+// Placement key of a kept j-atom (see above).
+__device__ inline uint32_t hmd_nbl_key(uint32_t m8, bool own) {
+  return own ? 0u : 1u + (uint32_t)__builtin_ctz(m8) + (31u - (uint32_t)__builtin_clz(m8));
+}
+
 #ifndef HMD_NBL_BUILD_WAVES_PER_EU
 #define HMD_NBL_BUILD_WAVES_PER_EU 3
 #endif
@@ -319,14 +336,22 @@ __device__ inline uint32_t hmd_wave_scan_incl(uint32_t v, uint32_t lane) {
 __device__ __forceinline__ void hmd_nbl_build_body(const NblBuildParams& p, const uint32_t hmd_bx,
     const uint32_t hmd_gx) {
   if (!hmd_gate(p.status, HMD_STATUS_DISPLACED)) return;
-  __shared__ uint64_t s_mask[HMD_NBL_MAX_CHUNKS];
-  __shared__ uint64_t s_nowrap[HMD_NBL_MAX_CHUNKS];
-  __shared__ uint32_t s_pre[HMD_NBL_MAX_CHUNKS + 1];
   __shared__ uint32_t s_cand[HMD_NBL_CAND_CAP];
   __shared__ uint64_t s_hit[HMD_NBL_CAND_CAP];
   __shared__ uint32_t s_off[HMD_NBL_CAND_CAP];
-  __shared__ uint32_t s_fill;
-  extern __shared__ uint32_t s_list[];   // [lds_slots]: the head of the cluster's j list
+  // per kept j-atom: sub-box mask | (rank among the candidate's hits of its key) << 8
+  __shared__ uint16_t s_lane[HMD_NBL_CAND_CAP][HMD_ENTRY];
+  // per (candidate, key): hit count, then list position of the first such hit
+  __shared__ uint16_t s_koff[HMD_NBL_CAND_CAP][HMD_NBL_KEYS];
+  __shared__ uint32_t s_emask[HMD_NBL_EMASK_CAP];
+  __shared__ uint32_t s_fill, s_nex;
+  // dynamic: candidate masks per chunk of 64 j-clusters, their prefix, and
+  // the head of the cluster's j list
+  extern __shared__ uint64_t s_dyn[];
+  uint64_t* s_mask = s_dyn;                                   // [max_chunks]
+  uint64_t* s_nowrap = s_dyn + p.max_chunks;                  // [max_chunks]
+  uint32_t* s_pre = reinterpret_cast<uint32_t*>(s_dyn + 2 * p.max_chunks);  // [max_chunks + 1]
+  uint32_t* s_list = s_pre + ((p.max_chunks + 2) & ~1u);      // [lds_slots]
   const uint32_t c = hmd_bx;
   const uint32_t wave = threadIdx.x / HMD_WAVE;
   const uint32_t lane = hmd_lane();
@@ -342,6 +367,7 @@ __device__ __forceinline__ void hmd_nbl_build_body(const NblBuildParams& p, cons
   // SGPRs) rather than as LDS broadcasts.
   const hmd_const_f4* csub = (const hmd_const_f4*)(reinterpret_cast<const float4*>(p.sub_bbox) + 16 * c);
 
+  for (uint32_t e = threadIdx.x; e < HMD_NBL_EMASK_CAP; e += blockDim.x) s_emask[e] = 0u;
   // wave 0: the sorted slots of each i-atom's exclusion partners (loaded
   // first, so that the chain of dependent loads overlaps the candidate
   // search), and their positions in the cluster's list once placed
@@ -414,6 +440,7 @@ __device__ __forceinline__ void hmd_nbl_build_body(const NblBuildParams& p, cons
   }
   __syncthreads();
   const uint32_t total = s_pre[n_chunks];
+  const bool sorted = total <= HMD_NBL_CAND_CAP;   // one round: key-sorted placement
 
   const size_t base_entry = (size_t)c * p.cap_entries;
   const uint32_t cap_slots = p.cap_entries * HMD_ENTRY;
@@ -465,10 +492,9 @@ __device__ __forceinline__ void hmd_nbl_build_body(const NblBuildParams& p, cons
       // nearest point of the sub-box within the search radius, with a relative
       // margin of 2e-4).  The j-atom is listed when it may reach one that holds
       // an i-atom below it in sorted order: every pair within r_search is
-      // listed, together with some farther ones (about 7% more j-atoms than an
-      // exact per-atom test would keep), which the force kernels mask by their
-      // own cutoff tests.  Empty sub-boxes (past the last atom) have negative
-      // extents and never pass.
+      // listed, together with some farther ones, which the force kernels mask
+      // by their own cutoff tests.  Empty sub-boxes (past the last atom) have
+      // negative extents and never pass.
       uint32_t smask = 0;
       if (sj < p.n) {
         if (nowrap) {
@@ -490,23 +516,58 @@ __device__ __forceinline__ void hmd_nbl_build_body(const NblBuildParams& p, cons
         }
       }
       // own cluster: only sub-clusters holding an i-atom below this lane's j
-      const uint32_t lim = (jcl == c) ? ((1u << ((lane + 7u) / 8u)) - 1u) : 0xFFu;
-      const bool hit = (smask & lim) != 0u;
+      const bool own = jcl == c;
+      const uint32_t lim = own ? ((1u << ((lane + 7u) / 8u)) - 1u) : 0xFFu;
+      const uint32_t m8 = smask & lim;
+      const bool hit = m8 != 0u;
       const uint64_t H = __ballot(hit);
+      // ranks among this candidate's hits of the same key, and per-key counts
+      const uint32_t key = hit ? hmd_nbl_key(m8, own) : HMD_NBL_KEYS;
+      if (lane < HMD_NBL_KEYS) s_koff[q][lane] = 0;
+      uint32_t rank = 0;
+      uint64_t pending = sorted ? H : 0ull;
+      while (pending) {
+        const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
+        const uint32_t k = hmd_readlane(key, leader);
+        const uint64_t B = __ballot(key == k);
+        if (key == k) rank = (uint32_t)__builtin_popcountll(B & lt_mask);
+        if (lane == 0) s_koff[q][k] = (uint16_t)__builtin_popcountll(B);
+        pending &= ~B;
+      }
+      s_lane[q][lane] = (uint16_t)(m8 | (rank << 8));
       if (lane == 0) s_hit[q] = H;
     }
     __syncthreads();
 
-    // list positions of each candidate's hits: exclusive scan of the hit
-    // counts in candidate order (wave 0)
+    // list positions (wave 0): key-major counting sort of the one round, or
+    // hit counts in candidate order
     if (wave == 0) {
       uint32_t carry = s_fill;
-      for (uint32_t b = 0; b < cnt; b += HMD_WAVE) {
-        const uint32_t k = b + lane;
-        const uint32_t h = k < cnt ? (uint32_t)__builtin_popcountll(s_hit[k]) : 0u;
-        const uint32_t incl = hmd_wave_scan_incl(h, lane);
-        if (k < cnt) s_off[k] = carry + incl - h;
+      if (sorted) {
+        // per key (lane < 16): exclusive prefix over the candidates, then the
+        // keys' bases by a prefix over the lanes
+        uint32_t run = 0;
+        for (uint32_t k = 0; k < cnt; ++k) {
+          if (lane < HMD_NBL_KEYS) {
+            const uint32_t t = s_koff[k][lane];
+            s_koff[k][lane] = (uint16_t)run;
+            run += t;
+          }
+        }
+        const uint32_t tot = lane < HMD_NBL_KEYS ? run : 0u;
+        const uint32_t incl = hmd_wave_scan_incl(tot, lane);
+        const uint32_t kbase = carry + incl - tot;
+        for (uint32_t k = 0; k < cnt; ++k)
+          if (lane < HMD_NBL_KEYS) s_koff[k][lane] = (uint16_t)(s_koff[k][lane] + kbase);
         carry += hmd_readlane(incl, HMD_WAVE - 1);
+      } else {
+        for (uint32_t b = 0; b < cnt; b += HMD_WAVE) {
+          const uint32_t k = b + lane;
+          const uint32_t h = k < cnt ? (uint32_t)__builtin_popcountll(s_hit[k]) : 0u;
+          const uint32_t incl = hmd_wave_scan_incl(h, lane);
+          if (k < cnt) s_off[k] = carry + incl - h;
+          carry += hmd_readlane(incl, HMD_WAVE - 1);
+        }
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -527,24 +588,35 @@ __device__ __forceinline__ void hmd_nbl_build_body(const NblBuildParams& p, cons
         if (lo < cnt && (s_cand[lo] & 0x7FFFFFFFu) == pc) {
           const uint64_t H = s_hit[lo];
           if ((H >> pb) & 1ull) {
-            const uint64_t below = (pb == 0) ? 0ull : (H & (~0ull >> (64 - pb)));
-            ex_pos[m] = s_off[lo] + (uint32_t)__builtin_popcountll(below);
+            if (sorted) {
+              const uint32_t v = s_lane[lo][pb];
+              ex_pos[m] = s_koff[lo][hmd_nbl_key(v & 0xFFu, pc == c)] + (v >> 8);
+            } else {
+              const uint64_t below = (pb == 0) ? 0ull : (H & (~0ull >> (64 - pb)));
+              ex_pos[m] = s_off[lo] + (uint32_t)__builtin_popcountll(below);
+            }
           }
         }
       }
     }
     __syncthreads();
 
-    // placement, all waves: lane k of candidate q puts its j-atom at list
-    // position s_off[q] + (hits of q below lane k): in the LDS list, or,
-    // past the slots staged there, straight into the entry array
+    // placement, all waves: a kept j-atom goes to its list position, in the
+    // LDS list or, past the slots staged there, straight into the entry
+    // array; its mask joins its entry's
     for (uint32_t k = wave; k < cnt; k += HMD_NBL_WAVES) {
       const uint64_t H = s_hit[k];
       if ((H >> lane) & 1ull) {
-        const uint32_t pos = s_off[k] + (uint32_t)__builtin_popcountll(H & lt_mask);
-        const uint32_t sj = (s_cand[k] & 0x7FFFFFFFu) * HMD_CLUSTER + lane;
+        const uint32_t cw = s_cand[k] & 0x7FFFFFFFu;
+        const uint32_t v = s_lane[k][lane];
+        const uint32_t m8 = v & 0xFFu;
+        const uint32_t pos = sorted ? s_koff[k][hmd_nbl_key(m8, cw == c)] + (v >> 8)
+                                    : s_off[k] + (uint32_t)__builtin_popcountll(H & lt_mask);
+        const uint32_t sj = cw * HMD_CLUSTER + lane;
         if (pos < p.lds_slots) s_list[pos] = sj;
         else if (pos < cap_slots) p.entry_j[base_entry * HMD_ENTRY + pos] = sj;
+        const uint32_t e = pos / HMD_ENTRY;
+        if (e < HMD_NBL_EMASK_CAP) atomicOr(&s_emask[e], m8);
       }
     }
     __syncthreads();
@@ -565,9 +637,11 @@ __device__ __forceinline__ void hmd_nbl_build_body(const NblBuildParams& p, cons
       p.entry_j[base_entry * HMD_ENTRY + pos] = HMD_SENTINEL;
   }
   if (wave == 0) {
-    // Exclusion masks only for the leading entries that hold a partner (the
-    // own cluster and its neighbours come first in the list); the force
-    // kernels read no mask past excl_entries[c].
+    // Exclusion rows (lane = i-atom, bits over the entry's j-slots) for the
+    // entries up to the last one that holds a partner; an entry that holds
+    // one is flagged by bit 8 of its mask (the force kernel loads rows only
+    // for flagged entries; entries past the kept masks are all flagged, and
+    // their rows always written).
     uint32_t e_hi = 0;
 #pragma unroll
     for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m)
@@ -581,15 +655,23 @@ __device__ __forceinline__ void hmd_nbl_build_body(const NblBuildParams& p, cons
       for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m)
         if (ex_pos[m] != HMD_SENTINEL && ex_pos[m] / HMD_ENTRY == e)
           mask |= 1ull << (ex_pos[m] % HMD_ENTRY);
-      p.entry_excl[(base_entry + e) * HMD_ENTRY + lane] = mask;
+      const bool any = __ballot(mask != 0ull) != 0ull;
+      if (any || e >= HMD_NBL_EMASK_CAP) p.entry_excl[(base_entry + e) * HMD_ENTRY + lane] = mask;
+      if (any && lane == 0 && e < HMD_NBL_EMASK_CAP) s_emask[e] |= 0x100u;
     }
     if (lane == 0) {
+      s_nex = n_ex;
       p.entry_count[c] = n_written;
       p.excl_entries[c] = n_ex;
       atomicMax(&p.status[2], n_entries);   // high-water mark: the host resizes
       if (n_entries > p.cap_entries) atomicOr(p.status, HMD_STATUS_ENTRY_OVERFLOW);
     }
   }
+  __syncthreads();
+  const uint32_t n_ex = s_nex;
+  for (uint32_t e = threadIdx.x; e < n_written; e += blockDim.x)
+    p.entry_mask[base_entry + e] =
+        e < HMD_NBL_EMASK_CAP ? s_emask[e] : (0xFFu | (e < n_ex ? 0x100u : 0u));
 }
 HMD_KERNEL(hmd_nbl_build, NblBuildParams, 64 * HMD_NBL_WAVES, HMD_NBL_BUILD_WAVES_PER_EU)
 

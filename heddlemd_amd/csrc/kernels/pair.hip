@@ -12,75 +12,93 @@
 // closed form by one LDS gather and three FMAs per pair.
 //
 // Mapping (wave64, VALU FP32; no MFMA -- there is no dense contraction):
-//  * one wave = one 64-atom i-cluster, one i-atom per lane, held in VGPRs;
-//  * an entry's 64 j-atoms are gathered with one load per lane (and shifted
-//    to the image nearest the cluster centre) into a per-wave LDS table,
-//    then broadcast one per step by a wave-uniform LDS read, so each step
-//    evaluates 64 pairs with no global memory access and no shuffles;
+//  * one wave walks a contiguous share of the work list; an entry holds 64
+//    j-atoms of one 64-atom i-cluster's list, one j-atom per lane, gathered
+//    once (and shifted to the image nearest the cluster centre);
+//  * the i-cluster's atoms sit in a per-wave LDS table and are broadcast one
+//    per step by a wave-uniform LDS read, so each step evaluates 64 pairs
+//    with no global memory access and no shuffles;
+//  * an entry visits only the 8-atom i-sub-clusters that some of its j-atoms
+//    can reach (the entry's 8-bit mask from the list build, whose placement
+//    groups j-atoms of similar reach into the same entries): about two
+//    thirds of the steps of visiting every i-atom;
 //  * slot combinations (LJ, Coulomb, energies) are compile-time variants,
-//    and LJ is skipped for j-types with no LJ partner (uniform branch);
+//    and LJ is skipped for i-types with no LJ partner (uniform branch);
 //  * entries whose j-atoms all lie in later clusters and carry no exclusion
-//    (all but the first one or two of a cluster) run a step variant with the
-//    cutoff test alone;
-//  * the j-side reaction forces of 8 consecutive j-atoms are summed across
-//    the wave by a transposed butterfly on the gfx950 lane swaps
-//    (v_permlane32_swap, v_permlane16_swap) and DPP row operations; the 3
-//    (with energy and virial 5) components of the 8 j-atoms are then packed
-//    into 24 (40) lanes that convert to fixed point and issue their atomics
-//    together;
-//  * i-forces stay in registers across every entry of the same cluster in the
-//    wave's contiguous share of the work list and are flushed once;
+//    (most of a cluster's list) run a step variant with the cutoff test
+//    alone; exclusion rows are transposed once per entry (lane = j-atom,
+//    bits over the i-atoms);
+//  * the i-side values of a sub-cluster's 8 atoms are summed across the wave
+//    by a transposed butterfly on the gfx950 lane swaps (v_permlane32_swap,
+//    v_permlane16_swap) and DPP row operations, packed one component per
+//    lane and added to the wave's i accumulators in LDS; the j reaction stays
+//    in the lane and is flushed once per entry;
 //  * accumulation is i64 fixed point (2^40), so the per-atom totals do not
 //    depend on the order in which waves finish: runs are bit-reproducible.
 #include "hmd_device.h"
 
 #define HMD_MAX_TYPES 16u
 
-struct HmdPairState {
-  float xi, yi, zi, kqi;
-  uint32_t si, ti, ti_row;  // sorted index; type; type * n_types (LJ table row)
-  bool vi;
-  float fix, fiy, fiz, eiv, wiv;
-};
+// 64 x 64 bit-matrix transpose across the wave: lane r holds row r (bit k =
+// column k) on entry and column r (bit k = row k) on return.  Six stages of
+// block swaps with the lane 2^s apart.
+__device__ inline uint64_t hmd_transpose64(uint64_t x, uint32_t lane) {
+  const uint64_t masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                             0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+  uint32_t j = 32;
+#pragma unroll
+  for (int k = 0; k < 6; ++k, j >>= 1) {
+    const uint64_t m = masks[k];
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, (int)j, HMD_WAVE);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), (int)j, HMD_WAVE);
+    const uint64_t y = ((uint64_t)hi << 32) | lo;
+    x = (lane & j) ? ((x & ~m) | ((y >> j) & m)) : ((x & m) | ((y << j) & ~m));
+  }
+  return x;
+}
 
-// Steps over the j-atoms of one entry (64 pairs per step).  PLAIN entries
-// (every valid j-atom in a later cluster, no exclusion bit in any lane) need
-// only the cutoff test: i < j holds by construction, the i-atoms are all
-// valid (only the last cluster is partial, and it has no later cluster), and
-// empty slots carry a far-away position so that they fail the cutoff test.
+// The i-atoms of the sub-clusters an entry reaches, 64 pairs per step: lane
+// k holds j-atom k of the entry; step u of sub-cluster b takes i-atom 8b + u
+// from the wave's LDS table (a broadcast read).  PLAIN entries (every valid
+// j-atom in a later cluster, no exclusion) need only the cutoff test: i < j
+// holds by construction, the i-atoms are all valid (only the last cluster is
+// partial, and it has no later cluster), and empty slots carry a far-away
+// position so that they fail the cutoff test.  The j reaction accumulates in
+// the lane (fj); the i-side values of a sub-cluster's 8 atoms are summed over
+// the wave by the transposed butterfly and added to the wave's i
+// accumulators in LDS.
 template <bool FEV, bool LJ, bool COUL, bool SHIFT, bool PLAIN>
-__device__ inline void hmd_pair_steps(const PairParams& p, const float4* s_lj,
-                                      const float4* s_ct, const float4* s_jp,
-                                      const uint2* s_jt,
-                                      HmdPairState& st, uint32_t my_j, uint64_t my_ex,
-                                      uint64_t lj_bits, uint32_t lane) {
+__device__ inline void hmd_pair_isteps(const PairParams& p, const float4* s_lj,
+                                       const float4* s_ct, const float4* s_ip,
+                                       const uint32_t* s_it, float* s_fi, float4 pj,
+                                       uint32_t tj, uint32_t my_j, uint64_t exT,
+                                       uint32_t emask, uint32_t c, float (&fj)[5],
+                                       uint32_t lane) {
   const HmdLattice L = p.L;
   const float x_max = (float)HMD_COUL_BINS - 0.01f;   // last bin: beyond r_cut, masked
   const float lj_rs2_d = p.lj_rs2 * p.lj_inv_delta;
   const float lj_12d = 12.0f * p.lj_inv_delta;
-  constexpr uint32_t NC = FEV ? 5u : 3u;        // reaction components per j
-  const uint32_t grp = lane & 7u;
-  const uint32_t comp = 7u - grp;               // component packed into this lane
-  long long* acc = p.acc;
-  const uint32_t stride = p.acc_stride;
-
-  for (uint32_t kb = 0; kb < HMD_ENTRY; kb += 8) {
-    if (hmd_readlane(my_j, kb) == HMD_SENTINEL) break;
-    // jr[c][u]: this lane's contribution to component c of j-atom kb + u
-    // (forces with the i-side sign; the j-side reaction is their negation)
-    float jr[NC][8];
+  constexpr uint32_t NC = FEV ? 5u : 3u;        // components per atom
+  const uint32_t comp = 7u - (lane & 7u);       // component packed into this lane
+  const bool valid_j = my_j != HMD_SENTINEL;
+  const uint32_t c0 = c * HMD_CLUSTER;
+  uint32_t pend = emask;
+  while (pend) {
+    const uint32_t sb = (uint32_t)__builtin_ctz(pend);   // i-sub-cluster
+    pend &= pend - 1u;
+    // ir[c][u]: this lane's contribution to component c of i-atom 8 sb + u
+    float ir[NC][8];
 #pragma unroll
     for (uint32_t u = 0; u < 8; ++u) {
-      const uint32_t k = kb + u;
-      const float4 bj = s_jp[k];
-      float dx = st.xi - bj.x, dy = st.yi - bj.y, dz = st.zi - bj.z;
+      const uint32_t i = 8u * sb + u;
+      const float4 a = s_ip[i];                   // (x, y, z, k_C q) of the i-atom
+      float dx = a.x - pj.x, dy = a.y - pj.y, dz = a.z - pj.z;
       if (!SHIFT) hmd_min_image(dx, dy, dz, L);
       const float r2 = dx * dx + dy * dy + dz * dz;
       bool ok = r2 <= p.rc2;
       if (!PLAIN) {
-        const uint32_t j = __builtin_amdgcn_readfirstlane(s_jt[k].x);   // wave-uniform
-        const bool excluded = (my_ex >> k) & 1ull;
-        ok = ok && (j != HMD_SENTINEL) && st.vi && (st.si < j) && !excluded;
+        const uint32_t si = c0 + i;
+        ok = ok && valid_j && (si < p.n) && (si < my_j) && !((exT >> i) & 1ull);
       }
       // PLAIN: every pair has r2 > 0 (distinct atoms; empty slots far away),
       // so the masked-out pairs evaluate to finite values without a guard
@@ -90,7 +108,7 @@ __device__ inline void hmd_pair_steps(const PairParams& p, const float4* s_lj,
       float factor = 0.0f, energy = 0.0f;
       if (COUL) {
         // tabulated erfc kernel: bin b = floor(r/h), cubic in t = r/h - b
-        const float qq = st.kqi * bj.w;
+        const float qq = a.w * pj.w;
         const float x = fminf(r2s * inv_r * p.coul_inv_h, x_max);
         const float t = __builtin_amdgcn_fractf(x);
         const uint32_t b = (uint32_t)x;
@@ -101,10 +119,11 @@ __device__ inline void hmd_pair_steps(const PairParams& p, const float4* s_lj,
           energy = qq * (((e.w * t + e.z) * t + e.y) * t + e.x);
         }
       }
+      const uint32_t ti = __builtin_amdgcn_readfirstlane(s_it[i]);   // wave-uniform
       if (LJ) {
-        if ((lj_bits >> k) & 1ull) {                        // uniform branch
+        if ((p.lj_type_mask >> ti) & 1u) {                  // uniform branch
           // (c6, c12, 6 c6, 12 c12)
-          const float4 cc = s_lj[st.ti_row + s_jt[k].y];
+          const float4 cc = s_lj[ti * p.n_types + tj];
           const float ir6 = inv_r2 * inv_r2 * inv_r2;
           const float e_lj = (cc.y * ir6 - cc.x) * ir6;
           const float f_lj = (cc.w * ir6 - cc.z) * ir6 * inv_r2;
@@ -123,31 +142,31 @@ __device__ inline void hmd_pair_steps(const PairParams& p, const float4* s_lj,
         // a user pair term composed in at run time: force / r and energy
         // from (types, r^2, 1/r); masked with the built-in terms below
         float f_u = 0.0f, e_u = 0.0f;
-        HMD_USER_FUNCTOR{}(p.user_params, st.ti, s_jt[k].y, r2s, inv_r, f_u, e_u);
+        HMD_USER_FUNCTOR{}(p.user_params, ti, tj, r2s, inv_r, f_u, e_u);
         factor += f_u;
         if (FEV) energy += e_u;
       }
 #endif
       factor = ok ? factor : 0.0f;
-      // the i-side sums use their own FMAs, so that the j-side values are
-      // dead once the lane swaps consume them (no copies)
-      st.fix = fmaf(factor, dx, st.fix);
-      st.fiy = fmaf(factor, dy, st.fiy);
-      st.fiz = fmaf(factor, dz, st.fiz);
-      jr[0][u] = factor * dx; jr[1][u] = factor * dy; jr[2][u] = factor * dz;
+      // the j reaction uses its own FMAs, so that the i-side values are dead
+      // once the lane swaps consume them (no copies)
+      fj[0] = fmaf(-factor, dx, fj[0]);
+      fj[1] = fmaf(-factor, dy, fj[1]);
+      fj[2] = fmaf(-factor, dz, fj[2]);
+      ir[0][u] = factor * dx; ir[1][u] = factor * dy; ir[2][u] = factor * dz;
       if (FEV) {
         const float he = ok ? 0.5f * energy : 0.0f;
         const float hw = 0.5f * factor * r2s;
-        st.eiv += he; st.wiv += hw;
-        jr[3][u] = he; jr[4][u] = hw;
+        fj[3] += he; fj[4] += hw;
+        ir[3 % NC][u] = he; ir[4 % NC][u] = hw;
       }
     }
-    // Wave sums of the 8 j-atoms' components (totals in lanes 8m + 7), then
-    // component c of j-atom kb + m moves to lane 8m + 7 - c, so that one
-    // conversion and one atomic per lane cover every component.
+    // Wave sums of the 8 i-atoms' components (totals in lanes 8m + 7), then
+    // component c of i-atom 8 sb + m moves to lane 8m + 7 - c, so that one
+    // LDS add per lane covers every component.
     float tot[NC];
 #pragma unroll
-    for (uint32_t c = 0; c < NC; ++c) tot[c] = hmd_wave_sum8(jr[c], lane);
+    for (uint32_t cc = 0; cc < NC; ++cc) tot[cc] = hmd_wave_sum8(ir[cc], lane);
     float sh[NC];
     sh[0] = tot[0];
     sh[1] = hmd_dpp<HMD_DPP_ROW_SHL(1)>(tot[1]);
@@ -159,69 +178,81 @@ __device__ inline void hmd_pair_steps(const PairParams& p, const float4* s_lj,
     // the moves run with all lanes enabled (an opaque use pins them here,
     // ahead of the branch below, where a disabled source lane would read 0)
 #pragma unroll
-    for (uint32_t c = 0; c < NC; ++c) asm volatile("" : "+v"(sh[c]));
+    for (uint32_t cc = 0; cc < NC; ++cc) asm volatile("" : "+v"(sh[cc]));
     float v = sh[0];
 #pragma unroll
-    for (uint32_t c = 1; c < NC; ++c) v = (comp == c) ? sh[c] : v;
-    const uint32_t jt = s_jt[kb + (lane >> 3)].x;
-    if (comp < NC && jt != HMD_SENTINEL) {
-      v = (comp < 3u) ? -v : v;                    // forces react; E, W halves add
-      hmd_atomic_add_fixed(&acc[comp * stride + jt], hmd_to_fixed(v, HMD_FIXED_SHIFT));
+    for (uint32_t cc = 1; cc < NC; ++cc) v = (comp == cc) ? sh[cc] : v;
+    if (comp < NC) {
+      float* a = &s_fi[comp * HMD_CLUSTER + 8u * sb + (lane >> 3)];
+      *a = *a + v;
     }
   }
 }
 
-// One entry: the 64 j-atoms are gathered once (one lane per j), shifted to
-// the image nearest the cluster centre when SHIFT, and stored in a per-wave
-// LDS table that the steps read at wave-uniform addresses (LDS broadcasts).
+// One entry: lane k gathers j-atom k (shifted to the image nearest the
+// cluster centre when SHIFT), the i-sub-clusters it reaches come from the
+// entry's mask, and the entry's exclusion rows (lane = i-atom, bits over the
+// j-slots) are transposed so that each j lane holds its bits over the
+// i-atoms.  The j reaction is flushed once per entry.
 template <bool FEV, bool LJ, bool COUL, bool SHIFT>
 __device__ inline void hmd_pair_entry(const PairParams& p, const float4* s_lj,
-                                      const float4* s_ct, float4* s_jp, uint2* s_jt, HmdPairState& st,
-                                      uint32_t slot, float cx, float cy, float cz,
-                                      uint32_t c_end, bool has_ex, uint32_t lane) {
+                                      const float4* s_ct, const float4* s_ip,
+                                      const uint32_t* s_it, float* s_fi, uint32_t slot,
+                                      uint32_t c, float cx, float cy, float cz, uint32_t lane) {
   const float4* ps = reinterpret_cast<const float4*>(p.posq_sorted);
   const HmdLattice L = p.L;
   const size_t ebase = (size_t)slot * HMD_ENTRY;
   const uint32_t my_j = p.entry_j[ebase + lane];
-  // masks exist only for the cluster's leading entries that hold a partner
-  const uint64_t my_ex = has_ex ? p.entry_excl[ebase + lane] : 0ull;
+  // bits 0-7: i-sub-clusters reached; bit 8: the entry holds an exclusion
+  const uint32_t emask = __builtin_amdgcn_readfirstlane(p.entry_mask[slot]);
+  const uint64_t ex = (emask & 0x100u) ? p.entry_excl[ebase + lane] : 0ull;
   const bool valid_j = my_j != HMD_SENTINEL;
   const uint32_t jj_l = valid_j ? my_j : 0u;
   float4 pj = ps[jj_l];
-  const uint32_t tj_l = p.type_sorted[jj_l];
+  const uint32_t tj = p.type_sorted[jj_l];
   if (SHIFT) {
     pj.x -= L.lx * floorf((pj.x - cx) * (1.0f / L.lx) + 0.5f);
     pj.y -= L.ly * floorf((pj.y - cy) * (1.0f / L.ly) + 0.5f);
     pj.z -= L.lz * floorf((pj.z - cz) * (1.0f / L.lz) + 0.5f);
     if (!valid_j) pj.x = pj.y = pj.z = 1.0e18f;   // fails every cutoff test
   }
-  const uint64_t lj_bits = LJ ? __ballot(valid_j && ((p.lj_type_mask >> tj_l) & 1u)) : 0ull;
-  const bool plain = SHIFT && __ballot(my_ex != 0ull || my_j < c_end) == 0ull;
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  s_jp[lane] = pj;
-  s_jt[lane] = make_uint2(my_j, tj_l);
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  __builtin_amdgcn_wave_barrier();
+  const bool any_ex = __ballot(ex != 0ull) != 0ull;
+  const bool plain = SHIFT && !any_ex && __ballot(my_j < (c + 1u) * HMD_CLUSTER) == 0ull;
+  const uint64_t exT = any_ex ? hmd_transpose64(ex, lane) : 0ull;
+  float fj[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   if (SHIFT && plain)
-    hmd_pair_steps<FEV, LJ, COUL, true, true>(p, s_lj, s_ct, s_jp, s_jt, st, my_j, my_ex,
-                                               lj_bits, lane);
+    hmd_pair_isteps<FEV, LJ, COUL, true, true>(p, s_lj, s_ct, s_ip, s_it, s_fi, pj, tj, my_j,
+                                                exT, emask & 0xFFu, c, fj, lane);
   else
-    hmd_pair_steps<FEV, LJ, COUL, SHIFT, false>(p, s_lj, s_ct, s_jp, s_jt, st, my_j, my_ex,
-                                                 lj_bits, lane);
+    hmd_pair_isteps<FEV, LJ, COUL, SHIFT, false>(p, s_lj, s_ct, s_ip, s_it, s_fi, pj, tj, my_j,
+                                                  exT, emask & 0xFFu, c, fj, lane);
+  if (valid_j) {
+    constexpr uint32_t NC = FEV ? 5u : 3u;
+    long long* acc = p.acc;
+#pragma unroll
+    for (uint32_t cc = 0; cc < NC; ++cc)
+      hmd_atomic_add_fixed(&acc[cc * p.acc_stride + my_j], hmd_to_fixed(fj[cc], HMD_FIXED_SHIFT));
+  }
 }
 
+// The wave's i accumulators (f32 sums over its share of the cluster's
+// entries) to fixed point, and cleared for the next cluster.
 template <bool FEV>
-__device__ inline void hmd_flush_i(const PairParams& p, const HmdPairState& st) {
-  if (!st.vi) return;
-  long long* acc = p.acc;
-  const uint32_t stride = p.acc_stride, si = st.si;
-  hmd_atomic_add_fixed(&acc[si], hmd_to_fixed(st.fix, HMD_FIXED_SHIFT));
-  hmd_atomic_add_fixed(&acc[stride + si], hmd_to_fixed(st.fiy, HMD_FIXED_SHIFT));
-  hmd_atomic_add_fixed(&acc[2 * stride + si], hmd_to_fixed(st.fiz, HMD_FIXED_SHIFT));
-  if (FEV) {
-    hmd_atomic_add_fixed(&acc[3 * stride + si], hmd_to_fixed(st.eiv, HMD_FIXED_SHIFT));
-    hmd_atomic_add_fixed(&acc[4 * stride + si], hmd_to_fixed(st.wiv, HMD_FIXED_SHIFT));
+__device__ inline void hmd_flush_i(const PairParams& p, float* s_fi, uint32_t si, bool vi,
+                                   uint32_t lane) {
+  constexpr uint32_t NC = FEV ? 5u : 3u;
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  float v[NC];
+#pragma unroll
+  for (uint32_t cc = 0; cc < NC; ++cc) {
+    v[cc] = s_fi[cc * HMD_CLUSTER + lane];
+    s_fi[cc * HMD_CLUSTER + lane] = 0.0f;
+  }
+  if (vi) {
+#pragma unroll
+    for (uint32_t cc = 0; cc < NC; ++cc)
+      hmd_atomic_add_fixed(&p.acc[cc * p.acc_stride + si], hmd_to_fixed(v[cc], HMD_FIXED_SHIFT));
   }
 }
 
@@ -231,8 +262,11 @@ __device__ inline void hmd_pair_body(const PairParams& p, const uint32_t hmd_bx,
   __shared__ float4 s_lj[HMD_MAX_TYPES * HMD_MAX_TYPES];   // (c6, c12, 6 c6, 12 c12)
   // erfc table: force coefficients, then (energy variants) energy coefficients
   __shared__ float4 s_ct[COUL ? (FEV ? 2 * HMD_COUL_BINS : HMD_COUL_BINS) : 1];
-  __shared__ float4 s_jp[4][HMD_ENTRY];
-  __shared__ uint2 s_jt[4][HMD_ENTRY];
+  // per wave: the current i-cluster (shifted positions with k_C q, types)
+  // and its f32 force (energy, virial) accumulators
+  __shared__ float4 s_ip[4][HMD_CLUSTER];
+  __shared__ uint32_t s_it[4][HMD_CLUSTER];
+  __shared__ float s_fi[4][(FEV ? 5 : 3) * HMD_CLUSTER];
   const uint32_t nt2 = p.n_types * p.n_types;
   for (uint32_t k = threadIdx.x; k < nt2; k += blockDim.x) {
     const float2 cc = reinterpret_cast<const float2*>(p.lj_c6c12)[k];
@@ -243,12 +277,16 @@ __device__ inline void hmd_pair_body(const PairParams& p, const uint32_t hmd_bx,
     for (uint32_t k = threadIdx.x; k < (FEV ? 2u : 1u) * HMD_COUL_BINS; k += blockDim.x)
       s_ct[k] = ct[k];
   }
+  const uint32_t wv = threadIdx.x / HMD_WAVE;
+  const uint32_t lane = hmd_lane();
+  float* fi = s_fi[wv];
+#pragma unroll
+  for (uint32_t cc = 0; cc < (FEV ? 5u : 3u); ++cc) fi[cc * HMD_CLUSTER + lane] = 0.0f;
   __syncthreads();
 
-  const uint32_t lane = hmd_lane();
   const uint32_t waves_per_block = blockDim.x / HMD_WAVE;
   const uint32_t n_waves = hmd_gx * waves_per_block;
-  const uint32_t gw = hmd_bx * waves_per_block + threadIdx.x / HMD_WAVE;
+  const uint32_t gw = hmd_bx * waves_per_block + wv;
   const uint32_t nw = *p.n_work;
   const uint32_t per = (nw + n_waves - 1) / n_waves;
   const uint32_t w0 = gw * per;
@@ -260,25 +298,21 @@ __device__ inline void hmd_pair_body(const PairParams& p, const uint32_t hmd_bx,
   const HmdLattice L = p.L;
   const bool ortho = (L.xy == 0.0f) && (L.xz == 0.0f) && (L.yz == 0.0f);
 
-  HmdPairState st{};
   uint32_t cur_c = HMD_SENTINEL;
-  uint32_t cur_nex = 0;
+  uint32_t si = 0;
+  bool vi = false;
   bool shift = false;
   float cx = 0, cy = 0, cz = 0;
   for (uint32_t w = w0; w < w1; ++w) {
     const uint32_t slot = p.work[w];
     const uint32_t c = slot / p.cap_entries;
     if (c != cur_c) {
-      if (cur_c != HMD_SENTINEL) hmd_flush_i<FEV>(p, st);
+      if (cur_c != HMD_SENTINEL) hmd_flush_i<FEV>(p, fi, si, vi, lane);
       cur_c = c;
-      cur_nex = p.excl_entries[c];
-      st.si = c * HMD_CLUSTER + lane;
-      st.vi = st.si < p.n;
-      const float4 a = ps[st.si];
-      st.xi = a.x; st.yi = a.y; st.zi = a.z;
-      st.kqi = p.k_coulomb * a.w;
-      st.ti = p.type_sorted[st.si];
-      st.ti_row = st.ti * p.n_types;
+      si = c * HMD_CLUSTER + lane;
+      vi = si < p.n;
+      float4 a = ps[si];
+      const uint32_t ti = p.type_sorted[si];
       const float4 cb = bb[2 * c], hb = bb[2 * c + 1];
       // Single-image fast path (compare jit_composed.rs:1396-1430): valid
       // when every atom that can interact with the cluster lies within half
@@ -290,22 +324,27 @@ __device__ inline void hmd_pair_body(const PairParams& p, const uint32_t hmd_bx,
               (hb.z + p.shift_margin < 0.5f * L.lz);
       cx = cb.x; cy = cb.y; cz = cb.z;
       if (shift) {
-        st.xi -= L.lx * floorf((st.xi - cx) * (1.0f / L.lx) + 0.5f);
-        st.yi -= L.ly * floorf((st.yi - cy) * (1.0f / L.ly) + 0.5f);
-        st.zi -= L.lz * floorf((st.zi - cz) * (1.0f / L.lz) + 0.5f);
+        a.x -= L.lx * floorf((a.x - cx) * (1.0f / L.lx) + 0.5f);
+        a.y -= L.ly * floorf((a.y - cy) * (1.0f / L.ly) + 0.5f);
+        a.z -= L.lz * floorf((a.z - cz) * (1.0f / L.lz) + 0.5f);
+        if (!vi) a.x = a.y = a.z = 1.0e18f;   // padding lanes fail every cutoff test
       }
-      st.fix = st.fiy = st.fiz = st.eiv = st.wiv = 0.0f;
+      a.w *= p.k_coulomb;
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      s_ip[wv][lane] = a;
+      s_it[wv][lane] = ti;
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
     }
-    const uint32_t wv = threadIdx.x / HMD_WAVE;
-    const bool has_ex = slot - c * p.cap_entries < cur_nex;
     if (shift)
-      hmd_pair_entry<FEV, LJ, COUL, true>(p, s_lj, s_ct, s_jp[wv], s_jt[wv], st, slot, cx, cy, cz,
-                                          (c + 1u) * HMD_CLUSTER, has_ex, lane);
+      hmd_pair_entry<FEV, LJ, COUL, true>(p, s_lj, s_ct, s_ip[wv], s_it[wv], fi, slot, c, cx, cy,
+                                          cz, lane);
     else
-      hmd_pair_entry<FEV, LJ, COUL, false>(p, s_lj, s_ct, s_jp[wv], s_jt[wv], st, slot, cx, cy, cz,
-                                           (c + 1u) * HMD_CLUSTER, has_ex, lane);
+      hmd_pair_entry<FEV, LJ, COUL, false>(p, s_lj, s_ct, s_ip[wv], s_it[wv], fi, slot, c, cx,
+                                           cy, cz, lane);
   }
-  hmd_flush_i<FEV>(p, st);
+  hmd_flush_i<FEV>(p, fi, si, vi, lane);
 }
 
 // Minimum waves per SIMD the register allocation must allow (1: no limit).
@@ -357,6 +396,39 @@ __device__ __forceinline__ void hmd_pair_finalize_body(const PairFinalizeParams&
   }
 }
 HMD_KERNEL(hmd_pair_finalize, PairFinalizeParams, 256)
+
+// A slot's per-atom results added into caller-owned outputs (+=): the pair
+// slots' fixed-point accumulators (cleared for the next evaluation as they
+// are read) or the reciprocal slot's f32 arrays.  Starting from zeroed
+// outputs, the pair slot then the reciprocal slot give the bytes the
+// finalize above writes (each sum is one f32 rounding of the same two
+// values).
+__device__ __forceinline__ void hmd_slot_add_body(const SlotAddParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const uint32_t a = hmd_bx * blockDim.x + threadIdx.x;
+  if (a >= p.n) return;
+  float v[5];
+  if (p.acc) {
+    const uint32_t s = p.slot_of[a];
+    const uint32_t st = p.acc_stride;
+    const uint32_t nc = p.fev ? 5u : 3u;
+    for (uint32_t d = 0; d < nc; ++d) {
+      v[d] = hmd_from_fixed(p.acc[d * st + s], HMD_FIXED_SHIFT);
+      p.acc_clear[d * st + s] = 0;
+    }
+  } else {
+    for (uint32_t d = 0; d < 3; ++d) v[d] = p.src_f[d * p.n + a];
+    if (p.fev) { v[3] = p.src_e[a]; v[4] = p.src_w[a]; }
+  }
+  p.out_x[a] = p.out_x[a] + v[0];
+  p.out_y[a] = p.out_y[a] + v[1];
+  p.out_z[a] = p.out_z[a] + v[2];
+  if (p.fev) {
+    if (p.out_e) p.out_e[a] = p.out_e[a] + v[3];
+    if (p.out_w) p.out_w[a] = p.out_w[a] + v[4];
+  }
+}
+HMD_KERNEL(hmd_slot_add, SlotAddParams, 256)
 
 // Scaled exclusions: one thread per listed pair with a non-zero scale, the
 // same per-pair arithmetic as the fused kernel (full minimum image, the

@@ -54,6 +54,13 @@ const char* hmd_version(void);
  * (src/gpu/device.rs:34-74, src/gpu/kernel_macros.rs:63-113): loads the
  * embedded gfx950 code object and resolves every kernel by name. */
 hmd_status hmd_ctx_create(int device_ordinal, hmd_ctx** out);
+/* A context that enqueues on the caller's HIP stream (a hipStream_t, passed
+ * as void*) instead of a stream of its own, so that a host's stream capture
+ * (the runner's graph capture, src/runner.rs:2402-2408) records the path's
+ * launches; the stream must outlive the context, which does not destroy
+ * it.  Force fields on such a context run SPME on that stream (no second
+ * stream unless hmd_forcefield_set_tuning asks for one). */
+hmd_status hmd_ctx_create_on_stream(int device_ordinal, void* stream, hmd_ctx** out);
 void hmd_ctx_destroy(hmd_ctx* ctx);
 hmd_status hmd_ctx_synchronize(hmd_ctx* ctx);
 hmd_status hmd_ctx_device_info(hmd_ctx* ctx, int* compute_units, char* name,
@@ -175,6 +182,37 @@ void hmd_forcefield_destroy(hmd_forcefield* ff);
 /* ForceField::run (src/forces/mod.rs:944-1432).  level 0 = forces,
  * 1 = forces + per-atom energies and virials (AggregateLevel). */
 hmd_status hmd_forcefield_compute(hmd_forcefield* ff, int32_t level);
+/* ---- slot level (Potential::compute with a SlotOutputView,
+ * src/forces/mod.rs:86-133, 206-212) ------------------------------------
+ * The same evaluation one slot at a time, each adding (+=) its per-atom
+ * results into caller-owned device arrays (canonical atom order), so that
+ * a host can keep slots of its own (bonded terms, a Morse term) beside
+ * these and sum them into one view.  Enqueue only, on the context's stream
+ * (capturable into a caller's graph, see hmd_ctx_create_on_stream): per
+ * evaluation call hmd_nbl_pre_step first (NeighborListState::pre_step,
+ * neighbor_list.rs:1265-1354: skin check, device-gated rebuild, sorted
+ * positions), then hmd_pair_force and hmd_spme_compute in either order, and
+ * at a batch boundary hmd_forcefield_check (the status read: an overflow is
+ * reported there, as hmd_forcefield_compute reports it).  Outputs the
+ * caller zeroes once per evaluation give the bytes hmd_forcefield_compute
+ * writes. */
+typedef struct {
+  float* force_x;   /* [n] each; += */
+  float* force_y;
+  float* force_z;
+  float* energy;    /* level 1: per-atom energy shares (may be NULL) */
+  float* virial;    /* level 1: per-atom virial shares (may be NULL) */
+} hmd_slot_output;
+hmd_status hmd_nbl_pre_step(hmd_forcefield* ff);
+/* The fast-class pair slots ("lennard-jones", "spme-real", with scaled
+ * exclusions): the fused kernel's fixed-point sums added into out. */
+hmd_status hmd_pair_force(hmd_forcefield* ff, int32_t level, const hmd_slot_output* out);
+/* The "spme-reciprocal" slot (spme.rs:858-914), with an attached network's
+ * correction when there is one, added into out. */
+hmd_status hmd_spme_compute(hmd_forcefield* ff, int32_t level, const hmd_slot_output* out);
+/* Synchronize the context's stream and read the neighbour status (the
+ * per-batch 4-byte read of pre_step, neighbor_list.rs:1308-1320). */
+hmd_status hmd_forcefield_check(hmd_forcefield* ff);
 /* Slot names in evaluation order, '\n' separated. */
 hmd_status hmd_forcefield_slots(hmd_forcefield* ff, char* buf, int32_t len);
 /* Reciprocal scalars of the last evaluation: out[0] = W_recip,
@@ -205,8 +243,9 @@ hmd_status hmd_nbl_count_pairs(hmd_forcefield* ff, double radius,
  * (neighbor_list.rs:415-445).  The library sizes the capacity itself (probe
  * at creation, 1.5x growth on overflow and past 0.8 fill); a smaller seed
  * exercises that growth.  lds_entries caps the entries the build stages in
- * LDS (0: as many as fit, 140); list positions past them are written to
- * the entry array directly, with the same result. */
+ * LDS (0: as many as fit beside its other arrays, about 150); list
+ * positions past them are written to the entry array directly, with the
+ * same result. */
 hmd_status hmd_nbl_set_capacity(hmd_forcefield* ff, uint32_t entries, uint32_t lds_entries);
 /* out[0] entries in the work list, [1] rebuild generation, [2] status bits,
  * [3] high-water entries per cluster, [4] capacity per cluster,
@@ -253,6 +292,27 @@ hmd_status hmd_integrator_step(hmd_integrator* it, uint32_t n_steps);
 /* Capture one step into a hipGraph (runner.rs:2321-2470) and replay it. */
 hmd_status hmd_integrator_capture(hmd_integrator* it);
 hmd_status hmd_integrator_replay(hmd_integrator* it, uint32_t n_steps);
+/* Switch the integrator to another force field on the same particles: the
+ * forces are re-evaluated with it at the current positions and the step
+ * graph is re-captured (the revert of a judged run; no counterpart in the
+ * reference, whose ForceField is fixed for a run). */
+hmd_status hmd_integrator_set_forcefield(hmd_integrator* it, hmd_forcefield* ff);
+/* The learned leg's guarded run: n_steps graph-replayed steps of an
+ * integrator whose force field carries a network (hmd_mlp_attach), in
+ * batches of judge_every steps; after each batch the fallback judge
+ * (hmd_mlp_judge) compares its forces with fine_ff's at the same positions,
+ * and the first relative RMS error above threshold switches the integrator
+ * to fine_ff (hmd_integrator_set_forcefield) for the rest of the run and
+ * after it.  judges[] receives up to max_judges judge values and *n_judges
+ * their count (may be NULL); *reverted_at the integrator's step count at the
+ * switch, or UINT64_MAX.  The north-star fallback (plain fine-mesh SPME every
+ * 50 steps, revert above 1e-3); the reference's nearest policy is its
+ * runner's fallback from graph replay to eager steps (src/runner.rs:
+ * 1650-1656). */
+hmd_status hmd_integrator_run_judged(hmd_integrator* it, hmd_forcefield* fine_ff,
+                                     uint32_t n_steps, uint32_t judge_every, double threshold,
+                                     double* judges, uint32_t max_judges, uint32_t* n_judges,
+                                     uint64_t* reverted_at);
 /* Steps taken so far. */
 uint64_t hmd_integrator_steps(hmd_integrator* it);
 /* Per-stage GPU time (HIP events) of n eager steps: writes up to

@@ -227,3 +227,96 @@ def test_training_beats_the_coarse_mesh(gpu_ctx):
     assert after < before
     for h in (net, coarse, integ, fine, part):
         h.close()
+
+
+def _fine_and_coarse(gpu_ctx, relax=40):
+    from heddlemd_amd import systems
+    sysw, ff_f, part, fine, integ = water_case(gpu_ctx, relax=relax, mesh=(32, 32, 32))
+    ff_c = systems.water_forcefield(grid=(16, 16, 16), exclusions=sysw["exclusions"])
+    coarse = api.ForceField(gpu_ctx, part, ff_c)
+    return sysw, ff_f, ff_c, part, fine, coarse, integ
+
+
+def test_judge_value_matches_the_oracle(gpu_ctx):
+    """hmd_mlp_judge = ||F_nn - F_fine|| / ||F_fine||: the same ratio from the
+    two force fields' forces downloaded one at a time (FP64 sums of the same
+    f32 forces, to 1e-9), and within 1% of the ratio the FP64 oracle gives
+    (pair + SPME on both meshes, the network in bf16 mode)."""
+    import oracle
+    from tests.gpu_helpers import oracle_pair
+    sysw, ff_f, ff_c, part, fine, coarse, integ = _fine_and_coarse(gpu_ctx)
+    net = make_net(gpu_ctx, scale=2e-3, seed=17)
+    try:
+        net.attach(coarse)
+        e = api.judge(coarse, fine)
+        fine.compute(0)
+        f_fine = part.forces().astype(np.float64)
+        coarse.compute(0)
+        f_nn = part.forces().astype(np.float64)
+        e_host = rel_rms(f_nn, f_fine)
+        posq = part.download()["posq"].astype(np.float64)
+        f_pair, _, _ = oracle_pair(sysw, ff_f, part.download()["posq"])
+        rec = {}
+        for m in ((32, 32, 32), (16, 16, 16)):
+            rec[m] = oracle.spme_recip(posq[:, :3], posq[:, 3], sysw["lattice"], m, 4,
+                                       ff_f["coulomb"]["alpha"])[1]
+        D = net.descriptors(coarse).astype(np.float64)
+        dF = N.forward(net.params.astype(np.float64), posq[:, :3], posq[:, 3], species(sysw),
+                       lattice_of(sysw), R_NN, 2e-3, D=D, bf16_mode=True)
+        f_fine_o = f_pair + rec[(32, 32, 32)]
+        f_nn_o = f_pair + rec[(16, 16, 16)] + dF
+        e_o = rel_rms(f_nn_o, f_fine_o)
+        print(f"judge {e:.6e}, from downloaded forces {e_host:.6e}, FP64 oracle {e_o:.6e}")
+        assert abs(e - e_host) <= 1e-9 * e_host
+        assert abs(e - e_o) <= 1e-2 * e_o
+    finally:
+        for h in (net, coarse, integ, fine, part):
+            h.close()
+
+
+def test_judged_run_reverts_and_then_steps_as_the_plain_mesh(gpu_ctx):
+    """A network with perturbed weights (set from the host) fails the first
+    judge (relative RMS force error above 1e-3 after 20 steps): the run
+    switches to the fine mesh at step 20, and its next 20 steps are byte for
+    byte those of a plain fine-mesh integrator started from the state at the
+    switch (separate particles and force field, same positions, velocities,
+    images and forces)."""
+    from heddlemd_amd import systems
+    sysw, ff_f, ff_c, part, fine, coarse, integ = _fine_and_coarse(gpu_ctx)
+    v = api.generate_velocities(sysw["mass"], U.kelvin(300.0), 3,
+                                n_constraints=3 * sysw["n_mol"])
+    part.upload(vel=v)
+    net = make_net(gpu_ctx, scale=2e-3, seed=23)
+    rng = np.random.default_rng(4)
+    net.params = net.params + rng.normal(scale=0.5, size=net.N_PARAMS).astype(np.float32)
+    net.attach(coarse)
+    it_nn = api.Integrator(gpu_ctx, part, coarse, dt=2.0 * U.FS, settle=sysw["settle"])
+    part_b = fine_b = it_b = None
+    try:
+        g_fine = fine.stats()["generation"]
+        judges, at = it_nn.run_judged(fine, 20, judge_every=20, threshold=1e-3)
+        print(f"judges {judges}, reverted at {at}")
+        assert len(judges) == 1 and judges[0] > 1e-3 and at == 20
+        # the judge's fine evaluation rebuilt the fine list at these positions
+        assert fine.stats()["generation"] > g_fine
+        s20 = part.download()
+        judges2, at2 = it_nn.run_judged(fine, 20, judge_every=20, threshold=1e-3)
+        assert judges2 == [] and at2 is None
+        got = part.download()
+        part_b = api.ParticleBuffers(gpu_ctx, s20["posq"], sysw["mass"], sysw["lattice"],
+                                     vel=s20["vel"], types=sysw["types"])
+        part_b.upload_images(s20["images"])
+        part_b.upload_forces(s20["force"])
+        fine_b = api.ForceField(gpu_ctx, part_b, ff_f)
+        # the same list capacity, hence the same pair-kernel grid and the same
+        # grouping of the f32 per-wave sums
+        fine_b.set_capacity(fine.stats()["cap_entries"])
+        it_b = api.Integrator(gpu_ctx, part_b, fine_b, dt=2.0 * U.FS, settle=sysw["settle"])
+        it_b.replay(20)
+        want = part_b.download()
+        for k in ("posq", "vel", "force", "images"):
+            assert np.array_equal(got[k].view(np.uint32), want[k].view(np.uint32)), k
+    finally:
+        for h in (it_b, fine_b, part_b, it_nn, net, coarse, integ, fine, part):
+            if h is not None:
+                h.close()
