@@ -153,33 +153,46 @@ def fp64_parity(sysw, ff, part, field):
             "against": "FP64 oracle (pair cell list + SPME 128^3) at the same f32 positions"}
 
 
-def cpu_baseline(sysw, ff, posq, budget_s=10.0):
-    """The FP64 oracle (the CPU restatement of the same force arithmetic) on
-    the host cores: full pair forces (cell list) + SPME 128^3 for the same
-    196,608-atom configuration, repeated while within budget_s."""
+def cpu_baseline(sysw, ff, state, steps_all=5, steps_one=2, repeats=5):
+    """The same MD step on the host cores (oracle/cpu_step.c: f32 cell +
+    Verlet list, LJ + tabulated real-space Ewald, SPME 128^3 order 4 in f32,
+    compensated velocity Verlet + SETTLE) from the GPU run's current state:
+    the median over `repeats` timings of a few steps, with every host thread
+    and with one."""
     import oracle
-    lj = ff["lj"]
-    off, par = ff["exclusions"]
-    ex = (off.astype(np.int32), par.astype(np.int32), np.zeros(par.size), np.zeros(par.size))
-    ps = oracle.PairSystem(posq[:, :3].astype(np.float64), posq[:, 3].astype(np.float64),
-                           sysw["types"].astype(np.int32), sysw["lattice"], lj=lj,
-                           coulomb=ff["coulomb"], exclusions=ex, erfc_mode=1)
-    evals = 0
-    t0 = time.perf_counter()
-    while True:
-        ps.forces(cells=True)
-        oracle.spme_recip(posq[:, :3], posq[:, 3], sysw["lattice"], ff["spme"]["grid"],
-                          ff["spme"]["order"], ff["coulomb"]["alpha"])
-        evals += 1
-        el = time.perf_counter() - t0
-        if el > budget_s:
-            break
-    steps_per_s = evals / el
     from heddlemd_amd import units as U
-    return {"value": round(U.ns_per_day(steps_per_s, DT_FS), 5), "unit": "ns/day",
-            "cores": oracle.num_threads(), "kind": "port",
-            "sample": f"{evals} force evaluation(s) (pair cell list + SPME 128^3, FP64) of "
-                      f"the 196,608-atom box in {el:.1f} s; integration excluded"}
+    cs = oracle.CpuStep(state["posq"], state["vel"], sysw["mass"], sysw["types"],
+                        sysw["lattice"], ff["lj"], ff["coulomb"], ff["r_skin"],
+                        ff["spme"]["grid"], ff["exclusions"], sysw["settle"], DT_FS * U.FS,
+                        compensated=True)
+    threads = oracle.CpuStep.max_threads()
+
+    def timed(k):
+        t = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            cs.step(k)
+            t.append((time.perf_counter() - t0) / k)
+        return float(np.median(t))
+
+    try:
+        cs.step(1)
+        s_all = timed(steps_all)
+        oracle.CpuStep.set_threads(1)
+        s_one = timed(steps_one)
+        oracle.CpuStep.set_threads(threads)
+        rebuilds = cs.rebuilds()
+    finally:
+        cs.close()
+    return {"value": round(U.ns_per_day(1.0 / s_all, DT_FS), 5), "unit": "ns/day",
+            "cores": threads, "kind": "port",
+            "one_core_ns_day": round(U.ns_per_day(1.0 / s_one, DT_FS), 5),
+            "ms_per_step": round(s_all * 1e3, 2), "ms_per_step_one_core": round(s_one * 1e3, 1),
+            "sample": f"the 196,608-atom box from the GPU run's state: median of {repeats} "
+                      f"timings of {steps_all} steps ({threads} threads) and of {steps_one} "
+                      f"steps (1 thread) of the same step in f32 on the host (cell + Verlet "
+                      f"list, {rebuilds} rebuilds, LJ + tabulated real-space Ewald, SPME "
+                      f"128^3, compensated VV + SETTLE)"}
 
 
 TRAFFIC_CALIB_BYTES = 256 << 20
@@ -297,7 +310,7 @@ def main():
         parity = fp64_parity(sysw, ff, part, field)
         log(f"FP64 parity at the bench positions: force rel RMS {parity['force_rel_rms']:.2e}, "
             f"PE rel {parity['pe_rel']:.2e}")
-        cpu = cpu_baseline(sysw, ff, part.download()["posq"])
+        cpu = cpu_baseline(sysw, ff, part.download())
 
     plain = {"ns_day": round(ns_day, 3), "ms_per_step": round(ms_per_step, 4),
              "rebuilds": int(gen1 - gen0), "mesh": 128}

@@ -337,8 +337,11 @@ class Integrator:
 
     def __init__(self, ctx, particles, forcefield, kind="velocity-verlet", dt=1.0,
                  compensated=False, temperature=0.0, gamma=0.0, seed=0, settle=None,
-                 fused=True, scalars=False):
+                 fused=True, scalars=False, thermostat=None, tau=0.0):
+        """thermostat: None or "csvr" (temperature = k_B T, tau, seed)."""
         d = IntegratorDesc()
+        d.thermostat = {None: 0, "csvr": 1}[thermostat]
+        d.tau = tau
         # HMD_INTEGRATOR_UNFUSED = 1, HMD_INTEGRATOR_SCALARS = 2
         d.flags = (0 if fused else 1) | (2 if scalars else 0)
         self._kind = kind.encode()
@@ -371,6 +374,12 @@ class Integrator:
 
     def steps(self):
         return int(lib().hmd_integrator_steps(self.h))
+
+    def thermostat_state(self):
+        """CSVR: (injected energy sum K' - K, draws so far)."""
+        out = np.zeros(2)
+        check(lib().hmd_integrator_thermostat_state(self.h, _ptr(out, _dp)))
+        return float(out[0]), int(out[1])
 
     def set_forcefield(self, forcefield):
         """Step with another force field on the same particles from here on

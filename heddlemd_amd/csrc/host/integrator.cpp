@@ -38,6 +38,14 @@ struct Integrator {
   uint32_t n_free = 0;
   uint64_t seed = 0;
   DevBuf<unsigned long long> step_counter;
+  // CSVR thermostat
+  bool csvr = false;
+  double csvr_c = 1.0, csvr_kt_half = 0.0;
+  uint32_t g_dof = 1;
+  static constexpr uint32_t kCsvrPartials = 128;
+  DevBuf<double> csvr_ke, csvr_partials, csvr_injection;
+  DevBuf<float> csvr_factor;
+  DevBuf<unsigned long long> csvr_counter;
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -99,9 +107,66 @@ struct Integrator {
       seed = d.seed;
       step_counter.alloc(1);
     }
+    if (d.thermostat == HMD_THERMOSTAT_CSVR) {
+      if (kind != Kind::VelocityVerlet)
+        throw Error(HMD_ERR_INVALID, "the CSVR thermostat goes with velocity Verlet");
+      if (!(d.temperature > 0.0) || !(d.tau > 0.0) || !std::isfinite(d.tau))
+        throw Error(HMD_ERR_INVALID, "CSVR needs a positive temperature and tau");
+      csvr = true;
+      seed = d.seed;
+      // thermal degrees of freedom 3N - n_constraints - 3, floored at 1
+      // (csvr.rs:96-106); SETTLE fixes three distances per group
+      const int64_t g = 3 * (int64_t)p.n - 3 * (int64_t)n_groups - 3;
+      g_dof = (uint32_t)std::max<int64_t>(1, g);
+      csvr_c = std::exp(-(double)dt / d.tau);
+      csvr_kt_half = 0.5 * d.temperature;
+      csvr_ke.alloc(1);
+      csvr_partials.alloc(kCsvrPartials);
+      csvr_injection.alloc(1);
+      csvr_factor.alloc(1);
+      csvr_counter.alloc(1);
+    } else if (d.thermostat != HMD_THERMOSTAT_NONE) {
+      throw Error(HMD_ERR_INVALID, "unknown thermostat");
+    }
     check(hipMemsetAsync(p.pos_lo.ptr, 0, p.pos_lo.count * sizeof(double), s), "memset");
     check(hipMemsetAsync(p.vel_lo.ptr, 0, p.vel_lo.count * sizeof(double), s), "memset");
     check(hipStreamSynchronize(s), "sync");
+  }
+
+  // The thermostat after a step (apply_post, csvr.rs:164-206): FP64 kinetic
+  // energy into device memory, the CSVR chain, the rescale; enqueue only.
+  void thermostat_post() {
+    if (!csvr) return;
+    ReduceParams r{};
+    r.v = p.vel.ptr;
+    r.m = p.mass.ptr;
+    r.n = p.n;
+    r.mode = 1;
+    r.partials = p.red_partials.ptr;
+    r.out = csvr_ke.ptr;
+    r.n_blocks = (uint32_t)p.red_partials.count;
+    launch(c, K_hmd_reduce_partial, r.n_blocks, 256, 0, r);
+    launch(c, K_hmd_reduce_final, 1, 256, 0, r);
+    CsvrParams q{};
+    q.ke = csvr_ke.ptr;
+    q.partials = csvr_partials.ptr;
+    q.n_partials = kCsvrPartials;
+    q.g_dof = g_dof;
+    q.counter = csvr_counter.ptr;
+    q.seed_lo = (uint32_t)(seed & 0xFFFFFFFFull);
+    q.seed_hi = (uint32_t)(seed >> 32);
+    q.c = csvr_c;
+    q.one_minus_c = 1.0 - csvr_c;
+    q.kt_half = csvr_kt_half;
+    q.factor = csvr_factor.ptr;
+    q.injection = csvr_injection.ptr;
+    q.v = p.vel.ptr;
+    q.vel_lo = compensated ? p.vel_lo.ptr : nullptr;
+    q.n = p.n;
+    launch(c, K_hmd_csvr_sample, kCsvrPartials, 256, 0, q);
+    launch(c, K_hmd_csvr_finish, 1, 256, 0, q);
+    launch(c, K_hmd_csvr_rescale, ceil_div(p.n, 256), 256, 0, q);
+    c.mark("thermostat");
   }
 
   ~Integrator() {
@@ -205,6 +270,7 @@ struct Integrator {
       ff->compute(level());
       launch(c, K_hmd_step_kick_last, nu, 256, 0, sp);
       c.mark("kick_settle");
+      thermostat_post();
       return;
     }
     if (kind == Kind::VelocityVerlet) {
@@ -217,6 +283,7 @@ struct Integrator {
       settle_velocities();                                      // AfterKick
       settle_virial();
       c.mark("kick_settle");
+      thermostat_post();
     } else {
       // B A O A (force) B, with SETTLE after each position and velocity
       // update (the reference's BAOAB plan has no constraint points).
@@ -914,6 +981,20 @@ hmd_status hmd_replicas_plan(hmd_replicas* h, uint32_t* n_batched, uint32_t* n_s
 hmd_status hmd_integrator_step(hmd_integrator* w, uint32_t n) { HMD_GUARD(w->it->step(n)); }
 hmd_status hmd_integrator_capture(hmd_integrator* w) { HMD_GUARD(w->it->capture()); }
 hmd_status hmd_integrator_replay(hmd_integrator* w, uint32_t n) { HMD_GUARD(w->it->replay(n)); }
+
+hmd_status hmd_integrator_thermostat_state(hmd_integrator* w, double out[2]) {
+  HMD_GUARD({
+    Integrator& I = *w->it;
+    out[0] = 0.0;
+    out[1] = 0.0;
+    if (I.csvr) {
+      I.csvr_injection.download(&out[0], 1, I.c.stream);
+      unsigned long long k = 0;
+      I.csvr_counter.download(&k, 1, I.c.stream);
+      out[1] = (double)k;
+    }
+  });
+}
 
 hmd_status hmd_integrator_set_forcefield(hmd_integrator* w, hmd_forcefield* f) {
   HMD_GUARD({

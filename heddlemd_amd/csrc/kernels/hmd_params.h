@@ -388,6 +388,25 @@ struct CounterParams {
   unsigned long long* counter;
 };
 
+// Stochastic velocity rescaling (CSVR; kernels/nose_hoover.cu:181-360,
+// src/integrator/csvr.rs:164-226).
+struct CsvrParams {
+  const double* ke;            // [1] FP64 kinetic energy of the velocities
+  double* partials;            // [n_partials] sums of xi^2
+  uint32_t n_partials;
+  uint32_t g_dof;              // thermal degrees of freedom 3N - n_constraints - 3
+  unsigned long long* counter; // [1] Philox draw counter, +1 per application
+  uint32_t seed_lo, seed_hi;
+  double c;                    // exp(-dt / tau)
+  double one_minus_c;
+  double kt_half;              // k_target / N_f = k_B T / 2
+  float* factor;               // [1] rescale factor
+  double* injection;           // [1] accumulated k_new - k_old
+  float* v;                    // [3][n]
+  double* vel_lo;              // compensated low parts (may be null)
+  uint32_t n;
+};
+
 struct ReduceParams {
   const float* x;              // [comps][n] or null
   const float* v;              // velocities [3][n] (KE mode)

@@ -358,13 +358,18 @@ __device__ inline void hmd_philox(uint32_t k0, uint32_t k1, uint32_t c[4]) {
 
 // Box-Muller cos branch on the first two words with (x + 1/2) 2^-32
 // uniforms, f64 internals (philox.cuh:136-150).
-__device__ inline float hmd_philox_gaussian(uint32_t sl, uint32_t sh, uint32_t cl,
-                                            uint32_t ch, uint32_t pid, uint32_t axis) {
+__device__ inline double hmd_philox_gaussian_f64(uint32_t sl, uint32_t sh, uint32_t cl,
+                                                 uint32_t ch, uint32_t pid, uint32_t axis) {
   uint32_t c[4] = {cl, ch, pid, axis};
   hmd_philox(sl, sh, c);
   const double u1 = ((double)c[0] + 0.5) * (1.0 / 4294967296.0);
   const double u2 = ((double)c[1] + 0.5) * (1.0 / 4294967296.0);
-  return (float)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+__device__ inline float hmd_philox_gaussian(uint32_t sl, uint32_t sh, uint32_t cl,
+                                            uint32_t ch, uint32_t pid, uint32_t axis) {
+  return (float)hmd_philox_gaussian_f64(sl, sh, cl, ch, pid, axis);
 }
 
 // A step: x += v dt/2 (one rounding, as the reference's contracted
@@ -588,6 +593,89 @@ __device__ __forceinline__ void hmd_reduce_final_body(const ReduceParams& p, con
   if (threadIdx.x == 0) p.out[0] = s[0];
 }
 HMD_KERNEL(hmd_reduce_final, ReduceParams, 256)
+
+// ------------------------------------------------------------------- CSVR --
+// Stochastic velocity rescaling (Bussi, Donadio & Parrinello, J. Chem. Phys.
+// 126, 014101 (2007)) as kernels/nose_hoover.cu:181-360 applies it after a
+// step: with the kinetic energy K, N_f = g_dof and the normals xi_i of
+// Philox (seed, draw counter, i, 0),
+//   K' = c K + (kT/2)(1 - c)(sum_{i=1}^{N_f-1} xi_i^2 + xi_0^2)
+//        + 2 xi_0 sqrt(c (1 - c) K kT/2),
+// v <- sqrt(K'/K) v, and the injected energy K' - K accumulates for the
+// conserved quantity.  All on the device (capturable); the draw counter
+// advances once per application.  Sums: a fixed chunk per block, a fixed
+// lane stride and LDS tree per block, a fixed tree over the blocks.
+__device__ inline double hmd_block_sum(double v, double* s) {
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (uint32_t st = blockDim.x / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+    __syncthreads();
+  }
+  return s[0];
+}
+
+__device__ __forceinline__ void hmd_csvr_sample_body(const CsvrParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  __shared__ double s[256];
+  const unsigned long long ctr = *p.counter;
+  const uint32_t cl = (uint32_t)(ctr & 0xFFFFFFFFull), ch = (uint32_t)(ctr >> 32);
+  const uint32_t m = p.g_dof > 0u ? p.g_dof - 1u : 0u;   // draws i = 1 .. g_dof - 1
+  const uint32_t chunk = (m + hmd_gx - 1u) / hmd_gx;
+  const uint32_t beg = hmd_bx * chunk, end = min(beg + chunk, m);
+  double acc = 0.0;
+  for (uint32_t j = beg + threadIdx.x; j < end; j += blockDim.x) {
+    const double xi = hmd_philox_gaussian_f64(p.seed_lo, p.seed_hi, cl, ch, j + 1u, 0u);
+    acc += xi * xi;
+  }
+  const double t = hmd_block_sum(acc, s);
+  if (threadIdx.x == 0) p.partials[hmd_bx] = t;
+}
+HMD_KERNEL(hmd_csvr_sample, CsvrParams, 256)
+
+__device__ __forceinline__ void hmd_csvr_finish_body(const CsvrParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  __shared__ double s[256];
+  double acc = 0.0;
+  for (uint32_t i = threadIdx.x; i < p.n_partials; i += blockDim.x) acc += p.partials[i];
+  const double sum = hmd_block_sum(acc, s);
+  if (threadIdx.x == 0) {
+    const unsigned long long ctr = *p.counter;
+    const uint32_t cl = (uint32_t)(ctr & 0xFFFFFFFFull), ch = (uint32_t)(ctr >> 32);
+    const double r = hmd_philox_gaussian_f64(p.seed_lo, p.seed_hi, cl, ch, 0u, 0u);
+    const double k = *p.ke;
+    const double cross = k > 0.0 ? 2.0 * r * sqrt(p.c * p.one_minus_c * k * p.kt_half) : 0.0;
+    double kn = p.c * k + p.kt_half * p.one_minus_c * (sum + r * r) + cross;
+    if (!isfinite(kn) || kn <= 0.0) kn = k;
+    const double f = (k > 0.0 && kn != k) ? sqrt(kn / k) : 1.0;
+    p.factor[0] = (float)f;
+    p.injection[0] += kn - k;
+    *p.counter = ctr + 1ull;
+  }
+}
+HMD_KERNEL(hmd_csvr_finish, CsvrParams, 256)
+
+// v <- factor v (f32, as rescale_velocities_device_factor); in compensated
+// mode the extended velocity hi + lo is scaled in f64 and split again.
+__device__ __forceinline__ void hmd_csvr_rescale_body(const CsvrParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  const uint32_t i = hmd_bx * blockDim.x + threadIdx.x;
+  if (i >= p.n) return;
+  const float f = p.factor[0];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const size_t k = (size_t)d * p.n + i;
+    if (p.vel_lo) {
+      const double ev = ((double)p.v[k] + p.vel_lo[k]) * (double)f;
+      const float vh = (float)ev;
+      p.vel_lo[k] = ev - (double)vh;
+      p.v[k] = vh;
+    } else {
+      p.v[k] = p.v[k] * f;
+    }
+  }
+}
+HMD_KERNEL(hmd_csvr_rescale, CsvrParams, 256)
 
 // ------------------------------------------------------- steepest descent --
 // x += h F, with |h F| capped at max_step per atom, then wrap (setup-time

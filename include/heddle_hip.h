@@ -278,9 +278,18 @@ typedef struct {
    * settle_virial_scatter does (kernels/settle.cu:498-510, called from
    * apply_after_kick, src/integrator/settle.rs:371-401). */
   uint32_t flags;
+  /* Thermostat applied after every step (velocity Verlet only):
+   * HMD_THERMOSTAT_CSVR is the stochastic velocity rescaling of
+   * src/integrator/csvr.rs:164-226 (kernels/nose_hoover.cu:181-360) with
+   * temperature = k_B T, tau = coupling time and seed = Philox key; N_f =
+   * 3N - 3 n_groups - 3.  The kinetic energy is the FP64 reduction. */
+  uint32_t thermostat;
+  double tau;
 } hmd_integrator_desc;
 #define HMD_INTEGRATOR_UNFUSED 1u
 #define HMD_INTEGRATOR_SCALARS 2u
+#define HMD_THERMOSTAT_NONE 0u
+#define HMD_THERMOSTAT_CSVR 1u
 
 hmd_status hmd_integrator_create(hmd_ctx* ctx, hmd_particles* particles,
                                  hmd_forcefield* ff,
@@ -313,6 +322,10 @@ hmd_status hmd_integrator_run_judged(hmd_integrator* it, hmd_forcefield* fine_ff
                                      uint32_t n_steps, uint32_t judge_every, double threshold,
                                      double* judges, uint32_t max_judges, uint32_t* n_judges,
                                      uint64_t* reverted_at);
+/* CSVR state (flush_pending_injection, csvr.rs:134-160): out[0] the energy
+ * the thermostat has injected, sum of K' - K (the conserved quantity is
+ * KE + PE - out[0]), out[1] the Philox draws so far. */
+hmd_status hmd_integrator_thermostat_state(hmd_integrator* it, double out[2]);
 /* Steps taken so far. */
 uint64_t hmd_integrator_steps(hmd_integrator* it);
 /* Per-stage GPU time (HIP events) of n eager steps: writes up to

@@ -80,6 +80,10 @@ def _declare(L):
     L.orc_philox4x32_10.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _up, _up]
     L.orc_philox_gaussian.restype = ctypes.c_double
     L.orc_philox_gaussian.argtypes = [ctypes.c_uint32] * 6
+    L.orc_csvr_knew.restype = ctypes.c_double
+    L.orc_csvr_knew.argtypes = [ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32,
+                                ctypes.c_uint64, ctypes.c_uint32, ctypes.c_double,
+                                ctypes.c_double]
     L.orc_kinetic_energy.restype = ctypes.c_double
     L.orc_kinetic_energy.argtypes = [ctypes.c_int, _dp, _dp]
     L.orc_num_threads.restype = ctypes.c_int
@@ -328,6 +332,12 @@ class VvState:
         return self._vlo.T.copy()
 
 
+def csvr_knew(k_old, seed, counter, g_dof, c, kt_half):
+    """CSVR's new kinetic energy (orc_csvr_knew)."""
+    return lib().orc_csvr_knew(float(k_old), seed & 0xFFFFFFFF, seed >> 32, counter, g_dof,
+                               float(c), float(kt_half))
+
+
 def kinetic_energy(v, m):
     v = _f64(v).reshape(-1, 3); m = _f64(m)
     return lib().orc_kinetic_energy(v.shape[0], _p(v, _dp), _p(m, _dp))
@@ -335,3 +345,88 @@ def kinetic_energy(v, m):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+_STEP_PATH = os.path.join(_HERE, "libcpustep.so")
+_step_lib = None
+
+
+def step_lib():
+    """The f32 CPU baseline of the MD step (cpu_step.c)."""
+    global _step_lib
+    if _step_lib is None:
+        if not os.path.exists(_STEP_PATH):
+            build()
+        L = ctypes.CDLL(_STEP_PATH)
+        L.cs_create.restype = ctypes.c_void_p
+        L.cs_create.argtypes = ([ctypes.c_int, _fp, _fp, _fp, _ip, ctypes.c_int, _fp, _fp, _fp,
+                                 ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_double,
+                                 ctypes.c_double, _ip, _ip, _ip, ctypes.c_int, _ip]
+                                + [ctypes.c_double] * 5 + [ctypes.c_float, ctypes.c_int])
+        L.cs_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.cs_rebuilds.restype = ctypes.c_int
+        L.cs_rebuilds.argtypes = [ctypes.c_void_p]
+        L.cs_forces.argtypes = [ctypes.c_void_p, _fp]
+        L.cs_destroy.argtypes = [ctypes.c_void_p]
+        L.cs_set_threads.argtypes = [ctypes.c_int]
+        L.cs_max_threads.restype = ctypes.c_int
+        _step_lib = L
+    return _step_lib
+
+
+class CpuStep:
+    """The same MD step on the host cores in f32 (cell + Verlet list, LJ +
+    tabulated real-space Ewald, SPME order 4, compensated or f32 velocity
+    Verlet, SETTLE): the CPU baseline bench.py times."""
+
+    def __init__(self, posq, vel, mass, types, lattice, lj, coulomb, r_skin, mesh,
+                 exclusions, settle, dt, compensated=True):
+        L = step_lib()
+        self.n = n = len(mass)
+        nt = int(round(np.sqrt(np.asarray(lj["sigma"]).size)))
+        sg = np.asarray(lj["sigma"], np.float64)
+        ep = np.asarray(lj["epsilon"], np.float64)
+        self._a = [np.ascontiguousarray(posq, np.float32).reshape(-1, 4),
+                   np.ascontiguousarray(vel, np.float32).reshape(-1, 3),
+                   np.ascontiguousarray(mass, np.float32),
+                   np.ascontiguousarray(types, np.int32),
+                   (4 * ep * sg ** 6).astype(np.float32), (4 * ep * sg ** 12).astype(np.float32),
+                   np.ascontiguousarray(np.asarray(lattice)[:3], np.float32),
+                   np.ascontiguousarray(exclusions[0], np.int32),
+                   np.ascontiguousarray(exclusions[1], np.int32),
+                   np.ascontiguousarray(mesh, np.int32),
+                   np.ascontiguousarray(settle["groups"], np.int32)]
+        a = self._a
+        geom = settle_geometry(settle["d_oh"], settle["d_hh"], settle["m_o"], settle["m_h"])
+        self.h = L.cs_create(n, _p(a[0], _fp), _p(a[1], _fp), _p(a[2], _fp), _p(a[3], _ip), nt,
+                             _p(a[4], _fp), _p(a[5], _fp), _p(a[6], _fp),
+                             float(np.asarray(lj["cutoff"]).flat[0]),
+                             float(np.asarray(lj["switch"]).flat[0]), float(r_skin),
+                             coulomb["alpha"], coulomb.get("k_coulomb", 1.0), _p(a[7], _ip),
+                             _p(a[8], _ip), _p(a[9], _ip), len(a[10]), _p(a[10], _ip),
+                             settle["m_o"], settle["m_h"], geom[0], geom[1], geom[2],
+                             float(dt), 1 if compensated else 0)
+
+    def step(self, n):
+        step_lib().cs_step(self.h, n)
+
+    def rebuilds(self):
+        return step_lib().cs_rebuilds(self.h)
+
+    def forces(self):
+        f = np.empty((self.n, 3), np.float32)
+        step_lib().cs_forces(self.h, _p(f, _fp))
+        return f
+
+    @staticmethod
+    def set_threads(k):
+        step_lib().cs_set_threads(int(k))
+
+    @staticmethod
+    def max_threads():
+        return step_lib().cs_max_threads()
+
+    def close(self):
+        if self.h:
+            step_lib().cs_destroy(self.h)
+            self.h = None

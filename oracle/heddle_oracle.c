@@ -1041,6 +1041,28 @@ double orc_philox_gaussian(uint32_t seed_lo, uint32_t seed_hi, uint32_t step_lo,
     return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
 
+/* CSVR (kernels/nose_hoover.cu:181-256, Bussi-Donadio-Parrinello 2007):
+ * the new kinetic energy from the old one and the Philox normals
+ * xi_i = philox_gaussian(seed, counter, i, 0), i = 0 .. g_dof - 1:
+ * K' = c K + kt_half (1 - c)(sum_{i>=1} xi_i^2 + xi_0^2)
+ *      + 2 xi_0 sqrt(c (1 - c) K kt_half), K' = K if not finite or <= 0.
+ * Returns K' (the rescale factor is sqrt(K'/K)). */
+double orc_csvr_knew(double k_old, uint32_t seed_lo, uint32_t seed_hi, uint64_t counter,
+                     uint32_t g_dof, double c, double kt_half)
+{
+    uint32_t cl = (uint32_t)(counter & 0xFFFFFFFFu), ch = (uint32_t)(counter >> 32);
+    double s = 0.0;
+    for (uint32_t i = 1; i < g_dof; ++i) {
+        double xi = orc_philox_gaussian(seed_lo, seed_hi, cl, ch, i, 0);
+        s += xi * xi;
+    }
+    double r = orc_philox_gaussian(seed_lo, seed_hi, cl, ch, 0, 0);
+    double cross = k_old > 0.0 ? 2.0 * r * sqrt(c * (1.0 - c) * k_old * kt_half) : 0.0;
+    double kn = c * k_old + kt_half * (1.0 - c) * (s + r * r) + cross;
+    if (!isfinite(kn) || kn <= 0.0) kn = k_old;
+    return kn;
+}
+
 /* Langevin BAOAB A and O steps in f32 (kernels/langevin.cu:13-80) with
  * the reference's contracted roundings: x + v dt/2 once (fmaf);
  * sigma = sqrt(fl(1 - a^2) kT / m) with 1 - a^2 rounded once; v = a v +

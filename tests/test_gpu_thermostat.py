@@ -1,0 +1,92 @@
+"""The CSVR thermostat (stochastic velocity rescaling, src/integrator/
+csvr.rs:164-226, kernels/nose_hoover.cu:181-360) on the GPU through the C
+ABI: one application against the oracle's K' (orc_csvr_knew, pinned by its
+limits and the stationarity of the canonical distribution), and a run that
+takes water from 200 K to the 300 K target and holds it there while the
+conserved quantity KE + PE - (injected energy) stays put."""
+import numpy as np
+import pytest
+
+import oracle
+from heddlemd_amd import api
+from heddlemd_amd import units as U
+from tests.gpu_helpers import water_case
+
+pytestmark = pytest.mark.gpu
+
+TAU = 100.0 * U.FS
+SEED = 0x1234567890AB
+
+
+def _csvr(ctx, case, scalars=False):
+    sysw, ff, part, field, _ = case
+    return api.Integrator(ctx, part, field, dt=2.0 * U.FS, settle=sysw["settle"],
+                          temperature=U.kelvin(300.0), seed=SEED, thermostat="csvr", tau=TAU,
+                          scalars=scalars)
+
+
+def test_csvr_step_matches_the_oracle(gpu_ctx):
+    a = water_case(gpu_ctx, relax=40)
+    b = water_case(gpu_ctx, relax=40)
+    it_a = _csvr(gpu_ctx, a)
+    try:
+        sysw = a[0]
+        v0 = api.generate_velocities(sysw["mass"], U.kelvin(250.0), 5,
+                                     n_constraints=3 * sysw["n_mol"])
+        a[2].upload(vel=v0)
+        b[2].upload(vel=v0)
+        it_a.step(1)
+        b[4].step(1)
+        va = a[2].download()["vel"]
+        vb = b[2].download()["vel"]
+        ke = b[2].kinetic_energy()          # the FP64 reduction CSVR read
+        g_dof = 3 * sysw["n"] - 3 * sysw["n_mol"] - 3
+        c = np.exp(-(2.0 * U.FS) / TAU)
+        kn = oracle.csvr_knew(ke, SEED, 0, g_dof, c, 0.5 * U.kelvin(300.0))
+        f_ref = np.float32(np.sqrt(kn / ke))
+        inj, draws = it_a.thermostat_state()
+        # the device factor: the oracle's f32 factor or one ulp from it (the
+        # sums run in another order), with v' = fl(f v) for every component
+        cands = [f_ref, np.nextafter(f_ref, np.float32(np.inf)), np.nextafter(f_ref, np.float32(0))]
+        hit = [k for k, cf in enumerate(cands) if np.array_equal(va, vb * cf)]
+        print(f"K {ke:.6e} -> {kn:.6e}; oracle factor {f_ref!r}, device factor "
+              f"{[cands[k] for k in hit]}; injection {inj:.6e}")
+        assert hit
+        assert inj == pytest.approx(kn - ke, rel=1e-10)
+        assert draws == 1
+    finally:
+        it_a.close()
+        for case in (a, b):
+            for h in case[2:][::-1]:
+                h.close()
+
+
+def test_csvr_reaches_and_holds_the_target(gpu_ctx):
+    case = water_case(gpu_ctx, relax=40)
+    sysw, ff, part, field, _ = case
+    it = _csvr(gpu_ctx, case, scalars=True)
+    try:
+        part.upload(vel=api.generate_velocities(sysw["mass"], U.kelvin(200.0), 6,
+                                                n_constraints=3 * sysw["n_mol"]))
+        g_dof = 3 * sysw["n"] - 3 * sysw["n_mol"] - 3
+        it.capture()
+        temps, cons = [], []
+        for _ in range(30):
+            it.replay(50)
+            ke = part.kinetic_energy()
+            pe, _ = part.energy_sums()
+            inj, _ = it.thermostat_state()
+            temps.append(2.0 * ke / g_dof / U.kelvin(1.0))
+            cons.append((ke, ke + pe - inj))
+        t = np.array(temps)
+        c = np.array(cons)
+        drift = np.abs(c[:, 1] - c[0, 1]).max()
+        print(f"T every 100 fs: {np.round(t, 1).tolist()}; conserved drift {drift:.3e} Ha "
+              f"({drift / c[:, 0].mean():.2e} of <KE>)")
+        assert t[0] < 290.0
+        assert abs(t[15:].mean() - 300.0) < 9.0
+        assert drift < 0.02 * c[:, 0].mean()
+    finally:
+        it.close()
+        for h in case[2:][::-1]:
+            h.close()

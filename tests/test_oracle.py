@@ -508,6 +508,26 @@ def test_settle_constraint_virials_equal_bond_form():
         assert oracle.settle_velocities_one(new, v, m_o, m_h, lat) == pytest.approx(v2)
 
 
+def test_csvr_limits_and_stationarity():
+    """CSVR's K' (Bussi, Donadio & Parrinello 2007): c = 1 keeps K; c = 0
+    gives kT/2 times the sum of g_dof squared Philox normals (summed here
+    from the Random123-pinned normals); and the canonical distribution of K
+    (Gamma(N_f/2, kT)) is stationary: K drawn from it maps to K' with the
+    same mean N_f kT/2 and variance N_f (kT)^2/2."""
+    kt_half = 0.5e-3
+    assert oracle.csvr_knew(0.37, 7, 3, 40, 1.0, kt_half) == pytest.approx(0.37, rel=1e-15)
+    xs = [oracle.philox_gaussian(7, 0, 3, 0, i, 0) for i in range(40)]
+    assert oracle.csvr_knew(0.37, 7, 3, 40, 0.0, kt_half) == pytest.approx(
+        kt_half * sum(x * x for x in xs), rel=1e-13)
+    g = 30
+    kt = 2.0 * kt_half
+    rng = np.random.default_rng(1)
+    k0 = rng.gamma(g / 2.0, kt, size=20000)
+    k1 = np.array([oracle.csvr_knew(k, 11, n, g, 0.8, kt_half) for n, k in enumerate(k0)])
+    assert k1.mean() == pytest.approx(g * kt / 2, rel=0.02)
+    assert k1.var() == pytest.approx(g * kt ** 2 / 2, rel=0.05)
+
+
 def test_kinetic_energy_closed_form():
     v = np.array([[1.0, 2.0, 3.0], [0.5, 0, 0]])
     m = np.array([2.0, 4.0])
