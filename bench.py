@@ -312,8 +312,14 @@ def main():
             f"PE rel {parity['pe_rel']:.2e}")
         cpu = cpu_baseline(sysw, ff, part.download())
 
+    st = field.stats()
     plain = {"ns_day": round(ns_day, 3), "ms_per_step": round(ms_per_step, 4),
-             "rebuilds": int(gen1 - gen0), "mesh": 128}
+             "rebuilds": int(gen1 - gen0), "mesh": 128,
+             "list_entries": st["n_work"],
+             "pair_steps": 8 * st["subcluster_visits"],
+             "pair_slot_efficiency": round(n_in_cut / (512.0 * max(st["subcluster_visits"], 1)), 4)}
+    log(f"list: {st['n_work']} entries, {8 * st['subcluster_visits']} pair steps of 64 pairs "
+        f"({plain['pair_slot_efficiency']:.3f} of the evaluated pairs inside r_c)")
     roofline = {"bound": "valu", "kernel": "hmd_pair_force_f",
                 "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),

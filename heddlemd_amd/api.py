@@ -323,7 +323,7 @@ class ForceField:
         out = np.zeros(8, np.uint32)
         check(lib().hmd_nbl_stats(self.h, _ptr(out, _u32p)))
         keys = ["n_work", "generation", "status", "high_water", "cap_entries",
-                "n_clusters", "fine_cells", "pad"]
+                "n_clusters", "fine_cells", "subcluster_visits"]
         return dict(zip(keys, (int(x) for x in out)))
 
     def close(self):

@@ -613,6 +613,17 @@ void ForceField::request_rebuild() {
   check(hipStreamSynchronize(c.stream), "sync");
 }
 
+// Status for a batch re-run from restored positions: any overflow and the
+// high-water mark of the abandoned batch are dropped (the capacity has
+// already grown) and the next evaluation rebuilds the list.
+void ForceField::restart_list() {
+  uint32_t st[4];
+  status.download(st, 4, c.stream);
+  uint32_t reset[4] = {HMD_STATUS_DISPLACED, st[1], 0u, 0u};
+  status.upload(reset, 4, c.stream);
+  check(hipStreamSynchronize(c.stream), "sync");
+}
+
 // ----------------------------------------------------------------- step ---
 void ForceField::rebuild_chain() {
   rebuild_sort();
@@ -1204,7 +1215,16 @@ hmd_status hmd_nbl_stats(hmd_forcefield* f, uint32_t out[8]) {
     out[4] = F.cap_entries;
     out[5] = F.n_clusters;
     out[6] = F.g[0];
-    out[7] = 0;
+    // i-sub-cluster visits of the pair kernel (8 steps of 64 pairs each):
+    // the listed entries' sub-cluster masks, summed
+    std::vector<uint32_t> cnt(F.n_clusters), mask((size_t)F.n_clusters * F.cap_entries);
+    F.entry_count.download(cnt.data(), cnt.size(), F.c.stream);
+    F.entry_mask.download(mask.data(), mask.size(), F.c.stream);
+    uint64_t visits = 0;
+    for (uint32_t c = 0; c < F.n_clusters; ++c)
+      for (uint32_t e = 0; e < cnt[c] && e < F.cap_entries; ++e)
+        visits += (uint64_t)__builtin_popcount(mask[(size_t)c * F.cap_entries + e] & 0xFFu);
+    out[7] = (uint32_t)std::min<uint64_t>(visits, 0xFFFFFFFFu);
   });
 }
 

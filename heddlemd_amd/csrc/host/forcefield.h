@@ -137,6 +137,7 @@ struct ForceField {
   uint32_t grown_capacity(uint32_t need) const;
   void probe_capacity();
   void request_rebuild();
+  void restart_list();
   void check_box();
   void compute_gv_table();
   void on_box_change();

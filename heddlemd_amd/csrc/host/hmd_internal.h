@@ -117,7 +117,13 @@ struct DevBuf {
     count = n;
     if (n == 0) return;
     check(hipMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T)), "hipMalloc");
-    if (zero) check(hipMemset(ptr, 0, n * sizeof(T)), "hipMemset");
+    if (zero) {
+      // hipMemset runs on the null stream, which does not order against the
+      // library's non-blocking streams: wait for it, so that a kernel on
+      // any stream sees the zeros.
+      check(hipMemset(ptr, 0, n * sizeof(T)), "hipMemset");
+      check(hipStreamSynchronize(nullptr), "sync");
+    }
   }
   void upload(const T* host, size_t n, hipStream_t s) {
     if (n == 0) return;

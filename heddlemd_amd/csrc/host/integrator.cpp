@@ -118,6 +118,7 @@ struct Integrator {
       // (csvr.rs:96-106); SETTLE fixes three distances per group
       const int64_t g = 3 * (int64_t)p.n - 3 * (int64_t)n_groups - 3;
       g_dof = (uint32_t)std::max<int64_t>(1, g);
+      // from the f32 step, widened (csvr.rs:190)
       csvr_c = std::exp(-(double)dt / d.tau);
       csvr_kt_half = 0.5 * d.temperature;
       csvr_ke.alloc(1);
@@ -559,6 +560,8 @@ struct ReplicaSet {
       check(hipMemcpy(batch_params.ptr, hp.data(), hp.size(), hipMemcpyHostToDevice), "H2D");
     if (!hs.empty())
       check(hipMemcpy(batch_starts.ptr, hs.data(), hs.size() * 4, hipMemcpyHostToDevice), "H2D");
+    // null-stream copies: landed before the replicas' non-blocking streams read them
+    check(hipStreamSynchronize(nullptr), "sync");
     planned.clear();
     for (const Item& it : plan) {
       if (it.blocks) {
@@ -639,6 +642,59 @@ struct ReplicaSet {
       }
     }
     if (code != HMD_OK) throw Error(code, err);
+  }
+};
+
+// Device copy of everything a batch of steps changes (positions, velocities,
+// images, forces, the compensated low parts, the random-stream and thermostat
+// counters), taken at the batch's start so that a batch that reported a list
+// overflow is re-run from there: the caller-side protocol of
+// HMD_ERR_NBL_OVERFLOW, kept inside entry points that run several batches.
+struct BatchStart {
+  DevBuf<float> posq, vel, force;
+  DevBuf<int32_t> images;
+  DevBuf<double> pos_lo, vel_lo, injection;
+  DevBuf<unsigned long long> step_counter, csvr_counter, draw_counter;
+  uint64_t steps = 0;
+
+  template <class T>
+  static void copy(DevBuf<T>& dst, const DevBuf<T>& src, hipStream_t s) {
+    if (!src.count) return;
+    if (dst.count != src.count) dst.alloc(src.count, false);
+    check(hipMemcpyAsync(dst.ptr, src.ptr, src.count * sizeof(T), hipMemcpyDeviceToDevice, s),
+          "D2D");
+  }
+  void save(Integrator& I) {
+    Particles& p = I.p;
+    hipStream_t s = I.c.stream;
+    copy(posq, p.posq, s);
+    copy(vel, p.vel, s);
+    copy(force, p.force, s);
+    copy(images, p.images, s);
+    copy(pos_lo, p.pos_lo, s);
+    copy(vel_lo, p.vel_lo, s);
+    copy(draw_counter, p.draw_counter, s);
+    copy(step_counter, I.step_counter, s);
+    copy(csvr_counter, I.csvr_counter, s);
+    copy(injection, I.csvr_injection, s);
+    steps = I.steps;
+  }
+  void restore(Integrator& I) {
+    Particles& p = I.p;
+    hipStream_t s = I.c.stream;
+    copy(p.posq, posq, s);
+    copy(p.vel, vel, s);
+    copy(p.force, force, s);
+    copy(p.images, images, s);
+    copy(p.pos_lo, pos_lo, s);
+    copy(p.vel_lo, vel_lo, s);
+    copy(p.draw_counter, draw_counter, s);
+    copy(I.step_counter, step_counter, s);
+    copy(I.csvr_counter, csvr_counter, s);
+    copy(I.csvr_injection, injection, s);
+    check(hipStreamSynchronize(s), "sync");
+    I.steps = steps;
+    I.ff->restart_list();
   }
 };
 
@@ -1020,14 +1076,29 @@ hmd_status hmd_integrator_run_judged(hmd_integrator* w, hmd_forcefield* fine, ui
     if (&fine->ff->p != &I.p) throw Error(HMD_ERR_INVALID, "fine force field is on other particles");
     *reverted_at = UINT64_MAX;
     uint32_t k = 0;
+    BatchStart start;
     for (uint32_t done = 0; done < n_steps;) {
       const uint32_t b = std::min(judge_every, n_steps - done);
-      I.replay(b);
+      // a block that overflowed the list is re-run from its start with the
+      // grown capacity (twice at most, as a batch runner would)
+      start.save(I);
+      for (int attempt = 0;; ++attempt) {
+        try {
+          I.replay(b);
+          break;
+        } catch (const Error& e) {
+          if (e.code != HMD_ERR_NBL_OVERFLOW || attempt == 2) throw;
+          start.restore(I);
+        }
+      }
       done += b;
       if (I.ff == fine->ff) continue;   // reverted: plain fine-mesh steps
       hmd_forcefield nn{I.ff};
       double e = 0.0;
-      const hmd_status st = hmd_mlp_judge(&nn, fine, &e);
+      hmd_status st = hmd_mlp_judge(&nn, fine, &e);
+      // the fine list overflowing at these positions: it has grown and been
+      // rebuilt here, so the judge is evaluated once more
+      if (st == HMD_ERR_NBL_OVERFLOW) st = hmd_mlp_judge(&nn, fine, &e);
       if (st != HMD_OK) throw Error(st, hmd_last_error());
       if (judges && k < max_judges) judges[k] = e;
       ++k;

@@ -95,6 +95,23 @@ __device__ inline float hmd_nn_rbf(float u15, float k) {
   return __builtin_amdgcn_exp2f(-0.5f * HMD_NN_LOG2E * t * t);
 }
 
+// phi_{k0 + e}(x), e = 0..7, for x = 15 s, by the Gaussian product
+// recurrence phi_{k+1} = phi_k exp(x - k - 1/2), whose ratio itself shrinks
+// by e^-1 per step: two exponentials instead of eight.  From k0 = 0 the
+// start value underflows for x > 13.2, where phi_0..7 are below 6e-9; from
+// k0 = 8 it never does.
+__device__ inline void hmd_nn_rbf8(float u15, float k0, float out[8]) {
+  const float t = u15 - k0;
+  float ph = __builtin_amdgcn_exp2f(-0.5f * HMD_NN_LOG2E * t * t);
+  float r = __builtin_amdgcn_exp2f(HMD_NN_LOG2E * (t - 0.5f));
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    out[e] = ph;
+    ph *= r;
+    r *= 0.36787944117144233f;
+  }
+}
+
 __device__ inline uint32_t hmd_nn_species(uint32_t mask, uint32_t type) {
   return (mask >> (type & 31u)) & 1u;
 }
@@ -197,9 +214,12 @@ hmd_nn_descriptors(const NnPassParams p) {
         const float s = __builtin_sqrtf(ok ? r2 : 0.0f) * p.inv_r_cut;
         const float fc = ok ? hmd_nn_cutoff(s) : 0.0f;
         const float u15 = 15.0f * s;
+        float ph[16];
+        hmd_nn_rbf8(u15, 0.0f, ph);
+        hmd_nn_rbf8(u15, 8.0f, ph + 8);
 #pragma unroll
         for (int kk = 0; kk < 16; ++kk) {
-          const float wk = hmd_nn_rbf(u15, (float)kk) * fc;
+          const float wk = ph[kk] * fc;
           d[kk] += qj * wk;
           jc[kk][u] = qi * wk;
         }
@@ -336,8 +356,7 @@ __device__ inline void hmd_nn_gather_pairs(const NnPassParams& p, const HmdNnTab
     oh[h] = W.sp[il] + W.sp[HMD_WAVE + jk];
     const float u15 = 15.0f * s;
     const float k0 = (float)(8u * (g & 1u));
-#pragma unroll
-    for (int e = 0; e < 8; ++e) phi[h][e] = hmd_nn_rbf(u15, k0 + (float)e);
+    hmd_nn_rbf8(u15, k0, phi[h]);
   }
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
@@ -517,8 +536,10 @@ __device__ inline void hmd_nn_forward_batch(const NnPassParams& p, const HmdNnSh
     const float u15 = 15.0f * sc;
     const float k0 = (float)(8u * hf);
     hmd_bf16x8 ph;
+    float phf[8];
+    hmd_nn_rbf8(u15, k0, phf);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) ph[e] = (__bf16)hmd_nn_rbf(u15, k0 + (float)e);
+    for (int e = 0; e < 8; ++e) ph[e] = (__bf16)phf[e];
     W.phi[pr][hf] = ph;
     if (hf == 0u) {
       const float fc = valid ? hmd_nn_cutoff(sc) : 0.0f;

@@ -72,8 +72,8 @@ __device__ inline void hmd_pair_isteps(const PairParams& p, const float4* s_lj,
                                        const float4* s_ct, const float4* s_ip,
                                        const uint32_t* s_it, float* s_fi, float4 pj,
                                        uint32_t tj, uint32_t my_j, uint64_t exT,
-                                       uint32_t emask, uint32_t c, float (&fj)[5],
-                                       uint32_t lane) {
+                                       uint32_t emask, uint32_t c, uint64_t lj_ibits,
+                                       float (&fj)[5], uint32_t lane) {
   const HmdLattice L = p.L;
   const float x_max = (float)HMD_COUL_BINS - 0.01f;   // last bin: beyond r_cut, masked
   const float lj_rs2_d = p.lj_rs2 * p.lj_inv_delta;
@@ -88,10 +88,15 @@ __device__ inline void hmd_pair_isteps(const PairParams& p, const float4* s_lj,
     pend &= pend - 1u;
     // ir[c][u]: this lane's contribution to component c of i-atom 8 sb + u
     float ir[NC][8];
+    // LJ flags of the sub-cluster's 8 i-atoms, in a scalar register
+    const uint32_t lj8 = __builtin_amdgcn_readfirstlane((uint32_t)(lj_ibits >> (8u * sb)) & 0xFFu);
+    // (x, y, z, k_C q) of the i-atoms, each read one step ahead
+    float4 a_next = s_ip[8u * sb];
 #pragma unroll
     for (uint32_t u = 0; u < 8; ++u) {
       const uint32_t i = 8u * sb + u;
-      const float4 a = s_ip[i];                   // (x, y, z, k_C q) of the i-atom
+      const float4 a = a_next;
+      if (u < 7) a_next = s_ip[i + 1];
       float dx = a.x - pj.x, dy = a.y - pj.y, dz = a.z - pj.z;
       if (!SHIFT) hmd_min_image(dx, dy, dz, L);
       const float r2 = dx * dx + dy * dy + dz * dz;
@@ -119,10 +124,10 @@ __device__ inline void hmd_pair_isteps(const PairParams& p, const float4* s_lj,
           energy = qq * (((e.w * t + e.z) * t + e.y) * t + e.x);
         }
       }
-      const uint32_t ti = __builtin_amdgcn_readfirstlane(s_it[i]);   // wave-uniform
       if (LJ) {
-        if ((p.lj_type_mask >> ti) & 1u) {                  // uniform branch
+        if ((lj8 >> u) & 1u) {                               // scalar branch
           // (c6, c12, 6 c6, 12 c12)
+          const uint32_t ti = __builtin_amdgcn_readfirstlane(s_it[i]);
           const float4 cc = s_lj[ti * p.n_types + tj];
           const float ir6 = inv_r2 * inv_r2 * inv_r2;
           const float e_lj = (cc.y * ir6 - cc.x) * ir6;
@@ -142,6 +147,7 @@ __device__ inline void hmd_pair_isteps(const PairParams& p, const float4* s_lj,
         // a user pair term composed in at run time: force / r and energy
         // from (types, r^2, 1/r); masked with the built-in terms below
         float f_u = 0.0f, e_u = 0.0f;
+        const uint32_t ti = __builtin_amdgcn_readfirstlane(s_it[i]);
         HMD_USER_FUNCTOR{}(p.user_params, ti, tj, r2s, inv_r, f_u, e_u);
         factor += f_u;
         if (FEV) energy += e_u;
@@ -182,10 +188,9 @@ __device__ inline void hmd_pair_isteps(const PairParams& p, const float4* s_lj,
     float v = sh[0];
 #pragma unroll
     for (uint32_t cc = 1; cc < NC; ++cc) v = (comp == cc) ? sh[cc] : v;
-    if (comp < NC) {
-      float* a = &s_fi[comp * HMD_CLUSTER + 8u * sb + (lane >> 3)];
-      *a = *a + v;
-    }
+    // LDS atomic add without return: no wait on the old value; one lane per
+    // address, in program order, so the sums are the same every run
+    if (comp < NC) atomicAdd(&s_fi[comp * HMD_CLUSTER + 8u * sb + (lane >> 3)], v);
   }
 }
 
@@ -198,7 +203,8 @@ template <bool FEV, bool LJ, bool COUL, bool SHIFT>
 __device__ inline void hmd_pair_entry(const PairParams& p, const float4* s_lj,
                                       const float4* s_ct, const float4* s_ip,
                                       const uint32_t* s_it, float* s_fi, uint32_t slot,
-                                      uint32_t c, float cx, float cy, float cz, uint32_t lane) {
+                                      uint32_t c, float cx, float cy, float cz,
+                                      uint64_t lj_ibits, uint32_t lane) {
   const float4* ps = reinterpret_cast<const float4*>(p.posq_sorted);
   const HmdLattice L = p.L;
   const size_t ebase = (size_t)slot * HMD_ENTRY;
@@ -222,10 +228,10 @@ __device__ inline void hmd_pair_entry(const PairParams& p, const float4* s_lj,
   float fj[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   if (SHIFT && plain)
     hmd_pair_isteps<FEV, LJ, COUL, true, true>(p, s_lj, s_ct, s_ip, s_it, s_fi, pj, tj, my_j,
-                                                exT, emask & 0xFFu, c, fj, lane);
+                                                exT, emask & 0xFFu, c, lj_ibits, fj, lane);
   else
     hmd_pair_isteps<FEV, LJ, COUL, SHIFT, false>(p, s_lj, s_ct, s_ip, s_it, s_fi, pj, tj, my_j,
-                                                  exT, emask & 0xFFu, c, fj, lane);
+                                                  exT, emask & 0xFFu, c, lj_ibits, fj, lane);
   if (valid_j) {
     constexpr uint32_t NC = FEV ? 5u : 3u;
     long long* acc = p.acc;
@@ -301,6 +307,7 @@ __device__ inline void hmd_pair_body(const PairParams& p, const uint32_t hmd_bx,
   uint32_t cur_c = HMD_SENTINEL;
   uint32_t si = 0;
   bool vi = false;
+  uint64_t lj_ibits = 0;   // i-atoms of the cluster whose type has an LJ partner
   bool shift = false;
   float cx = 0, cy = 0, cz = 0;
   for (uint32_t w = w0; w < w1; ++w) {
@@ -330,6 +337,12 @@ __device__ inline void hmd_pair_body(const PairParams& p, const uint32_t hmd_bx,
         if (!vi) a.x = a.y = a.z = 1.0e18f;   // padding lanes fail every cutoff test
       }
       a.w *= p.k_coulomb;
+      {
+        const uint64_t b = __ballot(vi && ((p.lj_type_mask >> ti) & 1u));
+        // held in scalar registers: the per-step LJ test is a scalar branch
+        lj_ibits = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(b >> 32)) << 32) |
+                   (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)b);
+      }
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       __builtin_amdgcn_wave_barrier();
       s_ip[wv][lane] = a;
@@ -339,10 +352,10 @@ __device__ inline void hmd_pair_body(const PairParams& p, const uint32_t hmd_bx,
     }
     if (shift)
       hmd_pair_entry<FEV, LJ, COUL, true>(p, s_lj, s_ct, s_ip[wv], s_it[wv], fi, slot, c, cx, cy,
-                                          cz, lane);
+                                          cz, lj_ibits, lane);
     else
       hmd_pair_entry<FEV, LJ, COUL, false>(p, s_lj, s_ct, s_ip[wv], s_it[wv], fi, slot, c, cx,
-                                           cy, cz, lane);
+                                           cy, cz, lj_ibits, lane);
   }
   hmd_flush_i<FEV>(p, fi, si, vi, lane);
 }

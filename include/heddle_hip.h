@@ -249,7 +249,8 @@ hmd_status hmd_nbl_count_pairs(hmd_forcefield* ff, double radius,
 hmd_status hmd_nbl_set_capacity(hmd_forcefield* ff, uint32_t entries, uint32_t lds_entries);
 /* out[0] entries in the work list, [1] rebuild generation, [2] status bits,
  * [3] high-water entries per cluster, [4] capacity per cluster,
- * [5] clusters, [6] fine cells per axis, [7] pad. */
+ * [5] clusters, [6] fine cells per axis, [7] i-sub-cluster visits of the
+ * pair kernel over the list (each 8 steps of 64 pairs). */
 hmd_status hmd_nbl_stats(hmd_forcefield* ff, uint32_t out[8]);
 
 /* ---- integrators ---------------------------------------------------------

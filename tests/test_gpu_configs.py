@@ -38,7 +38,18 @@ def _log(msg):
 
 
 def _check_against_oracle(tag, sysw, ff, part, field):
-    """Energy-level evaluation at the current positions vs the oracle."""
+    """Energy-level evaluation at the current positions vs the oracle.  The
+    list is complete within r_c on its own (the count of listed pairs within
+    r_c equals the oracle's); a rebuild at these positions then makes it the
+    exact r_c + r_skin set, compared pair for pair."""
+    field.compute(1)
+    posq0 = part.download()["posq"]
+    rc = np.float32(ff["coulomb"]["r_cut"])
+    n_rc = len(oracle.pair_set_f32(posq0, sysw["lattice"].astype(np.float32), rc * rc,
+                                   cells=True))
+    n_rc_gpu = field.count_pairs(float(ff["coulomb"]["r_cut"]))
+    assert n_rc_gpu == n_rc, (n_rc_gpu, n_rc)
+    field.request_rebuild()
     field.compute(1)
     d = part.download()
     posq = d["posq"]
@@ -71,8 +82,8 @@ def _check_against_oracle(tag, sysw, ff, part, field):
     assert err_wi <= FORCE_TOL and err_ws <= ENERGY_TOL
 
 
-def _case(ctx, grid, seed, mesh, **integ_kw):
-    sysw = systems.spc_water(grid=grid, seed=seed)
+def _case(ctx, grid, layout_seed, mesh, **integ_kw):
+    sysw = systems.spc_water(grid=grid, seed=layout_seed)
     ff = systems.water_forcefield(grid=mesh, exclusions=sysw["exclusions"])
     part = api.ParticleBuffers(ctx, sysw["posq"], sysw["mass"], sysw["lattice"],
                                types=sysw["types"])

@@ -275,32 +275,29 @@ def test_judge_value_matches_the_oracle(gpu_ctx):
 
 
 def test_judged_run_reverts_and_then_steps_as_the_plain_mesh(gpu_ctx):
-    """A network with perturbed weights (set from the host) fails the first
-    judge (relative RMS force error above 1e-3 after 20 steps): the run
-    switches to the fine mesh at step 20, and its next 20 steps are byte for
-    byte those of a plain fine-mesh integrator started from the state at the
-    switch (separate particles and force field, same positions, velocities,
-    images and forces)."""
-    from heddlemd_amd import systems
+    """A judge above the threshold (here an untrained network against a
+    threshold of 1e-9 on the relative RMS force error, which no coarse mesh
+    meets) switches the run to the fine mesh at step 20, and its next 20
+    steps are byte for byte those of a plain fine-mesh integrator started
+    from the state at the switch (separate particles and force field, same
+    positions, velocities, images and forces)."""
     sysw, ff_f, ff_c, part, fine, coarse, integ = _fine_and_coarse(gpu_ctx)
     v = api.generate_velocities(sysw["mass"], U.kelvin(300.0), 3,
                                 n_constraints=3 * sysw["n_mol"])
     part.upload(vel=v)
     net = make_net(gpu_ctx, scale=2e-3, seed=23)
-    rng = np.random.default_rng(4)
-    net.params = net.params + rng.normal(scale=0.5, size=net.N_PARAMS).astype(np.float32)
     net.attach(coarse)
     it_nn = api.Integrator(gpu_ctx, part, coarse, dt=2.0 * U.FS, settle=sysw["settle"])
     part_b = fine_b = it_b = None
     try:
         g_fine = fine.stats()["generation"]
-        judges, at = it_nn.run_judged(fine, 20, judge_every=20, threshold=1e-3)
+        judges, at = it_nn.run_judged(fine, 20, judge_every=20, threshold=1e-9)
         print(f"judges {judges}, reverted at {at}")
-        assert len(judges) == 1 and judges[0] > 1e-3 and at == 20
+        assert len(judges) == 1 and judges[0] > 1e-9 and at == 20
         # the judge's fine evaluation rebuilt the fine list at these positions
         assert fine.stats()["generation"] > g_fine
         s20 = part.download()
-        judges2, at2 = it_nn.run_judged(fine, 20, judge_every=20, threshold=1e-3)
+        judges2, at2 = it_nn.run_judged(fine, 20, judge_every=20, threshold=1e-9)
         assert judges2 == [] and at2 is None
         got = part.download()
         part_b = api.ParticleBuffers(gpu_ctx, s20["posq"], sysw["mass"], sysw["lattice"],
@@ -308,8 +305,6 @@ def test_judged_run_reverts_and_then_steps_as_the_plain_mesh(gpu_ctx):
         part_b.upload_images(s20["images"])
         part_b.upload_forces(s20["force"])
         fine_b = api.ForceField(gpu_ctx, part_b, ff_f)
-        # the same list capacity, hence the same pair-kernel grid and the same
-        # grouping of the f32 per-wave sums
         fine_b.set_capacity(fine.stats()["cap_entries"])
         it_b = api.Integrator(gpu_ctx, part_b, fine_b, dt=2.0 * U.FS, settle=sysw["settle"])
         it_b.replay(20)
@@ -320,3 +315,38 @@ def test_judged_run_reverts_and_then_steps_as_the_plain_mesh(gpu_ctx):
         for h in (it_b, fine_b, part_b, it_nn, net, coarse, integ, fine, part):
             if h is not None:
                 h.close()
+
+
+def test_judged_block_that_overflowed_is_re_run_from_its_start(gpu_ctx):
+    """The coarse list seeded with one entry per cluster overflows in the
+    first judged block: the run grows the capacity, puts back the state of
+    the block's start (positions, velocities, images, forces, counters) and
+    runs the block again.  Its end state and judge value equal, byte for
+    byte, those of a second system that had room from the start (same
+    start, same network weights, no overflow)."""
+    runs = []
+    for seed_capacity in (True, False):
+        sysw, ff_f, ff_c, part, fine, coarse, integ = _fine_and_coarse(gpu_ctx)
+        v = api.generate_velocities(sysw["mass"], U.kelvin(300.0), 3,
+                                    n_constraints=3 * sysw["n_mol"])
+        part.upload(vel=v)
+        net = make_net(gpu_ctx, scale=2e-3, seed=29)
+        net.attach(coarse)
+        it_nn = api.Integrator(gpu_ctx, part, coarse, dt=2.0 * U.FS, settle=sysw["settle"])
+        try:
+            if seed_capacity:
+                coarse.set_capacity(1)
+            judges, at = it_nn.run_judged(fine, 20, judge_every=20, threshold=1e9)
+            runs.append((part.download(), judges, coarse.stats()["cap_entries"],
+                         it_nn.steps()))
+            print(f"seeded capacity {seed_capacity}: judges {judges}, reverted at {at}, "
+                  f"capacity {runs[-1][2]}, steps {runs[-1][3]}")
+            assert at is None
+        finally:
+            for h in (it_nn, net, coarse, integ, fine, part):
+                h.close()
+    (a, ja, cap_a, steps_a), (b, jb, cap_b, steps_b) = runs
+    assert cap_a > 1 and steps_a == steps_b == 20
+    assert ja == jb
+    for k in ("posq", "vel", "force", "images"):
+        assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k

@@ -2,7 +2,8 @@
 hmd_spme_compute, hmd_forcefield_check) through the C ABI: the reference's
 Potential::compute writes each slot's per-atom results with += into a
 caller-owned SlotOutputView (src/forces/mod.rs:86-133, 206-212).  Device
-arrays here come from torch (plumbing only).  Checks: the two slots composed
+arrays and the caller's stream and graph come from hipMalloc,
+hipStreamCreate and stream capture on the HIP runtime the library links.  Checks: the two slots composed
 into zeroed external arrays equal hmd_forcefield_compute byte for byte
 (forces, energies, virials); += keeps what the caller had there; and on a
 context created on the caller's stream (hmd_ctx_create_on_stream) the slots
@@ -13,14 +14,16 @@ import pytest
 
 from heddlemd_amd import api
 from heddlemd_amd._lib import SlotOutput
-from tests.gpu_helpers import water_case
+from tests.gpu_helpers import HipRuntime, water_case
 
 pytestmark = pytest.mark.gpu
 
 
-def _outputs(torch, n):
-    t = torch.zeros((5, n), dtype=torch.float32, device="cuda")
-    return t, SlotOutput(*(t[k].data_ptr() for k in range(5)))
+def _outputs(hip, n):
+    """Five zeroed device arrays of n floats (force x, y, z, energy, virial)
+    in one allocation, and the hmd_slot_output naming them."""
+    buf = hip.zeros(5 * 4 * n)
+    return buf, SlotOutput(*(buf + 4 * n * k for k in range(5)))
 
 
 def _bits(a):
@@ -35,69 +38,77 @@ def _want(part, field):
 
 
 def test_slots_compose_to_the_force_field(gpu_ctx):
-    import torch
+    hip = HipRuntime()
     sysw, ff, part, field, integ = water_case(gpu_ctx, relax=40)
     try:
+        n = part.n
         want = _want(part, field)
-        t, out = _outputs(torch, part.n)
-        torch.cuda.synchronize()
+        buf, out = _outputs(hip, n)
         field.pre_step()
         field.pair_force(1, out)
         field.spme_compute(1, out)
         field.check()
-        got = t.cpu().numpy()
+        got = hip.download(buf, (5, n))
         assert np.array_equal(_bits(got), _bits(want))
         # += : the pair slot twice into the same arrays doubles it exactly
-        t.zero_()
-        torch.cuda.synchronize()
+        zero = np.zeros((5, n), np.float32)
+        hip.upload(buf, zero)
         field.pre_step()
         field.pair_force(0, out)
         field.pre_step()
         field.pair_force(0, out)
         field.check()
-        pair2 = t[:3].cpu().numpy()
-        t.zero_()
-        torch.cuda.synchronize()
+        pair2 = hip.download(buf, (5, n))[:3]
+        hip.upload(buf, zero)
         field.pre_step()
         field.pair_force(0, out)
         field.check()
-        pair1 = t[:3].cpu().numpy()
+        pair1 = hip.download(buf, (5, n))[:3]
         assert np.array_equal(_bits(pair2), _bits(2.0 * pair1))
     finally:
         for h in (integ, field, part):
             h.close()
+        hip.free_all()
 
 
 def test_slots_on_the_callers_stream_replay_in_its_graph():
-    import torch
-    stream = torch.cuda.Stream()
-    ctx = api.GpuContext(0, stream=stream.cuda_stream)
+    hip = HipRuntime()
+    stream = hip.stream_create()
+    ctx = api.GpuContext(0, stream=stream)
     sysw, ff, part, field, integ = water_case(ctx, relax=40)
+    graph = None
     try:
+        n = part.n
         want = _want(part, field)
-        t, out = _outputs(torch, part.n)
-        torch.cuda.synchronize()
+        buf, out = _outputs(hip, n)
+        zeros = hip.zeros(5 * 4 * n)
         # eager on the caller's stream first (the slots' one-time set-up)
         field.pre_step()
         field.pair_force(1, out)
         field.spme_compute(1, out)
         field.check()
-        assert np.array_equal(_bits(t.cpu().numpy()), _bits(want))
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g, stream=stream):
-            t.zero_()
-            field.pre_step()
-            field.pair_force(1, out)
-            field.spme_compute(1, out)
+        assert np.array_equal(_bits(hip.download(buf, (5, n))), _bits(want))
+        # the caller's graph: its own clear of the arrays (a copy node), then
+        # the slots
+        hip.begin_capture(stream)
+        hip.copy_async(buf, zeros, 5 * 4 * n, stream)
+        field.pre_step()
+        field.pair_force(1, out)
+        field.spme_compute(1, out)
+        graph = hip.end_capture(stream)
+        sevens = np.full((5, n), 7.0, np.float32)
         for rep in range(3):
-            t.fill_(7.0)                     # eager work between the replays
-            torch.cuda.synchronize()
-            g.replay()
-            torch.cuda.synchronize()
+            hip.upload(buf, sevens)          # eager work between the replays
+            hip.launch(graph[1], stream)
+            hip.stream_sync(stream)
             field.check()
-            assert np.array_equal(_bits(t.cpu().numpy()), _bits(want)), f"replay {rep}"
+            assert np.array_equal(_bits(hip.download(buf, (5, n))), _bits(want)), f"replay {rep}"
             field.compute(1)                 # the library's own eager evaluation
-        del g
     finally:
+        if graph is not None:
+            hip.graph_destroy(*graph)
         for h in (integ, field, part, ctx):
             h.close()
+        hip.stream_sync(stream)
+        hip.stream_destroy(stream)
+        hip.free_all()

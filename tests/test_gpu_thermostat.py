@@ -2,8 +2,8 @@
 csvr.rs:164-226, kernels/nose_hoover.cu:181-360) on the GPU through the C
 ABI: one application against the oracle's K' (orc_csvr_knew, pinned by its
 limits and the stationarity of the canonical distribution), and a run that
-takes water from 200 K to the 300 K target and holds it there while the
-conserved quantity KE + PE - (injected energy) stays put."""
+takes equilibrated water from 200 K to the 300 K target and holds it there
+while the conserved quantity KE + PE - (injected energy) stays put."""
 import numpy as np
 import pytest
 
@@ -41,7 +41,8 @@ def test_csvr_step_matches_the_oracle(gpu_ctx):
         vb = b[2].download()["vel"]
         ke = b[2].kinetic_energy()          # the FP64 reduction CSVR read
         g_dof = 3 * sysw["n"] - 3 * sysw["n_mol"] - 3
-        c = np.exp(-(2.0 * U.FS) / TAU)
+        # c from the f32 step widened to f64, as csvr.rs:190 forms it
+        c = np.exp(-float(np.float32(2.0 * U.FS)) / TAU)
         kn = oracle.csvr_knew(ke, SEED, 0, g_dof, c, 0.5 * U.kelvin(300.0))
         f_ref = np.float32(np.sqrt(kn / ke))
         inj, draws = it_a.thermostat_state()
@@ -62,14 +63,25 @@ def test_csvr_step_matches_the_oracle(gpu_ctx):
 
 
 def test_csvr_reaches_and_holds_the_target(gpu_ctx):
+    """A lightly relaxed lattice releases potential energy as it starts to
+    move, so the thermostat first equilibrates the structure at 300 K
+    (1,000 steps); the velocities are then redrawn at 200 K.  From there the
+    run must climb to the target (the first 100 fs sample below 290 K, the
+    mean of the last 1.5 ps within 9 K of 300 K: about 7 standard errors of
+    a mean over 15 samples with N_f ~ 7,800) while the conserved quantity
+    KE + PE - (injected energy) drifts by less than 2% of the mean KE."""
     case = water_case(gpu_ctx, relax=40)
     sysw, ff, part, field, _ = case
     it = _csvr(gpu_ctx, case, scalars=True)
     try:
-        part.upload(vel=api.generate_velocities(sysw["mass"], U.kelvin(200.0), 6,
-                                                n_constraints=3 * sysw["n_mol"]))
         g_dof = 3 * sysw["n"] - 3 * sysw["n_mol"] - 3
+        part.upload(vel=api.generate_velocities(sysw["mass"], U.kelvin(300.0), 6,
+                                                n_constraints=3 * sysw["n_mol"]))
         it.capture()
+        it.replay(1000)
+        t_eq = 2.0 * part.kinetic_energy() / g_dof / U.kelvin(1.0)
+        part.upload(vel=api.generate_velocities(sysw["mass"], U.kelvin(200.0), 8,
+                                                n_constraints=3 * sysw["n_mol"]))
         temps, cons = [], []
         for _ in range(30):
             it.replay(50)
@@ -81,7 +93,8 @@ def test_csvr_reaches_and_holds_the_target(gpu_ctx):
         t = np.array(temps)
         c = np.array(cons)
         drift = np.abs(c[:, 1] - c[0, 1]).max()
-        print(f"T every 100 fs: {np.round(t, 1).tolist()}; conserved drift {drift:.3e} Ha "
+        print(f"T after equilibration {t_eq:.1f} K; from 200 K, every 100 fs: "
+              f"{np.round(t, 1).tolist()}; conserved drift {drift:.3e} Ha "
               f"({drift / c[:, 0].mean():.2e} of <KE>)")
         assert t[0] < 290.0
         assert abs(t[15:].mean() - 300.0) < 9.0
