@@ -48,7 +48,8 @@ EXPORTED = [
     "hmd_settle_after_drift", "hmd_settle_after_kick",
     "hmd_forcefield_create", "hmd_forcefield_destroy", "hmd_forcefield_compute",
     "hmd_forcefield_slots", "hmd_spme_scalars", "hmd_spme_grids", "hmd_spme_forces",
-    "hmd_nbl_request_rebuild", "hmd_nbl_pair_snapshot", "hmd_nbl_count_pairs",
+    "hmd_nbl_request_rebuild", "hmd_nbl_pair_snapshot", "hmd_nbl_pair_snapshot_noalloc",
+    "hmd_nbl_count_pairs",
     "hmd_nbl_stats", "hmd_nbl_set_capacity",
     "hmd_nbl_pre_step", "hmd_pair_force", "hmd_spme_compute", "hmd_forcefield_check",
     "hmd_integrator_create", "hmd_integrator_destroy", "hmd_integrator_step",
@@ -210,6 +211,7 @@ def _declare(L):
         "hmd_spme_forces": (st, [vp, _c_fp]),
         "hmd_nbl_request_rebuild": (st, [vp]),
         "hmd_nbl_pair_snapshot": (st, [vp, _c_u32p, ctypes.c_uint64, _c_u64p]),
+        "hmd_nbl_pair_snapshot_noalloc": (st, [vp, _c_u32p, ctypes.c_uint64, _c_u64p]),
         "hmd_nbl_count_pairs": (st, [vp, ctypes.c_double, _c_u64p]),
         "hmd_nbl_stats": (st, [vp, _c_u32p]),
         "hmd_nbl_set_capacity": (st, [vp, ctypes.c_uint32, ctypes.c_uint32]),

@@ -232,6 +232,16 @@ hmd_status hmd_nbl_request_rebuild(hmd_forcefield* ff);
  * (unsorted).  *n_pairs receives the total even when it exceeds cap. */
 hmd_status hmd_nbl_pair_snapshot(hmd_forcefield* ff, uint32_t* pairs,
                                  uint64_t cap, uint64_t* n_pairs);
+/* hmd_nbl_pair_snapshot without any device allocation: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code near
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The pairs and their counter are staged in a fixed device array of the
+ * library (2,097,152 pairs, 16 MiB + 256 B per GPU, held until the process
+ * ends); cap above 2,097,152 is refused with HMD_ERR_INVALID before any
+ * device work, and *n_pairs still receives the full total.  Calls on one
+ * GPU share that array: calls that overlap on one GPU race. */
+hmd_status hmd_nbl_pair_snapshot_noalloc(hmd_forcefield* ff, uint32_t* pairs,
+                                         uint64_t cap, uint64_t* n_pairs);
 /* Number of listed pairs whose exact f32 min-image distance is <= radius
  * (radius <= r_c + r_skin); with radius = r_c this is the number of pairs
  * the pair kernel evaluates inside the cutoff. */
