@@ -72,22 +72,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def overflow_safe(run, part, field):
-    """run(n) that re-runs a batch which reported a neighbour-list overflow
-    (HMD_ERR_NBL_OVERFLOW: the library has already grown the capacity and
-    the batch is to be re-run, as the C ABI specifies).  The steps of such a
-    batch after the overflow ran on an incomplete list, so the particle state
-    from before the batch (positions, velocities, images, forces) is put back
-    and the list rebuilt before the re-run.  The state is saved before each
-    batch, or once by prepare() ahead of a timed batch run with
+def overflow_safe(integ, eager=False):
+    """run(n) of `integ` (eager steps or graph replays) that re-runs a batch
+    which reported a neighbour-list overflow (HMD_ERR_NBL_OVERFLOW: the
+    library has already grown the capacity and the batch is to be re-run, as
+    the C ABI specifies).  The steps of such a batch after the overflow ran
+    on an incomplete list, so the integrator rolls back to the checkpoint
+    taken at the batch's start (positions, velocities, images, forces, the
+    compensated low parts, counters and step count; hmd_integrator_rollback)
+    and the list is rebuilt before the re-run.  The checkpoint is taken
+    before each batch, or once by prepare() ahead of a timed batch run with
     snapshot=False.  Returns the number of re-runs through `run.reruns`."""
     from heddlemd_amd._lib import HMD_ERR_NBL_OVERFLOW, HmdError
+    run = integ.step if eager else integ.replay
 
     def prepare():
-        wrapped.snap = part.download()
+        integ.checkpoint()
+        wrapped.ready = True
 
     def wrapped(n, snapshot=True):
-        if snapshot or wrapped.snap is None:
+        if snapshot or not wrapped.ready:
             prepare()
         for attempt in range(3):
             try:
@@ -97,13 +101,9 @@ def overflow_safe(run, part, field):
                     raise
                 wrapped.reruns += 1
                 log(f"neighbour list overflow, batch re-run from its start: {e}")
-                st = wrapped.snap
-                part.upload(posq=st["posq"], vel=st["vel"])
-                part.upload_images(st["images"])
-                part.upload_forces(st["force"])
-                field.request_rebuild()
+                integ.rollback()
     wrapped.reruns = 0
-    wrapped.snap = None
+    wrapped.ready = False
     wrapped.prepare = prepare
     return wrapped
 
@@ -275,7 +275,7 @@ def main():
     if args.autotune:
         tuned = field.autotune(20)
         log(f"autotune: {tuned}")
-    run = overflow_safe(integ.step if args.eager else integ.replay, part, field)
+    run = overflow_safe(integ, args.eager)
     if not args.eager:
         integ.capture()
     run(args.warmup)
@@ -478,7 +478,7 @@ def nn_leg(api, systems, U, args, ctx, sysw, part, fine, integ_fine, run_fine):
     net.attach(coarse)
     integ = api.Integrator(ctx, part, coarse, dt=DT_FS * U.FS, compensated=True,
                            settle=sysw["settle"])
-    run = overflow_safe(integ.step if args.eager else integ.replay, part, coarse)
+    run = overflow_safe(integ, args.eager)
     if not args.eager:
         integ.capture()
     run(args.warmup)

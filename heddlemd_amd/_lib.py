@@ -56,7 +56,7 @@ EXPORTED = [
     "hmd_integrator_capture", "hmd_integrator_replay", "hmd_integrator_steps",
     "hmd_integrator_profile", "hmd_integrator_minimize",
     "hmd_integrator_set_forcefield", "hmd_integrator_run_judged",
-    "hmd_integrator_thermostat_state",
+    "hmd_integrator_thermostat_state", "hmd_integrator_checkpoint", "hmd_integrator_rollback",
     "hmd_generate_velocities", "hmd_chacha_block",
     "hmd_mlp_param_count", "hmd_mlp_create", "hmd_mlp_destroy", "hmd_mlp_set_params",
     "hmd_mlp_get_params", "hmd_mlp_set_force_scale", "hmd_mlp_descriptors",
@@ -228,6 +228,8 @@ def _declare(L):
         "hmd_integrator_steps": (ctypes.c_uint64, [vp]),
         "hmd_integrator_set_forcefield": (st, [vp, vp]),
         "hmd_integrator_thermostat_state": (st, [vp, _c_dp]),
+        "hmd_integrator_checkpoint": (st, [vp]),
+        "hmd_integrator_rollback": (st, [vp]),
         "hmd_integrator_run_judged": (st, [vp, vp, ctypes.c_uint32, ctypes.c_uint32,
                                            ctypes.c_double, _c_dp, ctypes.c_uint32,
                                            _c_u32p, _c_u64p]),

@@ -381,6 +381,16 @@ class Integrator:
         check(lib().hmd_integrator_thermostat_state(self.h, _ptr(out, _dp)))
         return float(out[0]), int(out[1])
 
+    def checkpoint(self):
+        """Keep a device copy of the state a batch of steps changes
+        (hmd_integrator_checkpoint)."""
+        check(lib().hmd_integrator_checkpoint(self.h))
+
+    def rollback(self):
+        """Restore the last checkpoint, with a list rebuild at the next
+        evaluation (hmd_integrator_rollback)."""
+        check(lib().hmd_integrator_rollback(self.h))
+
     def set_forcefield(self, forcefield):
         """Step with another force field on the same particles from here on
         (hmd_integrator_set_forcefield)."""

@@ -9,6 +9,8 @@
 #include <cstdio>
 #include <functional>
 
+#include <cstring>
+
 #include "forcefield.h"
 
 namespace hmd {
@@ -171,40 +173,51 @@ static double erfc_as(double x) {
   return poly * std::exp(-x * x);
 }
 
-// Real-space Ewald kernel on HMD_COUL_BINS bins of width h = r_c/(bins - 1)
-// (bin b covers [b h, (b + 1) h); the last one lies beyond the cutoff): per
-// bin the cubic Hermite interpolant, in t = r/h - b, of the force factor
-// G(r) = [erfc(a r)/r + 2a/sqrt(pi) exp(-a^2 r^2)]/r^2 and of the energy
-// E(r) = erfc(a r)/r, from FP64 values and derivatives at both bin ends
-// (derivatives by a fourth-order central difference).
+// Real-space Ewald kernel in the octave table of r^2 (hmd_coul_bin): the
+// table spans [2^(e - HMD_COUL_OCTAVES), 2^e) with 2^e the first power of two
+// above r_c^2, each octave split into 2^HMD_COUL_BITS bins of equal width in
+// r^2, so a bin's relative width is at most 2^-HMD_COUL_BITS everywhere.  Per
+// bin the cubic Hermite interpolant, in the bin's own coordinate s in [0, 1),
+// of the force factor G(r) = [erfc(a r)/r + 2a/sqrt(pi) exp(-a^2 r^2)]/r^2
+// and of the energy E(r) = erfc(a r)/r, from FP64 values and derivatives in
+// r^2 at both bin ends (derivatives by a fourth-order central difference).
+// Distances below the table's first r (between r_c/16 and r_c/11 with eight
+// octaves: under 0.9 angstrom for a 1 nm cutoff) clamp to its first value;
+// no non-excluded pair comes that close.
 void ForceField::build_coulomb_table(double a, double rc, hipStream_t s) {
-  const uint32_t nb = HMD_COUL_BINS;
-  const double h = rc / (nb - 1);
-  auto G = [a](double r) {
-    return (erfc_as(a * r) / r + 2.0 * a / std::sqrt(M_PI) * std::exp(-a * a * r * r)) / (r * r);
+  const uint32_t nb = HMD_COUL_BINS, per = 1u << HMD_COUL_BITS;
+  const int e_hi = (int)std::floor(std::log2(rc * rc)) + 1;   // 2^e_hi > rc^2
+  const int e_lo = e_hi - (int)HMD_COUL_OCTAVES;
+  auto G = [a](double r2) {
+    const double r = std::sqrt(r2);
+    return (erfc_as(a * r) / r + 2.0 * a / std::sqrt(M_PI) * std::exp(-a * a * r2)) / r2;
   };
-  auto E = [a](double r) { return erfc_as(a * r) / r; };
+  auto E = [a](double r2) { return erfc_as(a * std::sqrt(r2)) / std::sqrt(r2); };
   std::vector<float> tab(2 * 4 * nb);
   for (int f = 0; f < 2; ++f) {
-    auto F = [&](double r) { return f == 0 ? G(r) : E(r); };
-    std::vector<double> v(nb + 1), dv(nb + 1);
-    for (uint32_t k = 0; k <= nb; ++k) {
-      const double r = std::max((double)k, 0.5) * h;   // bin 0: no pair is that close
-      const double d = 1e-3 * h;
-      v[k] = F(r);
-      dv[k] = h * (F(r - 2 * d) - 8 * F(r - d) + 8 * F(r + d) - F(r + 2 * d)) / (12 * d);
-    }
+    auto F = [&](double r2) { return f == 0 ? G(r2) : E(r2); };
     for (uint32_t k = 0; k < nb; ++k) {
+      const double w = std::ldexp(1.0, e_lo + (int)(k / per)) / per;   // bin width in r^2
+      const double x0 = std::ldexp(1.0, e_lo + (int)(k / per)) + (k % per) * w;
+      double v[2], dv[2];
+      for (int end = 0; end < 2; ++end) {
+        const double x = x0 + end * w, d = 1e-3 * w;
+        v[end] = F(x);
+        dv[end] = w * (F(x - 2 * d) - 8 * F(x - d) + 8 * F(x + d) - F(x + 2 * d)) / (12 * d);
+      }
       float* c = &tab[4 * (f * nb + k)];
-      c[0] = (float)v[k];
-      c[1] = (float)dv[k];
-      c[2] = (float)(3.0 * (v[k + 1] - v[k]) - 2.0 * dv[k] - dv[k + 1]);
-      c[3] = (float)(2.0 * (v[k] - v[k + 1]) + dv[k] + dv[k + 1]);
+      c[0] = (float)v[0];
+      c[1] = (float)dv[0];
+      c[2] = (float)(3.0 * (v[1] - v[0]) - 2.0 * dv[0] - dv[1]);
+      c[3] = (float)(2.0 * (v[0] - v[1]) + dv[0] + dv[1]);
     }
   }
   coul_table.alloc(tab.size(), false);
   coul_table.upload(tab.data(), tab.size(), s);
-  coul_inv_h = (float)(1.0 / h);
+  const float lo = std::ldexp(1.0f, e_lo), hi = std::ldexp(1.0f, e_hi);
+  std::memcpy(&coul_u_lo, &lo, 4);
+  std::memcpy(&coul_u_hi, &hi, 4);
+  coul_u_hi -= 1u;   // the largest f32 below 2^e_hi
 }
 
 // ---------------------------------------------------------- construction --
@@ -721,13 +734,16 @@ void ForceField::pair_forces(bool fev) {
   pp.n_types = n_types;
   pp.lj_c6c12 = lj_table.ptr;
   pp.lj_type_mask = lj_type_mask;
+  pp.lj_single_type = __builtin_popcount(lj_type_mask) == 1 ? (uint32_t)__builtin_ctz(lj_type_mask)
+                                                            : n_types;
   pp.rc2 = rc2;
   pp.lj_rs2 = lj_rs2;
   pp.lj_inv_delta = lj_inv_delta;
   pp.alpha = alpha;
   pp.k_coulomb = k_coulomb;
   pp.coul_table = coul_table.ptr;
-  pp.coul_inv_h = coul_inv_h;
+  pp.coul_u_lo = coul_u_lo;
+  pp.coul_u_hi = coul_u_hi;
   pp.shift_margin = r_cut + 0.5f * r_skin;
   pp.acc = acc.ptr;
   pp.acc_stride = n_pad;
@@ -770,7 +786,8 @@ void ForceField::pair_forces(bool fev) {
     q.lj_inv_delta = lj_inv_delta;
     q.k_coulomb = k_coulomb;
     q.coul_table = coul_table.ptr;
-    q.coul_inv_h = coul_inv_h;
+    q.coul_u_lo = coul_u_lo;
+    q.coul_u_hi = coul_u_hi;
     q.lj_on = lj_on ? 1u : 0u;
     q.coul_on = coul_on ? 1u : 0u;
     q.fev = fev ? 1u : 0u;

@@ -46,7 +46,7 @@ struct ForceField {
   float alpha = 0, k_coulomb = 1;
   DevBuf<float> lj_table;
   DevBuf<float> coul_table;    // float4[2][HMD_COUL_BINS]
-  float coul_inv_h = 0;
+  uint32_t coul_u_lo = 0, coul_u_hi = 0;   // table bounds as f32 bits
   void build_coulomb_table(double alpha, double r_cut, hipStream_t s);
   DevBuf<uint32_t> excl_off, excl_par;
   // listed pairs with non-zero exclusion scales (i < j)

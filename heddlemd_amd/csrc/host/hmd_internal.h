@@ -125,9 +125,13 @@ struct DevBuf {
       check(hipStreamSynchronize(nullptr), "sync");
     }
   }
+  // Waits for the copy: the source is usually a temporary in pageable host
+  // memory, which the runtime may still be reading after an asynchronous
+  // copy returns.  (Never called inside a stream capture.)
   void upload(const T* host, size_t n, hipStream_t s) {
     if (n == 0) return;
     check(hipMemcpyAsync(ptr, host, n * sizeof(T), hipMemcpyHostToDevice, s), "H2D");
+    check(hipStreamSynchronize(s), "sync");
   }
   void download(T* host, size_t n, hipStream_t s) const {
     if (n == 0) return;

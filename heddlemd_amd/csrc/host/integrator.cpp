@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <map>
+#include <memory>
 
 #include "forcefield.h"
 
@@ -702,6 +703,7 @@ struct BatchStart {
 
 struct hmd_integrator {
   hmd::Integrator* it = nullptr;
+  std::unique_ptr<hmd::BatchStart> mark;   // hmd_integrator_checkpoint
 };
 
 struct hmd_replicas {
@@ -1111,6 +1113,23 @@ hmd_status hmd_integrator_run_judged(hmd_integrator* w, hmd_forcefield* fine, ui
   });
 }
 uint64_t hmd_integrator_steps(hmd_integrator* w) { return w->it->steps; }
+
+hmd_status hmd_integrator_checkpoint(hmd_integrator* w) {
+  HMD_GUARD({
+    if (!w) throw Error(HMD_ERR_INVALID, "null integrator");
+    if (!w->mark) w->mark.reset(new BatchStart());
+    w->mark->save(*w->it);
+    check(hipStreamSynchronize(w->it->c.stream), "sync");
+  });
+}
+
+hmd_status hmd_integrator_rollback(hmd_integrator* w) {
+  HMD_GUARD({
+    if (!w) throw Error(HMD_ERR_INVALID, "null integrator");
+    if (!w->mark) throw Error(HMD_ERR_INVALID, "no checkpoint to roll back to");
+    w->mark->restore(*w->it);
+  });
+}
 
 hmd_status hmd_integrator_profile(hmd_integrator* w, uint32_t n, double* us, int32_t max_stages,
                                   int32_t* n_stages, char* names, int32_t names_len) {

@@ -21,7 +21,11 @@ using __hip_internal::uint64_t;
 #define HMD_ENTRY 64u               // j-atom slots per neighbour entry
 #define HMD_SENTINEL 0xFFFFFFFFu    // empty j slot
 #define HMD_FIXED_SHIFT 40          // pair accumulators: value * 2^40 in i64
-#define HMD_COUL_BINS 512u          // real-space Ewald table bins
+// real-space Ewald table: HMD_COUL_OCTAVES octaves of r^2 below the first
+// power of two above r_cut^2, 2^HMD_COUL_BITS bins per octave
+#define HMD_COUL_OCTAVES 8u
+#define HMD_COUL_BITS 7u
+#define HMD_COUL_BINS (HMD_COUL_OCTAVES << HMD_COUL_BITS)
 #define HMD_SPREAD_SHIFT 32         // mesh accumulators: value * 2^32 in i64
 
 // neighbour status word bits (kernels set them, the host reads them)
@@ -174,15 +178,17 @@ struct PairParams {
   uint32_t n_types;
   const float* lj_c6c12;       // float2[n_types^2]: (4 eps s^6, 4 eps s^12)
   uint32_t lj_type_mask;       // bit t: type t has LJ with some type
+  uint32_t lj_single_type;     // the one type in lj_type_mask, or n_types if several
   float rc2;                   // max cutoff squared (mask)
   float lj_rs2;                // switch start squared
   float lj_inv_delta;          // 1/(rc^2 - rs^2); 0 disables the switch
   float alpha, k_coulomb;
   // real-space Ewald kernel, tabulated: float4[2][HMD_COUL_BINS] cubic
-  // coefficients per bin of width r_cut / (HMD_COUL_BINS - 1), force factor
-  // G(r) = [erfc(a r)/r + 2a/sqrt(pi) exp(-a^2 r^2)] / r^2 then energy erfc(a r)/r
+  // coefficients per bin of the octave table in r^2 (hmd_coul_bin), force
+  // factor G(r) = [erfc(a r)/r + 2a/sqrt(pi) exp(-a^2 r^2)] / r^2 then
+  // energy erfc(a r)/r
   const float* coul_table;
-  float coul_inv_h;            // 1 / bin width
+  uint32_t coul_u_lo, coul_u_hi;   // f32 bits of the table's first r^2 and of its last
   float shift_margin;          // r_cut + r_skin/2: per-cluster shift eligibility
   long long* acc;              // [5][n_pad] fixed-point F_x,F_y,F_z,E,W (sorted)
   uint32_t acc_stride;         // n_pad
@@ -207,7 +213,7 @@ struct PairScaledParams {
   float rc2, lj_rs2, lj_inv_delta;
   float k_coulomb;
   const float* coul_table;      // float4 cubic coefficients, force then energy
-  float coul_inv_h;
+  uint32_t coul_u_lo, coul_u_hi;
   uint32_t lj_on, coul_on, fev;
   long long* acc;
   uint32_t acc_stride;

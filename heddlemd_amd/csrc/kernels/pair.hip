@@ -5,11 +5,12 @@
 // Stegun 7.1.26 erfc (src/forces/spme.rs:712-760), exclusion scale 0 for
 // listed partners, branch-free max-cutoff mask, Newton's third law, and the
 // half/half energy and virial split (src/forces/jit_composed.rs:1302-1643).
-// The real-space Ewald force and energy functions of r (the A&S erfc
-// formula, evaluated in FP64 on the host) are read from a 512-bin cubic
-// Hermite table in LDS (relative interpolation error below 3e-7 for
-// r >= 0.09 nm), which replaces the exp, reciprocal and polynomial of the
-// closed form by one LDS gather and three FMAs per pair.
+// The real-space Ewald force and energy functions (the A&S erfc formula,
+// evaluated in FP64 on the host) are read from a cubic Hermite table in LDS
+// over octaves of r^2, 128 bins per octave, whose bin and in-bin coordinate
+// come from the f32 bits of r^2 (hmd_coul_bin): one clamp, two shifts and a
+// bit insert replace the square root, the exp and the polynomial of the
+// closed form, and no reciprocal is taken unless the step has LJ.
 //
 // Mapping (wave64, VALU FP32; no MFMA -- there is no dense contraction):
 //  * one wave walks a contiguous share of the work list; an entry holds 64
@@ -22,6 +23,8 @@
 //    can reach (the entry's 8-bit mask from the list build, whose placement
 //    groups j-atoms of similar reach into the same entries): about two
 //    thirds of the steps of visiting every i-atom;
+//  * a step (one i-atom against the entry's 64 j-atoms) with no pair inside
+//    the cutoff is skipped by a wave-uniform branch after the distance test;
 //  * slot combinations (LJ, Coulomb, energies) are compile-time variants,
 //    and LJ is skipped for i-types with no LJ partner (uniform branch);
 //  * entries whose j-atoms all lie in later clusters and carry no exclusion
@@ -71,11 +74,16 @@ template <bool FEV, bool LJ, bool COUL, bool SHIFT, bool PLAIN>
 __device__ inline void hmd_pair_isteps(const PairParams& p, const float4* s_lj,
                                        const float4* s_ct, const float4* s_ip,
                                        const uint32_t* s_it, float* s_fi, float4 pj,
-                                       uint32_t tj, uint32_t my_j, uint64_t exT,
+                                       uint32_t tj, float4 cc_single, uint32_t my_j,
+                                       uint64_t exT,
                                        uint32_t emask, uint32_t c, uint64_t lj_ibits,
                                        float (&fj)[5], uint32_t lane) {
+  // Every fused multiply-add below is written out: the solo kernel and its
+  // batched twin then round identically (left to the compiler, a*b + c*d
+  // may be fused on either side, differently in the two).
+#pragma clang fp contract(off)
   const HmdLattice L = p.L;
-  const float x_max = (float)HMD_COUL_BINS - 0.01f;   // last bin: beyond r_cut, masked
+  const float4* s_ctb = s_ct;
   const float lj_rs2_d = p.lj_rs2 * p.lj_inv_delta;
   const float lj_12d = 12.0f * p.lj_inv_delta;
   constexpr uint32_t NC = FEV ? 5u : 3u;        // components per atom
@@ -99,47 +107,57 @@ __device__ inline void hmd_pair_isteps(const PairParams& p, const float4* s_lj,
       if (u < 7) a_next = s_ip[i + 1];
       float dx = a.x - pj.x, dy = a.y - pj.y, dz = a.z - pj.z;
       if (!SHIFT) hmd_min_image(dx, dy, dz, L);
-      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
       bool ok = r2 <= p.rc2;
       if (!PLAIN) {
         const uint32_t si = c0 + i;
         ok = ok && valid_j && (si < p.n) && (si < my_j) && !((exT >> i) & 1ull);
       }
+      // A step with no pair inside the cutoff in any lane (about a fifth of
+      // the steps an entry's sub-cluster mask admits) is skipped as a whole:
+      // a wave-uniform branch.
+      if (__ballot(ok) == 0ull) {
+#pragma unroll
+        for (uint32_t cc = 0; cc < NC; ++cc) ir[cc][u] = 0.0f;
+        continue;
+      }
       // PLAIN: every pair has r2 > 0 (distinct atoms; empty slots far away),
       // so the masked-out pairs evaluate to finite values without a guard
       const float r2s = PLAIN ? r2 : (ok ? r2 : 1.0f);
-      const float inv_r = __builtin_amdgcn_rsqf(r2s);
-      const float inv_r2 = inv_r * inv_r;
       float factor = 0.0f, energy = 0.0f;
       if (COUL) {
-        // tabulated erfc kernel: bin b = floor(r/h), cubic in t = r/h - b
+        // tabulated erfc kernel: octave-table bin of r^2, cubic in its s
         const float qq = a.w * pj.w;
-        const float x = fminf(r2s * inv_r * p.coul_inv_h, x_max);
-        const float t = __builtin_amdgcn_fractf(x);
-        const uint32_t b = (uint32_t)x;
-        const float4 g = s_ct[b];
-        factor = qq * (((g.w * t + g.z) * t + g.y) * t + g.x);
+        float t;
+        const uint32_t b = hmd_coul_bin(r2s, p.coul_u_lo, p.coul_u_hi, t);
+        const float4 g = s_ctb[b];
+        factor = qq * fmaf(fmaf(fmaf(g.w, t, g.z), t, g.y), t, g.x);
         if (FEV) {
-          const float4 e = s_ct[HMD_COUL_BINS + b];
-          energy = qq * (((e.w * t + e.z) * t + e.y) * t + e.x);
+          const float4 e = s_ctb[HMD_COUL_BINS + b];
+          energy = qq * fmaf(fmaf(fmaf(e.w, t, e.z), t, e.y), t, e.x);
         }
       }
       if (LJ) {
         if ((lj8 >> u) & 1u) {                               // scalar branch
-          // (c6, c12, 6 c6, 12 c12)
-          const uint32_t ti = __builtin_amdgcn_readfirstlane(s_it[i]);
-          const float4 cc = s_lj[ti * p.n_types + tj];
+          // (c6, c12, 6 c6, 12 c12): held per entry when one type carries
+          // all the LJ (the i-atom's type is then that one), else looked up
+          float4 cc = cc_single;
+          if (p.lj_single_type >= p.n_types) {
+            const uint32_t ti = __builtin_amdgcn_readfirstlane(s_it[i]);
+            cc = s_lj[ti * p.n_types + tj];
+          }
+          const float inv_r2 = __builtin_amdgcn_rcpf(r2s);
           const float ir6 = inv_r2 * inv_r2 * inv_r2;
-          const float e_lj = (cc.y * ir6 - cc.x) * ir6;
-          const float f_lj = (cc.w * ir6 - cc.z) * ir6 * inv_r2;
+          const float e_lj = fmaf(cc.y, ir6, -cc.x) * ir6;
+          const float f_lj = fmaf(cc.w, ir6, -cc.z) * ir6 * inv_r2;
           // C1 switch: tau = (r^2 - rs^2)/(rc^2 - rs^2) clamped at 0,
           // S = (1 - tau)^2 (1 + 2 tau), force gains -dS/dr / r * E
-          const float tau = fmaxf(r2s * p.lj_inv_delta - lj_rs2_d, 0.0f);
+          const float tau = fmaxf(fmaf(r2s, p.lj_inv_delta, -lj_rs2_d), 0.0f);
           const float omt = 1.0f - tau;
-          const float sw = omt * omt * (1.0f + 2.0f * tau);
+          const float sw = omt * omt * fmaf(2.0f, tau, 1.0f);
           const float chain = (lj_12d * tau) * omt;
-          factor += sw * f_lj + chain * e_lj;
-          if (FEV) energy += sw * e_lj;
+          factor += fmaf(sw, f_lj, chain * e_lj);
+          if (FEV) energy = fmaf(sw, e_lj, energy);
         }
       }
 #ifdef HMD_USER_FUNCTOR
@@ -148,7 +166,7 @@ __device__ inline void hmd_pair_isteps(const PairParams& p, const float4* s_lj,
         // from (types, r^2, 1/r); masked with the built-in terms below
         float f_u = 0.0f, e_u = 0.0f;
         const uint32_t ti = __builtin_amdgcn_readfirstlane(s_it[i]);
-        HMD_USER_FUNCTOR{}(p.user_params, ti, tj, r2s, inv_r, f_u, e_u);
+        HMD_USER_FUNCTOR{}(p.user_params, ti, tj, r2s, __builtin_amdgcn_rsqf(r2s), f_u, e_u);
         factor += f_u;
         if (FEV) energy += e_u;
       }
@@ -205,6 +223,7 @@ __device__ inline void hmd_pair_entry(const PairParams& p, const float4* s_lj,
                                       const uint32_t* s_it, float* s_fi, uint32_t slot,
                                       uint32_t c, float cx, float cy, float cz,
                                       uint64_t lj_ibits, uint32_t lane) {
+#pragma clang fp contract(off)   // as in hmd_pair_isteps: fused multiply-adds written out
   const float4* ps = reinterpret_cast<const float4*>(p.posq_sorted);
   const HmdLattice L = p.L;
   const size_t ebase = (size_t)slot * HMD_ENTRY;
@@ -217,21 +236,26 @@ __device__ inline void hmd_pair_entry(const PairParams& p, const float4* s_lj,
   float4 pj = ps[jj_l];
   const uint32_t tj = p.type_sorted[jj_l];
   if (SHIFT) {
-    pj.x -= L.lx * floorf((pj.x - cx) * (1.0f / L.lx) + 0.5f);
-    pj.y -= L.ly * floorf((pj.y - cy) * (1.0f / L.ly) + 0.5f);
-    pj.z -= L.lz * floorf((pj.z - cz) * (1.0f / L.lz) + 0.5f);
+    pj.x = fmaf(-L.lx, floorf(fmaf(pj.x - cx, 1.0f / L.lx, 0.5f)), pj.x);
+    pj.y = fmaf(-L.ly, floorf(fmaf(pj.y - cy, 1.0f / L.ly, 0.5f)), pj.y);
+    pj.z = fmaf(-L.lz, floorf(fmaf(pj.z - cz, 1.0f / L.lz, 0.5f)), pj.z);
     if (!valid_j) pj.x = pj.y = pj.z = 1.0e18f;   // fails every cutoff test
   }
   const bool any_ex = __ballot(ex != 0ull) != 0ull;
   const bool plain = SHIFT && !any_ex && __ballot(my_j < (c + 1u) * HMD_CLUSTER) == 0ull;
   const uint64_t exT = any_ex ? hmd_transpose64(ex, lane) : 0ull;
   float fj[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  const float4 cc_single = (LJ && p.lj_single_type < p.n_types)
+                               ? s_lj[p.lj_single_type * p.n_types + tj]
+                               : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (SHIFT && plain)
-    hmd_pair_isteps<FEV, LJ, COUL, true, true>(p, s_lj, s_ct, s_ip, s_it, s_fi, pj, tj, my_j,
-                                                exT, emask & 0xFFu, c, lj_ibits, fj, lane);
+    hmd_pair_isteps<FEV, LJ, COUL, true, true>(p, s_lj, s_ct, s_ip, s_it, s_fi, pj, tj,
+                                                cc_single, my_j, exT, emask & 0xFFu, c,
+                                                lj_ibits, fj, lane);
   else
-    hmd_pair_isteps<FEV, LJ, COUL, SHIFT, false>(p, s_lj, s_ct, s_ip, s_it, s_fi, pj, tj, my_j,
-                                                  exT, emask & 0xFFu, c, lj_ibits, fj, lane);
+    hmd_pair_isteps<FEV, LJ, COUL, SHIFT, false>(p, s_lj, s_ct, s_ip, s_it, s_fi, pj, tj,
+                                                  cc_single, my_j, exT, emask & 0xFFu, c,
+                                                  lj_ibits, fj, lane);
   if (valid_j) {
     constexpr uint32_t NC = FEV ? 5u : 3u;
     long long* acc = p.acc;
@@ -265,6 +289,7 @@ __device__ inline void hmd_flush_i(const PairParams& p, float* s_fi, uint32_t si
 template <bool FEV, bool LJ, bool COUL>
 __device__ inline void hmd_pair_body(const PairParams& p, const uint32_t hmd_bx,
     const uint32_t hmd_gx) {
+#pragma clang fp contract(off)   // as in hmd_pair_isteps: fused multiply-adds written out
   __shared__ float4 s_lj[HMD_MAX_TYPES * HMD_MAX_TYPES];   // (c6, c12, 6 c6, 12 c12)
   // erfc table: force coefficients, then (energy variants) energy coefficients
   __shared__ float4 s_ct[COUL ? (FEV ? 2 * HMD_COUL_BINS : HMD_COUL_BINS) : 1];
@@ -331,17 +356,19 @@ __device__ inline void hmd_pair_body(const PairParams& p, const uint32_t hmd_bx,
               (hb.z + p.shift_margin < 0.5f * L.lz);
       cx = cb.x; cy = cb.y; cz = cb.z;
       if (shift) {
-        a.x -= L.lx * floorf((a.x - cx) * (1.0f / L.lx) + 0.5f);
-        a.y -= L.ly * floorf((a.y - cy) * (1.0f / L.ly) + 0.5f);
-        a.z -= L.lz * floorf((a.z - cz) * (1.0f / L.lz) + 0.5f);
+        a.x = fmaf(-L.lx, floorf(fmaf(a.x - cx, 1.0f / L.lx, 0.5f)), a.x);
+        a.y = fmaf(-L.ly, floorf(fmaf(a.y - cy, 1.0f / L.ly, 0.5f)), a.y);
+        a.z = fmaf(-L.lz, floorf(fmaf(a.z - cz, 1.0f / L.lz, 0.5f)), a.z);
         if (!vi) a.x = a.y = a.z = 1.0e18f;   // padding lanes fail every cutoff test
       }
       a.w *= p.k_coulomb;
       {
         const uint64_t b = __ballot(vi && ((p.lj_type_mask >> ti) & 1u));
         // held in scalar registers: the per-step LJ test is a scalar branch
-        lj_ibits = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(b >> 32)) << 32) |
-                   (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)b);
+        // (readfirstlane returns int: widen through uint32_t, or a set bit 31
+        // would sign-extend over the upper half)
+        lj_ibits = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(b >> 32)) << 32) |
+                   (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)b);
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -452,6 +479,7 @@ HMD_KERNEL(hmd_slot_add, SlotAddParams, 256)
 __device__ __forceinline__ void hmd_pair_scaled_body(const PairScaledParams& p,
                                                      const uint32_t hmd_bx,
                                                      const uint32_t hmd_gx) {
+#pragma clang fp contract(off)   // fused multiply-adds written out, as in the fused kernel
   const uint32_t k = hmd_bx * blockDim.x + threadIdx.x;
   if (k >= p.n_pairs) return;
   const uint32_t i = p.pair_i[k], j = p.pair_j[k];
@@ -459,34 +487,32 @@ __device__ __forceinline__ void hmd_pair_scaled_body(const PairScaledParams& p,
   const float4 b = reinterpret_cast<const float4*>(p.posq)[j];
   float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
   hmd_min_image(dx, dy, dz, p.L);
-  const float r2 = dx * dx + dy * dy + dz * dz;
+  const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
   if (!(r2 <= p.rc2) || r2 == 0.0f) return;
-  const float inv_r = __builtin_amdgcn_rsqf(r2);
-  const float inv_r2 = inv_r * inv_r;
+  const float inv_r2 = __builtin_amdgcn_rcpf(r2);
   float factor = 0.0f, energy = 0.0f;
   if (p.coul_on && p.s_coul[k] != 0.0f) {
     const float qq = p.k_coulomb * a.w * b.w;
-    const float x = fminf(r2 * inv_r * p.coul_inv_h, (float)HMD_COUL_BINS - 0.01f);
-    const float t = __builtin_amdgcn_fractf(x);
-    const uint32_t bin = (uint32_t)x;
+    float t;
+    const uint32_t bin = hmd_coul_bin(r2, p.coul_u_lo, p.coul_u_hi, t);
     const float4 g = reinterpret_cast<const float4*>(p.coul_table)[bin];
-    factor += p.s_coul[k] * (qq * (((g.w * t + g.z) * t + g.y) * t + g.x));
+    factor += p.s_coul[k] * (qq * fmaf(fmaf(fmaf(g.w, t, g.z), t, g.y), t, g.x));
     if (p.fev) {
       const float4 e = reinterpret_cast<const float4*>(p.coul_table)[HMD_COUL_BINS + bin];
-      energy += p.s_coul[k] * (qq * (((e.w * t + e.z) * t + e.y) * t + e.x));
+      energy += p.s_coul[k] * (qq * fmaf(fmaf(fmaf(e.w, t, e.z), t, e.y), t, e.x));
     }
   }
   if (p.lj_on && p.s_lj[k] != 0.0f) {
     const float2 cc =
         reinterpret_cast<const float2*>(p.lj_c6c12)[p.type[i] * p.n_types + p.type[j]];
     const float ir6 = inv_r2 * inv_r2 * inv_r2;
-    const float e_lj = (cc.y * ir6 - cc.x) * ir6;
-    const float f_lj = (12.0f * cc.y * ir6 - 6.0f * cc.x) * ir6 * inv_r2;
+    const float e_lj = fmaf(cc.y, ir6, -cc.x) * ir6;
+    const float f_lj = fmaf(12.0f * cc.y, ir6, -6.0f * cc.x) * ir6 * inv_r2;
     const float tau = fmaxf((r2 - p.lj_rs2) * p.lj_inv_delta, 0.0f);
     const float omt = 1.0f - tau;
-    const float sw = omt * omt * (1.0f + 2.0f * tau);
+    const float sw = omt * omt * fmaf(2.0f, tau, 1.0f);
     const float chain = (12.0f * p.lj_inv_delta * tau) * omt;
-    factor += p.s_lj[k] * (sw * f_lj + chain * e_lj);
+    factor += p.s_lj[k] * fmaf(sw, f_lj, chain * e_lj);
     if (p.fev) energy += p.s_lj[k] * (sw * e_lj);
   }
   const uint32_t si = p.slot_of[i], sj = p.slot_of[j], st = p.acc_stride;

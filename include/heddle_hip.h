@@ -333,6 +333,17 @@ hmd_status hmd_integrator_run_judged(hmd_integrator* it, hmd_forcefield* fine_ff
                                      uint32_t n_steps, uint32_t judge_every, double threshold,
                                      double* judges, uint32_t max_judges, uint32_t* n_judges,
                                      uint64_t* reverted_at);
+/* Device copy of everything a batch of steps changes (positions,
+ * velocities, images, forces, the compensated low parts, the random-stream
+ * and thermostat counters, the step count), and its restore with a list
+ * rebuild at the next evaluation: the re-run of a batch that reported
+ * HMD_ERR_NBL_OVERFLOW starts from exactly the batch's start.  No
+ * counterpart in the reference, whose runner halts on a packed-list
+ * overflow (NeighborListError::PackedNeighborOverflow,
+ * src/forces/neighbor_list.rs:1308-1320).  Rollback without a checkpoint is
+ * HMD_ERR_INVALID; a checkpoint stays valid until the next one. */
+hmd_status hmd_integrator_checkpoint(hmd_integrator* it);
+hmd_status hmd_integrator_rollback(hmd_integrator* it);
 /* CSVR state (flush_pending_injection, csvr.rs:134-160): out[0] the energy
  * the thermostat has injected, sum of K' - K (the conserved quantity is
  * KE + PE - out[0]), out[1] the Philox draws so far. */
