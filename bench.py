@@ -198,19 +198,38 @@ def cpu_baseline(sysw, ff, state, steps_all=5, steps_one=2, repeats=5):
 TRAFFIC_CALIB_BYTES = 256 << 20
 
 
+def calibration_copies(nbytes, n):
+    """n device-to-device copies of nbytes (the runtime's copy kernel,
+    __amd_rocclr_copyBuffer) through the HIP runtime the library links,
+    which a counter reading is scaled by."""
+    import ctypes
+    from heddlemd_amd._lib import lib
+    lib()
+    rt = ctypes.CDLL("libamdhip64.so.7")   # the runtime the library loaded
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for buf in (a, b):
+        if rt.hipMalloc(ctypes.byref(buf), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if rt.hipMemset(a, 1, ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMemset failed")
+        for _ in range(n):
+            if rt.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) != 0:   # device to device
+                raise RuntimeError("hipMemcpy failed")
+        if rt.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
+    finally:
+        rt.hipFree(a)
+        rt.hipFree(b)
+
+
 def traffic_probe(api, systems, U):
     """The run the HBM-traffic counters are read from (rocprofv3 --pmc
-    FETCH_SIZE, then WRITE_SIZE): a device copy of a known size (3 x 256 MiB,
-    through torch, which the counters are calibrated by), then five eager
-    force evaluations of the bench box with SPME on the main stream, so that
-    each kernel's counters are its own."""
-    import torch   # device buffers for the calibration copy only
-    a = torch.empty(TRAFFIC_CALIB_BYTES // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    a.fill_(1.0)
-    for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
+    FETCH_SIZE, then WRITE_SIZE): device copies of a known size (3 x 256 MiB,
+    which the counters are calibrated by), then five eager force evaluations
+    of the bench box with SPME on the main stream, so that each kernel's
+    counters are its own."""
+    calibration_copies(TRAFFIC_CALIB_BYTES, 3)
     log(f"calibration copies: 3 x {TRAFFIC_CALIB_BYTES} bytes")
     ctx, sysw, ff, part, field, integ = build_system(api, systems, U, minimize_iter=20)
     field.set_tuning(dict(field.tuning(), spme_stream=0))
@@ -249,6 +268,9 @@ def main():
                          "copy, then five eager force evaluations of the bench box)")
     ap.add_argument("--pair-blocks-per-cu", type=int, default=0,
                     help="pair-kernel workgroups per CU (0: the library default)")
+    ap.add_argument("--fft-slab", type=int, default=0, choices=(0, 1),
+                    help="1: the three-kernel slab FFT (128^3, 64^3); 0: the five line "
+                         "passes (the library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -271,6 +293,8 @@ def main():
         field.set_tuning(dict(field.tuning(), spme_stream=0))
     if args.pair_blocks_per_cu:
         field.set_tuning(dict(field.tuning(), pair_blocks_per_cu=args.pair_blocks_per_cu))
+    if args.fft_slab:
+        field.set_tuning(dict(field.tuning(), fft_slab=1))
     tuned = None
     if args.autotune:
         tuned = field.autotune(20)

@@ -90,6 +90,7 @@ class Tuning(ctypes.Structure):
         ("fft_blocks_per_cu", ctypes.c_uint32),
         ("brick", ctypes.c_uint32 * 3),
         ("spme_stream", ctypes.c_uint32),
+        ("fft_slab", ctypes.c_uint32),
         ("n_candidates", ctypes.c_uint32),
         ("us_initial", ctypes.c_double),
         ("us_tuned", ctypes.c_double),

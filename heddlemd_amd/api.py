@@ -248,7 +248,7 @@ class ForceField:
         return {"pair_blocks_per_cu": t.pair_blocks_per_cu,
                 "fft_blocks_per_cu": t.fft_blocks_per_cu,
                 "brick": tuple(t.brick), "spme_stream": t.spme_stream,
-                "n_candidates": t.n_candidates,
+                "fft_slab": t.fft_slab, "n_candidates": t.n_candidates,
                 "us_initial": t.us_initial, "us_tuned": t.us_tuned}
 
     def autotune(self, n_evals=20):
@@ -272,6 +272,7 @@ class ForceField:
         for k in range(3):
             t.brick[k] = int(br[k])
         t.spme_stream = int(tuning.get("spme_stream", 1))
+        t.fft_slab = int(tuning.get("fft_slab", 0))
         check(lib().hmd_forcefield_set_tuning(self.h, ctypes.byref(t)))
 
     def request_rebuild(self):

@@ -100,6 +100,7 @@ static hmd_status ctx_create(int device, hipStream_t caller_stream, hmd_ctx** ou
       hipDeviceProp_t prop;
       check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
       c.num_cus = prop.multiProcessorCount;
+      c.max_lds = prop.sharedMemPerBlock;
       c.device_name = prop.gcnArchName;
       if (c.device_name.rfind("gfx950", 0) != 0)
         throw Error(HMD_ERR_UNSUPPORTED,

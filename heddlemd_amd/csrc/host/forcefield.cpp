@@ -180,7 +180,9 @@ static double erfc_as(double x) {
 // bin the cubic Hermite interpolant, in the bin's own coordinate s in [0, 1),
 // of the force factor G(r) = [erfc(a r)/r + 2a/sqrt(pi) exp(-a^2 r^2)]/r^2
 // and of the energy E(r) = erfc(a r)/r, from FP64 values and derivatives in
-// r^2 at both bin ends (derivatives by a fourth-order central difference).
+// r^2 at both bin ends (derivatives by a fourth-order central difference),
+// stored as the same cubic in t = 1 + s (the kernels read t off the bits of
+// r^2 and evaluate it by Horner's rule; the basis change is done in FP64).
 // Distances below the table's first r (between r_c/16 and r_c/11 with eight
 // octaves: under 0.9 angstrom for a 1 nm cutoff) clamp to its first value;
 // no non-excluded pair comes that close.
@@ -205,11 +207,15 @@ void ForceField::build_coulomb_table(double a, double rc, hipStream_t s) {
         v[end] = F(x);
         dv[end] = w * (F(x - 2 * d) - 8 * F(x - d) + 8 * F(x + d) - F(x + 2 * d)) / (12 * d);
       }
+      // Hermite cubic in s, then the same cubic in t = 1 + s
+      const double a0 = v[0], a1 = dv[0];
+      const double a2 = 3.0 * (v[1] - v[0]) - 2.0 * dv[0] - dv[1];
+      const double a3 = 2.0 * (v[0] - v[1]) + dv[0] + dv[1];
       float* c = &tab[4 * (f * nb + k)];
-      c[0] = (float)v[0];
-      c[1] = (float)dv[0];
-      c[2] = (float)(3.0 * (v[1] - v[0]) - 2.0 * dv[0] - dv[1]);
-      c[3] = (float)(2.0 * (v[0] - v[1]) + dv[0] + dv[1]);
+      c[0] = (float)(a0 - a1 + a2 - a3);
+      c[1] = (float)(a1 - 2.0 * a2 + 3.0 * a3);
+      c[2] = (float)(a2 - 3.0 * a3);
+      c[3] = (float)a3;
     }
   }
   coul_table.alloc(tab.size(), false);
@@ -546,21 +552,36 @@ void ForceField::plan_fft() {
   pass_c2r = plan_lines(n2 / 2, n0 * n1, target, false);
   pass_b = plan_axis(n1, n0, nch, target);
   pass_a = plan_axis(n0, n1, nch, target, 2);
-  if (pass_a.p2) {
+  slab = FftSlab{};
+  if (n0 == n1 && n1 == n2 && (n0 == 128 || n0 == 64)) {
+    const size_t nc = n2 / 2;
+    const size_t lds_fwd = 8 * (nc + 1) * (size_t)(n1 + 1);
+    if (lds_fwd <= c.max_lds && fft_slab_ok) {
+      slab.on = true;
+      slab.fwd = n0 == 128 ? K_hmd_fft_slab_fwd_128 : K_hmd_fft_slab_fwd_64;
+      slab.solve = n0 == 128 ? K_hmd_fft_slab_solve_128 : K_hmd_fft_slab_solve_64;
+      slab.inv = n0 == 128 ? K_hmd_fft_slab_inv_128 : K_hmd_fft_slab_inv_64;
+      slab.lds_fwd = lds_fwd;
+      slab.lds_solve = 8 * (size_t)n0 * nch;
+    }
+  }
+  if (slab.on) {
+    partial_blocks = n1;
+  } else if (pass_a.p2) {
     partial_blocks = pass_a.grid;
   } else {
     const uint32_t La = lines_per_block(n0);
     partial_blocks = ceil_div((size_t)n1 * nch, La);
   }
   partials.alloc(2 * (size_t)partial_blocks);
-  if (pass_a.p2) gv.alloc(2 * (size_t)n0 * n1 * nch, false);
+  if (pass_a.p2 || slab.on) gv.alloc(2 * (size_t)n0 * n1 * nch, false);
   compute_gv_table();
 }
 
 // (G, vf) influence table of the power-of-two a-axis pass for the current
 // box (rebuilt when the box changes).
 void ForceField::compute_gv_table() {
-  if (pass_a.p2) {
+  if (pass_a.p2 || slab.on) {
     const uint32_t n0 = grid[0], n1 = grid[1], n2 = grid[2], nch = n2 / 2 + 1;
     const size_t m_c = (size_t)n0 * n1 * nch;
     FftParams g{};
@@ -829,6 +850,25 @@ void ForceField::spme_forces(bool fev) {
     f.n_stages = (uint32_t)r.size();
     for (size_t k = 0; k < r.size(); ++k) f.radix[k] = r[k];
   };
+  if (slab.on) {
+    // slab path: c-axis R2C + b forward per a; a forward * G * a inverse per
+    // b; b inverse + c-axis C2R per a
+    FftParams q = f;
+    q.twiddle = tw_c.ptr;
+    q.twiddle_r2c = tw_r2c.ptr;
+    q.twiddle2 = tw_b.ptr;
+    q.data = rho.ptr;
+    q.out = mesh_c.ptr;
+    launch(c, slab.fwd, n0, 1024, slab.lds_fwd, q);
+    FftParams qa = f;
+    qa.data = mesh_c.ptr;
+    qa.twiddle = tw_a.ptr;
+    qa.partials = partials.ptr;
+    launch(c, slab.solve, n1, 1024, slab.lds_solve, qa);
+    q.data = mesh_c.ptr;
+    q.out = V.ptr;
+    launch(c, slab.inv, n0, 1024, slab.lds_fwd, q);
+  } else {
   // 1. c axis, R2C
   {
     FftParams q = f;
@@ -921,6 +961,7 @@ void ForceField::spme_forces(bool fev) {
       launch(c, K_hmd_fft_c2r_c, ceil_div(q.n_lines, q.lines_per_block), 256,
              2 * (size_t)q.lines_per_block * (q.n + 1) * 8, q);
     }
+  }
   }
   c.mark("spme_fft_solve");
   SpmeReduceParams rp{partials.ptr, partial_blocks, spme_out.ptr};

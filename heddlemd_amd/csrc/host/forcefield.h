@@ -63,6 +63,9 @@ struct ForceField {
 #define HMD_FFT_BLOCKS_PER_CU 3
 #endif
   uint32_t fft_blocks_per_cu = HMD_FFT_BLOCKS_PER_CU;
+  // the slab FFT path where the mesh allows it (off by default: faster alone,
+  // but its one-workgroup-per-CU kernels co-run worse beside the pair kernel)
+  bool fft_slab_ok = false;
   // SPME enqueued on a second stream beside the pair kernel (fork / join)
   uint32_t spme_stream = 1;
   hipStream_t side = nullptr;
@@ -124,6 +127,14 @@ struct ForceField {
     size_t shmem = 0;
   };
   FftPass pass_r2c, pass_c2r, pass_b, pass_a;
+  // Slab path (128^3 and 64^3 meshes): three kernels in place of the five
+  // passes when a (b, c) slab fits in one workgroup's LDS.
+  struct FftSlab {
+    bool on = false;
+    KernelId fwd = K_COUNT, solve = K_COUNT, inv = K_COUNT;
+    size_t lds_fwd = 0, lds_solve = 0;
+  };
+  FftSlab slab;
   double u_self_total = 0.0;
 
   // ---- learned correction of the reciprocal forces (optional)

@@ -34,6 +34,8 @@
   X(hmd_fft_axis_64) X(hmd_fft_r2c_64) X(hmd_fft_c2r_64)                     \
   X(hmd_fft_axis_128) X(hmd_fft_r2c_128) X(hmd_fft_c2r_128)                  \
   X(hmd_fft_axis_256) X(hmd_fft_r2c_256) X(hmd_fft_c2r_256)                  \
+  X(hmd_fft_slab_fwd_128) X(hmd_fft_slab_solve_128) X(hmd_fft_slab_inv_128)   \
+  X(hmd_fft_slab_fwd_64) X(hmd_fft_slab_solve_64) X(hmd_fft_slab_inv_64)      \
   X(hmd_spme_gv_table)                                                        \
   X(hmd_vv_kick_drift) X(hmd_vv_kick)                                        \
   X(hmd_settle_snapshot) X(hmd_settle_positions) X(hmd_settle_velocities)    \
@@ -83,6 +85,7 @@ struct LaunchRec {
 struct Context {
   int device = 0;
   int num_cus = 0;
+  size_t max_lds = 65536;   // LDS bytes one workgroup may use
   std::string device_name;
   hipStream_t stream = nullptr;
   bool owns_stream = true;   // false: the caller's stream (hmd_ctx_create_on_stream)

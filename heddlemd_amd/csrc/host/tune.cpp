@@ -52,6 +52,7 @@ void ForceField::get_tuning(hmd_tuning* t) const {
     t->fft_blocks_per_cu = fft_blocks_per_cu;
     for (int k = 0; k < 3; ++k) t->brick[k] = brick[k];
     t->spme_stream = spme_stream;
+    t->fft_slab = fft_slab_ok ? 1u : 0u;
   }
 }
 
@@ -63,14 +64,16 @@ void ForceField::set_tuning(const hmd_tuning& t) {
       throw Error(HMD_ERR_INVALID, "fft_blocks_per_cu must be 1..64");
     check_brick(t.brick);
     if (t.spme_stream > 1) throw Error(HMD_ERR_INVALID, "spme_stream must be 0 or 1");
+    if (t.fft_slab > 1) throw Error(HMD_ERR_INVALID, "fft_slab must be 0 or 1");
   }
   check(hipStreamSynchronize(c.stream), "sync");   // nothing in flight uses the old buffers
   pair_blocks_per_cu = t.pair_blocks_per_cu;
   if (spme_on) {
     spme_stream = t.spme_stream;
     make_side_stream();
-    if (t.fft_blocks_per_cu != fft_blocks_per_cu) {
+    if (t.fft_blocks_per_cu != fft_blocks_per_cu || (t.fft_slab != 0) != fft_slab_ok) {
       fft_blocks_per_cu = t.fft_blocks_per_cu;
+      fft_slab_ok = t.fft_slab != 0;
       plan_fft();
     }
     if (t.brick[0] != brick[0] || t.brick[1] != brick[1] || t.brick[2] != brick[2]) {

@@ -398,10 +398,11 @@ __device__ inline void hmd_dft(float2* v) {
   else hmd_dft16<SG>(v);
 }
 
-// In-place transform of the W columns of tile[k * W + w] (k < N1 N2);
-// requires W * N2 <= 2 * blockDim.x.  tw[m] = e^{-2 pi i m / n}.
+// In-place transform of the W columns of tile[k * S + w] (k < N1 N2, row
+// stride S >= W); requires W * N2 <= 2 * blockDim.x.
+// tw[m] = e^{-2 pi i m / n}.
 template <int N1, int N2, int SG>
-__device__ inline void hmd_fft_cols(float2* tile, uint32_t W, const float2* tw) {
+__device__ inline void hmd_fft_cols_s(float2* tile, uint32_t W, uint32_t S, const float2* tw) {
   constexpr uint32_t N = (uint32_t)(N1 * N2);
   float2 v[2][N1];
   const uint32_t ntask1 = W * (uint32_t)N2;
@@ -411,7 +412,7 @@ __device__ inline void hmd_fft_cols(float2* tile, uint32_t W, const float2* tw) 
     if (task < ntask1) {
       const uint32_t w = task % W, n2 = task / W;
 #pragma unroll
-      for (int n1 = 0; n1 < N1; ++n1) v[a][n1] = tile[((uint32_t)N2 * n1 + n2) * W + w];
+      for (int n1 = 0; n1 < N1; ++n1) v[a][n1] = tile[((uint32_t)N2 * n1 + n2) * S + w];
       hmd_dft<N1, SG>(v[a]);
 #pragma unroll
       for (int k1 = 1; k1 < N1; ++k1) {
@@ -428,7 +429,7 @@ __device__ inline void hmd_fft_cols(float2* tile, uint32_t W, const float2* tw) 
     if (task < ntask1) {
       const uint32_t w = task % W, n2 = task / W;
 #pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) tile[((uint32_t)N1 * n2 + k1) * W + w] = v[a][k1];
+      for (int k1 = 0; k1 < N1; ++k1) tile[((uint32_t)N1 * n2 + k1) * S + w] = v[a][k1];
     }
   }
   __syncthreads();
@@ -437,12 +438,28 @@ __device__ inline void hmd_fft_cols(float2* tile, uint32_t W, const float2* tw) 
     const uint32_t w = task % W, k1 = task / W;
     float2 u[N2];
 #pragma unroll
-    for (int n2 = 0; n2 < N2; ++n2) u[n2] = tile[((uint32_t)N1 * n2 + k1) * W + w];
+    for (int n2 = 0; n2 < N2; ++n2) u[n2] = tile[((uint32_t)N1 * n2 + k1) * S + w];
     hmd_dft<N2, SG>(u);
 #pragma unroll
-    for (int k2 = 0; k2 < N2; ++k2) tile[(k1 + (uint32_t)N1 * k2) * W + w] = u[k2];
+    for (int k2 = 0; k2 < N2; ++k2) tile[(k1 + (uint32_t)N1 * k2) * S + w] = u[k2];
   }
   __syncthreads();
+}
+
+// The same with the rows packed (S = W).
+template <int N1, int N2, int SG>
+__device__ inline void hmd_fft_cols(float2* tile, uint32_t W, const float2* tw) {
+  hmd_fft_cols_s<N1, N2, SG>(tile, W, W, tw);
+}
+
+// The N twiddles of a transform copied into LDS, where the butterflies read
+// them (a global read per twiddle put an L2 round trip into every stage);
+// visible after the caller's next __syncthreads.
+template <uint32_t N>
+__device__ inline const float2* hmd_lds_twiddles(float2* s_tw, const float* tw) {
+  for (uint32_t k = threadIdx.x; k < N; k += blockDim.x)
+    s_tw[k] = reinterpret_cast<const float2*>(tw)[k];
+  return s_tw;
 }
 
 // Tile fill with the loads of HMD_FFT_BATCH elements per thread issued before
@@ -481,6 +498,8 @@ __device__ inline void hmd_fft_axis_slab(const FftParams& p, const uint32_t hmd_
   extern __shared__ float2 s_tile[];
   __shared__ double s_red[2][HMD_FFT_THREADS];
   constexpr uint32_t N = (uint32_t)(N1 * N2);
+  __shared__ float2 s_tw[N];
+  const float2* tw = hmd_lds_twiddles<N>(s_tw, p.twiddle);
   const uint32_t W = p.lines_per_block;
   const uint32_t n_tiles = (p.line_minor + W - 1) / W;
   const uint32_t outer = hmd_bx / n_tiles;
@@ -508,7 +527,6 @@ __device__ inline void hmd_fft_axis_slab(const FftParams& p, const uint32_t hmd_
         [](uint32_t t) { return t; });
   }
   __syncthreads();
-  const float2* tw = reinterpret_cast<const float2*>(p.twiddle);
   if (p.sign < 0)
     hmd_fft_cols<N1, N2, -1>(s_tile, W, tw);
   else
@@ -556,6 +574,8 @@ __device__ inline void hmd_fft_r2c_slab(const FftParams& p, const uint32_t hmd_b
     const uint32_t hmd_gx) {
   extern __shared__ float2 s_tile[];
   constexpr uint32_t N = (uint32_t)(N1 * N2);
+  __shared__ float2 s_tw[N];
+  const float2* tw = hmd_lds_twiddles<N>(s_tw, p.twiddle);
   const uint32_t W = p.lines_per_block;
   const uint32_t l0 = hmd_bx * W;
   const uint32_t Wc = min(W, p.n_lines - l0);
@@ -571,10 +591,11 @@ __device__ inline void hmd_fft_r2c_slab(const FftParams& p, const uint32_t hmd_b
         return m * W + w;
       });
   __syncthreads();
-  hmd_fft_cols<N1, N2, -1>(s_tile, W, reinterpret_cast<const float2*>(p.twiddle));
+  __shared__ float2 s_wr[N + 1];
+  const float2* wr = hmd_lds_twiddles<N + 1>(s_wr, p.twiddle_r2c);
+  hmd_fft_cols<N1, N2, -1>(s_tile, W, tw);   // ends with __syncthreads
   const uint32_t nch = N + 1;
   float2* out = reinterpret_cast<float2*>(p.out);
-  const float2* wr = reinterpret_cast<const float2*>(p.twiddle_r2c);
   for (uint32_t t = threadIdx.x; t < Wc * nch; t += blockDim.x) {
     const uint32_t w = t / nch, m = t - w * nch;
     const float2 zk = s_tile[(m & (N - 1u)) * W + w];
@@ -591,6 +612,8 @@ __device__ inline void hmd_fft_c2r_slab(const FftParams& p, const uint32_t hmd_b
     const uint32_t hmd_gx) {
   extern __shared__ float2 s_tile[];
   constexpr uint32_t N = (uint32_t)(N1 * N2);
+  __shared__ float2 s_tw[N];
+  const float2* tw = hmd_lds_twiddles<N>(s_tw, p.twiddle);
   const uint32_t W = p.lines_per_block;
   const uint32_t l0 = hmd_bx * W;
   const uint32_t Wc = min(W, p.n_lines - l0);
@@ -616,7 +639,7 @@ __device__ inline void hmd_fft_c2r_slab(const FftParams& p, const uint32_t hmd_b
         return m * W + w;
       });
   __syncthreads();
-  hmd_fft_cols<N1, N2, +1>(s_tile, W, reinterpret_cast<const float2*>(p.twiddle));
+  hmd_fft_cols<N1, N2, +1>(s_tile, W, tw);
   float2* out = reinterpret_cast<float2*>(p.out);
   for (uint32_t t = threadIdx.x; t < Wc * N; t += blockDim.x) {
     const uint32_t w = t / N, m = t - w * N;
@@ -645,3 +668,256 @@ HMD_FFT_P2(32, 8, 4)
 HMD_FFT_P2(64, 8, 8)
 HMD_FFT_P2(128, 16, 8)
 HMD_FFT_P2(256, 16, 16)
+
+// ---------------------------------------------------------------------------
+// Slab path, for meshes whose (b, c) slab of the complex mesh fits in LDS
+// (128^3: 66 KiB, transposed, of gfx950's 160 KiB per workgroup): three
+// kernels instead of five passes, each reading and writing the mesh once.
+//   slab_fwd  (one workgroup per a): the real rows of the slab, transposed
+//             into LDS as N_c = n2/2 complex values per row; the c-axis FFT
+//             and the real -> half-spectrum split; the b-axis FFT of the
+//             n2/2 + 1 columns; the complex slab written out;
+//   slab_solve (one workgroup per b): the (a, c) plane, the a-axis FFT, the
+//             influence multiply with its fixed-order virial / energy
+//             partial (one per workgroup), the inverse a-axis FFT;
+//   slab_inv  (one workgroup per a): inverse b-axis FFT, the half-spectrum
+//             -> complex pre-twiddle, inverse c-axis FFT, real rows out.
+// The arithmetic per line is that of the five-pass path (same butterflies,
+// twiddles and split formulas); only the grouping of lines into workgroups
+// differs.  Rows of the transposed slab are padded by one element.
+
+// Sums of v over the block in a fixed order: wave butterflies, then the
+// waves' results in wave order (the result is valid in thread 0).
+__device__ inline double hmd_block_sum_fixed(double v, double* s_w) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, HMD_WAVE);
+  const uint32_t wv = threadIdx.x / HMD_WAVE;
+  if ((threadIdx.x & (HMD_WAVE - 1)) == 0) s_w[wv] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (uint32_t k = 0; k < blockDim.x / HMD_WAVE; ++k) t += s_w[k];
+  __syncthreads();
+  return t;
+}
+
+// In-place transform of R contiguous rows tile[r * S + j] (j < N1 N2).
+template <int N1, int N2, int SG>
+__device__ inline void hmd_fft_rows_s(float2* tile, uint32_t R, uint32_t S, const float2* tw) {
+  constexpr uint32_t N = (uint32_t)(N1 * N2);
+  const uint32_t ntask1 = R * (uint32_t)N2;
+  for (uint32_t t0 = 0; t0 < ntask1; t0 += 2 * blockDim.x) {
+    float2 v[2][N1];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const uint32_t task = t0 + threadIdx.x + (uint32_t)a * blockDim.x;
+      if (task < ntask1) {
+        const uint32_t r = task / (uint32_t)N2, n2 = task - r * (uint32_t)N2;
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) v[a][n1] = tile[r * S + (uint32_t)N2 * n1 + n2];
+        hmd_dft<N1, SG>(v[a]);
+#pragma unroll
+        for (int k1 = 1; k1 < N1; ++k1) {
+          float2 t = tw[(n2 * (uint32_t)k1) & (N - 1u)];
+          if (SG > 0) t = cconj(t);
+          v[a][k1] = cmul(v[a][k1], t);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const uint32_t task = t0 + threadIdx.x + (uint32_t)a * blockDim.x;
+      if (task < ntask1) {
+        const uint32_t r = task / (uint32_t)N2, n2 = task - r * (uint32_t)N2;
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) tile[r * S + (uint32_t)N1 * n2 + k1] = v[a][k1];
+      }
+    }
+    __syncthreads();
+  }
+  const uint32_t ntask2 = R * (uint32_t)N1;
+  for (uint32_t task = threadIdx.x; task < ntask2; task += blockDim.x) {
+    const uint32_t r = task / (uint32_t)N1, k1 = task - r * (uint32_t)N1;
+    float2 u[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) u[n2] = tile[r * S + (uint32_t)N1 * n2 + k1];
+    hmd_dft<N2, SG>(u);
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) tile[r * S + k1 + (uint32_t)N1 * k2] = u[k2];
+  }
+  __syncthreads();
+}
+
+// One LDS buffer T of NC + 1 rows of NB (+1 pad) points: row m holds point m
+// of every c line, then, after the split, row k holds spectrum point k of
+// every b line, so the b-axis transform runs along the rows.
+template <int B1, int B2, int C1, int C2>
+__device__ inline void hmd_fft_slab_fwd(const FftParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  extern __shared__ float2 s_lds[];
+  constexpr uint32_t NB = (uint32_t)(B1 * B2), NC = (uint32_t)(C1 * C2), NCH = NC + 1;
+  constexpr uint32_t TS = NB + 1;
+  __shared__ float2 s_twc[NC], s_twb[NB], s_wr[NC + 1];
+  const float2* twc = hmd_lds_twiddles<NC>(s_twc, p.twiddle);
+  const float2* twb = hmd_lds_twiddles<NB>(s_twb, p.twiddle2);
+  const float2* wr = hmd_lds_twiddles<NC + 1>(s_wr, p.twiddle_r2c);
+  float2* T = s_lds;
+  const uint32_t a = hmd_bx;
+  const float2* in = reinterpret_cast<const float2*>(p.data) + (size_t)a * NB * NC;
+  hmd_fill_tile(T, NB * NC, [&](uint32_t t) { return in[t]; },
+                [&](uint32_t t) {
+                  const uint32_t b = t / NC, m = t - b * NC;
+                  return m * TS + b;
+                });
+  __syncthreads();
+  hmd_fft_cols_s<C1, C2, -1>(T, NB, TS, twc);
+  // real -> half spectrum in place: pair j produces points j and NC - j (and
+  // point NC from j = 0) from rows j and NC - j only
+  for (uint32_t t = threadIdx.x; t < (NC / 2 + 1) * NB; t += blockDim.x) {
+    const uint32_t j = t / NB, b = t - j * NB;
+    const uint32_t jp = (NC - j) & (NC - 1u);           // row of the partner point
+    const float2 zj = T[j * TS + b], zp = T[jp * TS + b];
+    const float2 zc = cconj(zp);
+    const float2 e = make_float2(0.5f * (zj.x + zc.x), 0.5f * (zj.y + zc.y));
+    const float2 o = cmul_si(make_float2(0.5f * (zj.x - zc.x), 0.5f * (zj.y - zc.y)), -1);
+    const float2 xj = cadd(e, cmul(wr[j], o));
+    // point NC - j: zk = zp, zc = conj(zj)
+    const float2 zc2 = cconj(zj);
+    const float2 e2 = make_float2(0.5f * (zp.x + zc2.x), 0.5f * (zp.y + zc2.y));
+    const float2 o2 = cmul_si(make_float2(0.5f * (zp.x - zc2.x), 0.5f * (zp.y - zc2.y)), -1);
+    const float2 xp = cadd(e2, cmul(wr[NC - j], o2));
+    T[j * TS + b] = xj;
+    if (j != NC / 2) T[(NC - j) * TS + b] = xp;
+  }
+  __syncthreads();
+  hmd_fft_rows_s<B1, B2, -1>(T, NCH, TS, twb);
+  float2* out = reinterpret_cast<float2*>(p.out) + (size_t)a * NB * NCH;
+  for (uint32_t t = threadIdx.x; t < NB * NCH; t += blockDim.x) {
+    const uint32_t b = t / NCH, k = t - b * NCH;
+    out[t] = T[k * TS + b];
+  }
+}
+
+template <int A1, int A2>
+__device__ inline void hmd_fft_slab_solve(const FftParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  extern __shared__ float2 s_lds[];
+  __shared__ double s_w[2][1024 / HMD_WAVE];
+  constexpr uint32_t NA = (uint32_t)(A1 * A2);
+  __shared__ float2 s_twa[NA];
+  const float2* tw = hmd_lds_twiddles<NA>(s_twa, p.twiddle);
+  const uint32_t nch = p.nch, n1 = p.n1, b = hmd_bx;
+  float2* Al = s_lds;              // [NA][nch]: the (a, c) plane at this b
+  float2* mesh = reinterpret_cast<float2*>(p.data);
+  hmd_fill_tile(Al, NA * nch,
+                [&](uint32_t t) {
+                  const uint32_t a = t / nch, k = t - a * nch;
+                  return mesh[((size_t)a * n1 + b) * nch + k];
+                },
+                [](uint32_t t) { return t; });
+  __syncthreads();
+  hmd_fft_cols_s<A1, A2, -1>(Al, nch, nch, tw);
+  const float2* gv = reinterpret_cast<const float2*>(p.gv);
+  double vir = 0.0, ene = 0.0;
+  // (G, vf) read HMD_FFT_BATCH at a time per thread, every load in flight
+  // before the first is used
+  const uint32_t total = NA * nch;
+  for (uint32_t t0 = threadIdx.x; t0 < total; t0 += HMD_FFT_BATCH * blockDim.x) {
+    float2 g[HMD_FFT_BATCH];
+#pragma unroll
+    for (int u = 0; u < HMD_FFT_BATCH; ++u) {
+      const uint32_t t = t0 + (uint32_t)u * blockDim.x;
+      const uint32_t a = t / nch, k = t - a * nch;
+      g[u] = t < total ? gv[((size_t)a * n1 + b) * nch + k] : make_float2(0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int u = 0; u < HMD_FFT_BATCH; ++u) {
+      const uint32_t t = t0 + (uint32_t)u * blockDim.x;
+      if (t >= total) continue;
+      const uint32_t k = t % nch;
+      const float2 x = Al[t];
+      const double hw = (k == 0u || (p.n2 % 2u == 0u && 2u * k == p.n2)) ? 1.0 : 2.0;
+      const double r2 = (double)x.x * x.x + (double)x.y * x.y;
+      vir += hw * (double)g[u].y * r2;
+      ene += hw * (double)g[u].x * r2;
+      Al[t] = make_float2(g[u].x * x.x, g[u].x * x.y);
+    }
+  }
+  const double vs = hmd_block_sum_fixed(vir, s_w[0]);
+  const double es = hmd_block_sum_fixed(ene, s_w[1]);
+  if (threadIdx.x == 0) {
+    p.partials[2 * b] = vs;
+    p.partials[2 * b + 1] = es;
+  }
+  hmd_fft_cols_s<A1, A2, +1>(Al, nch, nch, tw);
+  for (uint32_t t = threadIdx.x; t < NA * nch; t += blockDim.x) {
+    const uint32_t a = t / nch, k = t - a * nch;
+    mesh[((size_t)a * n1 + b) * nch + k] = Al[t];
+  }
+}
+
+template <int B1, int B2, int C1, int C2>
+__device__ inline void hmd_fft_slab_inv(const FftParams& p, const uint32_t hmd_bx,
+    const uint32_t hmd_gx) {
+  extern __shared__ float2 s_lds[];
+  constexpr uint32_t NB = (uint32_t)(B1 * B2), NC = (uint32_t)(C1 * C2), NCH = NC + 1;
+  constexpr uint32_t TS = NB + 1;
+  __shared__ float2 s_twc[NC], s_twb[NB], s_wr[NC + 1];
+  const float2* twc = hmd_lds_twiddles<NC>(s_twc, p.twiddle);
+  const float2* twb = hmd_lds_twiddles<NB>(s_twb, p.twiddle2);
+  const float2* wr = hmd_lds_twiddles<NC + 1>(s_wr, p.twiddle_r2c);
+  float2* T = s_lds;
+  const uint32_t a = hmd_bx;
+  const float2* in = reinterpret_cast<const float2*>(p.data) + (size_t)a * NB * NCH;
+  hmd_fill_tile(T, NB * NCH, [&](uint32_t t) { return in[t]; },
+                [&](uint32_t t) {
+                  const uint32_t b = t / NCH, k = t - b * NCH;
+                  return k * TS + b;
+                });
+  __syncthreads();
+  hmd_fft_rows_s<B1, B2, +1>(T, NCH, TS, twb);
+  // half spectrum -> complex c lines in place: pair j produces points j and
+  // NC - j from rows j and NC - j (row NC is read by j = 0)
+  for (uint32_t t = threadIdx.x; t < (NC / 2 + 1) * NB; t += blockDim.x) {
+    const uint32_t j = t / NB, b = t - j * NB;
+    const float2 xj = T[j * TS + b], xp = T[(NC - j) * TS + b];
+    const float2 xc = cconj(xp);
+    const float2 ee = cadd(xj, xc);
+    const float2 oo = cmul(csub(xj, xc), cconj(wr[j]));
+    const float2 zj = cadd(ee, cmul_si(oo, +1));
+    // point NC - j: xk = xp, xc = conj(xj)
+    const float2 xc2 = cconj(xj);
+    const float2 ee2 = cadd(xp, xc2);
+    const float2 oo2 = cmul(csub(xp, xc2), cconj(wr[NC - j]));
+    const float2 zp = cadd(ee2, cmul_si(oo2, +1));
+    T[j * TS + b] = zj;
+    if (j != 0 && j != NC / 2) T[(NC - j) * TS + b] = zp;
+  }
+  __syncthreads();
+  hmd_fft_cols_s<C1, C2, +1>(T, NB, TS, twc);
+  float2* out = reinterpret_cast<float2*>(p.out) + (size_t)a * NB * NC;
+  for (uint32_t t = threadIdx.x; t < NB * NC; t += blockDim.x) {
+    const uint32_t b = t / NC, m = t - b * NC;
+    out[t] = T[m * TS + b];
+  }
+}
+
+#define HMD_FFT_SLAB(TAG, A1, A2, B1, B2, C1, C2)                                   \
+  __device__ __forceinline__ void hmd_fft_slab_fwd_##TAG##_body(                   \
+      const FftParams& p, const uint32_t hmd_bx, const uint32_t hmd_gx) {           \
+    hmd_fft_slab_fwd<B1, B2, C1, C2>(p, hmd_bx, hmd_gx);                            \
+  }                                                                                 \
+  __device__ __forceinline__ void hmd_fft_slab_solve_##TAG##_body(                 \
+      const FftParams& p, const uint32_t hmd_bx, const uint32_t hmd_gx) {           \
+    hmd_fft_slab_solve<A1, A2>(p, hmd_bx, hmd_gx);                                  \
+  }                                                                                 \
+  __device__ __forceinline__ void hmd_fft_slab_inv_##TAG##_body(                   \
+      const FftParams& p, const uint32_t hmd_bx, const uint32_t hmd_gx) {           \
+    hmd_fft_slab_inv<B1, B2, C1, C2>(p, hmd_bx, hmd_gx);                            \
+  }                                                                                 \
+  HMD_KERNEL(hmd_fft_slab_fwd_##TAG, FftParams, 1024)                               \
+  HMD_KERNEL(hmd_fft_slab_solve_##TAG, FftParams, 1024)                             \
+  HMD_KERNEL(hmd_fft_slab_inv_##TAG, FftParams, 1024)
+HMD_FFT_SLAB(128, 16, 8, 16, 8, 8, 8)
+HMD_FFT_SLAB(64, 8, 8, 8, 8, 8, 4)

@@ -56,15 +56,17 @@ __device__ inline void hmd_cart_to_frac(float x, float y, float z,
   sa = (x - sb * L.xy - sc * L.xz) / L.lx;
 }
 
-// Bin of r^2 in the real-space Ewald table and the position s in [0, 1)
-// inside it, read straight from the f32 bits of r^2: the exponent and the
-// top HMD_COUL_BITS mantissa bits give the bin (2^-HMD_COUL_BITS of an octave
-// of r^2 wide), the next mantissa bits s.  r^2 is clamped to the table
-// ([u_lo, u_hi] as bits; NaN and values past the table clamp to its last
-// bin), so the index is always inside it.  No square root or reciprocal.
-__device__ inline uint32_t hmd_coul_bin(float r2, uint32_t u_lo, uint32_t u_hi, float& s) {
+// Bin of r^2 in the real-space Ewald table and the position t = 1 + s in
+// [1, 2) inside it (s in [0, 1) across the bin), read straight from the f32
+// bits of r^2: the exponent and the top HMD_COUL_BITS mantissa bits give the
+// bin (2^-HMD_COUL_BITS of an octave of r^2 wide), the next mantissa bits,
+// under the exponent of 1.0, give t.  The table's cubics are in t, so s is
+// never formed.  r^2 is clamped to the table ([u_lo, u_hi] as bits; NaN and
+// values past the table clamp to its last bin), so the index is always
+// inside it.  No square root or reciprocal.
+__device__ inline uint32_t hmd_coul_bin(float r2, uint32_t u_lo, uint32_t u_hi, float& t) {
   const uint32_t u = min(max(__float_as_uint(r2), u_lo), u_hi);
-  s = __uint_as_float(((u << HMD_COUL_BITS) & 0x007FFFFFu) | 0x3F800000u) - 1.0f;
+  t = __uint_as_float(((u << HMD_COUL_BITS) & 0x007FFFFFu) | 0x3F800000u);
   return (u >> (23u - HMD_COUL_BITS)) - (u_lo >> (23u - HMD_COUL_BITS));
 }
 

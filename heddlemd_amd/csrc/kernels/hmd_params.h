@@ -312,6 +312,7 @@ struct FftParams {
   uint64_t major_stride, minor_stride;  // complex elements
   const float* twiddle;        // float2[n] e^{-2 pi i m/n}
   const float* twiddle_r2c;    // float2[nc] e^{-2 pi i m/nc} (c-axis only)
+  const float* twiddle2;       // slab passes: float2[n1] e^{-2 pi i m/n1} (b axis)
   uint32_t radix[8];
   uint32_t n_stages;
   int32_t sign;                // -1 forward, +1 inverse

@@ -126,7 +126,7 @@ __device__ inline void hmd_pair_isteps(const PairParams& p, const float4* s_lj,
       const float r2s = PLAIN ? r2 : (ok ? r2 : 1.0f);
       float factor = 0.0f, energy = 0.0f;
       if (COUL) {
-        // tabulated erfc kernel: octave-table bin of r^2, cubic in its s
+        // tabulated erfc kernel: octave-table bin of r^2, cubic in its t
         const float qq = a.w * pj.w;
         float t;
         const uint32_t b = hmd_coul_bin(r2s, p.coul_u_lo, p.coul_u_hi, t);

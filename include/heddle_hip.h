@@ -439,6 +439,8 @@ typedef struct hmd_tuning {
   uint32_t fft_blocks_per_cu;  /* 0 without SPME */
   uint32_t brick[3];           /* 0 without SPME */
   uint32_t spme_stream;        /* 0 without SPME */
+  uint32_t fft_slab;           /* 1: three-kernel slab FFT where the mesh allows
+                                  (128^3, 64^3); 0 (default): five line passes */
   uint32_t n_candidates;       /* autotune: configurations timed */
   double us_initial;           /* autotune: us per evaluation with the shapes at entry */
   double us_tuned;             /* autotune: us per evaluation with the chosen shapes */

@@ -170,3 +170,36 @@ def test_spme_mesh_shapes_match_oracle(gpu_ctx, mesh):
     assert err_u <= ENERGY_TOL
     for h in (integ, field, part):
         h.close()
+
+
+@pytest.mark.parametrize("mesh", [(64, 64, 64), (128, 128, 128)])
+def test_slab_fft_matches_oracle_and_line_passes(gpu_ctx, mesh):
+    """The three-kernel slab FFT (tuning fft_slab = 1) on the meshes it
+    serves: reciprocal forces, U_recip and W_recip against the oracle to the
+    same bars as the five line passes, and against the line passes
+    themselves (the same per-line arithmetic, lines grouped differently)."""
+    sysw, ff, part, field, integ = water_case(gpu_ctx, mesh=mesh)
+    try:
+        out = {}
+        for slab in (0, 1):
+            field.set_tuning(dict(field.tuning(), fft_slab=slab))
+            assert field.tuning()["fft_slab"] == slab
+            field.compute(1)
+            out[slab] = (field.spme_forces(), field.spme_scalars())
+        posq = part.download()["posq"]
+        u_rec, f_rec, _, w_rec = oracle.spme_recip(posq[:, :3], posq[:, 3], sysw["lattice"], mesh,
+                                                   4, ff["coulomb"]["alpha"])
+        f1, s1 = out[1]
+        f0, s0 = out[0]
+        err_f = rel_rms(f1, f_rec)
+        err_u = abs(s1["u_recip"] - u_rec) / abs(u_rec)
+        err_w = abs(s1["w_recip"] - w_rec) / abs(w_rec)
+        d_f = rel_rms(f1, f0)
+        print(f"mesh {mesh} slab: recip force {err_f:.2e}, U {err_u:.2e}, W {err_w:.2e}; "
+              f"vs line passes {d_f:.2e}")
+        assert err_f <= FORCE_TOL and err_u <= ENERGY_TOL and err_w <= ENERGY_TOL
+        assert d_f <= 1e-6
+        assert abs(s1["u_recip"] - s0["u_recip"]) <= 1e-9 * abs(s0["u_recip"])
+    finally:
+        for h in (integ, field, part):
+            h.close()
