@@ -50,7 +50,7 @@ PEAK_BF16_TFLOPS = 2500.0     # MI355X published dense bf16 matrix peak
 FLOP_PER_PAIR = 59
 
 
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "traffic_spc196k_plain128.md")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "counters_spc196k_plain128.md")
 
 
 def profiled_traffic(kernel):
