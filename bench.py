@@ -266,6 +266,14 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only the run the HBM-traffic counters are read from (a calibration "
                          "copy, then five eager force evaluations of the bench box)")
+    ap.add_argument("--nn-spec", action="store_true",
+                    help="only the learned leg's training at its specification (frames "
+                         "every 10 plain steps, batch 8, seed 31, until the held-out error "
+                         "plateaus); prints the learning curve as one JSON line")
+    ap.add_argument("--spec-frames", type=int, default=2000)
+    ap.add_argument("--spec-heldout", type=int, default=20)
+    ap.add_argument("--spec-chunk", type=int, default=50, help="Adam steps between held-out checks")
+    ap.add_argument("--spec-max-steps", type=int, default=2000)
     ap.add_argument("--pair-blocks-per-cu", type=int, default=0,
                     help="pair-kernel workgroups per CU (0: the library default)")
     ap.add_argument("--fft-slab", type=int, default=0, choices=(0, 1),
@@ -284,6 +292,9 @@ def main():
         return
     if args.traffic_probe:
         traffic_probe(api, systems, U)
+        return
+    if args.nn_spec:
+        print(json.dumps(nn_spec(api, systems, U, args)))
         return
 
     t_setup = time.perf_counter()
@@ -467,6 +478,86 @@ def replica_workload(api, systems, U, args, minimize_iter=50):
                    "atoms": n_atoms, "launch": best, "legs": legs},
         "roofline": None, "cpu_baseline": None,
     }
+
+
+def nn_spec(api, systems, U, args):
+    """The learned leg's training at its specification: frames every 10
+    steps of the plain 128^3 run (2,000 by default), Adam with batches of 8
+    frames and seed 31, trained in chunks until the held-out error stops
+    improving by 1% (or a step cap).  Held-out states are taken from the
+    same run after the training frames; their error is the judge's relative
+    RMS force error of the corrected 64^3 forces against the 128^3 ones, next
+    to the 64^3 mesh's own.  Prints one JSON line with the learning curve."""
+    ctx, sysw, ff, part, field, integ = build_system(api, systems, U)
+    run = overflow_safe(integ)
+    integ.capture()
+    run(args.warmup)
+    ff64 = systems.water_forcefield(grid=(64, 64, 64), exclusions=sysw["exclusions"])
+    coarse = api.ForceField(ctx, part, ff64)
+    net = api.CorrectionNet(ctx, 1.0 * U.NM, 1.0, [0, 1], seed=31, lr=1e-3)
+    n_frames = args.spec_frames
+    net.reserve(part.n, n_frames)
+    t0 = time.perf_counter()
+    for k in range(n_frames):
+        run(10)
+        net.record_frame(field, coarse)
+        if (k + 1) % 250 == 0:
+            log(f"{k + 1} frames recorded ({time.perf_counter() - t0:.1f} s)")
+    held = []
+    for _ in range(args.spec_heldout):
+        run(50)
+        held.append(part.download()["posq"].copy())
+    t_rec = time.perf_counter() - t0
+    st = net.dataset_stats()
+    net.set_force_scale(st["target_rms"])
+
+    def heldout(with_net):
+        if with_net:
+            net.attach(coarse)
+        errs = []
+        for posq in held:
+            part.upload(posq=posq)
+            errs.append(api.judge(coarse, field))
+        if with_net:
+            api.CorrectionNet.detach(coarse)
+        return float(np.mean(errs)), float(np.max(errs))
+
+    e_coarse = heldout(False)
+    e0 = heldout(True)
+    log(f"{n_frames} frames in {t_rec:.1f} s; held-out error: 64^3 alone {e_coarse[0]:.4e}, "
+        f"untrained network {e0[0]:.4e}")
+    curve = [{"adam_steps": 0, "heldout_mean": e0[0], "heldout_max": e0[1], "train_loss": None}]
+    done, best, stall = 0, e0[0], 0
+    t0 = time.perf_counter()
+    while done < args.spec_max_steps:
+        k = min(args.spec_chunk, args.spec_max_steps - done)
+        losses = net.train(coarse, k, batch_frames=8)
+        done += k
+        e = heldout(True)
+        curve.append({"adam_steps": done, "heldout_mean": e[0], "heldout_max": e[1],
+                      "train_loss": float(np.mean(losses))})
+        log(f"{done} Adam steps ({time.perf_counter() - t0:.0f} s): train loss "
+            f"{np.mean(losses):.4f}, held-out {e[0]:.4e} (max {e[1]:.4e})")
+        if e[0] < best * 0.99:
+            best, stall = e[0], 0
+        else:
+            stall += 1
+            if stall >= 2:
+                break
+    out = {"report": "learned coarse-mesh correction, training at specification",
+           "workload": "196,608-atom SPC water, plain 128^3 run, corrected 64^3 mesh",
+           "frames": n_frames, "frame_every_steps": 10, "batch_frames": 8, "seed": 31,
+           "lr": 1e-3, "heldout_states": len(held), "heldout_every_steps": 50,
+           "target_rms": st["target_rms"], "fine_recip_rms": st["fine_recip_rms"],
+           "heldout_error_coarse_only": {"mean": e_coarse[0], "max": e_coarse[1]},
+           "curve": curve, "adam_steps": done,
+           "train_seconds": round(time.perf_counter() - t0, 1),
+           "record_seconds": round(t_rec, 1),
+           "best_heldout_mean": min(c["heldout_mean"] for c in curve),
+           "meets_dF_bound_1e-3": min(c["heldout_mean"] for c in curve) <= 1e-3}
+    for h in (net, coarse, integ, field, part, ctx):
+        h.close()
+    return out
 
 
 def nn_leg(api, systems, U, args, ctx, sysw, part, fine, integ_fine, run_fine):

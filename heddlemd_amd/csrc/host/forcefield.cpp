@@ -822,7 +822,7 @@ void ForceField::pair_forces(bool fev) {
 void ForceField::spme_forces(bool fev) {
   const HmdLattice L = p.lattice;
   const uint32_t n0 = grid[0], n1 = grid[1], n2 = grid[2];
-  SpmeBinParams bp{posq_sorted.ptr, sorted_id.ptr, n, L, n0, n1, n2, brick[0], brick[1], brick[2],
+  SpmeBinParams bp{p.posq.ptr, sorted_id.ptr, n, L, n0, n1, n2, brick[0], brick[1], brick[2],
                    nb[0], nb[1], nb[2], atom_bin.ptr, bin_count.ptr, bin_start.ptr,
                    bin_atoms.ptr};
   launch(c, K_hmd_spme_bin, ceil_div(n, 256), 256, 0, bp);

@@ -1097,10 +1097,8 @@ hmd_status hmd_integrator_run_judged(hmd_integrator* w, hmd_forcefield* fine, ui
       if (I.ff == fine->ff) continue;   // reverted: plain fine-mesh steps
       hmd_forcefield nn{I.ff};
       double e = 0.0;
-      hmd_status st = hmd_mlp_judge(&nn, fine, &e);
-      // the fine list overflowing at these positions: it has grown and been
-      // rebuilt here, so the judge is evaluated once more
-      if (st == HMD_ERR_NBL_OVERFLOW) st = hmd_mlp_judge(&nn, fine, &e);
+      // (an overflowing list is grown and the judge evaluated again inside)
+      const hmd_status st = hmd_mlp_judge(&nn, fine, &e);
       if (st != HMD_OK) throw Error(st, hmd_last_error());
       if (judges && k < max_judges) judges[k] = e;
       ++k;

@@ -192,6 +192,20 @@ static void from_soa(const std::vector<float>& soa, uint32_t n, float* rows) {
     for (int d = 0; d < 3; ++d) rows[3 * (size_t)i + d] = soa[(size_t)d * n + i];
 }
 
+// An evaluation at fixed positions (a recorded frame, the judge) whose list
+// overflowed has already had the capacity grown and the list rebuilt by
+// check_status; nothing of it was kept, so it is simply evaluated once more.
+// A second overflow is reported to the caller.
+template <class Body>
+static void evaluate_regrowing(Body&& body) {
+  try {
+    body();
+  } catch (const Error& e) {
+    if (e.code != HMD_ERR_NBL_OVERFLOW) throw;
+    body();
+  }
+}
+
 }  // namespace hmd
 
 struct hmd_mlp {
@@ -367,6 +381,7 @@ hmd_status hmd_mlp_record_frame(hmd_mlp* w, hmd_forcefield* fine, hmd_forcefield
     if (A.n != m.ds_n) throw Error(HMD_ERR_INVALID, "data set reserved for another size");
     if (m.ds_count >= m.ds_cap) throw Error(HMD_ERR_INVALID, "data set is full");
     m.ensure_size(B);
+    evaluate_regrowing([&] {
     A.neighbor_pre_step();
     A.spme_forces(false);
     B.neighbor_pre_step();
@@ -394,6 +409,7 @@ hmd_status hmd_mlp_record_frame(hmd_mlp* w, hmd_forcefield* fine, hmd_forcefield
     m.ds_sum_t2 += s[0];
     m.ds_sum_f2 += s[1];
     ++m.ds_count;
+    });
   });
 }
 
@@ -456,6 +472,7 @@ hmd_status hmd_mlp_judge(hmd_forcefield* nn, hmd_forcefield* fine, double* rel_r
     if (!A.mlp) throw Error(HMD_ERR_INVALID, "no network attached to the first force field");
     Mlp& m = *A.mlp;
     Context& c = A.c;
+    evaluate_regrowing([&] {
     B.compute(0);
     check(hipMemcpyAsync(m.scratch_f.ptr, B.p.force.ptr, 3 * (size_t)B.n * sizeof(float),
                          hipMemcpyDeviceToDevice, c.stream), "D2D");
@@ -477,6 +494,7 @@ hmd_status hmd_mlp_judge(hmd_forcefield* nn, hmd_forcefield* fine, double* rel_r
     A.check_status();
     B.check_status();
     *rel_rms = s[1] > 0.0 ? std::sqrt(s[0] / s[1]) : 0.0;
+    });
   });
 }
 

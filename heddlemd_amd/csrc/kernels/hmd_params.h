@@ -257,8 +257,8 @@ struct SlotAddParams {
 
 // ------------------------------------------------------------- spme ----
 struct SpmeBinParams {
-  const float* posq_sorted;    // neighbour-list (Hilbert) order
-  const uint32_t* sorted_id;   // sorted slot -> canonical id
+  const float* posq;           // canonical order (what the spread and gather read)
+  const uint32_t* sorted_id;   // sorted slot -> canonical id (visiting order)
   uint32_t n;
   HmdLattice L;
   uint32_t n0, n1, n2;         // mesh

@@ -120,7 +120,11 @@ __device__ inline void hmd_spme_setup(float4 a, const HmdLattice& L,
 
 // Bin each atom by the brick holding its primary point.  Atoms are visited
 // in the neighbour list's Hilbert order, so a wave's 64 atoms fall into a few
-// bins: one atomic per (wave, bin) instead of one per atom.
+// bins: one atomic per (wave, bin) instead of one per atom.  The primary
+// point is computed from the same f32 position the spread and the gather
+// read (posq, not the list's image-shifted copy): an atom within a rounding
+// of a box face can wrap to the other face from a shifted copy, and the
+// gather would then look for its stencil outside the brick it is binned in.
 __device__ inline uint32_t hmd_spme_bin_of(const SpmeBinParams& p, float4 pq) {
   float t[3];
   int g[3];
@@ -135,7 +139,7 @@ __device__ __forceinline__ void hmd_spme_bin_body(const SpmeBinParams& p, const 
   const bool valid = s < p.n;
   uint32_t bin = HMD_SENTINEL;
   if (valid) {
-    bin = hmd_spme_bin_of(p, reinterpret_cast<const float4*>(p.posq_sorted)[s]);
+    bin = hmd_spme_bin_of(p, reinterpret_cast<const float4*>(p.posq)[p.sorted_id[s]]);
     p.atom_bin[s] = bin;
   }
   uint64_t pending = __ballot(valid);

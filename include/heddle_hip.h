@@ -535,7 +535,9 @@ hmd_status hmd_mlp_adam_step(hmd_mlp* mlp, const float* grad);
 hmd_status hmd_mlp_attach(hmd_forcefield* ff, hmd_mlp* mlp);
 /* Training frames (device resident): positions and the target
  * F_recip(fine) - F_recip(coarse) of two force fields on the same
- * particles at their current positions. */
+ * particles at their current positions.  A neighbour list that overflows
+ * at these positions is grown and the frame evaluated again;
+ * HMD_ERR_NBL_OVERFLOW is returned only if it overflows twice. */
 hmd_status hmd_mlp_dataset_reserve(hmd_mlp* mlp, uint32_t n_atoms,
                                    uint32_t n_frames);
 hmd_status hmd_mlp_record_frame(hmd_mlp* mlp, hmd_forcefield* fine,
@@ -551,7 +553,8 @@ hmd_status hmd_mlp_train(hmd_mlp* mlp, hmd_forcefield* ff, uint32_t n_steps,
                          uint32_t batch_frames, double* losses);
 /* Fallback judge: total forces of nn_ff (with its network) and fine_ff at
  * the current positions; *rel_rms = ||F_nn - F_fine|| / ||F_fine||.  The
- * particle forces are left as nn_ff's. */
+ * particle forces are left as nn_ff's.  Overflow is handled as in
+ * hmd_mlp_record_frame. */
 hmd_status hmd_mlp_judge(hmd_forcefield* nn_ff, hmd_forcefield* fine_ff,
                          double* rel_rms);
 

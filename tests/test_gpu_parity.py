@@ -177,7 +177,8 @@ def test_slab_fft_matches_oracle_and_line_passes(gpu_ctx, mesh):
     """The three-kernel slab FFT (tuning fft_slab = 1) on the meshes it
     serves: reciprocal forces, U_recip and W_recip against the oracle to the
     same bars as the five line passes, and against the line passes
-    themselves (the same per-line arithmetic, lines grouped differently)."""
+    themselves to f32 rounding (the same transform with its radix stages
+    grouped differently)."""
     sysw, ff, part, field, integ = water_case(gpu_ctx, mesh=mesh)
     try:
         out = {}
@@ -198,8 +199,100 @@ def test_slab_fft_matches_oracle_and_line_passes(gpu_ctx, mesh):
         print(f"mesh {mesh} slab: recip force {err_f:.2e}, U {err_u:.2e}, W {err_w:.2e}; "
               f"vs line passes {d_f:.2e}")
         assert err_f <= FORCE_TOL and err_u <= ENERGY_TOL and err_w <= ENERGY_TOL
-        assert d_f <= 1e-6
-        assert abs(s1["u_recip"] - s0["u_recip"]) <= 1e-9 * abs(s0["u_recip"])
+        # the two transforms round differently: f32 rounding (2^-24) over the
+        # 21 butterfly stages of a 128^3 transform and the solve is ~1e-6 RMS
+        assert d_f <= 3e-6
+        # the energy sums |F(k)|^2 G(k) over the f32 spectra of the two transforms
+        assert abs(s1["u_recip"] - s0["u_recip"]) <= 1e-7 * abs(s0["u_recip"])
+    finally:
+        for h in (integ, field, part):
+            h.close()
+
+
+def _f32_wrap_index(x, lx):
+    """Image index of the device's orthorhombic wrap (f32 divide, floor of
+    s + 1/2), as the kernels evaluate it."""
+    f = np.float32
+    return float(np.floor(f(f(x / lx) + f(0.5))))
+
+
+def _f32_primary_point(x, lx, n):
+    """Mesh point g = floor((s + 1/2) n) mod n of the device's SPME set-up
+    for a coordinate x on an orthorhombic axis of length lx (f32)."""
+    f = np.float32
+    xw = f(x - f(f(_f32_wrap_index(x, lx)) * lx))
+    u = f(f(f(xw / lx) + f(0.5)) * f(n))
+    return int(np.floor(u)) % n
+
+
+def _face_case(lx, n):
+    """A coordinate a within a few f32 steps of a box face and a coordinate b
+    of the same point at the last list build, 0.25 bohr away and one box
+    length over (as after a wrap), such that the list's image-shifted copy
+    of a, a - k_b L, has its primary mesh point on the opposite face from
+    a's own."""
+    f = np.float32
+    for face in (1.0, -1.0):
+        base = f(f(0.5 * face) * lx)
+        for k in range(-64, 65):
+            a = f(base + f(k) * np.spacing(base))
+            for db in (0.25, -0.25):
+                for jump in (1.0, -1.0):
+                    b = f(f(a + f(db)) + f(jump) * lx)
+                    kb = _f32_wrap_index(b, lx)
+                    if kb == _f32_wrap_index(a, lx):
+                        continue
+                    shifted = f(a - f(f(kb) * lx))
+                    g_a, g_s = _f32_primary_point(a, lx, n), _f32_primary_point(shifted, lx, n)
+                    if {g_a, g_s} == {0, n - 1}:
+                        return a, b, g_a, g_s
+    return None
+
+
+def test_spme_atom_within_a_rounding_of_a_box_face(gpu_ctx):
+    """An atom one box length and 0.25 bohr from where it stood at the last
+    list build (a wrap) that now lies within a few f32 steps of a box face,
+    where the list's image-shifted copy of its position wraps to the
+    opposite face (mesh point 0 against n - 1 from the position itself).
+    Binning, spreading and gathering use one primary point, so the forces
+    match the oracle."""
+    sysw, ff, part, field, integ = water_case(gpu_ctx, relax=20)
+    try:
+        f = np.float32
+        lx = f(sysw["lattice"][0])
+        n = ff["spme"]["grid"][0]
+        case = _face_case(lx, n)
+        assert case is not None, "no face case for this box length"
+        a, b, g_a, g_s = case
+        print(f"x now {a!r}, at the build {b!r}: mesh point {g_a} from the position, "
+              f"{g_s} from the shifted copy")
+        posq0 = part.download()["posq"].copy()
+        i = int(np.argmax(posq0[:, 0]))
+        p_build = posq0.copy()
+        p_build[:, 0] = (p_build[:, 0] + f(b - posq0[i, 0])).astype(f)
+        p_build[i, 0] = b
+        part.upload(posq=p_build)
+        field.request_rebuild()
+        field.compute(0)
+        gen = field.stats()["generation"]
+        p_now = p_build.copy()
+        p_now[:, 0] = (p_now[:, 0] + f(a - b)).astype(f)   # 0.25 bohr by minimum image
+        p_now[i, 0] = a
+        part.upload(posq=p_now)
+        field.compute(0)
+        assert field.stats()["generation"] == gen      # the build's images are kept
+        got = part.forces().astype(np.float64)
+        f_pair, _, _ = oracle_pair(sysw, ff, p_now, erfc_mode=1)
+        _, f_rec, _, _ = oracle.spme_recip(p_now[:, :3], p_now[:, 3], sysw["lattice"],
+                                           ff["spme"]["grid"], ff["spme"]["order"],
+                                           ff["coulomb"]["alpha"])
+        ref = f_pair + f_rec
+        err_i = float(np.linalg.norm(got[i] - ref[i]))
+        rms = float(np.sqrt(np.mean(np.sum(ref ** 2, axis=1))))
+        print(f"atom {i}: force error {err_i:.3e} (its reciprocal force {np.linalg.norm(f_rec[i]):.3e}, "
+              f"RMS force {rms:.3e}); all atoms rel {rel_rms(got, ref):.3e}")
+        assert err_i <= 1e-4 * rms
+        assert rel_rms(got, ref) <= FORCE_TOL
     finally:
         for h in (integ, field, part):
             h.close()
