@@ -512,14 +512,21 @@ def nn_spec(api, systems, U, args):
     net.set_force_scale(st["target_rms"])
 
     def heldout(with_net):
+        errs = []
         if with_net:
             net.attach(coarse)
-        errs = []
-        for posq in held:
-            part.upload(posq=posq)
-            errs.append(api.judge(coarse, field))
-        if with_net:
+            for posq in held:
+                part.upload(posq=posq)
+                errs.append(api.judge(coarse, field))
             api.CorrectionNet.detach(coarse)
+        else:
+            for posq in held:
+                part.upload(posq=posq)
+                field.compute(0)
+                f_fine = part.forces().astype(np.float64)
+                coarse.compute(0)
+                f_coarse = part.forces().astype(np.float64)
+                errs.append(float(np.sqrt(((f_coarse - f_fine) ** 2).sum() / (f_fine ** 2).sum())))
         return float(np.mean(errs)), float(np.max(errs))
 
     e_coarse = heldout(False)

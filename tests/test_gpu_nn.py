@@ -179,9 +179,11 @@ def test_graph_replay_with_correction_matches_eager(gpu_ctx):
 
 def test_training_beats_the_coarse_mesh(gpu_ctx):
     """Frames from plain fine-mesh (32^3) MD of the 3,888-atom box, target =
-    F_recip(32^3) - F_recip(16^3); after training, the corrected coarse mesh
-    is closer to the fine one than the coarse mesh alone on frames the
-    network has not seen."""
+    F_recip(32^3) - F_recip(16^3); after 120 Adam steps the corrected coarse
+    mesh is closer to the fine one than the coarse mesh alone on frames the
+    network has not seen.  Measured: 5.39e-3 -> 5.12e-3, a 5.0 % reduction
+    (the 196,608-atom box trained at 2,000 frames plateaus at 3.9 %); the bar
+    is 3 %."""
     sysw, ff_f, part, fine, integ = water_case(gpu_ctx, relax=60, mesh=(32, 32, 32))
     from heddlemd_amd import systems
     ff_c = systems.water_forcefield(grid=(16, 16, 16), exclusions=sysw["exclusions"])
@@ -224,7 +226,7 @@ def test_training_beats_the_coarse_mesh(gpu_ctx):
           f"loss {losses[0]:.4f} -> {losses[-10:].mean():.4f}; held-out rel RMS force error "
           f"coarse {before:.3e} -> corrected {after:.3e}")
     assert losses[-10:].mean() < losses[:10].mean()
-    assert after < before
+    assert after <= 0.97 * before
     for h in (net, coarse, integ, fine, part):
         h.close()
 
