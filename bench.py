@@ -26,8 +26,8 @@ Two legs, as the metric names them:
 against the plain 128^3 forces: the NN leg only if every judge passed, else
 the plain leg; both legs are reported.
 
-Prints one JSON line (see the keys below).  --gpus N > 1 exits: this build
-runs on one GPU.
+Prints one JSON line (see the keys below).  --gpus N > 1 exits: the
+benchmark runs on one GPU.
 """
 import argparse
 import json
@@ -281,7 +281,7 @@ def main():
                          "passes (the library default)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "bench.py runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
 
     from heddlemd_amd import api, systems

@@ -2,7 +2,7 @@
 // PairForceFragment / composer path of the reference
 // (src/forces/jit_composed.rs:112-151 fragment, :697-720 NVRTC compile,
 // :1162-1643 composed kernel), with hiprtc.  The fragment is a functor
-// struct, inserted ahead of this build's own pair-kernel source (embedded as
+// struct, inserted ahead of the library's own pair-kernel source (embedded as
 // text at build time); the pair kernel calls it for every listed pair, with
 // the built-in LJ / Coulomb terms, the same cutoff and exclusion masking, and
 // the same fixed-point accumulation.  The result is a second code object

@@ -1,7 +1,7 @@
 // Learned coarse-mesh SPME correction: the network object, its training
 // data set and Adam state, and the passes that run it over a force field's
-// neighbour list.  No counterpart in the reference (the learned part is new
-// in this build); the C entry points follow the hmd_mlp_{create, forward,
+// neighbour list.  No counterpart in the reference (it has no learned
+// part); the C entry points follow the hmd_mlp_{create, forward,
 // backward, adam_step} boundary the hot path's C ABI reserves for it.
 #include <cmath>
 #include <memory>

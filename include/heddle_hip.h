@@ -325,8 +325,8 @@ hmd_status hmd_integrator_set_forcefield(hmd_integrator* it, hmd_forcefield* ff)
  * to fine_ff (hmd_integrator_set_forcefield) for the rest of the run and
  * after it.  judges[] receives up to max_judges judge values and *n_judges
  * their count (may be NULL); *reverted_at the integrator's step count at the
- * switch, or UINT64_MAX.  The north-star fallback (plain fine-mesh SPME every
- * 50 steps, revert above 1e-3); the reference's nearest policy is its
+ * switch, or UINT64_MAX.  The learned leg's fallback policy (plain fine-mesh
+ * SPME as the judge every 50 steps, revert above 1e-3); the reference's nearest policy is its
  * runner's fallback from graph replay to eager steps (src/runner.rs:
  * 1650-1656). */
 hmd_status hmd_integrator_run_judged(hmd_integrator* it, hmd_forcefield* fine_ff,
