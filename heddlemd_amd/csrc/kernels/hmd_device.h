@@ -65,7 +65,10 @@ __device__ inline void hmd_cart_to_frac(float x, float y, float z,
 // values past the table clamp to its last bin), so the index is always
 // inside it.  No square root or reciprocal.
 __device__ inline uint32_t hmd_coul_bin(float r2, uint32_t u_lo, uint32_t u_hi, float& t) {
-  const uint32_t u = min(max(__float_as_uint(r2), u_lo), u_hi);
+  // one v_med3_u32 for the clamp (u_lo <= u_hi; the compiler does not fold
+  // min(max()) into it without knowing that)
+  uint32_t u;
+  asm("v_med3_u32 %0, %1, %2, %3" : "=v"(u) : "v"(__float_as_uint(r2)), "s"(u_lo), "v"(u_hi));
   t = __uint_as_float(((u << HMD_COUL_BITS) & 0x007FFFFFu) | 0x3F800000u);
   return (u >> (23u - HMD_COUL_BITS)) - (u_lo >> (23u - HMD_COUL_BITS));
 }

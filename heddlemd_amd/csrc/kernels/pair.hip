@@ -36,8 +36,10 @@
 //    v_permlane16_swap) and DPP row operations, packed one component per
 //    lane and added to the wave's i accumulators in LDS; the j reaction stays
 //    in the lane and is flushed once per entry;
-//  * accumulation is i64 fixed point (2^40), so the per-atom totals do not
-//    depend on the order in which waves finish: runs are bit-reproducible.
+//  * accumulation is i64 fixed point (2^40), each entry's i- and j-side sums
+//    converted separately, so the per-atom totals depend neither on the
+//    order in which waves finish nor on how the work list is split among
+//    them: runs are bit-reproducible across launch shapes and GPUs.
 #include "hmd_device.h"
 
 #define HMD_MAX_TYPES 16u
@@ -265,8 +267,8 @@ __device__ inline void hmd_pair_entry(const PairParams& p, const float4* s_lj,
   }
 }
 
-// The wave's i accumulators (f32 sums over its share of the cluster's
-// entries) to fixed point, and cleared for the next cluster.
+// The wave's i accumulators (f32 sums over one entry) to fixed point, and
+// cleared for the next entry.
 template <bool FEV>
 __device__ inline void hmd_flush_i(const PairParams& p, float* s_fi, uint32_t si, bool vi,
                                    uint32_t lane) {
@@ -339,7 +341,6 @@ __device__ inline void hmd_pair_body(const PairParams& p, const uint32_t hmd_bx,
     const uint32_t slot = p.work[w];
     const uint32_t c = slot / p.cap_entries;
     if (c != cur_c) {
-      if (cur_c != HMD_SENTINEL) hmd_flush_i<FEV>(p, fi, si, vi, lane);
       cur_c = c;
       si = c * HMD_CLUSTER + lane;
       vi = si < p.n;
@@ -383,8 +384,11 @@ __device__ inline void hmd_pair_body(const PairParams& p, const uint32_t hmd_bx,
     else
       hmd_pair_entry<FEV, LJ, COUL, false>(p, s_lj, s_ct, s_ip[wv], s_it[wv], fi, slot, c, cx,
                                            cy, cz, lj_ibits, lane);
+    // the entry's i-side sums go to fixed point on their own: the totals are
+    // then the same however the work list is split among waves (launch
+    // shape, CU count), not only run to run
+    hmd_flush_i<FEV>(p, fi, si, vi, lane);
   }
-  hmd_flush_i<FEV>(p, fi, si, vi, lane);
 }
 
 // Minimum waves per SIMD the register allocation must allow (1: no limit).

@@ -72,11 +72,14 @@ def test_shape_invariance(gpu_ctx):
                   f"rel PE {abs(e - e0) / abs(e0):.1e}", flush=True)
             assert f.tobytes() == f0.tobytes()
             assert abs(e - e0) <= 1e-12 * abs(e0)
-        field.set_tuning(dict(base, pair_blocks_per_cu=8))
-        f, _ = forces(field, part)
-        err = rel_rms(f, f0)
-        print(f"pair blocks/CU 8: rel RMS force {err:.2e}", flush=True)
-        assert err <= 1e-6
+        # each work-list entry's sums go to fixed point on their own, so the
+        # split of the list among waves does not change a bit
+        for pb in (8, 64):
+            field.set_tuning(dict(base, pair_blocks_per_cu=pb))
+            f, e = forces(field, part)
+            print(f"pair blocks/CU {pb}: forces identical {f.tobytes() == f0.tobytes()}", flush=True)
+            assert f.tobytes() == f0.tobytes()
+            assert e == e0
         field.set_tuning(base)
         with pytest.raises(HmdError):
             field.set_tuning(dict(base, brick=(5, 8, 8)))
