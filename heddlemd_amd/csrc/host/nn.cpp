@@ -89,6 +89,7 @@ struct Mlp {
     q.work = ff.work.ptr;
     q.n_work = ff.n_work.ptr;
     q.entry_j = ff.entry_j.ptr;
+    q.entry_mask = ff.entry_mask.ptr;
     q.bbox = ff.bbox.ptr;
     q.shift_margin = ff.r_cut + 0.5f * ff.r_skin;
     q.L = ff.p.lattice;
@@ -112,7 +113,7 @@ struct Mlp {
   // Descriptors of the current positions (the list must be current).
   void descriptors(const ForceField& ff, bool canonical) {
     NnPassParams q = pass_params(ff);
-    launch(c, K_hmd_nn_descriptors, (uint32_t)c.num_cus * 4u, 256, 0, q);
+    launch(c, K_hmd_nn_descriptors, (uint32_t)c.num_cus * 8u, 256, 0, q);
     NnDescFinalizeParams fp{desc_acc.ptr, n_pad, n, ff.sorted_id.ptr, desc.ptr,
                             canonical ? desc_canon.ptr : nullptr};
     launch(c, K_hmd_nn_desc_finalize, ceil_div(n_pad, 256), 256, 0, fp);

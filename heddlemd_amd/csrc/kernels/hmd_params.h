@@ -460,6 +460,7 @@ struct NnPassParams {
   const uint32_t* work;
   const uint32_t* n_work;
   const uint32_t* entry_j;
+  const uint32_t* entry_mask;  // per entry: 8-atom i-sub-clusters any of its j-atoms reaches
   const float* bbox;           // cluster centres + half extents
   HmdLattice L;
   uint32_t n, cap_entries, n_pad;

@@ -151,13 +151,63 @@ hmd_nn_pack(const NnPackParams p) {
   if (tid == 0u) p.vec[192] = b3[0];
 }
 
+// Single-image shift of a cluster (compare jit_composed.rs:1396-1430, as in
+// the pair kernel): valid when every atom within r_cut + r_skin/2 of the
+// cluster lies within half a box of its centre; positions are then moved to
+// the image nearest the centre and pair differences need no minimum image.
+struct HmdNnShift {
+  bool on;
+  float cx, cy, cz;
+};
+
+__device__ inline HmdNnShift hmd_nn_cluster_shift(const NnPassParams& p, uint32_t c) {
+  const HmdLattice& L = p.L;
+  HmdNnShift sh{false, 0.0f, 0.0f, 0.0f};
+  if (L.xy != 0.0f || L.xz != 0.0f || L.yz != 0.0f) return sh;
+  const float4* bb = reinterpret_cast<const float4*>(p.bbox);
+  const float4 cb = bb[2 * c], hb = bb[2 * c + 1];
+  sh.on = (hb.x + p.shift_margin < 0.5f * L.lx) && (hb.y + p.shift_margin < 0.5f * L.ly) &&
+          (hb.z + p.shift_margin < 0.5f * L.lz);
+  sh.cx = cb.x; sh.cy = cb.y; sh.cz = cb.z;
+  return sh;
+}
+
+__device__ inline float4 hmd_nn_apply_shift(float4 a, const HmdNnShift& sh, const HmdLattice& L) {
+  if (sh.on) {
+    a.x -= L.lx * floorf((a.x - sh.cx) * (1.0f / L.lx) + 0.5f);
+    a.y -= L.ly * floorf((a.y - sh.cy) * (1.0f / L.ly) + 0.5f);
+    a.z -= L.lz * floorf((a.z - sh.cz) * (1.0f / L.lz) + 0.5f);
+  }
+  return a;
+}
+
 // ------------------------------------------------------- descriptors ----
+// The pair kernel's traversal: one wave per contiguous share of the work
+// list, lane = j-atom of the entry (shifted once to the image nearest the
+// i-cluster's centre), the i-atoms broadcast from a per-wave LDS table one
+// per step, over the 8-atom i-sub-clusters the entry's mask admits.  Per
+// pair the 16 basis values come from two exponentials each half (the
+// product recurrence of hmd_nn_rbf8) with the cutoff folded into their
+// start; the j-atom's sums stay in the lane for the whole entry; the
+// i-atom's 16 sums of the step are reduced over the wave by two transposed
+// butterflies (totals of component m and 8 + m in lane 8m + 7) and added to
+// the wave's i accumulators in LDS.  Sums go to i64 fixed point (2^32): the
+// j side per entry, the i side per cluster share of the wave.
 extern "C" __global__ void __launch_bounds__(256)
 hmd_nn_descriptors(const NnPassParams p) {
+#pragma clang fp contract(off)
+  __shared__ float4 s_ip[4][HMD_CLUSTER];
+  // i accumulators, component k of atom l at k * 65 + l (the odd stride
+  // keeps the 8 lanes that add one atom's components on different banks)
+  __shared__ float s_di[4][16 * 65];
   const uint32_t lane = hmd_lane();
+  const uint32_t wv = threadIdx.x / HMD_WAVE;
+  float* di = s_di[wv];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) di[k * 65 + lane] = 0.0f;
   const uint32_t waves_per_block = blockDim.x / HMD_WAVE;
   const uint32_t n_waves = gridDim.x * waves_per_block;
-  const uint32_t gw = blockIdx.x * waves_per_block + threadIdx.x / HMD_WAVE;
+  const uint32_t gw = blockIdx.x * waves_per_block + wv;
   const uint32_t nw = *p.n_work;
   const uint32_t per = (nw + n_waves - 1) / n_waves;
   const uint32_t w0 = gw * per;
@@ -168,83 +218,86 @@ hmd_nn_descriptors(const NnPassParams p) {
   long long* acc = p.desc_acc;
   const uint32_t stride = p.n_pad;
 
-  float d[16];
   uint32_t cur_c = HMD_SENTINEL, si = 0;
   bool vi = false;
-  float xi = 0, yi = 0, zi = 0, qi = 0;
+  HmdNnShift sh{false, 0.0f, 0.0f, 0.0f};
+  auto flush_i = [&]() {
+    hmd_wave_sync();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const float v = di[k * 65 + lane];
+      di[k * 65 + lane] = 0.0f;
+      if (vi) hmd_atomic_add_fixed(&acc[k * stride + si], hmd_to_fixed(v, HMD_NN_DESC_SHIFT));
+    }
+  };
   for (uint32_t w = w0; w < w1; ++w) {
     const uint32_t slot = p.work[w];
     const uint32_t c = slot / p.cap_entries;
+    const uint32_t c0 = c * HMD_CLUSTER;
     if (c != cur_c) {
-      if (cur_c != HMD_SENTINEL && vi) {
-#pragma unroll
-        for (int k = 0; k < 16; ++k)
-          hmd_atomic_add_fixed(&acc[k * stride + si], hmd_to_fixed(d[k], HMD_NN_DESC_SHIFT));
-      }
+      if (cur_c != HMD_SENTINEL) flush_i();
       cur_c = c;
-      si = c * HMD_CLUSTER + lane;
+      si = c0 + lane;
       vi = si < p.n;
-      const float4 a = ps[si];
-      xi = a.x; yi = a.y; zi = a.z; qi = a.w;
-#pragma unroll
-      for (int k = 0; k < 16; ++k) d[k] = 0.0f;
+      sh = hmd_nn_cluster_shift(p, c);
+      float4 a = hmd_nn_apply_shift(ps[si], sh, L);
+      if (!vi) a.x = a.y = a.z = 1.0e18f;   // padding lanes fail every cutoff test
+      hmd_wave_sync();
+      s_ip[wv][lane] = a;
+      hmd_wave_sync();
     }
     const uint32_t my_j = p.entry_j[(size_t)slot * HMD_ENTRY + lane];
-    const float4 pj = ps[my_j != HMD_SENTINEL ? my_j : 0u];
-    for (uint32_t kb = 0; kb < HMD_ENTRY; kb += 8) {
-      if (hmd_readlane(my_j, kb) == HMD_SENTINEL) break;
-      float jc[16][8];
+    const bool valid_j = my_j != HMD_SENTINEL;
+    float4 pj = hmd_nn_apply_shift(ps[valid_j ? my_j : 0u], sh, L);
+    if (!valid_j) pj.x = pj.y = pj.z = 1.0e18f;
+    // entries whose j-atoms all lie in later clusters need only the cutoff
+    // test (i < j holds); the cluster's own entry also needs i < j
+    const bool later = __ballot(valid_j && my_j < c0 + HMD_CLUSTER) == 0ull;
+    uint32_t pend = __builtin_amdgcn_readfirstlane(p.entry_mask[slot]) & 0xFFu;
+    float dj[16];
 #pragma unroll
+    for (int k = 0; k < 16; ++k) dj[k] = 0.0f;
+    while (pend) {
+      const uint32_t sb = (uint32_t)__builtin_ctz(pend);
+      pend &= pend - 1u;
+#pragma unroll 2
       for (uint32_t u = 0; u < 8; ++u) {
-        const uint32_t k = kb + u;
-        const uint32_t j = hmd_readlane(my_j, k);
-        float dx = xi - hmd_readlane_f(pj.x, k);
-        float dy = yi - hmd_readlane_f(pj.y, k);
-        float dz = zi - hmd_readlane_f(pj.z, k);
-        const float qj = hmd_readlane_f(pj.w, k);
-        hmd_min_image(dx, dy, dz, L);
+        const uint32_t i = 8u * sb + u;
+        const float4 a = s_ip[wv][i];
+        float dx = a.x - pj.x, dy = a.y - pj.y, dz = a.z - pj.z;
+        if (!sh.on) hmd_min_image(dx, dy, dz, L);
         const float r2 = dx * dx + dy * dy + dz * dz;
-        const bool ok = (j != HMD_SENTINEL) && vi && ((j >> 6) != c || si < j) &&
-                        (r2 < p.r_cut2) && (r2 > 0.0f);
-        if (__ballot(ok) == 0) {
-#pragma unroll
-          for (int kk = 0; kk < 16; ++kk) jc[kk][u] = 0.0f;
-          continue;
-        }
+        const bool ok = (r2 < p.r_cut2) && (r2 > 0.0f) && (later || c0 + i < my_j);
+        if (__ballot(ok) == 0ull) continue;
         const float s = __builtin_sqrtf(ok ? r2 : 0.0f) * p.inv_r_cut;
         const float fc = ok ? hmd_nn_cutoff(s) : 0.0f;
-        const float u15 = 15.0f * s;
-        float ph[16];
-        hmd_nn_rbf8(u15, 0.0f, ph);
-        hmd_nn_rbf8(u15, 8.0f, ph + 8);
+        float w[16];
+        hmd_nn_rbf8(15.0f * s, 0.0f, w);
+        hmd_nn_rbf8(15.0f * s, 8.0f, w + 8);
+        float vlo[8], vhi[8];
 #pragma unroll
-        for (int kk = 0; kk < 16; ++kk) {
-          const float wk = ph[kk] * fc;
-          d[kk] += qj * wk;
-          jc[kk][u] = qi * wk;
+        for (int k = 0; k < 8; ++k) {
+          const float wl = w[k] * fc, wh = w[8 + k] * fc;
+          dj[k] = fmaf(a.w, wl, dj[k]);
+          dj[8 + k] = fmaf(a.w, wh, dj[8 + k]);
+          vlo[k] = pj.w * wl;
+          vhi[k] = pj.w * wh;
+        }
+        const float tlo = hmd_wave_sum8(vlo, lane), thi = hmd_wave_sum8(vhi, lane);
+        if ((lane & 7u) == 7u) {
+          const uint32_t m = lane >> 3;
+          di[m * 65 + i] += tlo;          // one lane per address: program order
+          di[(8u + m) * 65 + i] += thi;
         }
       }
-      float t[16];
+    }
+    if (valid_j) {
 #pragma unroll
-      for (int kk = 0; kk < 16; ++kk) t[kk] = hmd_transpose_reduce8(jc[kk], lane);
-      const uint32_t jt = __shfl(my_j, (int)(kb + (lane >> 3)), HMD_WAVE);
-      const uint32_t e = lane & 7u;
-      float v0 = 0.0f, v1 = 0.0f;
-#pragma unroll
-      for (uint32_t q = 0; q < 8; ++q) {
-        if (e == q) { v0 = t[2 * q]; v1 = t[2 * q + 1]; }
-      }
-      if (jt != HMD_SENTINEL) {
-        hmd_atomic_add_fixed(&acc[(2 * e) * stride + jt], hmd_to_fixed(v0, HMD_NN_DESC_SHIFT));
-        hmd_atomic_add_fixed(&acc[(2 * e + 1) * stride + jt], hmd_to_fixed(v1, HMD_NN_DESC_SHIFT));
-      }
+      for (int k = 0; k < 16; ++k)
+        hmd_atomic_add_fixed(&acc[k * stride + my_j], hmd_to_fixed(dj[k], HMD_NN_DESC_SHIFT));
     }
   }
-  if (vi) {
-#pragma unroll
-    for (int k = 0; k < 16; ++k)
-      hmd_atomic_add_fixed(&acc[k * stride + si], hmd_to_fixed(d[k], HMD_NN_DESC_SHIFT));
-  }
+  flush_i();
 }
 
 // Fixed point -> f32 descriptors in sorted order (and canonical order when
@@ -288,36 +341,6 @@ struct HmdNnFwdWave : HmdNnTables {        // forward pass: per-batch pair table
   float4 geo[32];                          // fc, unit vector i - j
   uint32_t info[32];                       // i | j << 6 | one-hot << 12 | valid << 14
 };
-
-// Single-image shift of a cluster (compare jit_composed.rs:1396-1430, as in
-// the pair kernel): valid when every atom within r_cut + r_skin/2 of the
-// cluster lies within half a box of its centre; positions are then moved to
-// the image nearest the centre and pair differences need no minimum image.
-struct HmdNnShift {
-  bool on;
-  float cx, cy, cz;
-};
-
-__device__ inline HmdNnShift hmd_nn_cluster_shift(const NnPassParams& p, uint32_t c) {
-  const HmdLattice& L = p.L;
-  HmdNnShift sh{false, 0.0f, 0.0f, 0.0f};
-  if (L.xy != 0.0f || L.xz != 0.0f || L.yz != 0.0f) return sh;
-  const float4* bb = reinterpret_cast<const float4*>(p.bbox);
-  const float4 cb = bb[2 * c], hb = bb[2 * c + 1];
-  sh.on = (hb.x + p.shift_margin < 0.5f * L.lx) && (hb.y + p.shift_margin < 0.5f * L.ly) &&
-          (hb.z + p.shift_margin < 0.5f * L.lz);
-  sh.cx = cb.x; sh.cy = cb.y; sh.cz = cb.z;
-  return sh;
-}
-
-__device__ inline float4 hmd_nn_apply_shift(float4 a, const HmdNnShift& sh, const HmdLattice& L) {
-  if (sh.on) {
-    a.x -= L.lx * floorf((a.x - sh.cx) * (1.0f / L.lx) + 0.5f);
-    a.y -= L.ly * floorf((a.y - sh.cy) * (1.0f / L.ly) + 0.5f);
-    a.z -= L.lz * floorf((a.z - sh.cz) * (1.0f / L.lz) + 0.5f);
-  }
-  return a;
-}
 
 // Geometry and inputs of the two pairs lane l works on in a batch
 // (pairs l%16 and 16 + l%16 of the batch).
