@@ -267,24 +267,30 @@ __device__ inline void hmd_pair_entry(const PairParams& p, const float4* s_lj,
   }
 }
 
-// The wave's i accumulators (f32 sums over one entry) to fixed point, and
-// cleared for the next entry.
+// The wave's i accumulators (f32 sums over one entry) to fixed point, added
+// to the lane's i64 running totals (integer adds: any grouping of entries
+// gives the same bits), and cleared for the next entry.
 template <bool FEV>
-__device__ inline void hmd_flush_i(const PairParams& p, float* s_fi, uint32_t si, bool vi,
-                                   uint32_t lane) {
+__device__ inline void hmd_fold_i(float* s_fi, long long (&fix)[5], uint32_t lane) {
   constexpr uint32_t NC = FEV ? 5u : 3u;
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  float v[NC];
 #pragma unroll
   for (uint32_t cc = 0; cc < NC; ++cc) {
-    v[cc] = s_fi[cc * HMD_CLUSTER + lane];
+    fix[cc] += hmd_to_fixed(s_fi[cc * HMD_CLUSTER + lane], HMD_FIXED_SHIFT);
     s_fi[cc * HMD_CLUSTER + lane] = 0.0f;
   }
-  if (vi) {
+}
+
+// The running totals of the wave's share of one cluster to the accumulators.
+template <bool FEV>
+__device__ inline void hmd_flush_i(const PairParams& p, long long (&fix)[5], uint32_t si,
+                                   bool vi) {
+  constexpr uint32_t NC = FEV ? 5u : 3u;
 #pragma unroll
-    for (uint32_t cc = 0; cc < NC; ++cc)
-      hmd_atomic_add_fixed(&p.acc[cc * p.acc_stride + si], hmd_to_fixed(v[cc], HMD_FIXED_SHIFT));
+  for (uint32_t cc = 0; cc < NC; ++cc) {
+    if (vi) hmd_atomic_add_fixed(&p.acc[cc * p.acc_stride + si], fix[cc]);
+    fix[cc] = 0;
   }
 }
 
@@ -337,10 +343,12 @@ __device__ inline void hmd_pair_body(const PairParams& p, const uint32_t hmd_bx,
   uint64_t lj_ibits = 0;   // i-atoms of the cluster whose type has an LJ partner
   bool shift = false;
   float cx = 0, cy = 0, cz = 0;
+  long long fix[5] = {0, 0, 0, 0, 0};
   for (uint32_t w = w0; w < w1; ++w) {
     const uint32_t slot = p.work[w];
     const uint32_t c = slot / p.cap_entries;
     if (c != cur_c) {
+      if (cur_c != HMD_SENTINEL) hmd_flush_i<FEV>(p, fix, si, vi);
       cur_c = c;
       si = c * HMD_CLUSTER + lane;
       vi = si < p.n;
@@ -387,8 +395,9 @@ __device__ inline void hmd_pair_body(const PairParams& p, const uint32_t hmd_bx,
     // the entry's i-side sums go to fixed point on their own: the totals are
     // then the same however the work list is split among waves (launch
     // shape, CU count), not only run to run
-    hmd_flush_i<FEV>(p, fi, si, vi, lane);
+    hmd_fold_i<FEV>(fi, fix, lane);
   }
+  hmd_flush_i<FEV>(p, fix, si, vi);
 }
 
 // Minimum waves per SIMD the register allocation must allow (1: no limit).
