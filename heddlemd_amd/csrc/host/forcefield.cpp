@@ -433,6 +433,7 @@ void ForceField::init_neighbor_list(const hmd_forcefield_desc& d) {
   image_sorted.alloc(4 * (size_t)n_pad);
   bbox.alloc(8 * (size_t)n_clusters);
   sub_bbox.alloc(64 * (size_t)n_clusters);
+  excl_slot.alloc(8 * (size_t)n_pad, false);
   entry_count.alloc(n_clusters);
   excl_entries.alloc(n_clusters);
   entry_offset.alloc(n_clusters + 1);
@@ -687,7 +688,9 @@ void ForceField::rebuild_sort() {
 void ForceField::rebuild_list() {
   const uint32_t* gate = status.ptr;
   const HmdLattice L = p.lattice;
-  NblBboxParams bp{posq_sorted.ptr, n, n_clusters, bbox.ptr, sub_bbox.ptr, gate};
+  NblBboxParams bp{posq_sorted.ptr, n, n_clusters, bbox.ptr, sub_bbox.ptr, gate,
+                   sorted_id.ptr, slot_of.ptr, excl_off.ptr, excl_par.ptr,
+                   excl_slot.ptr, n_pad};
   launch(c, K_hmd_nbl_bbox, ceil_div(n_clusters, 4), 256, 0, bp);
   NblBuildParams bd{};
   bd.posq_sorted = posq_sorted.ptr;
@@ -712,8 +715,8 @@ void ForceField::rebuild_list() {
   bd.entry_excl = entry_excl.ptr;
   bd.entry_count = entry_count.ptr;
   bd.excl_entries = excl_entries.ptr;
-  bd.excl_offsets = excl_off.ptr;
-  bd.excl_partners = excl_par.ptr;
+  bd.excl_slot = excl_slot.ptr;
+  bd.n_pad = n_pad;
   bd.status = status.ptr;
 #ifndef HMD_NBL_WAVES
 #define HMD_NBL_WAVES 4u

@@ -49,6 +49,7 @@ struct ForceField {
   uint32_t coul_u_lo = 0, coul_u_hi = 0;   // table bounds as f32 bits
   void build_coulomb_table(double alpha, double r_cut, hipStream_t s);
   DevBuf<uint32_t> excl_off, excl_par;
+  DevBuf<uint32_t> excl_slot;   // [8][n_pad] partners' sorted slots (each rebuild)
   // listed pairs with non-zero exclusion scales (i < j)
   uint32_t n_scaled = 0;
   DevBuf<uint32_t> sc_i, sc_j;

@@ -101,6 +101,14 @@ struct NblBboxParams {
   float* bbox;              // float4[2*n_clusters]: (cx,cy,cz,-), (hx,hy,hz,-)
   float* sub_bbox;          // float4[16*n_clusters]: 8 sub-cluster boxes
   const uint32_t* status;
+  // exclusion partners as sorted slots, for the list build: slot s's m-th
+  // partner (canonical CSR order, at most 8) at excl_slot[m * n_pad + s]
+  const uint32_t* sorted_id;
+  const uint32_t* slot_of;
+  const uint32_t* excl_offsets;   // canonical CSR [n+1]
+  const uint32_t* excl_partners;
+  uint32_t* excl_slot;            // [8][n_pad], sentinel-filled
+  uint32_t n_pad;
 };
 
 struct NblBuildParams {
@@ -122,8 +130,8 @@ struct NblBuildParams {
   uint64_t* entry_excl;        // [n_clusters*cap_entries*64]
   uint32_t* entry_count;       // [n_clusters] (entries written / needed)
   uint32_t* excl_entries;      // [n_clusters] leading entries with masks
-  const uint32_t* excl_offsets;  // canonical CSR [n+1]
-  const uint32_t* excl_partners;
+  const uint32_t* excl_slot;   // [8][n_pad] partners' sorted slots (hmd_nbl_bbox)
+  uint32_t n_pad;
   uint32_t* status;
 };
 

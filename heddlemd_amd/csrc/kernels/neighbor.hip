@@ -245,6 +245,21 @@ __device__ __forceinline__ void hmd_nbl_bbox_body(const NblBboxParams& p, const 
     b[2 * c + 1] = make_float4(0.5f * (hi[0] - lo[0]), 0.5f * (hi[1] - lo[1]),
                                0.5f * (hi[2] - lo[2]), 0.0f);
   }
+  // The atom's exclusion partners as sorted slots (a chain of dependent
+  // loads: id, CSR row, partner, its slot), resolved here for every atom at
+  // once so that the list build reads them with one load.
+  uint32_t b0 = 0, b1 = 0;
+  if (valid) {
+    const uint32_t id = p.sorted_id[s];
+    b0 = p.excl_offsets[id];
+    b1 = p.excl_offsets[id + 1];
+  }
+#pragma unroll
+  for (uint32_t m = 0; m < 8; ++m) {
+    uint32_t slot = HMD_SENTINEL;
+    if (m < b1 - b0) slot = p.slot_of[p.excl_partners[b0 + m]];
+    p.excl_slot[(size_t)m * p.n_pad + s] = slot;
+  }
 }
 HMD_KERNEL(hmd_nbl_bbox, NblBboxParams, 256)
 
@@ -368,9 +383,8 @@ __device__ __forceinline__ void hmd_nbl_build_body(const NblBuildParams& p, cons
   const hmd_const_f4* csub = (const hmd_const_f4*)(reinterpret_cast<const float4*>(p.sub_bbox) + 16 * c);
 
   for (uint32_t e = threadIdx.x; e < HMD_NBL_EMASK_CAP; e += blockDim.x) s_emask[e] = 0u;
-  // wave 0: the sorted slots of each i-atom's exclusion partners (loaded
-  // first, so that the chain of dependent loads overlaps the candidate
-  // search), and their positions in the cluster's list once placed
+  // wave 0: the sorted slots of each i-atom's exclusion partners (resolved
+  // by hmd_nbl_bbox), and their positions in the cluster's list once placed
   const uint32_t si = c * HMD_CLUSTER + lane;
   uint32_t ex_slot[HMD_MAX_INLINE_EXCL], ex_pos[HMD_MAX_INLINE_EXCL];
 #pragma unroll
@@ -378,13 +392,10 @@ __device__ __forceinline__ void hmd_nbl_build_body(const NblBuildParams& p, cons
     ex_slot[m] = HMD_SENTINEL;
     ex_pos[m] = HMD_SENTINEL;
   }
-  if (wave == 0 && si < p.n && p.excl_offsets) {
-    const uint32_t id = p.sorted_id[si];
-    const uint32_t b = p.excl_offsets[id], e = p.excl_offsets[id + 1];
-    const uint32_t n_ex = min(e - b, (uint32_t)HMD_MAX_INLINE_EXCL);
+  if (wave == 0 && si < p.n) {
 #pragma unroll
     for (uint32_t m = 0; m < HMD_MAX_INLINE_EXCL; ++m)
-      if (m < n_ex) ex_slot[m] = p.slot_of[p.excl_partners[b + m]];
+      ex_slot[m] = p.excl_slot[(size_t)m * p.n_pad + si];
   }
   // candidate masks, 64 j-clusters per ballot; each wave loads the boxes of
   // four chunks before testing them, so that the loads' latencies overlap
