@@ -37,7 +37,7 @@ EXPORTED = [
     "hmd_particles_upload", "hmd_particles_download_energies", "hmd_particles_download_lo",
     "hmd_particles_kinetic_energy", "hmd_particles_energy_sums",
     "hmd_particles_negate_velocities", "hmd_particles_upload_forces",
-    "hmd_particles_upload_images", "hmd_particles_set_box",
+    "hmd_particles_upload_images", "hmd_particles_set_box", "hmd_particles_get_box",
     "hmd_vv_kick_drift", "hmd_vv_kick", "hmd_lan_drift_half", "hmd_lan_ou_step",
     "hmd_replicas_create", "hmd_replicas_destroy", "hmd_replicas_step", "hmd_replicas_steps",
     "hmd_replicas_plan",
@@ -57,6 +57,7 @@ EXPORTED = [
     "hmd_integrator_profile", "hmd_integrator_minimize",
     "hmd_integrator_set_forcefield", "hmd_integrator_run_judged",
     "hmd_integrator_thermostat_state", "hmd_integrator_checkpoint", "hmd_integrator_rollback",
+    "hmd_integrator_set_barostat", "hmd_integrator_barostat_state",
     "hmd_generate_velocities", "hmd_chacha_block",
     "hmd_mlp_param_count", "hmd_mlp_create", "hmd_mlp_destroy", "hmd_mlp_set_params",
     "hmd_mlp_get_params", "hmd_mlp_set_force_scale", "hmd_mlp_descriptors",
@@ -112,6 +113,13 @@ class IntegratorDesc(ctypes.Structure):
         ("flags", ctypes.c_uint32),
         ("thermostat", ctypes.c_uint32), ("tau", ctypes.c_double),
     ]
+
+
+class BarostatDesc(ctypes.Structure):
+    """hmd_barostat_desc (the c-rescale barostat's parameters)."""
+    _fields_ = [("kind", ctypes.c_uint32), ("pressure", ctypes.c_double),
+                ("temperature", ctypes.c_double), ("tau", ctypes.c_double),
+                ("compressibility", ctypes.c_double), ("seed", ctypes.c_uint64)]
 
 
 class SlotOutput(ctypes.Structure):
@@ -173,6 +181,7 @@ def _declare(L):
         "hmd_particles_upload_forces": (st, [vp, _c_fp]),
         "hmd_particles_upload_images": (st, [vp, _c_i32p]),
         "hmd_particles_set_box": (st, [vp, _c_fp]),
+        "hmd_particles_get_box": (st, [vp, _c_fp]),
         "hmd_vv_kick_drift": (st, [vp, ctypes.c_float, ctypes.c_int32]),
         "hmd_vv_kick": (st, [vp, ctypes.c_float, ctypes.c_int32]),
         "hmd_lan_drift_half": (st, [vp, ctypes.c_float]),
@@ -229,6 +238,8 @@ def _declare(L):
         "hmd_integrator_steps": (ctypes.c_uint64, [vp]),
         "hmd_integrator_set_forcefield": (st, [vp, vp]),
         "hmd_integrator_thermostat_state": (st, [vp, _c_dp]),
+        "hmd_integrator_set_barostat": (st, [vp, ctypes.POINTER(BarostatDesc)]),
+        "hmd_integrator_barostat_state": (st, [vp, _c_dp]),
         "hmd_integrator_checkpoint": (st, [vp]),
         "hmd_integrator_rollback": (st, [vp]),
         "hmd_integrator_run_judged": (st, [vp, vp, ctypes.c_uint32, ctypes.c_uint32,

@@ -5,7 +5,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import ForceFieldDesc, IntegratorDesc, MlpDesc, Tuning, check, lib
+from ._lib import BarostatDesc, ForceFieldDesc, IntegratorDesc, MlpDesc, Tuning, check, lib
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -75,7 +75,14 @@ class ParticleBuffers:
                                          _ptr(mass, _fp), _ptr(t, _u32p), _ptr(lat, _fp),
                                          ctypes.byref(h)))
         self.h = h
-        self.lattice = lat
+
+    @property
+    def lattice(self):
+        """The current box {lx, ly, lz, xy, xz, yz} (hmd_particles_get_box;
+        a barostat moves it)."""
+        lat = np.empty(6, np.float32)
+        check(lib().hmd_particles_get_box(self.h, _ptr(lat, _fp)))
+        return lat
 
     def download(self):
         n = self.n
@@ -118,7 +125,6 @@ class ParticleBuffers:
         """New box {lx, ly, lz, xy, xz, yz} (hmd_particles_set_box)."""
         lat = np.ascontiguousarray(lattice, np.float32)
         check(lib().hmd_particles_set_box(self.h, _ptr(lat, _fp)))
-        self.lattice = lat
 
     def upload_images(self, images):
         im = np.ascontiguousarray(images, np.int32).reshape(-1, 3)
@@ -381,6 +387,25 @@ class Integrator:
         out = np.zeros(2)
         check(lib().hmd_integrator_thermostat_state(self.h, _ptr(out, _dp)))
         return float(out[0]), int(out[1])
+
+    def set_barostat(self, kind="c-rescale", pressure=0.0, temperature=0.0, tau=0.0,
+                     compressibility=0.0, seed=0):
+        """Per-step c-rescale barostat (hmd_integrator_set_barostat): target
+        pressure, k_B T of the noise term, coupling time, compressibility,
+        Philox seed, all in atomic units; kind None removes it.  Barostat
+        steps run eagerly (replay steps eagerly, capture is refused)."""
+        d = BarostatDesc()
+        d.kind = {None: 0, "c-rescale": 1}[kind]
+        d.pressure, d.temperature, d.tau = pressure, temperature, tau
+        d.compressibility, d.seed = compressibility, seed
+        check(lib().hmd_integrator_set_barostat(self.h, ctypes.byref(d)))
+
+    def barostat_state(self):
+        """(pressure of the latest move, volume after it, accumulated
+        P_target dV, draws so far, latest mu)."""
+        out = np.zeros(5)
+        check(lib().hmd_integrator_barostat_state(self.h, _ptr(out, _dp)))
+        return float(out[0]), float(out[1]), float(out[2]), int(out[3]), float(out[4])
 
     def checkpoint(self):
         """Keep a device copy of the state a batch of steps changes

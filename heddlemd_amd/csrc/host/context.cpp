@@ -277,6 +277,15 @@ hmd_status hmd_particles_set_box(hmd_particles* hp, const float lat[6]) {
   });
 }
 
+hmd_status hmd_particles_get_box(hmd_particles* hp, float lat[6]) {
+  HMD_GUARD({
+    if (!hp || !lat) throw Error(HMD_ERR_INVALID, "null argument");
+    const HmdLattice& L = hp->p.lattice;
+    lat[0] = L.lx; lat[1] = L.ly; lat[2] = L.lz;
+    lat[3] = L.xy; lat[4] = L.xz; lat[5] = L.yz;
+  });
+}
+
 hmd_status hmd_particles_upload_images(hmd_particles* hp, const int32_t* images) {
   HMD_GUARD({
     if (!hp || !images) throw Error(HMD_ERR_INVALID, "null argument");

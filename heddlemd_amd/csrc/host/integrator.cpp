@@ -47,6 +47,12 @@ struct Integrator {
   DevBuf<double> csvr_ke, csvr_partials, csvr_injection;
   DevBuf<float> csvr_factor;
   DevBuf<unsigned long long> csvr_counter;
+  // c-rescale barostat (a per-step move after the last kick)
+  bool baro = false;
+  double baro_p = 0.0, baro_tau = 0.0, baro_beta = 0.0, baro_kt = 0.0;
+  uint64_t baro_seed = 0;
+  DevBuf<double> baro_ke, baro_w, baro_state;
+  DevBuf<unsigned long long> baro_counter;
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -135,7 +141,8 @@ struct Integrator {
     check(hipStreamSynchronize(s), "sync");
   }
 
-  // The thermostat after a step (apply_post, csvr.rs:164-206): FP64 kinetic
+  // The thermostat after the step's last kick (apply_post, csvr.rs:164-206;
+  // before the barostat and the SETTLE velocity projection): FP64 kinetic
   // energy into device memory, the CSVR chain, the rescale; enqueue only.
   void thermostat_post() {
     if (!csvr) return;
@@ -169,6 +176,104 @@ struct Integrator {
     launch(c, K_hmd_csvr_finish, 1, 256, 0, q);
     launch(c, K_hmd_csvr_rescale, ceil_div(p.n, 256), 256, 0, q);
     c.mark("thermostat");
+  }
+
+  // c_rescale_barostat.rs:282-300 (validate_params / build): pressure
+  // finite, temperature, tau and compressibility finite and positive.  A
+  // per-step barostat reads the virial, so every force evaluation is at the
+  // energy level from here on (runner.rs:2521-2539), with the SETTLE virial
+  // folded in after the last kick.
+  void set_barostat(const hmd_barostat_desc& d) {
+    if (d.kind == HMD_BAROSTAT_NONE) {
+      baro = false;
+      drop_graph();
+      return;
+    }
+    if (d.kind != HMD_BAROSTAT_C_RESCALE) throw Error(HMD_ERR_INVALID, "unknown barostat");
+    auto finite_pos = [](double v) { return std::isfinite(v) && v > 0.0; };
+    if (!std::isfinite(d.pressure)) throw Error(HMD_ERR_INVALID, "barostat pressure must be finite");
+    if (!finite_pos(d.temperature) || !finite_pos(d.tau) || !finite_pos(d.compressibility))
+      throw Error(HMD_ERR_INVALID,
+                  "barostat temperature, tau and compressibility must be finite and positive");
+    baro = true;
+    baro_p = d.pressure;
+    baro_kt = d.temperature;
+    baro_tau = d.tau;
+    baro_beta = d.compressibility;
+    baro_seed = d.seed;
+    scalars = true;
+    if (!baro_state.count) {
+      baro_ke.alloc(1);
+      baro_w.alloc(1);
+      baro_state.alloc(4);
+      baro_counter.alloc(1);
+    }
+    drop_graph();
+  }
+
+  void drop_graph() {
+    if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+    if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+  }
+
+  // BarostatPoint (c_rescale_barostat.rs:172-232): FP64 kinetic energy and
+  // virial sums, the move on the device, then the new box.  The box is a
+  // kernel argument here (the reference keeps it in device memory), so the
+  // host reads mu back, scales the lattice by (float) mu as the device
+  // lattice is scaled, and the position rescale and the SETTLE velocity
+  // projection that follow see the new box.  Not capturable.
+  void barostat_apply() {
+    if (!baro) return;
+    ReduceParams r{};
+    r.v = p.vel.ptr;
+    r.m = p.mass.ptr;
+    r.n = p.n;
+    r.mode = 1;
+    r.partials = p.red_partials.ptr;
+    r.out = baro_ke.ptr;
+    r.n_blocks = (uint32_t)p.red_partials.count;
+    launch(c, K_hmd_reduce_partial, r.n_blocks, 256, 0, r);
+    launch(c, K_hmd_reduce_final, 1, 256, 0, r);
+    r.mode = 0;
+    r.x = p.virial.ptr;
+    r.out = baro_w.ptr;
+    launch(c, K_hmd_reduce_partial, r.n_blocks, 256, 0, r);
+    launch(c, K_hmd_reduce_final, 1, 256, 0, r);
+    const HmdLattice L = p.lattice;
+    CRescaleParams q{};
+    q.ke = baro_ke.ptr;
+    q.w = baro_w.ptr;
+    q.v_pre = (double)L.lx * (double)L.ly * (double)L.lz;
+    q.counter = baro_counter.ptr;
+    q.seed_lo = (uint32_t)(baro_seed & 0xFFFFFFFFull);
+    q.seed_hi = (uint32_t)(baro_seed >> 32);
+    q.p_target = baro_p;
+    q.tau = baro_tau;
+    q.beta = baro_beta;
+    q.kt = baro_kt;
+    q.dt = (double)dt;
+    q.mu3_min = 1e-18;   // MU_MIN^3, MU_MIN = 1e-6 (c_rescale_barostat.rs:58)
+    q.state = baro_state.ptr;
+    q.posq = p.posq.ptr;
+    q.pos_lo = compensated ? p.pos_lo.ptr : nullptr;
+    q.n = p.n;
+    launch(c, K_hmd_crescale_mu, 1, 64, 0, q);
+    double st[4];
+    baro_state.download(st, 4, c.stream);
+    const float mu = (float)st[0];
+    p.lattice = HmdLattice{L.lx * mu, L.ly * mu, L.lz * mu, L.xy * mu, L.xz * mu, L.yz * mu};
+    launch(c, K_hmd_crescale_positions, ceil_div(p.n, 256), 256, 0, q);
+    c.mark("barostat");
+    box_moved = true;
+  }
+  bool box_moved = false;
+
+  // The force fields on these particles follow a box the barostat moved:
+  // a new SPME influence table and a list rebuild at the next evaluation.
+  void publish_box() {
+    if (!box_moved) return;
+    box_moved = false;
+    for (ForceField* f : p.observers) f->on_box_change();
   }
 
   ~Integrator() {
@@ -260,6 +365,23 @@ struct Integrator {
 
   int level() const { return scalars ? 1 : 0; }
 
+  // The trailing kick and the post-force markers in the reference's order
+  // (src/integrator/mod.rs:500-517): the thermostat's apply_post after the
+  // kick, then the barostat, then the SETTLE velocity projection
+  // (velocity_verlet.rs:92-109).
+  void post_force_tail(uint32_t nb) {
+    launch(c, K_hmd_vv_kick, nb, 256, 0, vv());               // KickHalf
+    thermostat_post();                                        // apply_post
+    barostat_apply();                                         // BarostatPoint
+    settle_velocities();                                      // AfterKick
+    settle_virial();
+    if (kind == Kind::LangevinBaoab) {
+      CounterParams cp{step_counter.ptr};
+      launch(c, K_hmd_counter_increment, 1, 64, 0, cp);
+    }
+    c.mark("kick_settle");
+  }
+
   // One step, enqueued only (capturable).
   void enqueue_step() {
     const uint32_t nb = ceil_div(p.n, 256);
@@ -270,9 +392,12 @@ struct Integrator {
              0, sp);
       c.mark(kind == Kind::VelocityVerlet ? "kick_drift_settle" : "baoa");
       ff->compute(level());
+      if (csvr || baro) {
+        post_force_tail(nb);
+        return;
+      }
       launch(c, K_hmd_step_kick_last, nu, 256, 0, sp);
       c.mark("kick_settle");
-      thermostat_post();
       return;
     }
     if (kind == Kind::VelocityVerlet) {
@@ -281,11 +406,7 @@ struct Integrator {
       settle_positions(dt);                                     // AfterDrift
       c.mark("kick_drift_settle");
       ff->compute(level());                                      // ForceEval
-      launch(c, K_hmd_vv_kick, nb, 256, 0, vv());               // KickHalf
-      settle_velocities();                                      // AfterKick
-      settle_virial();
-      c.mark("kick_settle");
-      thermostat_post();
+      post_force_tail(nb);
     } else {
       // B A O A (force) B, with SETTLE after each position and velocity
       // update (the reference's BAOAB plan has no constraint points).
@@ -302,12 +423,7 @@ struct Integrator {
       settle_positions(0.5f * dt);
       c.mark("baoa");
       ff->compute(level());
-      launch(c, K_hmd_vv_kick, nb, 256, 0, k);
-      settle_velocities();
-      settle_virial();
-      CounterParams cp{step_counter.ptr};
-      launch(c, K_hmd_counter_increment, 1, 64, 0, cp);
-      c.mark("kick_settle");
+      post_force_tail(nb);
     }
   }
 
@@ -321,13 +437,19 @@ struct Integrator {
 
   void step(uint32_t n) {
     prime();
-    for (uint32_t i = 0; i < n; ++i) enqueue_step();
+    for (uint32_t i = 0; i < n; ++i) {
+      enqueue_step();
+      publish_box();
+    }
     check(hipStreamSynchronize(c.stream), "sync");
     steps += n;
     ff->check_status();
   }
 
   void capture() {
+    if (baro)
+      throw Error(HMD_ERR_UNSUPPORTED,
+                  "a per-step barostat moves the box every step: its steps run eagerly");
     prime();
     if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
     if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
@@ -360,6 +482,10 @@ struct Integrator {
   }
 
   void replay(uint32_t n) {
+    if (baro) {   // the box is a kernel argument: barostat steps are eager
+      step(n);
+      return;
+    }
     if (!exec || ff->graph_invalid) capture();
     for (uint32_t i = 0; i < n; ++i) check(hipGraphLaunch(exec, c.stream), "hipGraphLaunch");
     check(hipStreamSynchronize(c.stream), "sync");
@@ -655,7 +781,9 @@ struct BatchStart {
   DevBuf<float> posq, vel, force;
   DevBuf<int32_t> images;
   DevBuf<double> pos_lo, vel_lo, injection;
-  DevBuf<unsigned long long> step_counter, csvr_counter, draw_counter;
+  DevBuf<unsigned long long> step_counter, csvr_counter, draw_counter, baro_counter;
+  DevBuf<double> baro_state;
+  HmdLattice lattice{};
   uint64_t steps = 0;
 
   template <class T>
@@ -678,6 +806,9 @@ struct BatchStart {
     copy(step_counter, I.step_counter, s);
     copy(csvr_counter, I.csvr_counter, s);
     copy(injection, I.csvr_injection, s);
+    copy(baro_counter, I.baro_counter, s);
+    copy(baro_state, I.baro_state, s);
+    lattice = p.lattice;
     steps = I.steps;
   }
   void restore(Integrator& I) {
@@ -693,8 +824,16 @@ struct BatchStart {
     copy(I.step_counter, step_counter, s);
     copy(I.csvr_counter, csvr_counter, s);
     copy(I.csvr_injection, injection, s);
+    copy(I.baro_counter, baro_counter, s);
+    copy(I.baro_state, baro_state, s);
     check(hipStreamSynchronize(s), "sync");
     I.steps = steps;
+    const HmdLattice& L = p.lattice;
+    if (L.lx != lattice.lx || L.ly != lattice.ly || L.lz != lattice.lz || L.xy != lattice.xy ||
+        L.xz != lattice.xz || L.yz != lattice.yz) {
+      p.lattice = lattice;   // a barostat moved the box within the batch
+      for (ForceField* f : p.observers) f->on_box_change();
+    }
     I.ff->restart_list();
   }
 };
@@ -1025,7 +1164,15 @@ void hmd_replicas_destroy(hmd_replicas* h) {
   delete h;
 }
 
-hmd_status hmd_replicas_step(hmd_replicas* h, uint32_t n) { HMD_GUARD(h->set->step(n)); }
+hmd_status hmd_replicas_step(hmd_replicas* h, uint32_t n) {
+  HMD_GUARD({
+    for (Integrator* I : h->set->its)
+      if (I->baro)
+        throw Error(HMD_ERR_UNSUPPORTED,
+                    "replica batches are captured: a per-step barostat's steps run eagerly");
+    h->set->step(n);
+  });
+}
 uint64_t hmd_replicas_steps(hmd_replicas* h) { return h->set->steps; }
 
 hmd_status hmd_replicas_plan(hmd_replicas* h, uint32_t* n_batched, uint32_t* n_single) {
@@ -1050,6 +1197,32 @@ hmd_status hmd_integrator_thermostat_state(hmd_integrator* w, double out[2]) {
       unsigned long long k = 0;
       I.csvr_counter.download(&k, 1, I.c.stream);
       out[1] = (double)k;
+    }
+  });
+}
+
+hmd_status hmd_integrator_set_barostat(hmd_integrator* w, const hmd_barostat_desc* d) {
+  HMD_GUARD({
+    if (!w || !d) throw Error(HMD_ERR_INVALID, "null argument");
+    w->it->set_barostat(*d);
+  });
+}
+
+hmd_status hmd_integrator_barostat_state(hmd_integrator* w, double out[5]) {
+  HMD_GUARD({
+    if (!w || !out) throw Error(HMD_ERR_INVALID, "null argument");
+    Integrator& I = *w->it;
+    for (int k = 0; k < 5; ++k) out[k] = 0.0;
+    if (I.baro_state.count) {
+      double st[4];
+      I.baro_state.download(st, 4, I.c.stream);
+      unsigned long long k = 0;
+      I.baro_counter.download(&k, 1, I.c.stream);
+      out[0] = st[1];
+      out[1] = st[2];
+      out[2] = st[3];
+      out[3] = (double)k;
+      out[4] = st[0];
     }
   });
 }
@@ -1149,6 +1322,7 @@ hmd_status hmd_integrator_profile(hmd_integrator* w, uint32_t n, double* us, int
         throw;
       }
       c.profiling = false;
+      I.publish_box();
       check(hipStreamSynchronize(c.stream), "sync");
       for (size_t k = 1; k < c.marks.size(); ++k) {
         float ms = 0.0f;
@@ -1196,6 +1370,7 @@ hmd_status hmd_integrator_timings(hmd_integrator* w, uint32_t n, int32_t max_sta
         throw;
       }
       c.profiling = false;
+      I.publish_box();
       check(hipStreamSynchronize(c.stream), "sync");
       for (size_t k = 1; k < c.marks.size(); ++k) {
         float ms = 0.0f;

@@ -422,6 +422,21 @@ struct CsvrParams {
   uint32_t n;
 };
 
+// C-rescale barostat (kernels/barostat.cu:169-227,
+// src/integrator/c_rescale_barostat.rs:172-232).
+struct CRescaleParams {
+  const double* ke;            // [1] FP64 kinetic energy after the step's last kick
+  const double* w;             // [1] FP64 sum of the per-atom virials
+  double v_pre;                // box volume before the move
+  unsigned long long* counter; // [1] Philox draw counter, +1 per application
+  uint32_t seed_lo, seed_hi;
+  double p_target, tau, beta, kt, dt, mu3_min;
+  double* state;               // [4] mu, P, V after the move, accumulated P_target dV
+  float* posq;                 // [n] float4, scaled by (float) mu
+  double* pos_lo;              // compensated low parts (may be null)
+  uint32_t n;
+};
+
 struct ReduceParams {
   const float* x;              // [comps][n] or null
   const float* v;              // velocities [3][n] (KE mode)

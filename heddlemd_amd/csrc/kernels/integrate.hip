@@ -677,6 +677,64 @@ __device__ __forceinline__ void hmd_csvr_rescale_body(const CsvrParams& p, const
 }
 HMD_KERNEL(hmd_csvr_rescale, CsvrParams, 256)
 
+// ------------------------------------------------------- c-rescale --
+// Stochastic cell rescaling (Bernetti & Bussi, J. Chem. Phys. 153, 114107
+// (2020)), one isotropic move per step as kernels/barostat.cu:169-227 makes
+// it, in FP64 on one thread: with the kinetic energy K, the virial sum W and
+// the volume V,
+//   P   = (2K + W) / (3V),
+//   mu3 = 1 - beta (dt / tau)(P_target - P) + sqrt(2 beta kT dt / (tau V)) r,
+// r the Philox normal (seed, draw counter, 0, 0); mu3 is floored at mu3_min
+// and mu = cbrt(mu3).  The energy P_target (V mu3 - V) accumulates for the
+// conserved quantity and the draw counter advances once.
+__device__ __forceinline__ void hmd_crescale_mu_body(const CRescaleParams& p,
+    const uint32_t hmd_bx, const uint32_t hmd_gx) {
+  if (threadIdx.x != 0 || hmd_bx != 0) return;
+  const unsigned long long ctr = *p.counter;
+  const uint32_t cl = (uint32_t)(ctr & 0xFFFFFFFFull), ch = (uint32_t)(ctr >> 32);
+  const double v = p.v_pre;
+  const double pressure = (2.0 * p.ke[0] + p.w[0]) / (3.0 * v);
+  const double r = hmd_philox_gaussian_f64(p.seed_lo, p.seed_hi, cl, ch, 0u, 0u);
+  const double drift = -p.beta * (p.dt / p.tau) * (p.p_target - pressure);
+  const double noise = sqrt(2.0 * p.beta * p.kt * p.dt / (p.tau * v));
+  double mu3 = 1.0 + drift + noise * r;
+  mu3 = mu3 > p.mu3_min ? mu3 : p.mu3_min;
+  const double v_post = v * mu3;
+  p.state[0] = cbrt(mu3);
+  p.state[1] = pressure;
+  p.state[2] = v_post;
+  p.state[3] += p.p_target * (v_post - v);
+  *p.counter = ctr + 1ull;
+}
+HMD_KERNEL(hmd_crescale_mu, CRescaleParams, 64)
+
+// x <- (float) mu x for every atom (rescale_positions_device_factor,
+// kernels/barostat.cu:93-107); fractional coordinates and image counts are
+// unchanged.  In compensated mode the extended position hi + lo is scaled in
+// f64 and split again.
+__device__ __forceinline__ void hmd_crescale_positions_body(const CRescaleParams& p,
+    const uint32_t hmd_bx, const uint32_t hmd_gx) {
+  const uint32_t i = hmd_bx * blockDim.x + threadIdx.x;
+  if (i >= p.n) return;
+  const float f = (float)p.state[0];
+  float4 pq = reinterpret_cast<float4*>(p.posq)[i];
+  float x[3] = {pq.x, pq.y, pq.z};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (p.pos_lo) {
+      const size_t k = (size_t)d * p.n + i;
+      const double e = ((double)x[d] + p.pos_lo[k]) * (double)f;
+      x[d] = (float)e;
+      p.pos_lo[k] = e - (double)x[d];
+    } else {
+      x[d] = x[d] * f;
+    }
+  }
+  pq.x = x[0]; pq.y = x[1]; pq.z = x[2];
+  reinterpret_cast<float4*>(p.posq)[i] = pq;
+}
+HMD_KERNEL(hmd_crescale_positions, CRescaleParams, 256)
+
 // ------------------------------------------------------- steepest descent --
 // x += h F, with |h F| capped at max_step per atom, then wrap (setup-time
 // relaxation of synthetic lattices; the reference relaxes its examples with

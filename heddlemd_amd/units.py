@@ -10,6 +10,7 @@ ENERGY_HARTREE_TO_J = 4.359744722207101e-18
 TIME_AU_TO_S = 2.4188843265857195e-17
 FORCE_AU_TO_N = 8.238723498254079e-08
 TEMPERATURE_AU_TO_K = 315775.0248040668
+PRESSURE_AU_TO_PA = 29421015696522.1      # Hartree / Bohr^3 in Pa
 AVOGADRO = 6.02214076e23
 DALTON_KG = 1.66053906660e-27
 

@@ -92,6 +92,9 @@ hmd_status hmd_particles_upload(hmd_particles* p, const float* posq,
  * old box in place), refreshes its SPME influence table, rebuilds its list
  * at the next evaluation and re-captures its step graphs. */
 hmd_status hmd_particles_set_box(hmd_particles* p, const float lattice[6]);
+/* The current box {lx, ly, lz, xy, xz, yz} (SimulationBox::lattice,
+ * src/pbc.rs; after a barostat move, the moved one). */
+hmd_status hmd_particles_get_box(hmd_particles* p, float lattice[6]);
 /* Image counters [n][3] (the images of an init state,
  * src/io/init_state.rs InitImages). */
 hmd_status hmd_particles_upload_images(hmd_particles* p, const int32_t* images);
@@ -348,6 +351,44 @@ hmd_status hmd_integrator_rollback(hmd_integrator* it);
  * the thermostat has injected, sum of K' - K (the conserved quantity is
  * KE + PE - out[0]), out[1] the Philox draws so far. */
 hmd_status hmd_integrator_thermostat_state(hmd_integrator* it, double out[2]);
+/* C-rescale barostat (src/integrator/c_rescale_barostat.rs:21-28
+ * CRescaleBarostatParams, :172-232 apply; kernels/barostat.cu:169-227
+ * c_rescale_compute_mu, :93-107 rescale_positions_device_factor): one
+ * isotropic move per step at the plan's BarostatPoint, after the last kick
+ * and the thermostat and before the SETTLE velocity projection
+ * (src/integrator/velocity_verlet.rs:97-109, mod.rs:500-517).  pressure =
+ * target (Hartree/Bohr^3), temperature = k_B T of the noise term, tau the
+ * coupling time, compressibility in 1/pressure, seed the Philox key (draw
+ * counter from 0).  With K and W the FP64 kinetic-energy and per-atom-virial
+ * sums at that point and V the volume, P = (2K + W)/(3V), mu^3 = 1 -
+ * beta (dt/tau)(P_target - P) + sqrt(2 beta kT dt/(tau V)) r, floored at
+ * 1e-18; positions and all six lattice components are multiplied by
+ * (float) mu.  Validation as validate_params (:282-290): pressure finite,
+ * the others finite and positive, else HMD_ERR_INVALID.  Turns on
+ * energy-level evaluations (HMD_INTEGRATOR_SCALARS), as the reference's
+ * runner does for a per-step barostat (src/runner.rs:2521-2539).  The box
+ * is a kernel argument here, so barostat steps run eagerly:
+ * hmd_integrator_replay steps eagerly, hmd_integrator_capture and
+ * hmd_replicas_step return HMD_ERR_UNSUPPORTED.  kind HMD_BAROSTAT_NONE
+ * removes the barostat (energy-level evaluations stay on). */
+typedef struct {
+  uint32_t kind;
+  double pressure;
+  double temperature;
+  double tau;
+  double compressibility;
+  uint64_t seed;
+} hmd_barostat_desc;
+#define HMD_BAROSTAT_NONE 0u
+#define HMD_BAROSTAT_C_RESCALE 1u
+hmd_status hmd_integrator_set_barostat(hmd_integrator* it, const hmd_barostat_desc* desc);
+/* Barostat state (flush_pending_injection, c_rescale_barostat.rs:141-170):
+ * out[0] the pressure of the latest move, out[1] the volume after it,
+ * out[2] the accumulated P_target (V_post - V_pre) (the conserved quantity
+ * is KE + PE + P_target V - out[2], log_column_values :240-252), out[3] the
+ * Philox draws so far, out[4] the latest mu.  Zeros before the first
+ * move. */
+hmd_status hmd_integrator_barostat_state(hmd_integrator* it, double out[5]);
 /* Steps taken so far. */
 uint64_t hmd_integrator_steps(hmd_integrator* it);
 /* Per-stage GPU time (HIP events) of n eager steps: writes up to

@@ -84,6 +84,8 @@ def _declare(L):
     L.orc_csvr_knew.argtypes = [ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32,
                                 ctypes.c_uint64, ctypes.c_uint32, ctypes.c_double,
                                 ctypes.c_double]
+    L.orc_crescale_mu.argtypes = ([ctypes.c_double] * 3 + [ctypes.c_uint32, ctypes.c_uint32,
+                                  ctypes.c_uint64] + [ctypes.c_double] * 6 + [_dp])
     L.orc_kinetic_energy.restype = ctypes.c_double
     L.orc_kinetic_energy.argtypes = [ctypes.c_int, _dp, _dp]
     L.orc_num_threads.restype = ctypes.c_int
@@ -336,6 +338,15 @@ def csvr_knew(k_old, seed, counter, g_dof, c, kt_half):
     """CSVR's new kinetic energy (orc_csvr_knew)."""
     return lib().orc_csvr_knew(float(k_old), seed & 0xFFFFFFFF, seed >> 32, counter, g_dof,
                                float(c), float(kt_half))
+
+
+def crescale_mu(k, w, v, seed, counter, p_target, tau, beta, kt, dt, mu3_min=1e-18):
+    """One c-rescale move (orc_crescale_mu): (mu, P, V after, P_target dV)."""
+    out = np.zeros(4)
+    lib().orc_crescale_mu(float(k), float(w), float(v), seed & 0xFFFFFFFF, seed >> 32, counter,
+                          float(p_target), float(tau), float(beta), float(kt), float(dt),
+                          float(mu3_min), _p(out, _dp))
+    return tuple(float(x) for x in out)
 
 
 def kinetic_energy(v, m):

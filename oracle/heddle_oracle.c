@@ -1063,6 +1063,29 @@ double orc_csvr_knew(double k_old, uint32_t seed_lo, uint32_t seed_hi, uint64_t 
     return kn;
 }
 
+/* C-rescale barostat move (kernels/barostat.cu:169-227; Bernetti & Bussi,
+ * J. Chem. Phys. 153, 114107 (2020)) in FP64: with the kinetic energy k,
+ * the virial sum w and the volume v,
+ *   P   = (2k + w) / (3v),
+ *   mu3 = 1 - beta (dt/tau)(P_target - P) + sqrt(2 beta kT dt/(tau v)) r,
+ * r = philox_gaussian(seed, counter, 0, 0), mu3 floored at mu3_min.
+ * out: [mu = cbrt(mu3), P, v mu3, P_target (v mu3 - v)]. */
+void orc_crescale_mu(double k, double w, double v, uint32_t seed_lo, uint32_t seed_hi,
+                     uint64_t counter, double p_target, double tau, double beta, double kt,
+                     double dt, double mu3_min, double out[4])
+{
+    uint32_t cl = (uint32_t)(counter & 0xFFFFFFFFu), ch = (uint32_t)(counter >> 32);
+    double pressure = (2.0 * k + w) / (3.0 * v);
+    double r = orc_philox_gaussian(seed_lo, seed_hi, cl, ch, 0, 0);
+    double mu3 = 1.0 - beta * (dt / tau) * (p_target - pressure)
+               + sqrt(2.0 * beta * kt * dt / (tau * v)) * r;
+    if (!(mu3 > mu3_min)) mu3 = mu3_min;
+    out[0] = cbrt(mu3);
+    out[1] = pressure;
+    out[2] = v * mu3;
+    out[3] = p_target * (v * mu3 - v);
+}
+
 /* Langevin BAOAB A and O steps in f32 (kernels/langevin.cu:13-80) with
  * the reference's contracted roundings: x + v dt/2 once (fmaf);
  * sigma = sqrt(fl(1 - a^2) kT / m) with 1 - a^2 rounded once; v = a v +

@@ -532,3 +532,47 @@ def test_kinetic_energy_closed_form():
     v = np.array([[1.0, 2.0, 3.0], [0.5, 0, 0]])
     m = np.array([2.0, 4.0])
     assert oracle.kinetic_energy(v, m) == pytest.approx(0.5 * 2 * 14 + 0.5 * 4 * 0.25)
+
+
+def _pressure_system(p_target):
+    """The reference's 8-particle construction (tests/barostats_c_rescale.rs:
+    132-158): K from +-500 m/s velocities of 1.66e-27 kg particles in a 1 nm
+    cubic box, W chosen so that (2K + W)/(3V) is the given pressure."""
+    mass = 1.66e-27 / U.MASS_ME_TO_KG
+    v_mag = 500.0 / 2187691.2636411153
+    k = 0.5 * mass * 8 * v_mag ** 2
+    v = (1.0e-9 / U.LENGTH_BOHR_TO_M) ** 3
+    return k, 3.0 * v * p_target - 2.0 * k, v
+
+
+def test_crescale_limits_and_noise():
+    """The c-rescale move (orc_crescale_mu).  At kT -> 0 it is the Berendsen
+    move mu^3 = 1 - beta (dt/tau)(P_target - P) with P recovered from the
+    constructed K and W (the reference's
+    temperature_zero_limit_matches_berendsen_barostat, P = P_target / 2);
+    at P = P_target and kT -> 0 mu is 1; the noise over draw counters has mean
+    0 and variance 2 beta kT dt / (tau V); a collapse is floored at mu3_min;
+    the injected energy is P_target (V_post - V)."""
+    p_si = 1.0e6
+    p_au = p_si / U.PRESSURE_AU_TO_PA
+    tau = 1.0e-12 / U.TIME_AU_TO_S
+    beta = 4.5e-10 * U.PRESSURE_AU_TO_PA
+    dt = float(np.float32(1.0e-13 / U.TIME_AU_TO_S))
+    k, w, v = _pressure_system(p_au / 2.0)
+    mu, pres, v_post, inj = oracle.crescale_mu(k, w, v, 1, 0, p_au, tau, beta, 1e-30, dt)
+    assert pres == pytest.approx(p_au / 2.0, rel=1e-12)
+    assert mu ** 3 == pytest.approx(1.0 - beta * (dt / tau) * (p_au / 2.0), rel=1e-12)
+    assert v_post == pytest.approx(v * mu ** 3, rel=1e-14)
+    assert inj == pytest.approx(p_au * (v_post - v), rel=1e-12)
+    k, w, v = _pressure_system(p_au)
+    mu, pres, _, _ = oracle.crescale_mu(k, w, v, 1, 0, p_au, tau, beta, 1e-30, dt)
+    assert abs(mu - 1.0) < 1e-14
+    kt = 300.0 / U.TEMPERATURE_AU_TO_K
+    x = np.array([oracle.crescale_mu(k, w, v, 5, n, p_au, tau, beta, kt, dt)[0] ** 3 - 1.0
+                  for n in range(20000)])
+    var = 2.0 * beta * kt * dt / (tau * v)
+    assert abs(x.mean()) < 4.0 * np.sqrt(var / x.size)
+    assert x.var() == pytest.approx(var, rel=0.05)
+    mu, _, v_post, _ = oracle.crescale_mu(k, -1e6 * abs(w) - 1.0, v, 1, 0, p_au, tau,
+                                          beta * 1e6, 1e-30, dt)
+    assert mu == pytest.approx(1e-6, rel=1e-12) and v_post == pytest.approx(1e-18 * v)
