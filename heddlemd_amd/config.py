@@ -13,9 +13,13 @@ velocity-verlet (optionally `lossless`) or langevin-baoab integrator and
 its `[phase.output]` cadence; `[[minimization]]` steepest descent;
 `[[particle_types]]`; `[[pair_interactions]]` of kind lennard-jones and a
 `[lennard_jones]` Lorentz-Berthelot table; `[spme]`; `[[constraint_types]]`
-of kind settle with topology `[constraints]` rows; `[neighbor_list]`.
-Everything else the reference accepts (thermostats, barostats, bonded
-terms, SHAKE, analysis) raises ConfigError naming the unsupported key.
+of kind settle with topology `[constraints]` rows; `[neighbor_list]`;
+`[phase.thermostat]` of kind csvr (src/integrator/csvr.rs) and
+`[phase.barostat]` of kind c-rescale (src/integrator/c_rescale_barostat.rs),
+with their log columns (csvr_conserved; pressure, box_volume,
+c_rescale_conserved).  Everything else the reference accepts (other
+thermostats and barostats, bonded terms, SHAKE, analysis) raises
+ConfigError naming the unsupported key.
 """
 import math
 import os
@@ -36,7 +40,8 @@ _SI = {"length": U.LENGTH_BOHR_TO_M, "inverse_length": 1.0 / U.LENGTH_BOHR_TO_M,
        "mass": U.MASS_ME_TO_KG, "charge": U.CHARGE_E_TO_C,
        "energy": U.ENERGY_HARTREE_TO_J, "time": U.TIME_AU_TO_S,
        "inverse_time": 1.0 / U.TIME_AU_TO_S, "temperature": U.TEMPERATURE_AU_TO_K,
-       "velocity": 2187691.2636411153}
+       "velocity": 2187691.2636411153, "pressure": U.PRESSURE_AU_TO_PA,
+       "inverse_pressure": 1.0 / U.PRESSURE_AU_TO_PA}
 
 
 class ConfigError(ValueError):
@@ -320,7 +325,8 @@ def load(path):
         cfg["phases"].append({"kind": "minimization", "name": m["name"],
                               "max_iterations": int(alg.get("max_iterations", 1000))})
     for p in raw.get("phase", []):
-        _keys(p, "[[phase]]", {"name", "n_steps", "dt", "integrator", "output"})
+        _keys(p, "[[phase]]", {"name", "n_steps", "dt", "integrator", "output", "thermostat",
+                               "barostat"})
         integ = dict(_require(p, "integrator", "[[phase]]"))
         kind = integ.pop("kind", None)
         if kind == "velocity-verlet":
@@ -335,6 +341,8 @@ def load(path):
                   "seed": int(_require(integ, "seed", kind))}
         else:
             raise ConfigError(f"integrator kind `{kind}` is not supported")
+        th = _thermostat(p.get("thermostat"), units, kind)
+        baro = _barostat(p.get("barostat"), units)
         out = p.get("output", {})
         _keys(out, "[phase.output]", {"trajectory_every", "include_velocities", "include_images",
                                       "log_every"})
@@ -342,7 +350,7 @@ def load(path):
             "kind": "md", "name": _require(p, "name", "[[phase]]"),
             "n_steps": int(_require(p, "n_steps", "[[phase]]")),
             "dt": _positive("dt", _from_user(units, "time", _require(p, "dt", "[[phase]]"))),
-            "integrator": it,
+            "integrator": it, "thermostat": th, "barostat": baro,
             "trajectory_every": int(out.get("trajectory_every", 100)),
             "include_velocities": bool(out.get("include_velocities", True)),
             "include_images": bool(out.get("include_images", True)),
@@ -350,6 +358,54 @@ def load(path):
     if not cfg["phases"]:
         raise ConfigError("no [[phase]] or [[minimization]] entries")
     return cfg
+
+
+def _thermostat(t, units, integrator_kind):
+    """`[phase.thermostat]`: CSVR (csvr.rs:21-30 params, :282-300 checks),
+    coupling every step."""
+    if t is None:
+        return None
+    t = dict(t)
+    kind = t.get("kind")
+    if kind != "csvr":
+        raise ConfigError(f"thermostat kind `{kind}` is not supported (csvr only)")
+    if integrator_kind != "velocity-verlet":
+        raise ConfigError(f"integrator `{integrator_kind}` owns its own thermostat and is "
+                          "incompatible with `[phase.thermostat]`")
+    _keys(t, "[phase.thermostat]", {"kind", "temperature", "tau", "seed", "coupling_interval"})
+    if int(t.get("coupling_interval", 1)) != 1:
+        raise ConfigError("thermostat.coupling_interval: only 1 (every step) is supported")
+    return {"kind": kind,
+            "temperature": _positive("thermostat.temperature", _from_user(
+                units, "temperature", _require(t, "temperature", "[phase.thermostat]"))),
+            "tau": _positive("thermostat.tau", _from_user(
+                units, "time", _require(t, "tau", "[phase.thermostat]"))),
+            "seed": int(_require(t, "seed", "[phase.thermostat]"))}
+
+
+def _barostat(b, units):
+    """`[phase.barostat]`: c-rescale (CRescaleBarostatParams,
+    c_rescale_barostat.rs:21-28; validate_params :282-290)."""
+    if b is None:
+        return None
+    b = dict(b)
+    kind = b.get("kind")
+    if kind != "c-rescale":
+        raise ConfigError(f"barostat kind `{kind}` is not supported (c-rescale only)")
+    _keys(b, "[phase.barostat]", {"kind", "pressure", "temperature", "tau", "compressibility",
+                                  "seed"})
+    where = "[phase.barostat]"
+    pressure = _from_user(units, "pressure", _require(b, "pressure", where))
+    if not math.isfinite(pressure):
+        raise ConfigError(f"barostat.pressure must be finite, got {pressure}")
+    return {"kind": kind, "pressure": pressure,
+            "temperature": _positive("barostat.temperature", _from_user(
+                units, "temperature", _require(b, "temperature", where))),
+            "tau": _positive("barostat.tau", _from_user(units, "time",
+                                                        _require(b, "tau", where))),
+            "compressibility": _positive("barostat.compressibility", _from_user(
+                units, "inverse_pressure", _require(b, "compressibility", where))),
+            "seed": int(_require(b, "seed", where))}
 
 
 def _lj_table(raw, units, names, types):
@@ -507,17 +563,34 @@ def _run_phase(api, ph, cfg, sysd, ctx, part, field, dof, out_base, pre):
         integ.close()
         return {"name": ph["name"], "kind": "minimization"}
     it = ph["integrator"]
-    integ = api.Integrator(ctx, part, field, kind=it["kind"], dt=ph["dt"],
-                           compensated=it["lossless"], temperature=it.get("temperature", 0.0),
-                           gamma=it.get("friction", 0.0), seed=it.get("seed", 0),
-                           settle=sysd["settle"])
+    th, baro = ph.get("thermostat"), ph.get("barostat")
+    if th is not None:
+        integ = api.Integrator(ctx, part, field, kind=it["kind"], dt=ph["dt"],
+                               compensated=it["lossless"], temperature=th["temperature"],
+                               seed=th["seed"], thermostat="csvr", tau=th["tau"],
+                               settle=sysd["settle"], scalars=True)
+    else:
+        integ = api.Integrator(ctx, part, field, kind=it["kind"], dt=ph["dt"],
+                               compensated=it["lossless"],
+                               temperature=it.get("temperature", 0.0),
+                               gamma=it.get("friction", 0.0), seed=it.get("seed", 0),
+                               settle=sysd["settle"], scalars=baro is not None)
+    if baro is not None:
+        integ.set_barostat("c-rescale", pressure=baro["pressure"],
+                           temperature=baro["temperature"], tau=baro["tau"],
+                           compressibility=baro["compressibility"], seed=baro["seed"])
+    # extra log columns in the reference's order: integrator (none),
+    # thermostat, barostat (runner.rs:1426-1433)
+    extra = ([("csvr_conserved", "energy")] if th is not None else []) + (
+        [("pressure", "pressure"), ("box_volume", "dimensionless"),
+         ("c_rescale_conserved", "energy")] if baro is not None else [])
     traj = log = None
     if ph["trajectory_every"] > 0:
         traj = api.TrajectoryWriter(out_base + ph["name"] + ".xyz", cfg["units"],
                                     ph["include_velocities"], ph["include_images"],
                                     sysd["type_names"])
     if ph["log_every"] > 0:
-        log = api.LogWriter(out_base + ph["name"] + ".log", cfg["units"])
+        log = api.LogWriter(out_base + ph["name"] + ".log", cfg["units"], extra_columns=extra)
 
     def outputs(step):
         want_t = traj is not None and step % ph["trajectory_every"] == 0
@@ -529,20 +602,29 @@ def _run_phase(api, ph, cfg, sysd, ctx, part, field, dof, out_base, pre):
         host["device_to_host_download"].append(time.perf_counter() - t0)
         if want_t:
             t0 = time.perf_counter()
-            traj.write_frame(step, ph["dt"], sysd["lattice"], sysd["types"], d["posq"][:, :3],
+            traj.write_frame(step, ph["dt"], part.lattice, sysd["types"], d["posq"][:, :3],
                              velocities=d["vel"] if ph["include_velocities"] else None,
                              images=d["images"] if ph["include_images"] else None)
             host["trajectory_write"].append(time.perf_counter() - t0)
         if want_l:
             t0 = time.perf_counter()
             ke = host_kinetic_energy(sysd["mass"], d["vel"])
-            log.write_row(step, step * ph["dt"], ke, 2.0 * ke / dof if dof else 0.0)
+            vals = []
+            if extra:
+                pe, _ = part.energy_sums()
+                if th is not None:      # csvr.rs:242-248
+                    vals.append(ke + pe - integ.thermostat_state()[0])
+                if baro is not None:    # c_rescale_barostat.rs:240-252
+                    p_last, v_last, inj, _, _ = integ.barostat_state()
+                    vals += [p_last, v_last, ke + pe + baro["pressure"] * v_last - inj]
+            log.write_row(step, step * ph["dt"], ke, 2.0 * ke / dof if dof else 0.0, vals)
             host["log_write"].append(time.perf_counter() - t0)
 
     try:
         field.compute(1)                       # warm-up: forces at the current positions
         outputs(0)
-        integ.capture()
+        if baro is None:               # barostat steps run eagerly
+            integ.capture()
         step = 0
         cad = [c for c in (ph["trajectory_every"], ph["log_every"]) if c > 0]
         while step < ph["n_steps"]:

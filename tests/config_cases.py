@@ -67,6 +67,23 @@ r_switch = 8.5e-10
     return d / "argon.in.toml"
 
 
+NPT_TABLES = """
+[phase.thermostat]
+kind = "csvr"
+temperature = 298.15
+tau = 1.0e-13
+seed = 12
+
+[phase.barostat]
+kind = "c-rescale"
+pressure = 1.01325e5
+temperature = 298.15
+tau = 1.0e-12
+compressibility = 4.5e-10
+seed = 13
+"""
+
+
 def water_run(d, grid=(8, 8, 8), n_steps=20, kind="langevin-baoab"):
     w = systems.spc_water(grid=grid, seed=3)
     L_m = w["L"] * U.LENGTH_BOHR_TO_M
@@ -75,7 +92,11 @@ def water_run(d, grid=(8, 8, 8), n_steps=20, kind="langevin-baoab"):
     rows = "\n".join(f"{3 * m} {3 * m + 1} {3 * m + 2} SPC" for m in range(w["n_mol"]))
     (d / "water.in.topology").write_text("# rigid SPC\n[constraints]\n" + rows + "\n")
     integ = ('kind = "langevin-baoab"\nfriction = 1.0e12\ntemperature = 300.0\nseed = 7'
-             if kind == "langevin-baoab" else 'kind = "velocity-verlet"\nlossless = true')
+             if kind == "langevin-baoab" else 'kind = "velocity-verlet"\nlossless = true'
+             if kind == "velocity-verlet" else 'kind = "velocity-verlet"\nlossless = false')
+    # kind "npt": velocity Verlet with the CSVR thermostat and the c-rescale
+    # barostat of the reference's examples/spc-water-8192 production phase
+    coupling = NPT_TABLES if kind == "npt" else ""
     (d / "water.in.toml").write_text(f"""schema_version = 1
 init = "water.in.xyz"
 topology = "water.in.topology"
@@ -96,6 +117,7 @@ n_steps = {n_steps}
 dt = 2.0e-15
 [phase.integrator]
 {integ}
+{coupling}
 [phase.output]
 trajectory_every = 10
 log_every = 10

@@ -137,6 +137,32 @@ def test_unsupported_or_invalid_is_an_error(tmp_path, patch, msg):
         C.load(p)
 
 
+def test_thermostat_and_barostat_tables(tmp_path):
+    """[phase.thermostat] csvr and [phase.barostat] c-rescale in SI, converted
+    once as value / factor (the values of the reference's
+    examples/spc-water-8192 production phase); CSVR on a Langevin integrator
+    and a non-positive compressibility are errors."""
+    p = K.water_run(tmp_path, kind="npt")
+    cfg = C.load(p)
+    ph = [x for x in cfg["phases"] if x["kind"] == "md"][0]
+    assert ph["thermostat"] == {"kind": "csvr", "temperature": 298.15 / U.TEMPERATURE_AU_TO_K,
+                                "tau": 1.0e-13 / U.TIME_AU_TO_S, "seed": 12}
+    assert ph["barostat"] == {"kind": "c-rescale", "pressure": 1.01325e5 / U.PRESSURE_AU_TO_PA,
+                              "temperature": 298.15 / U.TEMPERATURE_AU_TO_K,
+                              "tau": 1.0e-12 / U.TIME_AU_TO_S,
+                              "compressibility": 4.5e-10 / (1.0 / U.PRESSURE_AU_TO_PA),
+                              "seed": 13}
+    text = p.read_text()
+    p.write_text(text.replace('kind = "velocity-verlet"\nlossless = false',
+                              'kind = "langevin-baoab"\nfriction = 1.0e12\n'
+                              'temperature = 300.0\nseed = 7', 1))
+    with pytest.raises(C.ConfigError, match="owns its own thermostat"):
+        C.load(p)
+    p.write_text(text.replace("compressibility = 4.5e-10", "compressibility = 0.0", 1))
+    with pytest.raises(C.ConfigError, match="compressibility"):
+        C.load(p)
+
+
 def test_host_kinetic_energy_order():
     """Sequential f64 sum of m |v|^2 in index order, halved
     (src/io/log_output.rs:121-138)."""

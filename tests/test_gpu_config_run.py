@@ -1,6 +1,7 @@
 """End-to-end runs of `.in.toml` configurations through heddlemd_amd.config:
 an argon-like LJ box (f32 and lossless VV) and a rigid SPC water box
-(minimization, then Langevin BAOAB + SETTLE + SPME).  Each writes
+(minimization, then Langevin BAOAB + SETTLE + SPME, or velocity Verlet with
+the CSVR thermostat and the c-rescale barostat).  Each writes
 `<stem>.out.<phase>.xyz`, `.log` and `.timings`; two runs in separate
 directories must give byte-identical trajectories and logs, with the frame
 and row counts the cadence implies (step 0 included)."""
@@ -12,14 +13,15 @@ from tests import config_cases as K
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("case", ["argon", "argon_lossless", "water"])
+@pytest.mark.parametrize("case", ["argon", "argon_lossless", "water", "water_npt"])
 def test_config_run_outputs_reproducible(tmp_path, case):
     outs = []
     for tag in ("a", "b"):
         d = tmp_path / tag
         d.mkdir()
-        if case == "water":
-            p, stem, phase, n_atoms, rows, frames = K.water_run(d), "water", "md", 1536, 3, 3
+        if case.startswith("water"):
+            p = K.water_run(d, kind="npt" if case == "water_npt" else "langevin-baoab")
+            stem, phase, n_atoms, rows, frames = "water", "md", 1536, 3, 3
         else:
             p = K.argon_run(d, lossless=(case == "argon_lossless"))
             stem, phase, n_atoms, rows, frames = "argon", "run", 512, 5, 3
@@ -31,6 +33,13 @@ def test_config_run_outputs_reproducible(tmp_path, case):
         assert len(xyz.splitlines()) == frames * (n_atoms + 2)
         assert len(log.splitlines()) == 1 + rows
         assert tim.splitlines()[0].split()[:2] == ["stage", "count"]
+        if case == "water_npt":
+            head = log.decode().splitlines()[0]
+            for col in ("csvr_conserved", "pressure", "box_volume", "c_rescale_conserved"):
+                assert col in head
+            lat = [x.split('Lattice="')[1].split('"')[0] for x in xyz.decode().splitlines()
+                   if x.startswith("Lattice=") or 'Lattice="' in x]
+            assert lat[0] != lat[-1]          # the barostat moved the box
         outs.append((xyz, log))
     print(f"{case}: last log row {outs[0][1].decode().splitlines()[-1]}")
     assert outs[0] == outs[1]
