@@ -58,6 +58,7 @@ EXPORTED = [
     "hmd_integrator_set_forcefield", "hmd_integrator_run_judged",
     "hmd_integrator_thermostat_state", "hmd_integrator_checkpoint", "hmd_integrator_rollback",
     "hmd_integrator_set_barostat", "hmd_integrator_barostat_state",
+    "hmd_forcefield_add_bonded", "hmd_bonded_compute",
     "hmd_generate_velocities", "hmd_chacha_block",
     "hmd_mlp_param_count", "hmd_mlp_create", "hmd_mlp_destroy", "hmd_mlp_set_params",
     "hmd_mlp_get_params", "hmd_mlp_set_force_scale", "hmd_mlp_descriptors",
@@ -120,6 +121,12 @@ class BarostatDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("pressure", ctypes.c_double),
                 ("temperature", ctypes.c_double), ("tau", ctypes.c_double),
                 ("compressibility", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+class BondedDesc(ctypes.Structure):
+    """hmd_bonded_desc (one set of bonded terms)."""
+    _fields_ = [("kind", ctypes.c_uint32), ("n_terms", ctypes.c_uint32),
+                ("terms", _c_u32p), ("n_types", ctypes.c_uint32), ("params", _c_dp)]
 
 
 class SlotOutput(ctypes.Structure):
@@ -239,6 +246,8 @@ def _declare(L):
         "hmd_integrator_set_forcefield": (st, [vp, vp]),
         "hmd_integrator_thermostat_state": (st, [vp, _c_dp]),
         "hmd_integrator_set_barostat": (st, [vp, ctypes.POINTER(BarostatDesc)]),
+        "hmd_forcefield_add_bonded": (st, [vp, ctypes.POINTER(BondedDesc)]),
+        "hmd_bonded_compute": (st, [vp, ctypes.c_int32, ctypes.POINTER(SlotOutput)]),
         "hmd_integrator_barostat_state": (st, [vp, _c_dp]),
         "hmd_integrator_checkpoint": (st, [vp]),
         "hmd_integrator_rollback": (st, [vp]),

@@ -5,7 +5,8 @@ import ctypes
 
 import numpy as np
 
-from ._lib import BarostatDesc, ForceFieldDesc, IntegratorDesc, MlpDesc, Tuning, check, lib
+from ._lib import (BarostatDesc, BondedDesc, ForceFieldDesc, IntegratorDesc, MlpDesc, Tuning,
+                   check, lib)
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -317,6 +318,30 @@ class ForceField:
     def pair_force(self, level, out):
         """The pair slots added into a SlotOutput's device arrays, enqueued."""
         check(lib().hmd_pair_force(self.h, level, ctypes.byref(out)))
+
+    BONDED_KINDS = {"harmonic-bond": (1, 2), "morse": (2, 2), "harmonic-angle": (3, 3),
+                    "periodic-dihedral": (4, 4)}
+
+    def add_bonded(self, kind, terms, params):
+        """One set of bonded terms (hmd_forcefield_add_bonded): kind one of
+        BONDED_KINDS; terms (n, arity + 1) canonical atoms then the type;
+        params (n_types, <= 3) per type, atomic units (harmonic-bond k, r0;
+        morse De, a, r0; harmonic-angle k_theta, theta0; periodic-dihedral
+        k_phi, phi0, n)."""
+        code, arity = self.BONDED_KINDS[kind]
+        t = np.ascontiguousarray(terms, np.uint32).reshape(-1, arity + 1)
+        q = np.asarray(params, np.float64)
+        q = q.reshape(q.shape[0] if q.ndim > 1 else 1, -1)
+        pp = np.zeros((q.shape[0], 3))
+        pp[:, :q.shape[1]] = q
+        d = BondedDesc()
+        d.kind, d.n_terms, d.terms = code, t.shape[0], _ptr(t, _u32p)
+        d.n_types, d.params = pp.shape[0], _ptr(pp, _dp)
+        check(lib().hmd_forcefield_add_bonded(self.h, ctypes.byref(d)))
+
+    def bonded_compute(self, level, out):
+        """Every bonded set as one slot, added into a SlotOutput, enqueued."""
+        check(lib().hmd_bonded_compute(self.h, level, ctypes.byref(out)))
 
     def spme_compute(self, level, out):
         """The reciprocal slot added into a SlotOutput's device arrays, enqueued."""

@@ -1012,6 +1012,7 @@ void ForceField::compute(int level) {
   if (fork) neighbor_pre_step(fork_spme);
   else neighbor_pre_step();
   pair_forces(fev);
+  bonded_forces(fev);
   if (spme_on && !fork) spme_forces(fev);
   if (fork) check(hipStreamWaitEvent(c.stream, ev_join, 0), "hipStreamWaitEvent");
   if (mlp) mlp_correct(*mlp, *this);
@@ -1029,6 +1030,79 @@ void ForceField::compute(int level) {
   fp.acc_clear = acc.ptr;
   launch(c, K_hmd_pair_finalize, ceil_div(n, 256), 256, 0, fp);
   c.mark("finalize");
+}
+
+// Bonded terms (src/forces/harmonic_bond.rs, morse.rs, angle.rs,
+// dihedral.rs; validate_params of each): terms [n][arity + 1] (canonical
+// atoms, then the type), params [n_types][3].  The step graphs are
+// re-captured.
+void ForceField::add_bonded(uint32_t kind, uint32_t n_terms, const uint32_t* terms,
+                            uint32_t n_types, const double* params) {
+  uint32_t arity = 0;
+  switch (kind) {
+    case HMD_BONDED_HARMONIC_BOND: case HMD_BONDED_MORSE_BOND: arity = 2; break;
+    case HMD_BONDED_HARMONIC_ANGLE: arity = 3; break;
+    case HMD_BONDED_PERIODIC_DIHEDRAL: arity = 4; break;
+    default: throw Error(HMD_ERR_INVALID, "unknown bonded kind");
+  }
+  if (n_terms && (!terms || !n_types || !params)) throw Error(HMD_ERR_INVALID, "null argument");
+  const double pi = 3.14159265358979323846;
+  std::vector<float> ty(4 * (size_t)n_types, 0.0f);
+  for (uint32_t t = 0; t < n_types; ++t) {
+    const double* q = params + 3 * (size_t)t;
+    auto pos = [](double v) { return std::isfinite(v) && v > 0.0; };
+    bool ok = true;
+    if (kind == HMD_BONDED_HARMONIC_BOND) ok = pos(q[0]) && pos(q[1]);
+    else if (kind == HMD_BONDED_MORSE_BOND) ok = pos(q[0]) && pos(q[1]) && pos(q[2]);
+    else if (kind == HMD_BONDED_HARMONIC_ANGLE)
+      ok = pos(q[0]) && std::isfinite(q[1]) && q[1] >= 0.0 && q[1] <= pi;
+    else
+      ok = std::isfinite(q[0]) && std::isfinite(q[1]) && q[2] >= 0.0 && q[2] <= 64.0 &&
+           q[2] == std::floor(q[2]);
+    if (!ok) throw Error(HMD_ERR_INVALID, "bonded type " + std::to_string(t) + ": invalid parameters");
+    for (int k = 0; k < 3; ++k) ty[4 * (size_t)t + k] = (float)q[k];
+  }
+  const uint32_t stride = arity + 1;
+  for (uint32_t m = 0; m < n_terms; ++m) {
+    const uint32_t* q = terms + (size_t)stride * m;
+    for (uint32_t a = 0; a < arity; ++a) {
+      if (q[a] >= n) throw Error(HMD_ERR_INVALID, "bonded term atom out of range");
+      for (uint32_t b = 0; b < a; ++b)
+        if (q[a] == q[b]) throw Error(HMD_ERR_INVALID, "bonded term repeats an atom");
+    }
+    if (q[arity] >= n_types) throw Error(HMD_ERR_INVALID, "bonded term type out of range");
+  }
+  auto set = std::make_unique<BondedSet>();
+  set->kind = kind;
+  set->n_terms = n_terms;
+  if (n_terms) {
+    set->terms.alloc((size_t)stride * n_terms, false);
+    set->terms.upload(terms, (size_t)stride * n_terms, c.stream);
+    set->types.alloc(ty.size(), false);
+    set->types.upload(ty.data(), ty.size(), c.stream);
+  }
+  bonded.push_back(std::move(set));
+  graph_invalid = true;
+}
+
+// Every bonded set adds into the pair accumulators (enqueue only).
+void ForceField::bonded_forces(bool fev) {
+  for (const auto& b : bonded) {
+    if (!b->n_terms) continue;
+    BondedParams q{};
+    q.posq = p.posq.ptr;
+    q.terms = b->terms.ptr;
+    q.types = reinterpret_cast<const float4*>(b->types.ptr);
+    q.slot_of = slot_of.ptr;
+    q.acc = acc.ptr;
+    q.acc_stride = n_pad;
+    q.L = p.lattice;
+    q.n_terms = b->n_terms;
+    q.kind = b->kind;
+    q.fev = fev ? 1u : 0u;
+    launch(c, K_hmd_bonded_terms, ceil_div(b->n_terms, 256), 256, 0, q);
+  }
+  if (!bonded.empty()) c.mark("bonded");
 }
 
 // Host-side status check at a batch boundary: the analogue of the 4-byte
@@ -1121,6 +1195,33 @@ hmd_status hmd_pair_force(hmd_forcefield* f, int32_t level, const hmd_slot_outpu
     check_slot_output(F, level, out);
     const bool fev = level != 0;
     F.pair_forces(fev);
+    SlotAddParams q{};
+    q.acc = F.acc.ptr;
+    q.acc_stride = F.n_pad;
+    q.slot_of = F.slot_of.ptr;
+    q.acc_clear = F.acc.ptr;
+    q.out_x = out->force_x; q.out_y = out->force_y; q.out_z = out->force_z;
+    q.out_e = out->energy; q.out_w = out->virial;
+    q.n = F.n;
+    q.fev = fev ? 1u : 0u;
+    launch(F.c, K_hmd_slot_add, ceil_div(F.n, 256), 256, 0, q);
+  });
+}
+
+hmd_status hmd_forcefield_add_bonded(hmd_forcefield* f, const hmd_bonded_desc* d) {
+  HMD_GUARD({
+    if (!f || !d) throw Error(HMD_ERR_INVALID, "null argument");
+    f->ff->add_bonded(d->kind, d->n_terms, d->terms, d->n_types, d->params);
+  });
+}
+
+hmd_status hmd_bonded_compute(hmd_forcefield* f, int32_t level, const hmd_slot_output* out) {
+  HMD_GUARD({
+    if (!f) throw Error(HMD_ERR_INVALID, "null force field");
+    ForceField& F = *f->ff;
+    check_slot_output(F, level, out);
+    const bool fev = level != 0;
+    F.bonded_forces(fev);
     SlotAddParams q{};
     q.acc = F.acc.ptr;
     q.acc_stride = F.n_pad;

@@ -140,6 +140,16 @@ struct ForceField {
 
   // ---- learned correction of the reciprocal forces (optional)
   Mlp* mlp = nullptr;
+  // bonded terms (one set per hmd_forcefield_add_bonded call)
+  struct BondedSet {
+    uint32_t kind = 0, n_terms = 0;
+    DevBuf<uint32_t> terms;
+    DevBuf<float> types;   // [n_types][4]
+  };
+  std::vector<std::unique_ptr<BondedSet>> bonded;
+  void add_bonded(uint32_t kind, uint32_t n_terms, const uint32_t* terms, uint32_t n_types,
+                  const double* params);
+  void bonded_forces(bool fev);
 
   ForceField(Context& ctx, Particles& part, const hmd_forcefield_desc& d);
 

@@ -43,7 +43,7 @@
   X(hmd_lan_drift_half) X(hmd_lan_ou) X(hmd_counter_increment)               \
   X(hmd_step_vv_first) X(hmd_step_baoa) X(hmd_step_kick_last)                \
   X(hmd_sd_step) X(hmd_reduce_partial) X(hmd_reduce_final)                 \
-  X(hmd_csvr_sample) X(hmd_csvr_finish) X(hmd_csvr_rescale) X(hmd_crescale_mu) X(hmd_crescale_positions)                 \
+  X(hmd_csvr_sample) X(hmd_csvr_finish) X(hmd_csvr_rescale) X(hmd_crescale_mu) X(hmd_crescale_positions) X(hmd_bonded_terms)                 \
   X(hmd_nn_pack) X(hmd_nn_descriptors) X(hmd_nn_desc_finalize)               \
   X(hmd_nn_forward) X(hmd_nn_backward) X(hmd_nn_grad_reduce)                 \
   X(hmd_nn_finalize) X(hmd_nn_residual) X(hmd_nn_adam) X(hmd_nn_record)      \

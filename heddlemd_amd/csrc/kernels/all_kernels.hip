@@ -8,3 +8,4 @@
 #include "fft.hip"
 #include "integrate.hip"
 #include "nn.hip"
+#include "bonded.hip"

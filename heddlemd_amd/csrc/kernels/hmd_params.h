@@ -227,6 +227,25 @@ struct PairScaledParams {
   uint32_t acc_stride;
 };
 
+// Bonded terms (harmonic_bond.rs, morse.rs, angle.rs, dihedral.rs), added
+// into the pair accumulators.
+#define HMD_BONDED_HARMONIC_BOND 1u
+#define HMD_BONDED_MORSE_BOND 2u
+#define HMD_BONDED_HARMONIC_ANGLE 3u
+#define HMD_BONDED_PERIODIC_DIHEDRAL 4u
+struct BondedParams {
+  const float* posq;           // [n] float4, canonical
+  const uint32_t* terms;       // [n_terms][arity + 1]: canonical atoms, then the type
+  const float4* types;         // per type (k, r0) / (De, a, r0) / (k, theta0) / (k, phi0, n)
+  const uint32_t* slot_of;     // canonical -> sorted
+  long long* acc;              // [5][acc_stride] fixed point (2^-40)
+  uint32_t acc_stride;
+  HmdLattice L;
+  uint32_t n_terms;
+  uint32_t kind;
+  uint32_t fev;
+};
+
 struct PairFinalizeParams {
   const long long* acc;
   uint32_t acc_stride;

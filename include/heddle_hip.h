@@ -185,6 +185,37 @@ void hmd_forcefield_destroy(hmd_forcefield* ff);
 /* ForceField::run (src/forces/mod.rs:944-1432).  level 0 = forces,
  * 1 = forces + per-atom energies and virials (AggregateLevel). */
 hmd_status hmd_forcefield_compute(hmd_forcefield* ff, int32_t level);
+/* ---- bonded terms ---------------------------------------------------------
+ * The intramolecular potentials the reference composes into one module per
+ * arity (src/forces/jit_composed.rs:1739-2406): HMD_BONDED_HARMONIC_BOND
+ * (harmonic_bond.rs:316-360; params k, r0), HMD_BONDED_MORSE_BOND
+ * (morse.rs:330-360; De, a, r0), HMD_BONDED_HARMONIC_ANGLE
+ * (angle.rs:350-410; k_theta, theta0 in radians, angle i-j-k at j) and
+ * HMD_BONDED_PERIODIC_DIHEDRAL (dihedral.rs:440-530; k_phi, phi0, n).
+ * terms: [n_terms][arity + 1] canonical atom ids then the type index;
+ * params: [n_types][3] in atomic units (unused entries ignored).  Each call
+ * adds one set (a slot) to the force field; hmd_forcefield_compute adds
+ * every set's forces, and at level 1 the energy and virial shares (1/2,
+ * 1/3, 1/4 of the term's per endpoint), to the fast-class totals.  Invalid
+ * parameters (k, r0, De, a not finite and positive; theta0 outside [0, pi];
+ * a dihedral n not an integer in [0, 64]), atoms out of range or repeated
+ * within a term, and types out of range are HMD_ERR_INVALID.  Exclusions of
+ * bonded partners from the pair terms are the caller's, through the
+ * force-field descriptor's exclusion lists, as the reference's topology
+ * supplies them. */
+typedef struct {
+  uint32_t kind;
+  uint32_t n_terms;
+  const uint32_t* terms;
+  uint32_t n_types;
+  const double* params;
+} hmd_bonded_desc;
+#define HMD_BONDED_HARMONIC_BOND 1u
+#define HMD_BONDED_MORSE_BOND 2u
+#define HMD_BONDED_HARMONIC_ANGLE 3u
+#define HMD_BONDED_PERIODIC_DIHEDRAL 4u
+hmd_status hmd_forcefield_add_bonded(hmd_forcefield* ff, const hmd_bonded_desc* desc);
+
 /* ---- slot level (Potential::compute with a SlotOutputView,
  * src/forces/mod.rs:86-133, 206-212) ------------------------------------
  * The same evaluation one slot at a time, each adding (+=) its per-atom
@@ -213,6 +244,10 @@ hmd_status hmd_pair_force(hmd_forcefield* ff, int32_t level, const hmd_slot_outp
 /* The "spme-reciprocal" slot (spme.rs:858-914), with an attached network's
  * correction when there is one, added into out. */
 hmd_status hmd_spme_compute(hmd_forcefield* ff, int32_t level, const hmd_slot_output* out);
+/* Every bonded set of the force field as one slot (the composed bonded,
+ * angle and dihedral modules' launches, jit_composed.rs:1759-2406), += into
+ * out; enqueue only. */
+hmd_status hmd_bonded_compute(hmd_forcefield* ff, int32_t level, const hmd_slot_output* out);
 /* Synchronize the context's stream and read the neighbour status (the
  * per-batch 4-byte read of pre_step, neighbor_list.rs:1308-1320). */
 hmd_status hmd_forcefield_check(hmd_forcefield* ff);

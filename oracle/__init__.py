@@ -86,6 +86,8 @@ def _declare(L):
                                 ctypes.c_double]
     L.orc_crescale_mu.argtypes = ([ctypes.c_double] * 3 + [ctypes.c_uint32, ctypes.c_uint32,
                                   ctypes.c_uint64] + [ctypes.c_double] * 6 + [_dp])
+    L.orc_bonded.restype = ctypes.c_double
+    L.orc_bonded.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _up, _dp, _dp, _dp, _dp]
     L.orc_kinetic_energy.restype = ctypes.c_double
     L.orc_kinetic_energy.argtypes = [ctypes.c_int, _dp, _dp]
     L.orc_num_threads.restype = ctypes.c_int
@@ -347,6 +349,28 @@ def crescale_mu(k, w, v, seed, counter, p_target, tau, beta, kt, dt, mu3_min=1e-
                           float(p_target), float(tau), float(beta), float(kt), float(dt),
                           float(mu3_min), _p(out, _dp))
     return tuple(float(x) for x in out)
+
+
+BONDED_KINDS = {"harmonic-bond": (1, 2), "morse": (2, 2), "harmonic-angle": (3, 3),
+                "periodic-dihedral": (4, 4)}
+
+
+def bonded(kind, pos, lattice, terms, params):
+    """Bonded terms in FP64 (orc_bonded): (forces (n, 3), per-atom energy,
+    per-atom virial, total energy)."""
+    code, arity = BONDED_KINDS[kind]
+    x = _f64(pos).reshape(-1, 3)
+    L = _f64(lattice)
+    t = np.ascontiguousarray(terms, np.uint32).reshape(-1, arity + 1)
+    q = np.asarray(params, np.float64)
+    q = q.reshape(q.shape[0] if q.ndim > 1 else 1, -1)
+    pp = np.zeros((q.shape[0], 3))
+    pp[:, :q.shape[1]] = q
+    n = x.shape[0]
+    f = np.zeros((n, 3)); e = np.zeros(n); w = np.zeros(n)
+    u = lib().orc_bonded(code, _p(x, _dp), _p(L, _dp), t.shape[0], _p(t, _up), _p(pp, _dp),
+                         _p(f, _dp), _p(e, _dp), _p(w, _dp))
+    return f, e, w, u
 
 
 def kinetic_energy(v, m):

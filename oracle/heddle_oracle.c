@@ -1146,3 +1146,111 @@ int orc_num_threads(void)
     return 1;
 #endif
 }
+
+/* ---- bonded terms ----------------------------------------------------------
+ * Harmonic bond U = k (r - r0)^2 / 2 (src/forces/harmonic_bond.rs:316-360),
+ * Morse U = De (1 - exp(-a (r - r0)))^2 (morse.rs:330-360), harmonic angle
+ * U = k (theta - theta0)^2 / 2 at the middle atom (angle.rs:350-410) and
+ * periodic dihedral U = k (1 + cos(n phi - phi0)) with phi = atan2(|b2| b1 .
+ * (b2 x b3), (b1 x b2) . (b2 x b3)), b1 = r_i - r_j, b2 = r_k - r_j,
+ * b3 = r_k - r_l (dihedral.rs:440-530), in FP64.  kind 1..4 as above;
+ * terms [n_terms][arity + 1]; params [n_types][3].  Adds each endpoint's
+ * force to f[atom][3] and the term's energy and virial (sum of (r_a - r_j)
+ * . F_a) in equal shares to e[atom], w[atom].  Returns the total energy. */
+double orc_bonded(int kind, const double *pos, const double *L, int n_terms,
+                  const uint32_t *terms, const double *params, double *f, double *e,
+                  double *w)
+{
+    int arity = kind <= 2 ? 2 : kind == 3 ? 3 : 4;
+    double total = 0.0;
+    for (int t = 0; t < n_terms; ++t) {
+        const uint32_t *q = terms + (size_t)(arity + 1) * t;
+        const double *c = params + 3 * (size_t)q[arity];
+        double d[3][3];            /* displacements: a -> (q[0]-q[1]), b, c */
+        double fa[4][3] = {{0}};
+        double u = 0.0;
+        int pairs[3][2] = {{0, 1}, {2, 1}, {2, 3}};
+        int nd = arity == 2 ? 1 : arity == 3 ? 2 : 3;
+        for (int m = 0; m < nd; ++m) {
+            const double *pa = pos + 3 * (size_t)q[pairs[m][0]];
+            const double *pb = pos + 3 * (size_t)q[pairs[m][1]];
+            d[m][0] = pa[0] - pb[0]; d[m][1] = pa[1] - pb[1]; d[m][2] = pa[2] - pb[2];
+            mi_f64(&d[m][0], &d[m][1], &d[m][2], L);
+        }
+        if (arity == 2) {
+            double r = sqrt(d[0][0] * d[0][0] + d[0][1] * d[0][1] + d[0][2] * d[0][2]);
+            double g = 0.0;        /* -dU/dr */
+            if (r > 0.0) {
+                if (kind == 1) {
+                    u = 0.5 * c[0] * (r - c[1]) * (r - c[1]);
+                    g = -c[0] * (r - c[1]);
+                } else {
+                    double ex = exp(-c[1] * (r - c[2]));
+                    u = c[0] * (1.0 - ex) * (1.0 - ex);
+                    g = -2.0 * c[0] * c[1] * (1.0 - ex) * ex;
+                }
+                for (int k = 0; k < 3; ++k) { fa[0][k] = g * d[0][k] / r; fa[1][k] = -fa[0][k]; }
+            }
+        } else if (arity == 3) {
+            double *a = d[0], *b = d[1];
+            double la = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+            double lb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+            double dot = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+            double cx[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2],
+                            a[0] * b[1] - a[1] * b[0]};
+            double lc = sqrt(cx[0] * cx[0] + cx[1] * cx[1] + cx[2] * cx[2]);
+            if (la > 0.0 && lb > 0.0 && lc > 1e-7 * la * lb) {
+                double theta = atan2(lc, dot), ct = dot / (la * lb), st = lc / (la * lb);
+                double dth = theta - c[1];
+                u = 0.5 * c[0] * dth * dth;
+                /* dtheta/dr_i = (cos a/|a|^2 - b/(|a||b|)) / sin, likewise k */
+                double g = -c[0] * dth / st;
+                for (int k = 0; k < 3; ++k) {
+                    fa[0][k] = g * (ct * a[k] / (la * la) - b[k] / (la * lb));
+                    fa[2][k] = g * (ct * b[k] / (lb * lb) - a[k] / (la * lb));
+                    fa[1][k] = -(fa[0][k] + fa[2][k]);
+                }
+            }
+        } else {
+            double *b1 = d[0], *b2 = d[1], *b3 = d[2];
+            double m[3] = {b1[1] * b2[2] - b1[2] * b2[1], b1[2] * b2[0] - b1[0] * b2[2],
+                           b1[0] * b2[1] - b1[1] * b2[0]};
+            double nn[3] = {b2[1] * b3[2] - b2[2] * b3[1], b2[2] * b3[0] - b2[0] * b3[2],
+                            b2[0] * b3[1] - b2[1] * b3[0]};
+            double m2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+            double n2 = nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2];
+            double bb = b2[0] * b2[0] + b2[1] * b2[1] + b2[2] * b2[2];
+            if (m2 >= 1e-14 && n2 >= 1e-14 && bb >= 1e-14) {
+                double lb = sqrt(bb);
+                double phi = atan2(lb * (b1[0] * nn[0] + b1[1] * nn[1] + b1[2] * nn[2]),
+                                   m[0] * nn[0] + m[1] * nn[1] + m[2] * nn[2]);
+                double delta = c[2] * phi - c[1];
+                u = c[0] * (1.0 + cos(delta));
+                double fphi = c[0] * c[2] * sin(delta);     /* -dU/dphi */
+                double s = (b1[0] * b2[0] + b1[1] * b2[1] + b1[2] * b2[2]) / bb;
+                double tt = (b3[0] * b2[0] + b3[1] * b2[1] + b3[2] * b2[2]) / bb;
+                for (int k = 0; k < 3; ++k) {
+                    fa[0][k] = fphi * lb / m2 * m[k];
+                    fa[3][k] = -fphi * lb / n2 * nn[k];
+                    fa[1][k] = (s - 1.0) * fa[0][k] - tt * fa[3][k];
+                    fa[2][k] = -s * fa[0][k] + (tt - 1.0) * fa[3][k];
+                }
+            }
+        }
+        /* virial about atom q[1]: r_i - r_j = d[0], r_k - r_j = d[1],
+         * r_l - r_j = d[1] - d[2] */
+        double wv = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            wv += d[0][k] * fa[0][k];
+            if (arity >= 3) wv += d[1][k] * fa[2][k];
+            if (arity == 4) wv += (d[1][k] - d[2][k]) * fa[3][k];
+        }
+        for (int a2 = 0; a2 < arity; ++a2) {
+            for (int k = 0; k < 3; ++k) f[3 * (size_t)q[a2] + k] += fa[a2][k];
+            e[q[a2]] += u / arity;
+            w[q[a2]] += wv / arity;
+        }
+        total += u;
+    }
+    return total;
+}
