@@ -59,6 +59,7 @@ EXPORTED = [
     "hmd_integrator_thermostat_state", "hmd_integrator_checkpoint", "hmd_integrator_rollback",
     "hmd_integrator_set_barostat", "hmd_integrator_barostat_state",
     "hmd_forcefield_add_bonded", "hmd_bonded_compute",
+    "hmd_forcefield_set_fragment_exclusion_scales",
     "hmd_generate_velocities", "hmd_chacha_block",
     "hmd_mlp_param_count", "hmd_mlp_create", "hmd_mlp_destroy", "hmd_mlp_set_params",
     "hmd_mlp_get_params", "hmd_mlp_set_force_scale", "hmd_mlp_descriptors",
@@ -247,6 +248,7 @@ def _declare(L):
         "hmd_integrator_thermostat_state": (st, [vp, _c_dp]),
         "hmd_integrator_set_barostat": (st, [vp, ctypes.POINTER(BarostatDesc)]),
         "hmd_forcefield_add_bonded": (st, [vp, ctypes.POINTER(BondedDesc)]),
+        "hmd_forcefield_set_fragment_exclusion_scales": (st, [vp, _c_fp, ctypes.c_uint32]),
         "hmd_bonded_compute": (st, [vp, ctypes.c_int32, ctypes.POINTER(SlotOutput)]),
         "hmd_integrator_barostat_state": (st, [vp, _c_dp]),
         "hmd_integrator_checkpoint": (st, [vp]),

@@ -250,6 +250,13 @@ class ForceField:
         check(lib().hmd_forcefield_set_pair_fragment(self.h, name.encode(), source.encode(),
                                                      _ptr(prm, _fp), len(params)))
 
+    def set_fragment_exclusion_scales(self, scales):
+        """The composed term's scale for each listed exclusion partner, in
+        the exclusion lists' CSR order
+        (hmd_forcefield_set_fragment_exclusion_scales)."""
+        s = np.ascontiguousarray(scales, np.float32)
+        check(lib().hmd_forcefield_set_fragment_exclusion_scales(self.h, _ptr(s, _fp), s.size))
+
     @staticmethod
     def _tuning_dict(t):
         return {"pair_blocks_per_cu": t.pair_blocks_per_cu,

@@ -75,10 +75,10 @@ void ForceField::set_pair_fragment(const std::string& name, const std::string& s
   std::vector<char> code = compile_pair_fragment(name, source, nullptr);
   auto m = std::make_unique<JitPairModule>();
   check(hipModuleLoadData(&m->module, code.data()), "hipModuleLoadData (composed pair kernels)");
-  const KernelId ks[6] = {K_hmd_pair_force_f, K_hmd_pair_force_fev, K_hmd_pair_force_lj_f,
+  const KernelId ks[7] = {K_hmd_pair_force_f, K_hmd_pair_force_fev, K_hmd_pair_force_lj_f,
                           K_hmd_pair_force_lj_fev, K_hmd_pair_force_coul_f,
-                          K_hmd_pair_force_coul_fev};
-  for (int k = 0; k < 6; ++k) {
+                          K_hmd_pair_force_coul_fev, K_hmd_pair_scaled};
+  for (int k = 0; k < 7; ++k) {
     m->kernel[k] = ks[k];
     check(hipModuleGetFunction(&m->fn[k], m->module, kKernelNames[ks[k]]),
           kKernelNames[ks[k]]);
@@ -94,7 +94,7 @@ void ForceField::set_pair_fragment(const std::string& name, const std::string& s
 
 hipFunction_t ForceField::pair_function(KernelId k) const {
   if (jit)
-    for (int i = 0; i < 6; ++i)
+    for (int i = 0; i < 7; ++i)
       if (jit->kernel[i] == k) return jit->fn[i];
   return c.fn[k];
 }

@@ -294,34 +294,19 @@ ForceField::ForceField(Context& ctx, Particles& part, const hmd_forcefield_desc&
     epar.assign(d.excl_partners, d.excl_partners + eoff[n]);
     for (uint32_t m = 0; m < eoff[n]; ++m)
       if (epar[m] >= n) throw Error(HMD_ERR_INVALID, "exclusion partner out of range");
-    // pairs with a non-zero scale (taken from i's list for i < j), added
-    // back after the fused kernel by hmd_pair_scaled
-    std::vector<uint32_t> si, sj;
-    std::vector<float> slj, sco;
-    for (uint32_t i = 0; i < n; ++i)
-      for (uint32_t m = eoff[i]; m < eoff[i + 1]; ++m) {
-        const float a = d.excl_lj_scale ? d.excl_lj_scale[m] : 0.0f;
-        const float b = d.excl_coul_scale ? d.excl_coul_scale[m] : 0.0f;
-        if (epar[m] <= i || (a == 0.0f && b == 0.0f)) continue;
-        if (!std::isfinite(a) || !std::isfinite(b))
-          throw Error(HMD_ERR_INVALID, "exclusion scale not finite");
-        si.push_back(i);
-        sj.push_back(epar[m]);
-        slj.push_back(lj_on ? a : 0.0f);
-        sco.push_back(coul_on ? b : 0.0f);
-      }
-    n_scaled = (uint32_t)si.size();
-    if (n_scaled) {
-      sc_i.alloc(n_scaled, false);
-      sc_j.alloc(n_scaled, false);
-      sc_lj.alloc(n_scaled, false);
-      sc_coul.alloc(n_scaled, false);
-      sc_i.upload(si.data(), n_scaled, s);
-      sc_j.upload(sj.data(), n_scaled, s);
-      sc_lj.upload(slj.data(), n_scaled, s);
-      sc_coul.upload(sco.data(), n_scaled, s);
-      check(hipStreamSynchronize(s), "sync");
+    h_eoff = eoff;
+    h_epar = epar;
+    h_slj.assign(eoff[n], 0.0f);
+    h_sco.assign(eoff[n], 0.0f);
+    for (uint32_t m = 0; m < eoff[n]; ++m) {
+      const float a = d.excl_lj_scale ? d.excl_lj_scale[m] : 0.0f;
+      const float b = d.excl_coul_scale ? d.excl_coul_scale[m] : 0.0f;
+      if (!std::isfinite(a) || !std::isfinite(b))
+        throw Error(HMD_ERR_INVALID, "exclusion scale not finite");
+      h_slj[m] = lj_on ? a : 0.0f;
+      h_sco[m] = coul_on ? b : 0.0f;
     }
+    build_scaled_list(s);
   }
   if (epar.empty()) epar.push_back(0u);
   excl_off.alloc(eoff.size(), false);
@@ -817,7 +802,21 @@ void ForceField::pair_forces(bool fev) {
     q.fev = fev ? 1u : 0u;
     q.acc = acc.ptr;
     q.acc_stride = n_pad;
-    launch(c, K_hmd_pair_scaled, ceil_div(n_scaled, 256), 256, 0, q);
+    q.s_user = jit && sc_user.count == n_scaled ? sc_user.ptr : nullptr;
+    q.user_params = user_params.ptr;
+    const uint32_t sb = ceil_div(n_scaled, 256);
+    if (jit && c.rec) {
+      const unsigned char* b = reinterpret_cast<const unsigned char*>(&q);
+      c.rec->push_back(LaunchRec{K_hmd_pair_scaled, pair_function(K_hmd_pair_scaled), nullptr,
+                                 sb, 256, 0, std::vector<unsigned char>(b, b + sizeof q)});
+    } else if (jit) {
+      void* args[] = {&q};
+      check(hipModuleLaunchKernel(pair_function(K_hmd_pair_scaled), sb, 1, 1, 256, 1, 1, 0,
+                                  c.stream, args, nullptr),
+            "composed scaled-pair kernel");
+    } else {
+      launch(c, K_hmd_pair_scaled, sb, 256, 0, q);
+    }
   }
   c.mark("pair_force");
 }
@@ -1032,6 +1031,56 @@ void ForceField::compute(int level) {
   c.mark("finalize");
 }
 
+// Listed pairs with a non-zero scale for any term (taken from i's list for
+// i < j), added back after the fused kernel by hmd_pair_scaled.
+void ForceField::build_scaled_list(hipStream_t s) {
+  std::vector<uint32_t> si, sj;
+  std::vector<float> slj, sco, sus;
+  for (uint32_t i = 0; i + 1 < (uint32_t)h_eoff.size(); ++i)
+    for (uint32_t m = h_eoff[i]; m < h_eoff[i + 1]; ++m) {
+      const float a = h_slj[m], b = h_sco[m], u = h_suser.empty() ? 0.0f : h_suser[m];
+      if (h_epar[m] <= i || (a == 0.0f && b == 0.0f && u == 0.0f)) continue;
+      si.push_back(i);
+      sj.push_back(h_epar[m]);
+      slj.push_back(a);
+      sco.push_back(b);
+      sus.push_back(u);
+    }
+  n_scaled = (uint32_t)si.size();
+  if (n_scaled) {
+    sc_i.alloc(n_scaled, false);
+    sc_j.alloc(n_scaled, false);
+    sc_lj.alloc(n_scaled, false);
+    sc_coul.alloc(n_scaled, false);
+    sc_i.upload(si.data(), n_scaled, s);
+    sc_j.upload(sj.data(), n_scaled, s);
+    sc_lj.upload(slj.data(), n_scaled, s);
+    sc_coul.upload(sco.data(), n_scaled, s);
+    if (!h_suser.empty()) {
+      sc_user.alloc(n_scaled, false);
+      sc_user.upload(sus.data(), n_scaled, s);
+    }
+  }
+  graph_invalid = true;
+}
+
+// The composed user term's exclusion scale for each listed partner (CSR
+// entry m of the force field's exclusion lists): the per-fragment
+// exclusion_scale(i, j) of src/forces/lj.rs:451 and jit_composed.rs:939-972.
+// Without it a composed term has scale 0 for listed partners.
+void ForceField::set_fragment_exclusion_scales(const float* scales, uint32_t n_entries) {
+  if (!jit) throw Error(HMD_ERR_INVALID, "no composed pair term on this force field");
+  const uint32_t want = h_eoff.empty() ? 0u : h_eoff.back();
+  if (n_entries != want)
+    throw Error(HMD_ERR_INVALID, "expected one scale per listed exclusion partner (" +
+                                     std::to_string(want) + ")");
+  for (uint32_t m = 0; m < n_entries; ++m)
+    if (!std::isfinite(scales[m])) throw Error(HMD_ERR_INVALID, "exclusion scale not finite");
+  h_suser.assign(scales, scales + n_entries);
+  build_scaled_list(c.stream);
+  check(hipStreamSynchronize(c.stream), "sync");
+}
+
 // Bonded terms (src/forces/harmonic_bond.rs, morse.rs, angle.rs,
 // dihedral.rs; validate_params of each): terms [n][arity + 1] (canonical
 // atoms, then the type), params [n_types][3].  The step graphs are
@@ -1205,6 +1254,14 @@ hmd_status hmd_pair_force(hmd_forcefield* f, int32_t level, const hmd_slot_outpu
     q.n = F.n;
     q.fev = fev ? 1u : 0u;
     launch(F.c, K_hmd_slot_add, ceil_div(F.n, 256), 256, 0, q);
+  });
+}
+
+hmd_status hmd_forcefield_set_fragment_exclusion_scales(hmd_forcefield* f, const float* scales,
+                                                        uint32_t n_entries) {
+  HMD_GUARD({
+    if (!f || (n_entries && !scales)) throw Error(HMD_ERR_INVALID, "null argument");
+    f->ff->set_fragment_exclusion_scales(scales, n_entries);
   });
 }
 

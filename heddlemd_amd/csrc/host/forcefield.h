@@ -12,8 +12,8 @@ struct Mlp;
 // Pair kernels recompiled at run time with a user pair term (compose.cpp).
 struct JitPairModule {
   hipModule_t module = nullptr;
-  KernelId kernel[6] = {};
-  hipFunction_t fn[6] = {};
+  KernelId kernel[7] = {};
+  hipFunction_t fn[7] = {};
   JitPairModule() = default;
   JitPairModule(const JitPairModule&) = delete;
   JitPairModule& operator=(const JitPairModule&) = delete;
@@ -53,7 +53,14 @@ struct ForceField {
   // listed pairs with non-zero exclusion scales (i < j)
   uint32_t n_scaled = 0;
   DevBuf<uint32_t> sc_i, sc_j;
-  DevBuf<float> sc_lj, sc_coul;
+  DevBuf<float> sc_lj, sc_coul, sc_user;
+  // the exclusion lists and their scales on the host (CSR, one entry per
+  // listed partner), to rebuild the scaled list when a composed term brings
+  // scales of its own
+  std::vector<uint32_t> h_eoff, h_epar;
+  std::vector<float> h_slj, h_sco, h_suser;
+  void build_scaled_list(hipStream_t s);
+  void set_fragment_exclusion_scales(const float* scales, uint32_t n_entries);
 #ifndef HMD_PAIR_BLOCKS_PER_CU
 #define HMD_PAIR_BLOCKS_PER_CU 32
 #endif

@@ -225,6 +225,8 @@ struct PairScaledParams {
   uint32_t lj_on, coul_on, fev;
   long long* acc;
   uint32_t acc_stride;
+  const float* s_user;          // [n_pairs] scale of a composed user term, or null
+  const float* user_params;
 };
 
 // Bonded terms (harmonic_bond.rs, morse.rs, angle.rs, dihedral.rs), added

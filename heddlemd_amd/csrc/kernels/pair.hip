@@ -528,6 +528,17 @@ __device__ __forceinline__ void hmd_pair_scaled_body(const PairScaledParams& p,
     factor += p.s_lj[k] * fmaf(sw, f_lj, chain * e_lj);
     if (p.fev) energy += p.s_lj[k] * (sw * e_lj);
   }
+#ifdef HMD_USER_FUNCTOR
+  // the composed term's own exclusion_scale(i, j) (src/forces/
+  // jit_composed.rs:939-972): s_user x the term for each listed pair
+  if (p.s_user && p.s_user[k] != 0.0f) {
+    float f_u = 0.0f, e_u = 0.0f;
+    HMD_USER_FUNCTOR{}(p.user_params, p.type[i], p.type[j], r2, __builtin_amdgcn_rsqf(r2), f_u,
+                       e_u);
+    factor += p.s_user[k] * f_u;
+    if (p.fev) energy += p.s_user[k] * e_u;
+  }
+#endif
   const uint32_t si = p.slot_of[i], sj = p.slot_of[j], st = p.acc_stride;
   const float f3[3] = {factor * dx, factor * dy, factor * dz};
 #pragma unroll

@@ -471,6 +471,17 @@ hmd_status hmd_timings_write(const char* path, uint32_t n_stages, const char* co
 hmd_status hmd_forcefield_set_pair_fragment(hmd_forcefield* ff, const char* functor_name,
                                             const char* functor_source, const float* params,
                                             uint32_t n_params);
+/* The composed term's exclusion_scale(i, j) (src/forces/lj.rs:451, the
+ * per-fragment scale the composer multiplies into each fragment's result,
+ * jit_composed.rs:939-972): one scale per entry of the force field's
+ * exclusion lists (excl_offsets[n] entries, in CSR order; the pair (i, j) is
+ * taken from i's list for i < j).  Listed pairs within the cutoff then get
+ * scale x the composed term, beside the built-in terms' own scales; without
+ * this call the composed term has scale 0 for listed partners.  Needs a
+ * composed term (HMD_ERR_INVALID otherwise), the exact entry count and
+ * finite scales. */
+hmd_status hmd_forcefield_set_fragment_exclusion_scales(hmd_forcefield* ff, const float* scales,
+                                                        uint32_t n_entries);
 /* Compile a fragment without a device (validation); the compiler log goes
  * to `log` (log_len bytes, may be 0). */
 hmd_status hmd_pair_fragment_compile(const char* functor_name, const char* functor_source,

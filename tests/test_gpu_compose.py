@@ -85,3 +85,58 @@ def test_morse_fragment_matches_oracle(gpu_ctx):
     print(f"LJ + Morse fragment: rel RMS force {err_f:.3e}, rel PE {err_e:.3e} "
           f"({keep.sum()} pairs within r_c)")
     assert err_f <= 1e-5 and err_e <= 1e-6
+
+
+def test_fragment_exclusion_scale_on_listed_pairs(gpu_ctx):
+    """The composed term's own exclusion_scale (lj.rs:451): on the 3,888-atom
+    water box with a Morse fragment, all-zero scales give the bytes of the
+    fragment without scales (listed partners skipped), and scale 0.5 adds
+    0.5 x the Morse term of every listed pair (i < j from i's list), summed
+    in FP64 at the same positions: forces to 1e-5 relative RMS, energy to
+    1e-5 relative.  Scales need a composed term and one entry per listed
+    partner."""
+    from heddlemd_amd._lib import HmdError
+    from tests.gpu_helpers import water_case
+    sysw, ff, part, field, integ = water_case(gpu_ctx, relax=40)
+    D, alpha, r0 = 0.01, 1.0, 2.5
+    off = np.asarray(ff["exclusions"][0], np.int64)
+    par = np.asarray(ff["exclusions"][1], np.int64)
+    n_entries = int(off[-1])
+    try:
+        with pytest.raises(HmdError, match="no composed"):
+            field.set_fragment_exclusion_scales(np.zeros(n_entries))
+        field.set_pair_fragment("MorsePair", F.MORSE, [D, alpha, r0])
+        d0, _, _ = evaluate(field, part)
+        pe0, _ = part.energy_sums()
+        with pytest.raises(HmdError, match="one scale per listed"):
+            field.set_fragment_exclusion_scales(np.zeros(n_entries + 1))
+        field.set_fragment_exclusion_scales(np.zeros(n_entries))
+        dz, _, _ = evaluate(field, part)
+        assert dz["force"].tobytes() == d0["force"].tobytes()
+        s = 0.5
+        field.set_fragment_exclusion_scales(np.full(n_entries, s))
+        d1, _, _ = evaluate(field, part)
+        pe1, _ = part.energy_sums()
+    finally:
+        for h in (integ, field, part):
+            h.close()
+    x = d0["posq"][:, :3].astype(np.float64)
+    L = float(sysw["lattice"][0])
+    owner = np.repeat(np.arange(off.size - 1), np.diff(off))
+    keep = par > owner
+    pi, pj = owner[keep], par[keep]
+    dvec = x[pi] - x[pj]
+    dvec -= L * np.round(dvec / L)
+    r = np.linalg.norm(dvec, axis=1)
+    ex = np.exp(-alpha * (r - r0))
+    fr = -2.0 * D * alpha * (1 - ex) * ex / r
+    f_m = np.zeros_like(x)
+    np.add.at(f_m, pi, fr[:, None] * dvec)
+    np.add.at(f_m, pj, -fr[:, None] * dvec)
+    e_m = np.sum(D * (1 - ex) ** 2 - D)
+    got = d1["force"].astype(np.float64) - d0["force"].astype(np.float64)
+    err = rel_rms(got, s * f_m)
+    print(f"{pi.size} listed pairs; scaled Morse forces rel RMS {err:.2e}; "
+          f"energy {pe1 - pe0:.9e} vs {s * e_m:.9e}")
+    assert err < 1e-5
+    assert pe1 - pe0 == pytest.approx(s * e_m, rel=1e-5)
