@@ -715,11 +715,18 @@ void ForceField::rebuild_list() {
   launch(c, K_hmd_nbl_close, 1, 64, 0, wp);
 }
 
-void ForceField::neighbor_pre_step(const std::function<void()>& after_sort) {
-  NblDispParams dp{p.posq.ptr, ref_pos.ptr, n, p.lattice,
-                   0.25f * r_skin * r_skin, status.ptr};
-  launch(c, K_hmd_nbl_displacement, ceil_div(n, 256), 256, 0, dp);
-  c.mark("nbl_displacement");
+NblDispParams ForceField::disp_params() const {
+  return NblDispParams{p.posq.ptr, ref_pos.ptr, n, p.lattice, 0.25f * r_skin * r_skin,
+                       status.ptr};
+}
+
+void ForceField::neighbor_pre_step(const std::function<void()>& after_sort,
+                                   bool displacement_checked) {
+  if (!displacement_checked) {
+    const NblDispParams dp = disp_params();
+    launch(c, K_hmd_nbl_displacement, ceil_div(n, 256), 256, 0, dp);
+    c.mark("nbl_displacement");
+  }
   rebuild_sort();
   if (after_sort) after_sort();
   rebuild_list();
@@ -982,7 +989,7 @@ void ForceField::spme_forces(bool fev) {
   c.mark("spme_gather");
 }
 
-void ForceField::compute(int level) {
+void ForceField::compute(int level, bool displacement_checked) {
   const bool fev = level != 0;
   PairFinalizeParams fp{};
   // SPME on a second stream: forked once the sorted positions are written
@@ -1008,8 +1015,8 @@ void ForceField::compute(int level) {
     std::swap(c.stream, side);
     check(hipEventRecord(ev_join, side), "hipEventRecord");
   };
-  if (fork) neighbor_pre_step(fork_spme);
-  else neighbor_pre_step();
+  if (fork) neighbor_pre_step(fork_spme, displacement_checked);
+  else neighbor_pre_step({}, displacement_checked);
   pair_forces(fev);
   bonded_forces(fev);
   if (spme_on && !fork) spme_forces(fev);

@@ -174,12 +174,16 @@ struct ForceField {
   void rebuild_chain();
   void rebuild_sort();
   void rebuild_list();
-  void neighbor_pre_step(const std::function<void()>& after_sort = {});
+  void neighbor_pre_step(const std::function<void()>& after_sort = {},
+                         bool displacement_checked = false);
+  NblDispParams disp_params() const;
   void pair_forces(bool fev);
   void spme_forces(bool fev);
   // One force evaluation, enqueued only (no host synchronisation): safe to
   // capture into a graph.
-  void compute(int level);
+  // displacement_checked: the caller's kernel has already run the skin
+  // check on the current positions (the fused first integrator kernel)
+  void compute(int level, bool displacement_checked = false);
   void check_status();
 };
 

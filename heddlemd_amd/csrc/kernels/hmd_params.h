@@ -418,6 +418,8 @@ struct StepParams {
   uint32_t n_free;
   float dt;
   unsigned long long* counter; // Langevin draw counter (null for VV)
+  NblDispParams disp;          // the force field's skin check on the new
+                               // positions (disp.status null: not done here)
 };
 
 struct CounterParams {

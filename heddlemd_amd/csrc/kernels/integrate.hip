@@ -474,7 +474,22 @@ __device__ inline bool hmd_unit(const StepParams& p, uint32_t u, uint32_t& g, ui
   return k < p.n_free;
 }
 
-// velocity Verlet: [snapshot] kick-drift [SETTLE positions]
+// The neighbour list's skin check (hmd_nbl_displacement, the same f32
+// arithmetic) on an atom's final position of the step, folded into the
+// kernel that wrote it: one dependent launch less before the force
+// evaluation.
+__device__ inline void hmd_step_disp_check(const NblDispParams& d, uint32_t atom) {
+#pragma clang fp contract(off)
+  if (!d.status) return;
+  const float4 a = reinterpret_cast<const float4*>(d.posq)[atom];
+  const float4 r = reinterpret_cast<const float4*>(d.ref_pos)[atom];
+  float dx = a.x - r.x, dy = a.y - r.y, dz = a.z - r.z;
+  hmd_min_image(dx, dy, dz, d.L);
+  const float d2 = dx * dx + dy * dy + dz * dz;
+  if (d2 > d.threshold_sq) atomicOr(d.status, HMD_STATUS_DISPLACED);
+}
+
+// velocity Verlet: [snapshot] kick-drift [SETTLE positions] [skin check]
 __device__ __forceinline__ void hmd_step_vv_first_body(const StepParams& p, const uint32_t hmd_bx,
     const uint32_t hmd_gx) {
   const uint32_t u = hmd_bx * blockDim.x + threadIdx.x;
@@ -485,8 +500,11 @@ __device__ __forceinline__ void hmd_step_vv_first_body(const StepParams& p, cons
 #pragma unroll 1
     for (int a = 0; a < 3; ++a) hmd_vv_kick_drift_atom(p.vv, p.st.group_atoms[3 * g + a]);
     hmd_settle_positions_group(p.st, g, p.dt);
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) hmd_step_disp_check(p.disp, p.st.group_atoms[3 * g + a]);
   } else {
     hmd_vv_kick_drift_atom(p.vv, atom);
+    hmd_step_disp_check(p.disp, atom);
   }
 }
 HMD_KERNEL(hmd_step_vv_first, StepParams, 256)
