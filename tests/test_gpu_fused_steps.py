@@ -6,7 +6,8 @@ velocities, image counters and compensated low parts byte-identical after
 40 graph-replayed steps, for compensated VV + SETTLE, Langevin BAOAB +
 SETTLE, VV without constraints (argon) and a water box with half of its
 molecules left unconstrained (groups and free atoms in one launch; there the
-hydrogens get a small LJ core)."""
+hydrogens get a small LJ core); and the same number of list rebuilds, the
+fused velocity-Verlet plan running the skin check inside its first kernel."""
 import numpy as np
 import pytest
 
@@ -59,6 +60,7 @@ def run(kind, fused):
         integ.replay(STEPS)
         d = part.download()
         d["pos_lo"], d["vel_lo"] = part.lo_parts()
+        d["generation"] = field.stats()["generation"]
         return d
     finally:
         integ.close(); field.close(); part.close(); ctx.close()
@@ -72,3 +74,9 @@ def test_fused_plan_byte_identical(kind):
         same = a[k].tobytes() == b[k].tobytes()
         print(f"{kind}: {k} identical {same}", flush=True)
         assert same
+    # the fused velocity-Verlet plan runs the skin check inside its first
+    # kernel: the same list rebuilds as the displacement kernel's
+    print(f"{kind}: list generation fused {a['generation']}, unfused {b['generation']}")
+    assert a["generation"] == b["generation"]
+    if kind == "compensated":
+        assert a["generation"] > 2
