@@ -360,8 +360,8 @@ struct Integrator {
     sp.n_free = n_free;
     sp.dt = dt;
     sp.counter = kind == Kind::LangevinBaoab ? step_counter.ptr : nullptr;
-    // the velocity-Verlet first kernel runs the force field's skin check
-    if (kind == Kind::VelocityVerlet) sp.disp = ff->disp_params();
+    // the first fused kernel runs the force field's skin check
+    sp.disp = ff->disp_params();
     return sp;
   }
 
@@ -393,7 +393,7 @@ struct Integrator {
       launch(c, kind == Kind::VelocityVerlet ? K_hmd_step_vv_first : K_hmd_step_baoa, nu, 256,
              0, sp);
       c.mark(kind == Kind::VelocityVerlet ? "kick_drift_settle" : "baoa");
-      ff->compute(level(), kind == Kind::VelocityVerlet);
+      ff->compute(level(), true);
       if (csvr || baro) {
         post_force_tail(nb);
         return;

@@ -510,7 +510,7 @@ __device__ __forceinline__ void hmd_step_vv_first_body(const StepParams& p, cons
 HMD_KERNEL(hmd_step_vv_first, StepParams, 256)
 
 // Langevin BAOAB up to the force evaluation: B [SETTLE v] [snapshot] A
-// [SETTLE x, dt/2] O [SETTLE v] [snapshot] A [SETTLE x, dt/2]
+// [SETTLE x, dt/2] O [SETTLE v] [snapshot] A [SETTLE x, dt/2] [skin check]
 __device__ __forceinline__ void hmd_step_baoa_body(const StepParams& p, const uint32_t hmd_bx,
     const uint32_t hmd_gx) {
   const uint32_t u = hmd_bx * blockDim.x + threadIdx.x;
@@ -533,11 +533,14 @@ __device__ __forceinline__ void hmd_step_baoa_body(const StepParams& p, const ui
 #pragma unroll 1
     for (int a = 0; a < 3; ++a) hmd_lan_drift_half_atom(p.lan, ga[a]);
     hmd_settle_positions_group(p.st, g, h);
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) hmd_step_disp_check(p.disp, ga[a]);
   } else {
     hmd_vv_kick_atom(p.vv, atom);
     hmd_lan_drift_half_atom(p.lan, atom);
     hmd_lan_ou_atom(p.lan, atom);
     hmd_lan_drift_half_atom(p.lan, atom);
+    hmd_step_disp_check(p.disp, atom);
   }
 }
 HMD_KERNEL(hmd_step_baoa, StepParams, 256)
