@@ -1138,6 +1138,9 @@ hmd_status hmd_replicas_create(hmd_integrator* const* its, uint32_t n, uint32_t 
     std::vector<Integrator*> v;
     for (uint32_t r = 0; r < n; ++r) {
       if (!its[r]) throw Error(HMD_ERR_INVALID, "null integrator");
+      if (its[r]->it->baro)
+        throw Error(HMD_ERR_UNSUPPORTED,
+                    "replica batches are captured: a per-step barostat's steps run eagerly");
       v.push_back(its[r]->it);
     }
     auto* h = new hmd_replicas();

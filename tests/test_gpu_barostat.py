@@ -164,6 +164,8 @@ def test_crescale_rollback_repeats_and_capture_is_refused(gpu_ctx):
         assert it.thermostat_state() == t1
         with pytest.raises(HmdError, match="eagerly"):
             it.capture()
+        with pytest.raises(HmdError, match="eagerly"):
+            api.ReplicaSet([it])
         it.set_barostat(None)
         it.capture()
         it.replay(5)
