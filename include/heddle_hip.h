@@ -327,7 +327,9 @@ typedef struct {
    * settle_virial_scatter does (kernels/settle.cu:498-510, called from
    * apply_after_kick, src/integrator/settle.rs:371-401). */
   uint32_t flags;
-  /* Thermostat applied after every step (velocity Verlet only):
+  /* Thermostat applied after every step's last kick, before the SETTLE
+   * velocity projection (apply_post, src/integrator/mod.rs:500-517;
+   * velocity Verlet only):
    * HMD_THERMOSTAT_CSVR is the stochastic velocity rescaling of
    * src/integrator/csvr.rs:164-226 (kernels/nose_hoover.cu:181-360) with
    * temperature = k_B T, tau = coupling time and seed = Philox key; N_f =
