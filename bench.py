@@ -646,6 +646,9 @@ def nn_leg(api, systems, U, args, ctx, sysw, part, fine, integ_fine, run_fine):
            "roofline": {"bound": "mfma", "kernel": "hmd_nn_forward",
                         "achieved": round(ach, 3), "peak": PEAK_BF16_TFLOPS,
                         "unit": "TFLOP/s", "frac": round(ach / PEAK_BF16_TFLOPS, 5),
+                        # the same time priced at one orientation per pair
+                        # (14,848 FLOP), the network's work for each pair once
+                        "frac_one_orientation": round(0.5 * ach / PEAK_BF16_TFLOPS, 5),
                         "traffic": None, "kernel_us": round(fwd_us, 1)}}
     out["coarse_only_rel_rms"] = round(coarse_err, 6)
     for h in (integ, net, coarse):
