@@ -14,11 +14,14 @@ its `[phase.output]` cadence; `[[minimization]]` steepest descent;
 `[[particle_types]]`; `[[pair_interactions]]` of kind lennard-jones and a
 `[lennard_jones]` Lorentz-Berthelot table; `[spme]`; `[[constraint_types]]`
 of kind settle with topology `[constraints]` rows; `[neighbor_list]`;
+`[[bond_types]]` / `[[angle_types]]` / `[[dihedral_types]]` with topology
+`[bonds]` / `[angles]` / `[dihedrals]` / `[exclusions]` rows and the derived
+exclusion list (src/forces/topology.rs:1166-1260);
 `[phase.thermostat]` of kind csvr (src/integrator/csvr.rs) and
 `[phase.barostat]` of kind c-rescale (src/integrator/c_rescale_barostat.rs),
 with their log columns (csvr_conserved; pressure, box_volume,
 c_rescale_conserved).  Everything else the reference accepts (other
-thermostats and barostats, bonded terms, SHAKE, analysis) raises
+thermostats and barostats, SHAKE, analysis) raises
 ConfigError naming the unsupported key.
 """
 import math
@@ -41,6 +44,7 @@ _SI = {"length": U.LENGTH_BOHR_TO_M, "inverse_length": 1.0 / U.LENGTH_BOHR_TO_M,
        "energy": U.ENERGY_HARTREE_TO_J, "time": U.TIME_AU_TO_S,
        "inverse_time": 1.0 / U.TIME_AU_TO_S, "temperature": U.TEMPERATURE_AU_TO_K,
        "velocity": 2187691.2636411153, "pressure": U.PRESSURE_AU_TO_PA,
+       "stiffness": U.ENERGY_HARTREE_TO_J / U.LENGTH_BOHR_TO_M ** 2,
        "inverse_pressure": 1.0 / U.PRESSURE_AU_TO_PA}
 
 
@@ -186,11 +190,18 @@ def read_init(path, type_names, units="si"):
 
 # ---------------------------------------------------------------- topology --
 
-def read_topology(path, n, constraint_type_names):
-    """`[constraints]` rows (`i j k TYPE`, one group each) and an optional
-    `[charges]` section (`atom charge`, every atom or none); any other
-    section is outside this subset."""
+def read_topology(path, n, constraint_type_names, bonded_types=None):
+    """`[constraints]` rows (`i j k TYPE`, one group each), an optional
+    `[charges]` section (`atom charge`, every atom or none), `[bonds]`
+    (`i j TYPE`), `[angles]` (`i j k TYPE`), `[dihedrals]` (`i j k l TYPE`)
+    with types declared in the config, and `[exclusions]` (`i j s_lj
+    s_coul`) (src/forces/topology.rs).  The bonded rows land in
+    `bonded_types`-keyed lists returned as a fourth value when
+    `bonded_types` is given."""
     groups, group_types, charges, section = [], [], {}, None
+    bt = bonded_types or {"bond": {}, "angle": {}, "dihedral": {}}
+    bonded = {"bonds": [], "angles": [], "dihedrals": [], "exclusions": {}}
+    arity = {"bonds": (2, "bond"), "angles": (3, "angle"), "dihedrals": (4, "dihedral")}
     with open(path) as f:
         for ln_no, raw in enumerate(f, 1):
             line = raw.split("#", 1)[0].strip()
@@ -198,7 +209,8 @@ def read_topology(path, n, constraint_type_names):
                 continue
             if line.startswith("[") and line.endswith("]"):
                 section = line[1:-1].strip()
-                if section not in ("constraints", "charges"):
+                if section not in ("constraints", "charges", "bonds", "angles", "dihedrals",
+                                   "exclusions"):
                     raise ConfigError(f"topology line {ln_no}: section `[{section}]` is not "
                                       "supported by this loader")
                 continue
@@ -214,6 +226,23 @@ def read_topology(path, n, constraint_type_names):
                     raise ConfigError(f"topology line {ln_no}: atom index out of range")
                 groups.append(atoms)
                 group_types.append(cols[3])
+            elif section in arity:
+                k, cls = arity[section]
+                if len(cols) != k + 1 or cols[k] not in bt[cls]:
+                    raise ConfigError(f"topology line {ln_no}: expected {k} atoms and a "
+                                      f"declared {cls} type")
+                atoms = [int(c) for c in cols[:k]]
+                if any(a < 0 or a >= n for a in atoms) or len(set(atoms)) != k:
+                    raise ConfigError(f"topology line {ln_no}: atom index out of range or "
+                                      "repeated")
+                bonded[section].append(atoms + [cols[k]])
+            elif section == "exclusions":
+                if len(cols) != 4:
+                    raise ConfigError(f"topology line {ln_no}: expected `i j s_lj s_coul`")
+                i, j = int(cols[0]), int(cols[1])
+                if not (0 <= i < n and 0 <= j < n) or i == j:
+                    raise ConfigError(f"topology line {ln_no}: atom index out of range")
+                bonded["exclusions"][(min(i, j), max(i, j))] = (float(cols[2]), float(cols[3]))
             else:
                 charges[int(cols[0])] = float(cols[1])
     flat = [a for g in groups for a in g]
@@ -221,6 +250,8 @@ def read_topology(path, n, constraint_type_names):
         raise ConfigError("an atom appears in more than one [constraints] row")
     if charges and len(charges) != n:
         raise ConfigError(f"[charges] section assigns charges to {len(charges)} of {n} atoms")
+    if bonded_types is not None:
+        return groups, group_types, charges, bonded
     return groups, group_types, charges
 
 
@@ -228,7 +259,7 @@ def read_topology(path, n, constraint_type_names):
 
 _TOP = {"schema_version", "units", "init", "topology", "simulation", "phase", "minimization",
         "particle_types", "pair_interactions", "lennard_jones", "spme", "constraint_types",
-        "neighbor_list"}
+        "neighbor_list", "bond_types", "angle_types", "dihedral_types"}
 
 
 def load(path):
@@ -314,6 +345,7 @@ def load(path):
         if d_hh >= 2.0 * d_oh:
             raise ConfigError("settle: d_HH must be below 2 d_OH")
         cfg["constraint_types"][ct["name"]] = {"d_oh": d_oh, "d_hh": d_hh}
+    cfg["bonded_types"] = _bonded_types(raw, units)
     # phases, in document order ([[minimization]] tables keep their position
     # only relative to each other here: they run before the MD phases)
     cfg["phases"] = []
@@ -358,6 +390,86 @@ def load(path):
     if not cfg["phases"]:
         raise ConfigError("no [[phase]] or [[minimization]] entries")
     return cfg
+
+
+def _bonded_types(raw, units):
+    """[[bond_types]] (harmonic k, r0; morse de, a, re), [[angle_types]]
+    (harmonic k_theta, theta_0) and [[dihedral_types]] (periodic k_phi, n,
+    phi_0, scale_lj_14 = 0.5, scale_coul_14 = 1/1.2) by name
+    (src/io/config.rs:505-600; harmonic_bond.rs, morse.rs, angle.rs,
+    dihedral.rs parameter tables).  Values: (force-field kind, parameters
+    in atomic units[, 1-4 scales])."""
+    out = {"bond": {}, "angle": {}, "dihedral": {}}
+    spec = {("bond_types", "harmonic"): ("harmonic-bond", [("k", "stiffness"), ("r0", "length")]),
+            ("bond_types", "morse"): ("morse", [("de", "energy"), ("a", "inverse_length"),
+                                                ("re", "length")]),
+            ("angle_types", "harmonic"): ("harmonic-angle", [("k_theta", "energy"),
+                                                             ("theta_0", None)]),
+            ("dihedral_types", "periodic"): ("periodic-dihedral", [("k_phi", "energy"),
+                                                                   ("phi_0", None), ("n", None)])}
+    for table, cls in (("bond_types", "bond"), ("angle_types", "angle"),
+                       ("dihedral_types", "dihedral")):
+        for t in raw.get(table, []):
+            t = dict(t)
+            name = _require(t, "name", f"[[{table}]]")
+            key = (table, t.get("kind"))
+            if key not in spec:
+                raise ConfigError(f"[[{table}]] `{name}`: kind `{t.get('kind')}` is not supported")
+            kind, fields = spec[key]
+            allowed = {"name", "kind"} | {f for f, _ in fields}
+            if cls == "dihedral":
+                allowed |= {"scale_lj_14", "scale_coul_14"}
+            _keys(t, f"[[{table}]] `{name}`", allowed)
+            vals = []
+            for f, dim in fields:
+                v = _require(t, f, f"[[{table}]] `{name}`")
+                vals.append(float(v) if dim is None else _from_user(units, dim, v))
+            if cls == "dihedral":
+                if float(vals[2]) != int(vals[2]) or vals[2] < 0:
+                    raise ConfigError(f"[[dihedral_types]] `{name}`: n must be a non-negative "
+                                      "integer")
+                # the force field's parameter order: k_phi, phi_0, n
+                out[cls][name] = (kind, vals, (float(t.get("scale_lj_14", 0.5)),
+                                               float(t.get("scale_coul_14", 1.0 / 1.2))))
+            else:
+                out[cls][name] = (kind, vals, None)
+    return out
+
+
+def _effective_exclusions(n, explicit, bonds, angles, groups, dihedrals):
+    """The effective exclusion list of src/forces/topology.rs:1166-1260:
+    explicit rows as given; bonds and angles' 1-3 pairs at scale 0; every
+    pair inside a constraint group at scale 0; then each dihedral's 1-4 pair
+    (in sorted order, first wins) at its type's 1-4 scales, where no entry
+    covers it yet.  Returns a symmetric CSR (offsets, partners, lj scales,
+    coulomb scales)."""
+    eff = {}
+    for (i, j), sc in explicit.items():
+        eff[(min(i, j), max(i, j))] = sc
+    def add(i, j, sc):
+        key = (min(i, j), max(i, j))
+        if key not in eff:
+            eff[key] = sc
+    for i, j, _ in bonds:
+        add(i, j, (0.0, 0.0))
+    for i, _, k, _ in angles:
+        add(i, k, (0.0, 0.0))
+    for g in groups:
+        for a in range(len(g)):
+            for b in range(a + 1, len(g)):
+                add(g[a], g[b], (0.0, 0.0))
+    for i, _, _, l, sc in sorted(dihedrals, key=lambda d: (min(d[0], d[3]), max(d[0], d[3]))):
+        add(i, l, sc)
+    rows = [[] for _ in range(n)]
+    for (i, j), (slj, sco) in sorted(eff.items()):
+        rows[i].append((j, slj, sco))
+        rows[j].append((i, slj, sco))
+    off = np.zeros(n + 1, np.uint32)
+    off[1:] = np.cumsum([len(r) for r in rows])
+    par = np.array([p for r in rows for p, _, _ in sorted(r)], np.uint32)
+    slj = np.array([s for r in rows for _, s, _ in sorted(r)], np.float32)
+    sco = np.array([s for r in rows for _, _, s in sorted(r)], np.float32)
+    return off, par, slj, sco
 
 
 def _thermostat(t, units, integrator_kind):
@@ -467,8 +579,11 @@ def build(cfg):
     n = init["n"]
     q = np.array([cfg["types"][t]["charge"] for t in init["types"]], np.float64)
     groups, gtypes = [], []
+    bonded = {"bonds": [], "angles": [], "dihedrals": [], "exclusions": {}}
     if cfg["topology"]:
-        groups, gtypes, charges = read_topology(cfg["topology"], n, list(cfg["constraint_types"]))
+        groups, gtypes, charges, bonded = read_topology(cfg["topology"], n,
+                                                        list(cfg["constraint_types"]),
+                                                        cfg["bonded_types"])
         if charges:
             q = np.array([charges[i] for i in range(n)], np.float64)
     mass = np.array([cfg["types"][t]["mass"] for t in init["types"]], np.float64)
@@ -489,19 +604,29 @@ def build(cfg):
         g = np.array(groups, np.uint32)
         settle = {"groups": g, "d_oh": geo["d_oh"], "d_hh": geo["d_hh"],
                   "m_o": float(mass[g[0, 0]]), "m_h": float(mass[g[0, 1]])}
-        # implicit exclusions within each constraint group
-        # (src/forces/topology.rs:1171-1177), symmetric CSR
-        part = [[] for _ in range(n)]
-        for a, b, c in groups:
-            part[a] += [b, c]; part[b] += [a, c]; part[c] += [a, b]
-        off = np.zeros(n + 1, np.uint32)
-        off[1:] = np.cumsum([len(x) for x in part])
-        par = np.array([p for x in part for p in sorted(x)], np.uint32)
-        ff["exclusions"] = (off, par)
+    # bonded sets (one per force-field kind, types numbered in declaration
+    # order) and the effective exclusion list
+    bt = cfg["bonded_types"]
+    sets = []
+    for section, cls in (("bonds", "bond"), ("angles", "angle"), ("dihedrals", "dihedral")):
+        by_kind = {}
+        for row in bonded[section]:
+            kind = bt[cls][row[-1]][0]
+            by_kind.setdefault(kind, []).append(row)
+        for kind, rows in by_kind.items():
+            names_k = [nm for nm, v in bt[cls].items() if v[0] == kind]
+            params = [bt[cls][nm][1] for nm in names_k]
+            terms = np.array([r[:-1] + [names_k.index(r[-1])] for r in rows], np.uint32)
+            sets.append((kind, terms, params))
+    dihed = [r[:4] + [bt["dihedral"][r[4]][2]] for r in bonded["dihedrals"]]
+    if groups or bonded["exclusions"] or bonded["bonds"] or bonded["angles"] or dihed:
+        ff["exclusions"] = _effective_exclusions(
+            n, bonded["exclusions"], bonded["bonds"], bonded["angles"], groups, dihed)
     n_constraints = 3 * len(groups)
     return {"n": n, "lattice": init["lattice"], "posq": posq, "mass": mass,
             "types": init["types"], "vel": init["vel"], "images": init["images"], "ff": ff,
-            "settle": settle, "n_constraints": n_constraints, "type_names": names}
+            "settle": settle, "n_constraints": n_constraints, "type_names": names,
+            "bonded": sets}
 
 
 def host_kinetic_energy(mass, vel):
@@ -542,6 +667,8 @@ def run(path, device=0):
     if sysd["images"] is not None:
         part.upload_images(sysd["images"])
     field = api.ForceField(ctx, part, sysd["ff"])
+    for kind, terms, params in sysd["bonded"]:
+        field.add_bonded(kind, terms, params)
     dof = max(0, 3 * n - sysd["n_constraints"] - 3)
     summary = []
     try:

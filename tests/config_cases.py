@@ -169,3 +169,77 @@ mode = "cell-list"
 r_skin = 1.0e-10
 """)
     return d / "water.in.toml"
+
+
+def butane_run(d, n_side=4, n_steps=20):
+    """United-atom butane (4 LJ sites per chain) with harmonic bonds and
+    angles and a periodic dihedral from `[[bond_types]]` / `[[angle_types]]`
+    / `[[dihedral_types]]` and the topology's `[bonds]` / `[angles]` /
+    `[dihedrals]` rows; no constraints, no charges."""
+    L = 25.0e-10
+    a = L / n_side
+    th = np.deg2rad(112.0)
+    b = 1.54e-10
+    chain = np.array([[0, 0, 0], [b, 0, 0], [b - b * np.cos(th), b * np.sin(th), 0],
+                      [2 * b - b * np.cos(th), b * np.sin(th), 0]])
+    idx = np.stack(np.meshgrid(*[np.arange(n_side)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    pos = np.concatenate([(c + 0.2) * a + chain - 0.5 * L for c in idx])
+    n_mol = idx.shape[0]
+    write_xyz(d / "butane.in.xyz", (L, L, L), ["C"] * (4 * n_mol), pos)
+    rows = ["[bonds]"] + [f"{4 * m + k} {4 * m + k + 1} cc" for m in range(n_mol) for k in range(3)]
+    rows += ["[angles]"] + [f"{4 * m + k} {4 * m + k + 1} {4 * m + k + 2} ccc"
+                            for m in range(n_mol) for k in range(2)]
+    rows += ["[dihedrals]"] + [f"{4 * m} {4 * m + 1} {4 * m + 2} {4 * m + 3} cccc"
+                               for m in range(n_mol)]
+    (d / "butane.in.topology").write_text("\n".join(rows) + "\n")
+    na = 6.02214076e23
+    (d / "butane.in.toml").write_text(f"""schema_version = 1
+init = "butane.in.xyz"
+topology = "butane.in.topology"
+
+[simulation]
+seed = 2
+temperature = 300.0
+
+[[phase]]
+name = "md"
+n_steps = {n_steps}
+dt = 1.0e-15
+[phase.integrator]
+kind = "velocity-verlet"
+lossless = false
+[phase.output]
+trajectory_every = 10
+log_every = 10
+
+[[particle_types]]
+name = "C"
+mass = {14.027 * U.DALTON_KG!r}
+
+[[pair_interactions]]
+between = ["C", "C"]
+kind = "lennard-jones"
+sigma = 3.95e-10
+epsilon = {0.38e3 / na!r}
+cutoff = 9.0e-10
+
+[[bond_types]]
+name = "cc"
+kind = "harmonic"
+k = {2.5e5 * 1e3 / na / 1e-18!r}
+r0 = 1.54e-10
+
+[[angle_types]]
+name = "ccc"
+kind = "harmonic"
+k_theta = {520e3 / na!r}
+theta_0 = 1.9548
+
+[[dihedral_types]]
+name = "cccc"
+kind = "periodic"
+k_phi = {2.9e3 / na!r}
+n = 3
+phi_0 = 0.0
+""")
+    return d / "butane.in.toml"

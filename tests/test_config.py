@@ -89,8 +89,9 @@ def test_water_build_settle_and_exclusions(tmp_path):
     assert n == 3 * 512 and sysd["n_constraints"] == 3 * 512
     g = sysd["settle"]["groups"]
     assert g.shape == (512, 3) and g[5].tolist() == [15, 16, 17]
-    off, par = sysd["ff"]["exclusions"]
+    off, par, slj, sco = sysd["ff"]["exclusions"]
     assert off[-1] == 2 * n and par[off[16]:off[17]].tolist() == [15, 17]
+    assert not slj.any() and not sco.any()
     assert sysd["posq"][1, 3] == np.float32(0.41 * U.CHARGE_E_TO_C / U.CHARGE_E_TO_C)
 
 
@@ -174,3 +175,26 @@ def test_host_kinetic_energy_order():
         a, b, c = (float(x) for x in v[i])
         s += float(np.float32(m[i])) * (a * a + b * b + c * c)
     assert C.host_kinetic_energy(m, v) == 0.5 * s
+
+
+def test_bonded_topology_and_effective_exclusions(tmp_path):
+    """Bonded types and topology rows (src/io/config.rs:505-600,
+    src/forces/topology.rs): one set per kind with its types in declaration
+    order, parameters converted once; the effective exclusions of
+    topology.rs:1166-1260: bonds and the angles' 1-3 pairs at scale 0, the
+    dihedral's 1-4 pair at its type's default AMBER scales (0.5, 1/1.2)."""
+    cfg = C.load(K.butane_run(tmp_path))
+    sysd = C.build(cfg)
+    kinds = {k: (t, q) for k, t, q in sysd["bonded"]}
+    assert sorted(kinds) == ["harmonic-angle", "harmonic-bond", "periodic-dihedral"]
+    t, q = kinds["harmonic-bond"]
+    assert t.shape == (3 * 64, 3) and t[1].tolist() == [1, 2, 0]
+    k_au = 2.5e5 * 1e3 / 6.02214076e23 / 1e-18 / (U.ENERGY_HARTREE_TO_J / U.LENGTH_BOHR_TO_M ** 2)
+    assert q[0][0] == pytest.approx(k_au, rel=1e-14) and q[0][1] == pytest.approx(1.54e-10 / U.LENGTH_BOHR_TO_M)
+    t, q = kinds["periodic-dihedral"]
+    assert t[0].tolist() == [0, 1, 2, 3, 0] and q[0][1:] == [0.0, 3.0]
+    off, par, slj, sco = sysd["ff"]["exclusions"]
+    assert par[off[0]:off[1]].tolist() == [1, 2, 3]
+    assert slj[off[0]:off[1]].tolist() == [0.0, 0.0, 0.5]
+    assert sco[off[0]:off[1]] == pytest.approx([0.0, 0.0, 1.0 / 1.2])
+    assert par[off[5]:off[6]].tolist() == [4, 6, 7]
